@@ -1,0 +1,251 @@
+"""CPU oracle for the AnalogSat integrator -- test infrastructure only.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` import this package, and only as the checker.  It wraps two C
+libraries built by ``oracle/Makefile``:
+
+* ``libasat_oracle.so`` -- the C restatement (``asat_oracle.c``) of the
+  reference CPU path: ``CpuSat`` (AnalogSAT/cpu_sat/CpuSat.cpp:64-241),
+  ``CpuRungeKutta`` (AnalogSAT/cpu_sat/CpuRungeKutta.cpp:65-267),
+  ``SatSolver::DoSolve`` (AnalogSAT/solver/SatSolver.h:273-355), the problem
+  container (AnalogSAT/problem/SatProblem.cpp:64-95), ``FastSatMaker``
+  (AnalogSAT/problem/FastSatMaker.cpp:59-107) and ``CpuRandom``
+  (AnalogSAT/cpu_sat/CpuRandom.cpp:25-52).  Two precision modes: ``f64``
+  (the reference's double instantiation) and ``mix`` (float state, double step
+  control -- the GPU FP32 mode, which has no reference counterpart).
+* ``_ref/libasat_ref.so`` -- the reference's own CPU sources compiled in place
+  (optional; present only where it was built).
+
+Pinning status of each function is listed in the module docstrings of
+``tests/test_oracle_golden.py`` and ``tests/test_oracle_math.py``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libasat_oracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libasat_ref.so")
+
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_vp = ctypes.c_void_p
+
+_lib = None
+_ref = None
+
+RESULT_NAMES = {0: "solution found", 1: "underflow", 2: "user interrupt",
+                3: "maximum time reached", 4: "maximum steps reached", 5: "unknown result"}
+
+
+def build():
+    """Compile the restatement (and the reference checker where its sources exist)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+    if os.path.isdir("/root/reference/AnalogSAT"):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        for sfx, fp, ftype in (("_f64", _f64p, ctypes.c_double), ("_mix", _f32p, ctypes.c_float)):
+            getattr(L, "orc_rhs" + sfx).argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, ftype, fp, fp]
+            getattr(L, "orc_violations" + sfx).argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, fp]
+            getattr(L, "orc_violations" + sfx).restype = ctypes.c_int
+            f = getattr(L, "orc_rk_create" + sfx)
+            f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_double]
+            f.restype = _vp
+            getattr(L, "orc_rk_destroy" + sfx).argtypes = [_vp]
+            getattr(L, "orc_rk_reset" + sfx).argtypes = [_vp]
+            getattr(L, "orc_rk_set_b" + sfx).argtypes = [_vp, ctypes.c_double]
+            getattr(L, "orc_rk_set_state" + sfx).argtypes = [_vp, fp]
+            getattr(L, "orc_rk_get_state" + sfx).argtypes = [_vp, fp]
+            getattr(L, "orc_rk_step" + sfx).argtypes = [_vp, ctypes.c_int]
+            getattr(L, "orc_rk_step" + sfx).restype = ctypes.c_int
+            getattr(L, "orc_rk_violations" + sfx).argtypes = [_vp]
+            getattr(L, "orc_rk_violations" + sfx).restype = ctypes.c_int
+            for nm in ("time", "hdid", "hnext", "errmax"):
+                g = getattr(L, "orc_rk_" + nm + sfx)
+                g.argtypes = [_vp]
+                g.restype = ctypes.c_double
+            for nm in ("steps", "attempts", "rhscount"):
+                g = getattr(L, "orc_rk_" + nm + sfx)
+                g.argtypes = [_vp]
+                g.restype = ctypes.c_long
+            f = getattr(L, "orc_solve" + sfx)
+            f.argtypes = [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_int,
+                          ctypes.POINTER(ctypes.c_int)]
+            f.restype = ctypes.c_int
+        L.orc_problem_build.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _vp]
+        L.orc_uniform_f64.argtypes = [ctypes.c_uint, _f64p, ctypes.c_int]
+        L.orc_random_state_f64.argtypes = [ctypes.c_uint, ctypes.c_int, ctypes.c_int, _f64p]
+        L.orc_fastsat.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulong, _i32p]
+        _lib = L
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    """The reference's own CPU path (oracle/_ref), or None where it was not built."""
+    global _ref
+    if _ref is None and ref_available():
+        R = ctypes.CDLL(REF_PATH)
+        R.ref_problem_sizes.argtypes = [_i32p, _i32p, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
+        R.ref_rhs_f64.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_double, _f64p, _f64p]
+        R.ref_violations_f64.argtypes = [_i32p, _i32p, ctypes.c_int, _f64p]
+        R.ref_violations_f64.restype = ctypes.c_int
+        R.ref_rk_create.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_double]
+        R.ref_rk_create.restype = _vp
+        R.ref_rk_destroy.argtypes = [_vp]
+        R.ref_rk_set_state.argtypes = [_vp, _f64p]
+        R.ref_rk_get_state.argtypes = [_vp, _f64p]
+        R.ref_rk_step.argtypes = [_vp, ctypes.c_int]
+        R.ref_rk_step.restype = ctypes.c_int
+        for nm in ("time", "hdid"):
+            getattr(R, "ref_rk_" + nm).argtypes = [_vp]
+            getattr(R, "ref_rk_" + nm).restype = ctypes.c_double
+        for nm in ("steps", "attempts", "rhscount"):
+            getattr(R, "ref_rk_" + nm).argtypes = [_vp]
+            getattr(R, "ref_rk_" + nm).restype = ctypes.c_long
+        R.ref_rk_violations.argtypes = [_vp]
+        R.ref_rk_violations.restype = ctypes.c_int
+        R.ref_solve.argtypes = [_i32p, _i32p, ctypes.c_int, _f64p, ctypes.c_double, ctypes.c_int,
+                                ctypes.c_int, ctypes.c_double, _i32p, ctypes.c_int,
+                                ctypes.POINTER(ctypes.c_int), _f64p,
+                                np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS"),
+                                ctypes.POINTER(ctypes.c_double)]
+        R.ref_solve.restype = ctypes.c_int
+        R.ref_fastsat.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulong, _i32p]
+        R.ref_fastsat.restype = ctypes.c_int
+        R.ref_uniform_f64.argtypes = [ctypes.c_uint, _f64p, ctypes.c_int]
+        _ref = R
+    return _ref
+
+
+# ----------------------------------------------------------------------------
+# problem helpers
+# ----------------------------------------------------------------------------
+class Problem:
+    """Padded clause-major matrix as the reference's CpuSat stores it.
+
+    ``literals``/``lengths`` are the flat DIMACS-style clause list
+    (SatProblem::AddClauses input); ``C`` is the m*k matrix with zero padding.
+    """
+
+    def __init__(self, literals, lengths):
+        self.literals = np.ascontiguousarray(literals, dtype=np.int32)
+        self.lengths = np.ascontiguousarray(lengths, dtype=np.int32)
+        n, m, k = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        L = lib()
+        L.orc_problem_build(self.literals, self.lengths, len(self.lengths), ctypes.byref(n),
+                            ctypes.byref(m), ctypes.byref(k), None)
+        self.n, self.m, self.k = n.value, m.value, k.value
+        self.C = np.zeros(max(self.m * self.k, 1), dtype=np.int32)
+        L.orc_problem_build(self.literals, self.lengths, len(self.lengths), ctypes.byref(n),
+                            ctypes.byref(m), ctypes.byref(k), self.C.ctypes.data)
+
+    @classmethod
+    def from_clauses(cls, clauses):
+        lits = [l for c in clauses for l in c]
+        return cls(np.array(lits, dtype=np.int32), np.array([len(c) for c in clauses], dtype=np.int32))
+
+    @property
+    def nclauses(self):
+        return len(self.lengths)
+
+
+def fastsat(n, m, k, seed):
+    """FastSatMaker(seed) instance as a Problem (restated generator)."""
+    out = np.zeros(m * k, dtype=np.int32)
+    lib().orc_fastsat(n, m, k, seed, out)
+    return Problem(out, np.full(m, k, dtype=np.int32))
+
+
+def uniform_f64(seed, length):
+    out = np.zeros(length, dtype=np.float64)
+    lib().orc_uniform_f64(seed, out, length)
+    return out
+
+
+def random_state_f64(seed, n, m):
+    out = np.zeros(n + m, dtype=np.float64)
+    lib().orc_random_state_f64(seed, n, m, out)
+    return out
+
+
+def rhs(problem, x, mode="f64", b=0.0):
+    """One RHS on the internal layout x = [-1, s, a] (length n+m+1)."""
+    dt = np.float64 if mode == "f64" else np.float32
+    x = np.ascontiguousarray(x, dtype=dt)
+    out = np.zeros_like(x)
+    getattr(lib(), "orc_rhs_" + mode)(problem.n, problem.m, problem.k, problem.C, b, x, out)
+    return out
+
+
+def violations(problem, pub, mode="f64"):
+    dt = np.float64 if mode == "f64" else np.float32
+    x = np.concatenate([[-1.0], np.asarray(pub, dtype=np.float64)]).astype(dt)
+    return getattr(lib(), "orc_violations_" + mode)(problem.n, problem.m, problem.k, problem.C, x)
+
+
+class RK:
+    """Restated CpuRungeKutta bound to one trajectory (mode 'f64' or 'mix')."""
+
+    def __init__(self, problem, eps=1e-4, mode="f64", b=0.0):
+        self.problem = problem
+        self.mode = mode
+        self.dt = np.float64 if mode == "f64" else np.float32
+        self._L = lib()
+        self._sfx = "_" + mode
+        self._h = getattr(self._L, "orc_rk_create" + self._sfx)(problem.n, problem.m, problem.k, problem.C, eps)
+        if b:
+            self._call("orc_rk_set_b", b)
+
+    def _call(self, name, *args):
+        return getattr(self._L, name + self._sfx)(self._h, *args)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            getattr(self._L, "orc_rk_destroy" + self._sfx)(self._h)
+            self._h = None
+
+    def set_state(self, pub):
+        self._call("orc_rk_set_state", np.ascontiguousarray(pub, dtype=self.dt))
+
+    def get_state(self):
+        out = np.zeros(self.problem.n + self.problem.m, dtype=self.dt)
+        self._call("orc_rk_get_state", out)
+        return out
+
+    def step(self, count=1):
+        return bool(self._call("orc_rk_step", count))
+
+    def reset(self):
+        self._call("orc_rk_reset")
+
+    def violations(self):
+        return self._call("orc_rk_violations")
+
+    time = property(lambda s: s._call("orc_rk_time"))
+    hdid = property(lambda s: s._call("orc_rk_hdid"))
+    hnext = property(lambda s: s._call("orc_rk_hnext"))
+    errmax = property(lambda s: s._call("orc_rk_errmax"))
+    steps = property(lambda s: s._call("orc_rk_steps"))
+    attempts = property(lambda s: s._call("orc_rk_attempts"))
+    rhscount = property(lambda s: s._call("orc_rk_rhscount"))
+
+    def solve(self, max_time=100.0, max_steps=100000, batch=20, max_batches=100000):
+        viol = np.zeros(max_batches, dtype=np.int32)
+        nb = ctypes.c_int()
+        res = self._call("orc_solve", max_time, max_steps, batch, viol, max_batches, ctypes.byref(nb))
+        return res, viol[: min(nb.value, max_batches)]

@@ -1,0 +1,185 @@
+/*
+ * CPU oracle for the AnalogSat CTDS integrator -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This library is the checker for the HIP implementation.  Only the test
+ * suite, the smoke check and the CPU-baseline leg of the benchmark may load
+ * it; the product library never links or calls it.
+ *
+ * Contents (each restates a piece of the reference's CPU path, see the
+ * file:line citations next to each routine):
+ *   - asat_oracle_impl.h, instantiated twice:
+ *       *_f64  state double, control double  (== CpuSat<double>+CpuRungeKutta<double>)
+ *       *_mix  state float,  control double  (definition of the GPU FP32 mode)
+ *   - problem container semantics (SatProblem::AddClauses)
+ *   - FastSatMaker's clause generator and CpuRandom's uniform stream, both
+ *     driven by a restatement of libstdc++'s minstd_rand0 + generate_canonical
+ *
+ * Pinning (see tests/test_oracle_*.py): FP64 RHS, step sequences, violation
+ * counts, generator and uniform streams are compared bit for bit with the
+ * reference's own sources compiled into oracle/_ref (golden fixtures under
+ * tests/golden/), and the RHS/tableau/controller against closed forms.
+ * The *_mix mode has no reference counterpart: it is pinned by closed forms,
+ * by agreement with *_f64 to float accuracy on single RHS evaluations, and by
+ * the GPU FP32 path agreeing with it.
+ *
+ * Build: gcc -O2 -ffp-contract=off -fPIC -shared (oracle/Makefile).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ST double
+#define CT double
+#define SFX _f64
+#define ST_SIN sin
+#include "asat_oracle_impl.h"
+#undef ST
+#undef CT
+#undef SFX
+#undef ST_SIN
+
+#define ST float
+#define CT double
+#define SFX _mix
+#define ST_SIN sinf
+#include "asat_oracle_impl.h"
+#undef ST
+#undef CT
+#undef SFX
+#undef ST_SIN
+
+/* ---------------------------------------------------------------------------
+ * Problem container: SatProblem::AddClauses (AnalogSAT/problem/SatProblem.cpp:64-95)
+ * Drops zero literals and empty clauses; N = max |lit|, K = max clause length.
+ * Writes the padded clause-major matrix C[j*K + q] (zeros pad short clauses,
+ * CpuSat::InitProblem, AnalogSAT/cpu_sat/CpuSat.cpp:352-377).
+ * Call with C == NULL to get the sizes only.
+ * ------------------------------------------------------------------------- */
+int orc_problem_build(const int *literals, const int *lengths, int nclauses,
+                      int *out_n, int *out_m, int *out_k, int *C)
+{
+    int n = 0, m = 0, k = 0;
+    size_t off = 0;
+    for (int j = 0; j < nclauses; j++) {
+        int len1 = 0;
+        for (int i = 0; i < lengths[j]; i++) {
+            int lit = literals[off + i];
+            if (lit == 0) continue;
+            int a = lit < 0 ? -lit : lit;
+            if (a > n) n = a;
+            len1++;
+        }
+        if (len1 > 0) { m++; if (len1 > k) k = len1; }
+        off += (size_t)lengths[j];
+    }
+    *out_n = n; *out_m = m; *out_k = k;
+    if (!C) return 0;
+    memset(C, 0, sizeof(int) * (size_t)m * (size_t)k);
+    off = 0;
+    int row = 0;
+    for (int j = 0; j < nclauses; j++) {
+        int q = 0;
+        for (int i = 0; i < lengths[j]; i++) {
+            int lit = literals[off + i];
+            if (lit == 0) continue;
+            C[(size_t)row * k + q++] = lit;
+        }
+        if (q > 0) row++;
+        off += (size_t)lengths[j];
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * libstdc++ std::default_random_engine (= minstd_rand0: x <- 16807 x mod 2^31-1,
+ * seed s -> s mod m, 0 -> 1) and uniform_real_distribution<double>(0,1), which
+ * draws generate_canonical<double,53>: two engine outputs combined as
+ * ((u1-1) + (u2-1) r) / r^2 with r = 2^31 - 2, clamped below 1.
+ * (Published behaviour of GCC's <random>; the reference uses it in
+ * AnalogSAT/problem/FastSatMaker.h:54-55 and AnalogSAT/cpu_sat/CpuRandom.h.)
+ * ------------------------------------------------------------------------- */
+typedef struct { uint64_t x; } orc_lcg_t;
+
+static void lcg_seed(orc_lcg_t *g, uint64_t s)
+{
+    uint64_t v = s % 2147483647ULL;
+    g->x = v == 0 ? 1 : v;
+}
+
+static uint64_t lcg_next(orc_lcg_t *g)
+{
+    g->x = (g->x * 16807ULL) % 2147483647ULL;
+    return g->x;
+}
+
+static double canonical_double(orc_lcg_t *g)
+{
+    const double r = 2147483646.0;
+    double sum = 0.0, tmp = 1.0;
+    for (int i = 0; i < 2; i++) {
+        sum += (double)(lcg_next(g) - 1) * tmp;
+        tmp *= r;
+    }
+    double ret = sum / tmp;
+    if (ret >= 1.0) ret = nextafter(1.0, 0.0);
+    return ret;
+}
+
+/* CpuRandom<double>(seed).GenerateUniform(out, len)  (AnalogSAT/cpu_sat/CpuRandom.cpp:25-52) */
+void orc_uniform_f64(unsigned int seed, double *out, int len)
+{
+    orc_lcg_t g;
+    lcg_seed(&g, seed);
+    for (int i = 0; i < len; i++) out[i] = canonical_double(&g);
+}
+
+/* CpuSat::SetRandomState from a CpuRandom<double>(seed) stream
+   (AnalogSAT/cpu_sat/CpuSat.cpp:122-133): public [s = 2u-1, a = u+1] */
+void orc_random_state_f64(unsigned int seed, int n, int m, double *pub)
+{
+    orc_uniform_f64(seed, pub, n + m);
+    for (int i = 0; i < n; i++) pub[i] = pub[i] * 2.0 - 1.0;
+    for (int i = n; i < n + m; i++) pub[i] = pub[i] + 1.0;
+}
+
+/* LocalSort (AnalogSAT/util/utils.cpp:44-56): ascending */
+static void local_sort(int *v, int len)
+{
+    if (len <= 1) return;
+    if (len == 2) { if (v[0] > v[1]) { int t = v[0]; v[0] = v[1]; v[1] = t; } return; }
+    if (len == 3) {
+        int t;
+        if (v[0] > v[1]) { t = v[0]; v[0] = v[1]; v[1] = t; }
+        if (v[0] > v[2]) { t = v[0]; v[0] = v[2]; v[2] = t; }
+        if (v[1] > v[2]) { t = v[1]; v[1] = v[2]; v[2] = t; }
+        return;
+    }
+    for (int i = 1; i < len; i++) {          /* insertion sort: same ascending result */
+        int x = v[i], j = i - 1;
+        while (j >= 0 && v[j] > x) { v[j + 1] = v[j]; j--; }
+        v[j + 1] = x;
+    }
+}
+
+/* FastSatMaker(seed) with Set_N/Set_M/Set_K, MakeSatProblem
+   (AnalogSAT/problem/FastSatMaker.cpp:59-107): m clauses of k distinct
+   variables floor(u*n)+1, sorted, redrawn on repeats, then each literal kept
+   positive if u < 0.5 else negated.  out: m*k literals, clause-major. */
+void orc_fastsat(int n, int m, int k, unsigned long seed, int *out)
+{
+    orc_lcg_t g;
+    lcg_seed(&g, (uint64_t)seed);
+    int clause[64];
+    for (int j = 0; j < m; j++) {
+        int rep;
+        do {
+            for (int i = 0; i < k; i++) clause[i] = (int)floor(canonical_double(&g) * n) + 1;
+            local_sort(clause, k);
+            rep = 0;
+            for (int i = 0; i < k - 1; i++) if (clause[i] == clause[i + 1]) rep = 1;
+        } while (rep);
+        for (int i = 0; i < k; i++) clause[i] *= canonical_double(&g) < 0.5 ? 1 : -1;
+        memcpy(out + (size_t)j * k, clause, sizeof(int) * (size_t)k);
+    }
+}

@@ -1,0 +1,323 @@
+/*
+ * Type-generic body of the CPU oracle (test infrastructure only).
+ *
+ * Included by asat_oracle.c once per precision mode with these macros set:
+ *   ST   state / right-hand-side arithmetic type (double or float)
+ *   CT   step-control type: time, h, error norm, accept/reject (double)
+ *   SFX  symbol suffix (_f64 for ST=double, _mix for ST=float)
+ *
+ * Every routine restates a routine of the reference CPU path and follows its
+ * floating-point operation order, so that with -ffp-contract=off the FP64 mode
+ * reproduces the reference's results bit for bit:
+ *   rhs        AnalogSAT/cpu_sat/CpuSat.cpp:64-94 (GetDerivatives) and
+ *              :197-241 (CalculateAllEx<K>), products as in
+ *              AnalogSAT/cpu_sat/cpu_helper.h:23-113 (CpuProdWrapper/CpuKmiWrapper)
+ *   violations AnalogSAT/cpu_sat/CpuSat.cpp:135-174
+ *   rk step    AnalogSAT/cpu_sat/CpuRungeKutta.cpp:65-100 (Step),
+ *              :180-215 (rkck), :217-267 (rkqs), :168-177 (Reset);
+ *              tableau AnalogSAT/cpu_sat/CpuRungeKutta.h:72-112
+ *   solve      AnalogSAT/solver/SatSolver.h:273-355 (DoSolve, no callback)
+ *
+ * In the mixed mode (ST=float, CT=double) the state arithmetic stays in float
+ * exactly as CpuSat<float> does it, while xscal, the error estimate, the max
+ * norm and the step controller run in double (h enters the float stage sums
+ * as (float)h).  This mode has no counterpart in the reference; it is the
+ * definition the GPU FP32 path is checked against.
+ */
+
+#define CAT2(a, b) a##b
+#define CAT(a, b) CAT2(a, b)
+#define FN(name) CAT(name, SFX)
+
+/* one Cash-Karp integrator bound to a problem; state layout [-1, s_1..s_N, a_1..a_M] */
+typedef struct {
+    int n, m, k;
+    const int *C;        /* clause matrix, clause-major: C[j*k + q], 0 = padding */
+    ST knorm;
+    ST b;                /* bias prefactor; 0 disables the sine term */
+    int len;             /* n + m + 1 */
+    ST *x, *xnew, *temp, *dxdt, *ak2, *ak3, *ak4, *ak5, *ak6;
+    CT *xscal, *xerr;
+    CT time, hdid, hnext, eps;
+    long steps, attempts, rhscount;
+    CT last_errmax;
+} FN(orc_rk_t);
+
+/* ---- right-hand side (CpuSat::GetDerivatives / CalculateAllEx<K>) ---- */
+
+/* product of s[0..kk-1] skipping position skip, left to right
+   (CpuProdWrapper<TFloat,K,Q>::KmiProd) */
+static inline ST FN(kmi_prod)(const ST *s, int kk, int skip)
+{
+    ST p = 0;
+    int first = 1;
+    for (int q = 0; q < kk; q++) {
+        if (q == skip) continue;
+        if (first) { p = s[q]; first = 0; }
+        else p = p * s[q];
+    }
+    if (first) p = (ST)1;   /* K == 1 */
+    return p;
+}
+
+void FN(orc_rhs)(int n, int m, int k, const int *C, ST b, const ST *state, ST *rhs)
+{
+    const int D = n + m + 1;
+    const ST one = (ST)1.0, zero = (ST)0.0;
+    const ST knorm = (ST)pow(2.0, -k);   /* CpuSat.cpp:355 */
+    const ST bb = b * knorm * knorm;      /* CpuSat.cpp:76 */
+    const ST alpha = (ST)m / (ST)n;       /* CpuSat.cpp:55 */
+    ST s[64];
+    int index[64];
+    ST sum = 0;
+
+    memset(rhs, 0, sizeof(ST) * (size_t)D);             /* dxdt.SetZero() */
+    for (int j = 0; j < m; j++) {
+        ST km = knorm;
+        for (int q = 0; q < k; q++) index[q] = C[(size_t)j * k + q];
+        for (int q = 0; q < k; q++) {
+            s[q] = index[q] != 0 ? state[abs(index[q])] : (ST)-1.0;
+            s[q] = index[q] < 0 ? one + s[q] : one - s[q];
+            km *= s[q];
+        }
+        int p = n + 1 + j;
+        ST am = state[p];
+        rhs[p] = am * km * km;
+        sum += am;
+        if (k <= 10) {
+            ST r = km * knorm * am;
+            /* CpuKmiWrapper::KmiFunc: stops at the first zero (padding) index */
+            for (int q = 0; q < k; q++) {
+                if (index[q] == 0) break;
+                ST rr = r * FN(kmi_prod)(s, k, q);
+                rr *= index[q] < 0 ? (ST)-2.0 : (ST)2.0;
+                rhs[abs(index[q])] += rr;
+            }
+        } else {
+            /* CpuSat::CalculateAll (CpuSat.cpp:290-349), used for K > 10:
+               temp = km * prod_{q' != q} f_q', then ((2*temp)*knorm)*am*sign;
+               padding slots add into rhs[0], which is cleared afterwards */
+            for (int q = 0; q < k; q++) {
+                ST temp = km;
+                for (int qq = 0; qq < k; qq++) temp *= (qq == q) ? one : s[qq];
+                ST sign = index[q] < 0 ? (ST)-1.0 : one;
+                rhs[abs(index[q])] += (ST)2.0 * temp * knorm * am * sign;
+            }
+        }
+    }
+    if (bb > zero) {
+        ST a = sum / (ST)m;
+        for (int i = 1; i < n + 1; i++)
+            rhs[i] += a * alpha * bb * (ST)1.570796326794897 * ST_SIN(state[i] * (ST)3.141592653589793);
+    }
+    rhs[0] = zero;
+}
+
+/* ---- violated-clause count (CpuSat::CalculateClauses) ---- */
+int FN(orc_violations)(int n, int m, int k, const int *C, const ST *state)
+{
+    (void)n;
+    int count = 0;
+    for (int j = 0; j < m; j++) {
+        int clause = 0;
+        for (int i = 0; i < k; i++) {
+            int item = C[(size_t)j * k + i];
+            int index = item < 0 ? -item : item;
+            int lit = state[index] > (ST)0;
+            if (item < 0) lit = !lit;
+            clause |= lit;
+            if (clause) break;
+        }
+        count += clause ? 0 : 1;
+    }
+    return count;
+}
+
+/* ---- Cash-Karp RK45 (CpuRungeKutta) ---- */
+
+FN(orc_rk_t) *FN(orc_rk_create)(int n, int m, int k, const int *C, CT eps)
+{
+    FN(orc_rk_t) *r = (FN(orc_rk_t) *)calloc(1, sizeof(*r));
+    r->n = n; r->m = m; r->k = k; r->C = C;
+    r->knorm = (ST)pow(2.0, -k);
+    r->len = n + m + 1;
+    size_t L = (size_t)r->len;
+    r->x = (ST *)calloc(L, sizeof(ST));    r->xnew = (ST *)calloc(L, sizeof(ST));
+    r->temp = (ST *)calloc(L, sizeof(ST)); r->dxdt = (ST *)calloc(L, sizeof(ST));
+    r->ak2 = (ST *)calloc(L, sizeof(ST));  r->ak3 = (ST *)calloc(L, sizeof(ST));
+    r->ak4 = (ST *)calloc(L, sizeof(ST));  r->ak5 = (ST *)calloc(L, sizeof(ST));
+    r->ak6 = (ST *)calloc(L, sizeof(ST));
+    r->xscal = (CT *)calloc(L, sizeof(CT)); r->xerr = (CT *)calloc(L, sizeof(CT));
+    r->x[0] = (ST)-1.0; r->xnew[0] = (ST)-1.0;
+    r->eps = eps;
+    r->b = 0;
+    /* Reset(): CpuRungeKutta.cpp:168-177 */
+    r->hnext = (CT)1e-9; r->steps = 0; r->attempts = 0; r->time = 0; r->rhscount = 0;
+    r->hdid = 0;
+    return r;
+}
+
+void FN(orc_rk_destroy)(FN(orc_rk_t) *r)
+{
+    if (!r) return;
+    free(r->x); free(r->xnew); free(r->temp); free(r->dxdt);
+    free(r->ak2); free(r->ak3); free(r->ak4); free(r->ak5); free(r->ak6);
+    free(r->xscal); free(r->xerr);
+    free(r);
+}
+
+void FN(orc_rk_reset)(FN(orc_rk_t) *r)
+{
+    r->hnext = (CT)1e-9; r->steps = 0; r->attempts = 0; r->time = 0; r->rhscount = 0;
+}
+
+void FN(orc_rk_set_b)(FN(orc_rk_t) *r, double b) { r->b = (ST)b; }
+
+/* public [s, a] -> internal [-1, s, a] (CpuSatStateImpl::CopyPartsFrom) */
+void FN(orc_rk_set_state)(FN(orc_rk_t) *r, const ST *pub)
+{
+    r->x[0] = (ST)-1.0;
+    memcpy(r->x + 1, pub, sizeof(ST) * (size_t)(r->n + r->m));
+}
+
+void FN(orc_rk_get_state)(const FN(orc_rk_t) *r, ST *pub)
+{
+    memcpy(pub, r->x + 1, sizeof(ST) * (size_t)(r->n + r->m));
+}
+
+static void FN(rhs_call)(FN(orc_rk_t) *r, ST *out, const ST *in)
+{
+    FN(orc_rhs)(r->n, r->m, r->k, r->C, r->b, in, out);
+    r->rhscount++;
+}
+
+/* rkck: CpuRungeKutta.cpp:180-215.  h is the control-type step; the state sums
+   use hs = (ST)h, which is h itself in the FP64 mode. */
+static void FN(rkck)(FN(orc_rk_t) *r, CT h)
+{
+    const int n = r->len;
+    const ST hs = (ST)h;
+    const ST b21 = (ST)0.2, b31 = (ST)(3.0 / 40.0), b32 = (ST)(9.0 / 40.0),
+             b41 = (ST)0.3, b42 = (ST)-0.9, b43 = (ST)1.2,
+             b51 = (ST)(-11.0 / 54.0), b52 = (ST)2.5, b53 = (ST)(-70.0 / 27.0), b54 = (ST)(35.0 / 27.0),
+             b61 = (ST)(1631.0 / 55296.0), b62 = (ST)(175.0 / 512.0), b63 = (ST)(575.0 / 13824.0),
+             b64 = (ST)(44275.0 / 110592.0), b65 = (ST)(253.0 / 4096.0),
+             c1 = (ST)(37.0 / 378.0), c3 = (ST)(250.0 / 621.0), c4 = (ST)(125.0 / 594.0),
+             c6 = (ST)(512.0 / 1771.0);
+    /* the error coefficients live in the control type (CpuRungeKutta.h:107-112) */
+    const CT C1 = (CT)(37.0 / 378.0), C3 = (CT)(250.0 / 621.0), C4 = (CT)(125.0 / 594.0),
+             C6 = (CT)(512.0 / 1771.0);
+    const CT dc5 = (CT)(-277.0 / 14336.0);
+    const CT dc1 = C1 - (CT)(2825.0 / 27648.0), dc3 = C3 - (CT)(18575.0 / 48384.0),
+             dc4 = C4 - (CT)(13525.0 / 55296.0), dc6 = C6 - (CT)0.25;
+    const ST *x = r->x;
+    ST *t = r->temp, *k1 = r->dxdt, *k2 = r->ak2, *k3 = r->ak3, *k4 = r->ak4, *k5 = r->ak5, *k6 = r->ak6;
+
+    for (int i = 0; i < n; i++) t[i] = x[i] + b21 * hs * k1[i];
+    FN(rhs_call)(r, k2, t);
+    for (int i = 0; i < n; i++) t[i] = x[i] + hs * (b31 * k1[i] + b32 * k2[i]);
+    FN(rhs_call)(r, k3, t);
+    for (int i = 0; i < n; i++) t[i] = x[i] + hs * (b41 * k1[i] + b42 * k2[i] + b43 * k3[i]);
+    FN(rhs_call)(r, k4, t);
+    for (int i = 0; i < n; i++) t[i] = x[i] + hs * (b51 * k1[i] + b52 * k2[i] + b53 * k3[i] + b54 * k4[i]);
+    FN(rhs_call)(r, k5, t);
+    for (int i = 0; i < n; i++) t[i] = x[i] + hs * (b61 * k1[i] + b62 * k2[i] + b63 * k3[i] + b64 * k4[i] + b65 * k5[i]);
+    FN(rhs_call)(r, k6, t);
+    for (int i = 0; i < n; i++) r->xnew[i] = x[i] + hs * (c1 * k1[i] + c3 * k3[i] + c4 * k4[i] + c6 * k6[i]);
+    for (int i = 0; i < n; i++)
+        r->xerr[i] = h * (dc1 * (CT)k1[i] + dc3 * (CT)k3[i] + dc4 * (CT)k4[i] + dc5 * (CT)k5[i] + dc6 * (CT)k6[i]);
+}
+
+/* rkqs: CpuRungeKutta.cpp:217-267.  Returns 1 on success, 0 on step underflow. */
+static int FN(rkqs)(FN(orc_rk_t) *r)
+{
+    const CT SAFETY = (CT)0.9, PGROW = (CT)-0.2, PSHRNK = (CT)-0.25, ERRCON = (CT)1.89e-4,
+             HFACTOR = (CT)0.1;
+    CT h = r->hnext;
+    CT ermax = 0;
+    const int n = r->len;
+    for (;;) {
+        FN(rkck)(r, h);
+        r->attempts++;
+        for (int i = 0; i < n; i++) r->xerr[i] = fabs(r->xerr[i] / r->xscal[i]);
+        ermax = 0;
+        for (int i = 0; i < n; i++) ermax = fmax(ermax, r->xerr[i]);
+        ermax /= r->eps;
+        r->last_errmax = ermax;
+        if (ermax <= (CT)1.0) break;
+        CT htemp = SAFETY * h * pow(ermax, PSHRNK);
+        h = h >= (CT)0 ? fmax(htemp, HFACTOR * h) : fmin(htemp, HFACTOR * h);
+        CT tnew = r->time + h;
+        if (tnew == r->time) return 0;
+    }
+    if (ermax > ERRCON) r->hnext = SAFETY * h * pow(ermax, PGROW);
+    else r->hnext = (CT)5.0 * h;
+    r->hdid = h;
+    r->time += r->hdid;
+    return 1;
+}
+
+/* Step(count): CpuRungeKutta.cpp:65-100.  Returns 1, or 0 on underflow. */
+int FN(orc_rk_step)(FN(orc_rk_t) *r, int count)
+{
+    const CT TINY = (CT)1e-20;
+    const int n = r->len;
+    for (int c = 0; c < count; c++) {
+        FN(rhs_call)(r, r->dxdt, r->x);
+        for (int i = 0; i < n; i++)
+            r->xscal[i] = fabs((CT)r->x[i]) + fabs((CT)r->dxdt[i] * r->hnext) + TINY;
+        if (FN(rkqs)(r)) {
+            r->steps++;
+            ST *tmp = r->x; r->x = r->xnew; r->xnew = tmp;
+        } else return 0;
+    }
+    return 1;
+}
+
+int FN(orc_rk_violations)(const FN(orc_rk_t) *r)
+{
+    return FN(orc_violations)(r->n, r->m, r->k, r->C, r->x);
+}
+
+/* SatSolver::DoSolve without a callback (SatSolver.h:317-339).
+   result codes follow AnalogSAT/solver/SatResult.h:24-32:
+   0 solution, 1 underflow, 2 interrupted, 3 max time, 4 max steps, 5 unknown.
+   out_batches (optional, size >= max_batches): violation count after each batch. */
+int FN(orc_solve)(FN(orc_rk_t) *r, CT max_time, int max_steps, int batch,
+                  int *out_batches, int max_batches, int *n_batches)
+{
+    int result = 5;
+    long steps = 0;
+    CT time = 0;
+    int nb = 0;
+    FN(orc_rk_reset)(r);
+    while (time < max_time && steps < max_steps) {
+        long before = r->steps;
+        if (!FN(orc_rk_step)(r, batch)) { result = 1; break; }
+        steps += r->steps - before;
+        time = r->time;
+        int viol = FN(orc_rk_violations)(r);
+        if (out_batches && nb < max_batches) out_batches[nb] = viol;
+        nb++;
+        if (viol == 0) { result = 0; break; }
+    }
+    if (result == 5) {
+        if (time >= max_time) result = 3;
+        else if (steps >= max_steps) result = 4;
+    }
+    if (n_batches) *n_batches = nb;
+    return result;
+}
+
+/* accessors for the Python side */
+double FN(orc_rk_time)(const FN(orc_rk_t) *r) { return (double)r->time; }
+double FN(orc_rk_hdid)(const FN(orc_rk_t) *r) { return (double)r->hdid; }
+double FN(orc_rk_hnext)(const FN(orc_rk_t) *r) { return (double)r->hnext; }
+double FN(orc_rk_errmax)(const FN(orc_rk_t) *r) { return (double)r->last_errmax; }
+long FN(orc_rk_steps)(const FN(orc_rk_t) *r) { return r->steps; }
+long FN(orc_rk_attempts)(const FN(orc_rk_t) *r) { return r->attempts; }
+long FN(orc_rk_rhscount)(const FN(orc_rk_t) *r) { return r->rhscount; }
+
+#undef FN
+#undef CAT
+#undef CAT2

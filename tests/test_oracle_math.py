@@ -1,0 +1,181 @@
+"""Pin the oracle against mathematics that fixes its answers (CPU only).
+
+These checks are independent of the reference build: closed forms of the
+CTDS right-hand side (AnalogSAT/cpu_sat/CpuSat.cpp:197-241), the gradient
+identity f_s = -dV/ds for V = sum_m a_m K_m^2, an exactly solvable one-clause
+system for the Cash-Karp tableau and controller
+(AnalogSAT/cpu_sat/CpuRungeKutta.cpp:180-267), brute-force violation counts,
+and the SatSolver loop's limit semantics (AnalogSAT/solver/SatSolver.h:317-339).
+They also pin the mixed (float state / double control) mode, which the
+reference does not have.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def internal(p, s, a):
+    return np.concatenate([[-1.0], np.asarray(s, float), np.asarray(a, float)])
+
+
+def energy(p, x):
+    """V = sum_m a_m K_m^2 with K_m = 2^-len prod (1 - c s), computed directly from the clauses."""
+    V = 0.0
+    off = 0
+    for j, L in enumerate(p.lengths):
+        lits = p.literals[off:off + L]
+        off += L
+        lits = lits[lits != 0]
+        K = 2.0 ** (-p.k)
+        for lit in lits:
+            c = 1.0 if lit > 0 else -1.0
+            K *= 1.0 - c * x[abs(lit)]
+        K *= 2.0 ** (p.k - len(lits))          # padding literals contribute a factor 2
+        V += x[p.n + 1 + j] * K * K
+    return V
+
+
+def test_zero_spins_closed_form():
+    p = O.fastsat(30, 120, 3, 5)
+    a = 1.0 + np.arange(p.m) / p.m
+    x = internal(p, np.zeros(p.n), a)
+    d = O.rhs(p, x)
+    assert np.array_equal(d[p.n + 1:], a * 4.0 ** -3)
+    want = np.zeros(p.n + 1)
+    C = p.C.reshape(p.m, p.k)
+    for j in range(p.m):
+        for lit in C[j]:
+            want[abs(lit)] += 2 * a[j] * np.sign(lit) * 4.0 ** -3
+    np.testing.assert_allclose(d[1:p.n + 1], want[1:], rtol=1e-14, atol=0)
+    assert d[0] == 0.0
+
+
+def test_satisfied_literal_kills_clause():
+    p = O.Problem.from_clauses([[1, -2, 3]])
+    for s1, s2, s3 in ([1.0, 0.3, -0.2], [0.4, -1.0, 0.9], [-0.5, 0.5, 1.0]):
+        d = O.rhs(p, internal(p, [s1, s2, s3], [1.7]))
+        assert np.all(d == 0.0)
+
+
+def test_padding_factor_two():
+    # a 2-literal clause inside a k=3 problem has K = 2^-2 (1-c1 s1)(1-c2 s2)
+    p = O.Problem.from_clauses([[1, -2], [1, 2, 3]])
+    assert p.k == 3
+    s = np.array([0.25, -0.5, 0.125])
+    x = internal(p, s, [1.5, 2.0])
+    d = O.rhs(p, x)
+    K0 = 0.25 * (1 - s[0]) * (1 + s[1])
+    assert d[p.n + 1] == pytest.approx(1.5 * K0 * K0, rel=1e-15)
+
+
+def test_gradient_identity_fd():
+    p = O.fastsat(25, 100, 3, 8)
+    rng = np.random.default_rng(0)
+    x = internal(p, rng.uniform(-1, 1, p.n), rng.uniform(1, 2, p.m))
+    d = O.rhs(p, x)
+    h = 1e-6
+    for i in range(1, p.n + 1):
+        xp, xm = x.copy(), x.copy()
+        xp[i] += h
+        xm[i] -= h
+        g = (energy(p, xp) - energy(p, xm)) / (2 * h)
+        assert d[i] == pytest.approx(-g, rel=1e-6, abs=1e-10)
+    # f_a = a K^2 exactly = V restricted to one clause
+    for j in range(p.m):
+        y = x.copy()
+        y[p.n + 1:] = 0.0
+        y[p.n + 1 + j] = x[p.n + 1 + j]
+        assert d[p.n + 1 + j] == pytest.approx(energy(p, y), rel=1e-14)
+
+
+def test_violations_bruteforce_and_planted():
+    rng = np.random.default_rng(1)
+    p = O.fastsat(12, 60, 3, 2)
+    C = p.C.reshape(p.m, p.k)
+    for _ in range(20):
+        s = rng.uniform(-1, 1, p.n)
+        truth = s > 0
+        want = 0
+        for row in C:
+            sat = any((truth[abs(l) - 1] if l > 0 else not truth[abs(l) - 1]) for l in row)
+            want += not sat
+        assert O.violations(p, np.concatenate([s, np.ones(p.m)])) == want
+        assert O.violations(p, np.concatenate([s, np.ones(p.m)]), mode="mix") == want
+    # planted: clauses built to satisfy a fixed assignment
+    plant = rng.random(20) < 0.5
+    clauses = []
+    while len(clauses) < 80:
+        v = rng.choice(np.arange(1, 21), 3, replace=False)
+        lits = [int(x) if rng.random() < 0.5 else -int(x) for x in v]
+        if any((plant[abs(l) - 1] if l > 0 else not plant[abs(l) - 1]) for l in lits):
+            clauses.append(lits)
+    pp = O.Problem.from_clauses(clauses)
+    s = np.where(plant, 0.5, -0.5)[: pp.n]
+    assert O.violations(pp, np.concatenate([s, np.ones(pp.m)])) == 0
+
+
+def test_mixed_mode_rhs_matches_f64_to_float_precision():
+    p = O.fastsat(200, 850, 3, 3)
+    rng = np.random.default_rng(2)
+    x = internal(p, rng.uniform(-1, 1, p.n), rng.uniform(1, 2, p.m)).astype(np.float32).astype(np.float64)
+    d64 = O.rhs(p, x)
+    d32 = O.rhs(p, x, mode="mix").astype(np.float64)
+    scale = np.abs(d64).max()
+    assert np.abs(d32 - d64).max() <= 2e-6 * scale
+
+
+def _one_clause_exact(u0, a0, t):
+    """One clause (x1), k=1: u = 1-s obeys w=u^2, w' = -C w + w^2/4 with C = a + u^2/4 conserved."""
+    C = a0 + u0 * u0 / 4.0
+    w = 1.0 / (1.0 / (4 * C) + (1.0 / (u0 * u0) - 1.0 / (4 * C)) * np.exp(C * t))
+    return w
+
+
+@pytest.mark.parametrize("mode", ["f64", "mix"])
+def test_tableau_and_controller_on_exact_solution(mode):
+    p = O.Problem.from_clauses([[1]])
+    s0, a0 = -0.5, 1.25
+    eps = 1e-10 if mode == "f64" else 1e-6
+    rk = O.RK(p, eps=eps, mode=mode)
+    rk.set_state([s0, a0])
+    for _ in range(40):
+        assert rk.step(5)
+        s, a = rk.get_state().astype(np.float64)
+        w = _one_clause_exact(1 - s0, a0, rk.time)
+        if mode == "f64":
+            assert (1 - s) ** 2 == pytest.approx(w, rel=1e-8)
+            assert a + (1 - s) ** 2 / 4 == pytest.approx(a0 + (1 - s0) ** 2 / 4, rel=1e-9)
+        else:
+            # float state: compare s itself (1 - s cancels near the fixed point s = 1)
+            assert s == pytest.approx(1 - np.sqrt(w), abs=3e-6)
+            assert a + (1 - s) ** 2 / 4 == pytest.approx(a0 + (1 - s0) ** 2 / 4, rel=3e-6)
+
+
+def test_initial_step_growth_is_five_fold():
+    # from h0 = 1e-9 the error is far below ERRCON, so hdid grows 5x per step
+    p = O.fastsat(50, 213, 3, 0)
+    pub = O.random_state_f64(1, p.n, p.m)
+    pub[p.n:] = 1.0
+    rk = O.RK(p, eps=1e-6)
+    rk.set_state(pub)
+    h = 1e-9
+    for _ in range(4):
+        rk.step(1)
+        assert rk.hdid == h
+        h = 5.0 * h
+    assert rk.attempts == 4 and rk.rhscount == 24
+
+
+def test_solve_limits_checked_between_batches():
+    p = O.fastsat(60, 270, 3, 4)
+    pub = O.random_state_f64(2, p.n, p.m)
+    pub[p.n:] = 1.0
+    rk = O.RK(p, eps=1e-6)
+    rk.set_state(pub)
+    res, viol = rk.solve(max_time=1e8, max_steps=1, batch=20)
+    # maxSteps is compared only after a whole batch
+    assert res in (0, 4) and rk.steps == 20 and len(viol) == 1
+    rk.set_state(pub)
+    res, viol = rk.solve(max_time=1e-12, max_steps=100000, batch=7)
+    assert res in (0, 3) and rk.steps == 7
