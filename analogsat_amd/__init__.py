@@ -1,0 +1,576 @@
+"""AnalogSat CTDS integrator for AMD Instinct MI355X (gfx950).
+
+Python mirror of the reference's plugin interface over the C ABI in
+``include/analogsat_hip.h`` (library ``analogsat_amd/libanalogsat_hip.so``):
+
+=====================  =====================================================
+this module            reference (AnalogSAT/...)
+=====================  =====================================================
+``HipSat``             ``ISat<TFloat,TState>`` / ``CudaSat3`` (solver/ISat.h:31-87)
+``HipState``           ``ISatState`` / ``CudaSatState2`` (solver/ISatState.h:31-161)
+``HipRungeKutta``      ``IODEInt`` / ``CudaRungeKutta`` (solver/IODEInt.h:28-60)
+``HipRandom``          ``ISatRandom`` / ``CudaRandom`` (solver/ISatRandom.h:25-33)
+``SatSolver``          ``SatSolver<TFloat,TState>`` (solver/SatSolver.h:41-356)
+``SatProblem``         ``SatProblem`` (problem/SatProblem.h:32-73)
+``FastSatMaker``       ``FastSatMaker`` (problem/FastSatMaker.h:29-61)
+``SatResult``          ``SatResult`` (solver/SatResult.h:24-32)
+=====================  =====================================================
+
+Methods keep the reference's names (``SetProblem``, ``GetDerivatives``,
+``Step``, ``Solve`` ...) and raise the reference's exception types
+(``ValueError`` for ``std::invalid_argument``, ``RuntimeError`` for
+``std::runtime_error``).  The library is required: importing works without
+it, but every object constructor raises if ``libanalogsat_hip.so`` is missing.
+"""
+import ctypes
+import enum
+import os
+import time as _time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libanalogsat_hip.so")
+
+ASAT_OK, ASAT_ERR_INVALID_ARGUMENT, ASAT_ERR_NOT_CONFIGURED, ASAT_ERR_HIP, ASAT_UNDERFLOW, ASAT_ERR_UNSUPPORTED = range(6)
+ASAT_F64, ASAT_F32 = 0, 1
+
+_lib = None
+
+
+class SatResult(enum.IntEnum):
+    """AnalogSAT/solver/SatResult.h:24-32."""
+    SAT_SOLUTION_FOUND = 0
+    SAT_UNDERFLOW = 1
+    SAT_ODE_INTERRUPTED = 2
+    SAT_MAXTIME_REACHED = 3
+    SAT_MAXITER_REACHED = 4
+    SAT_UNKNOWN = 5
+
+
+_RESULT_MSG = {0: "solution found", 1: "underflow", 2: "user interrupt", 3: "maximum time reached",
+               4: "maximum steps reached", 5: "unknown result"}
+
+
+def GetSatResultMessage(result):
+    """AnalogSAT/solver/SatResult.cpp:20-32."""
+    return _RESULT_MSG.get(int(result), "unknown result")
+
+
+class RkStats(ctypes.Structure):
+    _fields_ = [("time", ctypes.c_double), ("hdid", ctypes.c_double), ("hnext", ctypes.c_double),
+                ("errmax", ctypes.c_double), ("steps", ctypes.c_longlong), ("attempts", ctypes.c_longlong),
+                ("rhscount", ctypes.c_longlong)]
+
+
+def build():
+    from . import builder
+    return builder.build()
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libanalogsat_hip.so is not built: run python -m analogsat_amd.builder "
+                               "(or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, i = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int
+        pvp = ctypes.POINTER(ctypes.c_void_p)
+        sig = {
+            "asat_last_error": ([], ctypes.c_char_p),
+            "asat_version": ([], ctypes.c_char_p),
+            "asat_device_count": ([], i),
+            "asat_problem_fastsat": ([i, i, i, ctypes.c_ulong, vp], i),
+            "asat_problem_seed_engine": ([ctypes.c_ulong, ctypes.POINTER(ctypes.c_ulonglong)], i),
+            "asat_problem_fastsat_stream": ([i, i, i, ctypes.POINTER(ctypes.c_ulonglong), vp], i),
+            "asat_sys_create": ([i, pvp], i),
+            "asat_sys_destroy": ([vp], i),
+            "asat_sys_set_problem": ([vp, vp, vp, i], i),
+            "asat_sys_get_nmk": ([vp, ip, ip, ip], i),
+            "asat_sys_set_b": ([vp, ctypes.c_double], i),
+            "asat_sys_get_b": ([vp, ctypes.POINTER(ctypes.c_double)], i),
+            "asat_sys_rhs": ([vp, vp, vp, ctypes.c_double], i),
+            "asat_sys_violations": ([vp, vp, ip], i),
+            "asat_sys_set_random_state": ([vp, vp, vp], i),
+            "asat_state_create": ([vp, pvp], i),
+            "asat_state_destroy": ([vp], i),
+            "asat_state_set_public": ([vp, vp], i),
+            "asat_state_get_public": ([vp, vp], i),
+            "asat_state_get_boolean": ([vp, vp], i),
+            "asat_state_copy": ([vp, vp], i),
+            "asat_state_set_zero": ([vp], i),
+            "asat_state_alloc_size": ([vp, ip], i),
+            "asat_state_device_ptr": ([vp, pvp, ip], i),
+            "asat_rk_create": ([pvp], i),
+            "asat_rk_destroy": ([vp], i),
+            "asat_rk_set_eps": ([vp, ctypes.c_double], i),
+            "asat_rk_get_eps": ([vp, ctypes.POINTER(ctypes.c_double)], i),
+            "asat_rk_configure": ([vp, vp], i),
+            "asat_rk_step": ([vp, vp, vp, i], i),
+            "asat_rk_reset": ([vp], i),
+            "asat_rk_stats": ([vp, ctypes.POINTER(RkStats)], i),
+            "asat_rk_last_step_ms": ([vp, ctypes.POINTER(ctypes.c_float)], i),
+            "asat_rk_profile": ([vp, vp, vp, i, vp], i),
+            "asat_rng_create": ([ctypes.c_ulonglong, pvp], i),
+            "asat_rng_destroy": ([vp], i),
+            "asat_rng_uniform": ([vp, vp, i, i], i),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc == ASAT_OK:
+        return rc
+    msg = lib().asat_last_error().decode()
+    if rc == ASAT_ERR_INVALID_ARGUMENT:
+        raise ValueError(msg)
+    if rc == ASAT_UNDERFLOW:
+        return rc
+    raise RuntimeError(msg)
+
+
+def _prec(precision):
+    if precision in ("f64", "double", np.float64, ASAT_F64):
+        return ASAT_F64, np.float64
+    if precision in ("f32", "float", np.float32, ASAT_F32):
+        return ASAT_F32, np.float32
+    raise ValueError("precision must be 'f64' or 'f32'")
+
+
+# ----------------------------------------------------------------------------
+# problems
+# ----------------------------------------------------------------------------
+class SatProblem:
+    """Flat clause list (AnalogSAT/problem/SatProblem.h:32-73).
+
+    Holds the clauses as given; zero literals and empty clauses are dropped by
+    the library exactly as ``SatProblem::AddClauses`` does.
+    """
+
+    def __init__(self, literals=None, lengths=None):
+        self.literals = np.zeros(0, np.int32) if literals is None else np.ascontiguousarray(literals, np.int32)
+        self.lengths = np.zeros(0, np.int32) if lengths is None else np.ascontiguousarray(lengths, np.int32)
+
+    def AddClause(self, clause):
+        self.AddClauses(list(clause), [len(clause)])
+
+    def AddClauses(self, literals, lengths):
+        self.literals = np.concatenate([self.literals, np.asarray(literals, np.int32)])
+        self.lengths = np.concatenate([self.lengths, np.asarray(lengths, np.int32)])
+
+    def Clear(self):
+        self.literals = np.zeros(0, np.int32)
+        self.lengths = np.zeros(0, np.int32)
+
+    def _nonzero(self):
+        keep_l = self.literals != 0
+        clause_id = np.repeat(np.arange(len(self.lengths)), self.lengths)
+        nz_len = np.bincount(clause_id[keep_l], minlength=len(self.lengths))
+        return keep_l, nz_len
+
+    def Get_N(self):
+        return int(np.abs(self.literals).max()) if len(self.literals) else 0
+
+    def Get_M(self):
+        return int((self._nonzero()[1] > 0).sum()) if len(self.lengths) else 0
+
+    def Get_K(self):
+        return int(self._nonzero()[1].max()) if len(self.lengths) else 0
+
+    @classmethod
+    def from_clauses(cls, clauses):
+        return cls([l for c in clauses for l in c], [len(c) for c in clauses])
+
+    @classmethod
+    def read_cnf(cls, path):
+        """DIMACS reader with CnfReader semantics (AnalogSAT/io/CnfReader.cpp:48-141)."""
+        with open(path) as f:
+            text = f.read()
+        n = m = -1
+        toks = []
+        for line in text.splitlines():
+            t = line.strip()
+            if not t or t[0] in "c%":
+                continue
+            if t[0] == "p":
+                parts = t.split()
+                if len(parts) < 4 or parts[1] != "cnf":
+                    raise ValueError("expected: p cnf <variables> <clauses>")
+                n, m = int(parts[2]), int(parts[3])
+                continue
+            toks.extend(int(v) for v in t.split())
+        if n < 0 or m < 0:
+            raise ValueError("could not find a SAT declaration")
+        lits, lens, cur = [], [], 0
+        for v in toks:
+            if len(lens) == m:
+                break
+            if v == 0:
+                lens.append(cur)
+                cur = 0
+            else:
+                lits.append(v)
+                cur += 1
+        if len(lens) < m:
+            raise ValueError("file is missing %d clauses" % (m - len(lens)))
+        return cls(lits, lens)
+
+    def write_cnf(self, path):
+        """CnfWriter semantics (AnalogSAT/io/CnfWriter.cpp:49-74)."""
+        keep_l, nz = self._nonzero()
+        lits = self.literals[keep_l]
+        with open(path, "w") as f:
+            f.write("p cnf %d %d\n" % (self.Get_N(), int((nz > 0).sum())))
+            off = 0
+            for L in nz:
+                if L == 0:
+                    continue
+                f.write("".join("%d " % v for v in lits[off:off + L]) + "0\n")
+                off += L
+
+
+class FastSatMaker:
+    """AnalogSAT/problem/FastSatMaker.h:29-61 -- same seed, same instance stream.
+
+    Successive MakeSatProblem calls continue one generator stream, like the
+    reference's object.
+    """
+
+    def __init__(self, seed=None):
+        seed = int(_time.time_ns() & 0xFFFFFFFF) if seed is None else int(seed)
+        self._engine = ctypes.c_ulonglong()
+        _check(lib().asat_problem_seed_engine(seed, ctypes.byref(self._engine)))
+        self.k, self.n, self.m = 3, 100, 325   # FastSatMaker::SetDefaults
+
+    def Set_K(self, k): self.k = int(k)
+    def Set_N(self, n): self.n = int(n)
+    def Set_M(self, m): self.m = int(m)
+    def Get_K(self): return self.k
+    def Get_N(self): return self.n
+    def Get_M(self): return self.m
+
+    def MakeSatProblem(self, problem=None):
+        out = np.zeros(self.m * self.k, np.int32)
+        _check(lib().asat_problem_fastsat_stream(self.n, self.m, self.k, ctypes.byref(self._engine),
+                                                 out.ctypes.data))
+        p = SatProblem(out, np.full(self.m, self.k, np.int32))
+        if problem is not None:
+            problem.literals, problem.lengths = p.literals, p.lengths
+        return p
+
+
+def fastsat(n, m, k, seed):
+    """One FastSatMaker(seed) instance."""
+    out = np.zeros(m * k, np.int32)
+    _check(lib().asat_problem_fastsat(n, m, k, seed, out.ctypes.data))
+    return SatProblem(out, np.full(m, k, np.int32))
+
+
+# ----------------------------------------------------------------------------
+# system, states, integrator, rng
+# ----------------------------------------------------------------------------
+class HipState:
+    """ISatState over a device buffer (AnalogSAT/solver/ISatState.h:31-161)."""
+
+    def __init__(self, sat):
+        self.sat = sat
+        self._h = ctypes.c_void_p()
+        _check(lib().asat_state_create(sat._h, ctypes.byref(self._h)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value and _lib is not None:
+            _lib.asat_state_destroy(h)
+            self._h = None
+
+    def Get_N(self): return self.sat.Get_N()
+    def Get_M(self): return self.sat.Get_M()
+
+    def CopyTo(self, target=None):
+        out = np.zeros(self.sat.Get_N() + self.sat.Get_M(), self.sat.dtype)
+        _check(lib().asat_state_get_public(self._h, out.ctypes.data))
+        if target is not None:
+            target[:] = out
+        return out
+
+    def CopyFrom(self, source):
+        if isinstance(source, HipState):
+            _check(lib().asat_state_copy(self._h, source._h))
+            return
+        src = np.ascontiguousarray(source, dtype=self.sat.dtype)
+        if src.size < self.sat.Get_N() + self.sat.Get_M():
+            raise ValueError("source vector is too short")
+        _check(lib().asat_state_set_public(self._h, src.ctypes.data))
+
+    def CopyBooleanTo(self):
+        out = np.zeros(self.sat.Get_N(), np.uint8)
+        _check(lib().asat_state_get_boolean(self._h, out.ctypes.data))
+        return out.astype(bool)
+
+    def SetZero(self):
+        _check(lib().asat_state_set_zero(self._h))
+
+    def GetAllocSize(self):
+        v = ctypes.c_int()
+        _check(lib().asat_state_alloc_size(self._h, ctypes.byref(v)))
+        return v.value
+
+    def GetState(self):
+        """(device pointer, aux offset) of the internal layout."""
+        p, off = ctypes.c_void_p(), ctypes.c_int()
+        _check(lib().asat_state_device_ptr(self._h, ctypes.byref(p), ctypes.byref(off)))
+        return p.value, off.value
+
+
+class HipRandom:
+    """ISatRandom (AnalogSAT/solver/ISatRandom.h:25-33) on a Philox4x32-10 stream."""
+
+    def __init__(self, seed=None):
+        self._h = ctypes.c_void_p()
+        seed = int(_time.time_ns()) & 0xFFFFFFFFFFFFFFFF if seed is None else int(seed)
+        _check(lib().asat_rng_create(seed, ctypes.byref(self._h)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value and _lib is not None:
+            _lib.asat_rng_destroy(h)
+
+    def GenerateUniform(self, dev_ptr, length, precision="f64"):
+        _check(lib().asat_rng_uniform(self._h, ctypes.c_void_p(dev_ptr), int(length), _prec(precision)[0]))
+
+
+class HipSat:
+    """ISat on the GPU (AnalogSAT/solver/ISat.h:31-87); precision 'f64' or 'f32'."""
+
+    def __init__(self, precision="f64"):
+        self.precision, self.dtype = _prec(precision)
+        self._h = ctypes.c_void_p()
+        _check(lib().asat_sys_create(self.precision, ctypes.byref(self._h)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value and _lib is not None:
+            _lib.asat_sys_destroy(h)
+
+    def SetProblem(self, problem):
+        lits = np.ascontiguousarray(problem.literals, np.int32)
+        lens = np.ascontiguousarray(problem.lengths, np.int32)
+        _check(lib().asat_sys_set_problem(self._h, lits.ctypes.data, lens.ctypes.data, len(lens)))
+
+    def _nmk(self):
+        n, m, k = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().asat_sys_get_nmk(self._h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(k)))
+        return n.value, m.value, k.value
+
+    def Get_N(self): return self._nmk()[0]
+    def Get_M(self): return self._nmk()[1]
+    def Get_K(self): return self._nmk()[2]
+
+    def Get_B(self):
+        b = ctypes.c_double()
+        _check(lib().asat_sys_get_b(self._h, ctypes.byref(b)))
+        return b.value
+
+    def Set_B(self, b):
+        _check(lib().asat_sys_set_b(self._h, float(b)))
+
+    def MakeState(self):
+        return HipState(self)
+
+    def MakeRandomState(self, random):
+        st = self.MakeState()
+        self.SetRandomState(st, random)
+        return st
+
+    def SetRandomState(self, state, random):
+        _check(lib().asat_sys_set_random_state(self._h, state._h, random._h))
+
+    def GetDerivatives(self, dxdt, state, time=0.0):
+        _check(lib().asat_sys_rhs(self._h, dxdt._h, state._h, float(time)))
+
+    def GetClauseViolationCount(self, state):
+        if not isinstance(state, HipState):
+            tmp = self.MakeState()
+            tmp.CopyFrom(state)
+            state = tmp
+        v = ctypes.c_int()
+        _check(lib().asat_sys_violations(self._h, state._h, ctypes.byref(v)))
+        return v.value
+
+
+class HipRungeKutta:
+    """IODEInt (AnalogSAT/solver/IODEInt.h:28-60) -- Cash-Karp RK45, device-side step control."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        _check(lib().asat_rk_create(ctypes.byref(self._h)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value and _lib is not None:
+            _lib.asat_rk_destroy(h)
+
+    def SetEpsilon(self, eps):
+        _check(lib().asat_rk_set_eps(self._h, float(eps)))
+
+    def GetEpsilon(self):
+        e = ctypes.c_double()
+        _check(lib().asat_rk_get_eps(self._h, ctypes.byref(e)))
+        return e.value
+
+    def Configure(self, sat):
+        self._sat = sat
+        _check(lib().asat_rk_configure(self._h, sat._h))
+
+    def Step(self, old_state, new_state, count):
+        return _check(lib().asat_rk_step(self._h, old_state._h, new_state._h, int(count))) == ASAT_OK
+
+    def Reset(self):
+        _check(lib().asat_rk_reset(self._h))
+
+    def stats(self):
+        s = RkStats()
+        _check(lib().asat_rk_stats(self._h, ctypes.byref(s)))
+        return s
+
+    def GetTime(self): return self.stats().time
+    def GetLastStepSize(self): return self.stats().hdid
+    def GetRHSCount(self): return int(self.stats().rhscount)
+    def GetStepCount(self): return int(self.stats().steps)
+    def GetAttemptCount(self): return int(self.stats().attempts)
+
+    def profile_kernels(self, old_state, new_state, reps=20):
+        """Average microseconds per launch: clause J=1..6, var J=1..6, control."""
+        us = np.zeros(13, np.float32)
+        _check(lib().asat_rk_profile(self._h, old_state._h, new_state._h, int(reps), us.ctypes.data))
+        return us
+
+    def last_step_ms(self):
+        v = ctypes.c_float()
+        _check(lib().asat_rk_last_step_ms(self._h, ctypes.byref(v)))
+        return v.value
+
+
+# ----------------------------------------------------------------------------
+# SatSolver (AnalogSAT/solver/SatSolver.h:41-356)
+# ----------------------------------------------------------------------------
+class SatSolver:
+    """Batch loop around an ISat and an IODEInt; same semantics as the reference.
+
+    SAT is checked and limits are compared only between batches of
+    ``batchSize`` accepted steps (SatSolver.h:295-331).
+    """
+
+    def __init__(self, sat, odeint):
+        self.sat = sat
+        self.odeint = odeint
+        self.maxTime = 100.0
+        self.maxSteps = 100000
+        self.batchSize = 20
+        self.callback = None
+        self.running = False
+        self.oldState = self.newState = None
+        self._Reset()
+
+    def _Reset(self):
+        self.rhsCount = 0
+        self.walltime = 0.0
+        self.steps = 0
+        self.time = 0.0
+        self.lastStepSize = 0.0
+        self.lastViolationCount = 0
+
+    def GetStepCount(self): return self.steps
+    def GetRhsCount(self): return self.rhsCount
+    def GetElapsedTime(self): return self.time
+    def GetLastStepSize(self): return self.lastStepSize
+    def GetLastClauseViolationCount(self): return self.lastViolationCount
+
+    def GetElapsedWallTime(self):
+        if self.running:
+            self.walltime = _time.perf_counter() - self._wallstart
+        return self.walltime
+
+    def SetCallback(self, fn): self.callback = fn
+    def ClearCallback(self): self.callback = None
+    def GetBatchSize(self): return self.batchSize
+
+    def SetBatchSize(self, b):
+        if b < 1:
+            raise ValueError("Batch size must be positive")
+        self.batchSize = int(b)
+
+    def GetMaxTime(self): return self.maxTime
+
+    def SetMaxTime(self, t):
+        if t <= 0:
+            raise ValueError("Max time must be positive")
+        self.maxTime = float(t)
+
+    def GetMaxSteps(self): return self.maxSteps
+
+    def SetMaxSteps(self, s):
+        if s < 1:
+            raise ValueError("Max iterations must be positive")
+        self.maxSteps = int(s)
+
+    def SetProblem(self, problem):
+        self.sat.SetProblem(problem)
+        self._Reset()
+        self.oldState = self.sat.MakeState()
+        self.newState = self.sat.MakeState()
+        self.lastViolationCount = self.sat.GetClauseViolationCount(self.oldState)
+
+    def GetSatState(self):
+        return self.oldState
+
+    def SetSatState(self, state):
+        self.oldState.CopyFrom(state)
+        self.lastViolationCount = self.sat.GetClauseViolationCount(self.oldState)
+
+    def SetRandomInitialState(self, random):
+        self.sat.SetRandomState(self.oldState, random)
+        self.lastViolationCount = self.sat.GetClauseViolationCount(self.oldState)
+
+    def Solve(self):
+        self.running = True
+        self._Reset()
+        self.odeint.Configure(self.sat)
+        self._wallstart = _time.perf_counter()
+        result = SatResult.SAT_UNKNOWN
+        while self.time < self.maxTime and self.steps < self.maxSteps:
+            before = self.odeint.GetStepCount()
+            if not self.odeint.Step(self.oldState, self.newState, self.batchSize):
+                result = SatResult.SAT_UNDERFLOW
+                break
+            st = self.odeint.stats()
+            self.steps += int(st.steps) - before
+            self.time = st.time
+            self.lastViolationCount = self.sat.GetClauseViolationCount(self.oldState)
+            self.rhsCount = int(st.rhscount)
+            self.lastStepSize = st.hdid
+            if self.callback is not None and not self.callback(self):
+                result = SatResult.SAT_ODE_INTERRUPTED
+                break
+            if self.lastViolationCount == 0:
+                result = SatResult.SAT_SOLUTION_FOUND
+                break
+        if result == SatResult.SAT_UNKNOWN:
+            if self.time >= self.maxTime:
+                result = SatResult.SAT_MAXTIME_REACHED
+            elif self.steps >= self.maxSteps:
+                result = SatResult.SAT_MAXITER_REACHED
+        self.walltime = _time.perf_counter() - self._wallstart
+        st = self.odeint.stats()
+        self.time = st.time
+        self.rhsCount = int(st.rhscount)
+        self.lastStepSize = st.hdid
+        self.running = False
+        return result

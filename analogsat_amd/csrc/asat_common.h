@@ -1,0 +1,199 @@
+// Shared definitions of the AnalogSat MI355X library: error handling,
+// the device-side step-control block, the Cash-Karp tableau, wave64 helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "analogsat_hip.h"
+
+namespace asat {
+
+// ---------------------------------------------------------------------------
+// errors: every failing HIP call becomes an exception carrying a status code;
+// the extern "C" layer turns it into a return code + thread-local message.
+// ---------------------------------------------------------------------------
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void hip_check(hipError_t e, const char* what, const char* file, int line)
+{
+    if (e != hipSuccess) {
+        char buf[512];
+        snprintf(buf, sizeof(buf), "HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file,
+                 line);
+        throw Error(ASAT_ERR_HIP, buf);
+    }
+}
+#define HIP_CHECK(x) ::asat::hip_check((x), #x, __FILE__, __LINE__)
+
+inline void require(bool ok, int code, const char* msg)
+{
+    if (!ok) throw Error(code, msg);
+}
+
+// ---------------------------------------------------------------------------
+// device-resident step control for one trajectory (plain solver).  Written by
+// the control kernel only; every stage kernel reads it at entry.
+// ---------------------------------------------------------------------------
+struct alignas(64) Ctrl {
+    double h;          // step size of the current attempt
+    double hnext;      // step size proposed at the start of the current step (xscal uses it)
+    double time;       // analog time
+    double hdid;       // last accepted step
+    double errmax;     // last error norm / eps
+    double eps;
+    long long steps;      // accepted steps
+    long long attempts;   // attempts incl. rejected
+    long long rhscount;   // RHS evaluations
+    long long target;     // stop once steps reaches this
+    int parity;        // which of the two state buffers holds x
+    int step_start;    // 1: this attempt begins a new step (k1 must be computed)
+    int active;        // 0 once target reached or underflow
+    int underflow;
+    int violations;    // scratch for the violated-clause count
+    int pad_[3];
+};
+
+// ---------------------------------------------------------------------------
+// Cash-Karp tableau (Numerical Recipes in C, 2nd ed. rkck; reference values
+// AnalogSAT/cpu_sat/CpuRungeKutta.h:83-112).  State-type values are the
+// double constants rounded to T, the error coefficients are double.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Tab {
+    static constexpr T b21 = (T)0.2;
+    static constexpr T b31 = (T)(3.0 / 40.0), b32 = (T)(9.0 / 40.0);
+    static constexpr T b41 = (T)0.3, b42 = (T)-0.9, b43 = (T)1.2;
+    static constexpr T b51 = (T)(-11.0 / 54.0), b52 = (T)2.5, b53 = (T)(-70.0 / 27.0), b54 = (T)(35.0 / 27.0);
+    static constexpr T b61 = (T)(1631.0 / 55296.0), b62 = (T)(175.0 / 512.0), b63 = (T)(575.0 / 13824.0),
+                       b64 = (T)(44275.0 / 110592.0), b65 = (T)(253.0 / 4096.0);
+    static constexpr T c1 = (T)(37.0 / 378.0), c3 = (T)(250.0 / 621.0), c4 = (T)(125.0 / 594.0),
+                       c6 = (T)(512.0 / 1771.0);
+};
+struct TabD {
+    static constexpr double c1 = 37.0 / 378.0, c3 = 250.0 / 621.0, c4 = 125.0 / 594.0, c6 = 512.0 / 1771.0;
+    static constexpr double dc5 = -277.0 / 14336.0;
+    static constexpr double dc1 = c1 - 2825.0 / 27648.0, dc3 = c3 - 18575.0 / 48384.0,
+                            dc4 = c4 - 13525.0 / 55296.0, dc6 = c6 - 0.25;
+    static constexpr double TINY = 1e-20, SAFETY = 0.9, PGROW = -0.2, PSHRNK = -0.25, ERRCON = 1.89e-4,
+                            HFACTOR = 0.1;
+};
+
+// ---------------------------------------------------------------------------
+// Correctly rounded pow for the step controller's exponents.
+// y = -1/q for q = 4 (PSHRNK = -0.25, exact) and q = 5 (PGROW = -0.2, which is
+// -(1/5 + d) with d = double(0.2) - 1/5 = 1.1102230246251565e-17).
+// Start from r = pow(x, y) (a few ulp), apply one Newton step for x r^q = 1
+// with x r^q formed in double-double (FMA two-product), and the first-order
+// factor x^-d = 1 - d ln x; the result is x^y to ~1e-30 relative, rounded once.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline void dd_mul_d(double& hi, double& lo, double b)
+{
+    double p = hi * b;
+    double e = fma(hi, b, -p);
+    e = fma(lo, b, e);
+    double s = p + e;
+    lo = e - (s - p);
+    hi = s;
+}
+
+__host__ __device__ inline double pow_ctrl(double x, int q)
+{
+    const double y = q == 4 ? -0.25 : -0.2;
+    if (!(x > 0.0) || !(x < 1.0e300) || !(x > 1.0e-300)) return pow(x, y);
+    double r = pow(x, y);
+    double hi = r, lo = 0.0;
+    for (int i = 1; i < q; i++) dd_mul_d(hi, lo, r);
+    dd_mul_d(hi, lo, x);
+    double e = (1.0 - hi) - lo;              // 1 - x r^q, exact leading part (Sterbenz)
+    double corr = r * e / (double)q;
+    if (q == 5) corr -= r * (1.1102230246251565404e-17 * log(x));
+    return r + corr;
+}
+
+// stage input of stage J (2..6) from x and k1..k(J-1), in the reference's
+// operation order (AnalogSAT/cpu_sat/CpuRungeKutta.cpp:190-210)
+template <typename T, int J>
+__device__ __forceinline__ T stage_input(T x, T k1, T k2, T k3, T k4, T k5, T hs)
+{
+    using B = Tab<T>;
+    if constexpr (J == 2) return x + B::b21 * hs * k1;
+    else if constexpr (J == 3) return x + hs * (B::b31 * k1 + B::b32 * k2);
+    else if constexpr (J == 4) return x + hs * (B::b41 * k1 + B::b42 * k2 + B::b43 * k3);
+    else if constexpr (J == 5) return x + hs * (B::b51 * k1 + B::b52 * k2 + B::b53 * k3 + B::b54 * k4);
+    else return x + hs * (B::b61 * k1 + B::b62 * k2 + B::b63 * k3 + B::b64 * k4 + B::b65 * k5);
+}
+
+// final combination (xnew) and error term |xerr / xscal| of one element
+template <typename T>
+__device__ __forceinline__ void final_combination(T x, T k1, T k3, T k4, T k5, T k6, T hs, double h, double hnext,
+                                                  T& xnew, double& err)
+{
+    using B = Tab<T>;
+    xnew = x + hs * (B::c1 * k1 + B::c3 * k3 + B::c4 * k4 + B::c6 * k6);
+    double xerr = h * (TabD::dc1 * (double)k1 + TabD::dc3 * (double)k3 + TabD::dc4 * (double)k4 +
+                       TabD::dc5 * (double)k5 + TabD::dc6 * (double)k6);
+    double xscal = fabs((double)x) + fabs((double)k1 * hnext) + TabD::TINY;
+    err = fabs(xerr / xscal);
+}
+
+// ---------------------------------------------------------------------------
+// wave64 / block reductions (DPP-free, via __shfl_xor on 64 lanes)
+// fmax ignores NaN like the CPU reference's std::fmax.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ double block_max(double v, double* sh)
+{
+    v = wave_max(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x < 64) {
+        r = threadIdx.x < BLOCK / 64 ? sh[threadIdx.x] : 0.0;
+        r = wave_max(r);
+    }
+    __syncthreads();
+    return r;   // valid in thread 0
+}
+
+template <int BLOCK>
+__device__ __forceinline__ int block_sum(int v, int* sh)
+{
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    int r = 0;
+    if (threadIdx.x < 64) {
+        r = threadIdx.x < BLOCK / 64 ? sh[threadIdx.x] : 0;
+        r = wave_sum(r);
+    }
+    __syncthreads();
+    return r;
+}
+
+inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace asat

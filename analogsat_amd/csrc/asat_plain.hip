@@ -1,0 +1,1257 @@
+// Plain single-trajectory CTDS solver for MI355X (gfx950): one problem, one
+// trajectory, the whole chip.  Replaces CudaSat1/2/3 + CudaRungeKutta
+// (AnalogSAT/cuda_sat/CudaSat3.cu:33-430, AnalogSAT/cuda_sat/CudaRungeKutta.cu:56-484).
+//
+// Data layout in HBM (T = double for ASAT_F64, float for ASAT_F32):
+//   lit[m*K + q]      int32 signed literal (+v / -v, 0 = padding), clause-major
+//   csr_ptr[0..N+1]   var-major CSR row pointers over literal slots
+//   csr_slot[e]       slot index m*K+q, ascending within each variable
+//   contrib[m*K + q]  per-slot contribution 2 c a K K_mi of the current stage
+//   state buffer      [ -1, s_1..s_N, pad | a_1..a_M, pad ], aux at aoff
+//   k1..k5            same layout as the state (stage derivatives)
+//   tmp               spin part of the next stage's input (aux stage inputs are
+//                     recomputed inside the clause pass and never stored)
+//
+// One RK45 attempt = 6 x (clause pass + var pass) + 1 control kernel:
+//   clause pass J: gather the stage-J spins, K_m, aux stage input and
+//                  k_J(a_m) = a K^2, per-slot contributions (coalesced writes)
+//   var pass J:    deterministic per-variable sum over the CSR (slot order ==
+//                  the reference's accumulation order), k_J(s_i), and the
+//                  spin part of stage J+1's input
+//   J = 6 passes also form x_new and |xerr/xscal| and a per-block FP64 max;
+//   the control kernel finishes the max-norm and runs the Cash-Karp
+//   accept/reject/step-size logic in FP64 on the device.
+// Step(count) enqueues attempts without host round trips and reads the
+// control block back once per round of attempts.
+#include "asat_common.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <vector>
+
+namespace asat {
+
+thread_local std::string g_last_error;
+
+constexpr int BLK = 256;          // stage-kernel block size (4 waves)
+constexpr int CTRL_BLK = 1024;    // control kernel block size
+constexpr int GEN_KMAX = 32;      // longest clause supported by the generic kernels
+
+static inline long long round_up(long long a, long long b) { return (a + b - 1) / b * b; }
+
+// ===========================================================================
+// problem (host)
+// ===========================================================================
+struct HostProblem {
+    int n = 0, m = 0, k = 0;
+    std::vector<int> lit;       // m*k, zero padded
+    std::vector<int> csr_ptr;   // n+2
+    std::vector<int> csr_slot;  // number of non-padding slots
+};
+
+// SatProblem::AddClauses semantics (AnalogSAT/problem/SatProblem.cpp:64-95) +
+// CpuSat::InitProblem clause matrix (AnalogSAT/cpu_sat/CpuSat.cpp:352-377) +
+// the transposed var-major slot list (role of CudaSat3::InitProblem,
+// AnalogSAT/cuda_sat/CudaSat3.cu:266-307, without the pad-to-32).
+static void build_problem(const int* literals, const int* lengths, int nclauses, HostProblem& P)
+{
+    require(nclauses >= 0, ASAT_ERR_INVALID_ARGUMENT, "negative clause count");
+    require(nclauses == 0 || (literals && lengths), ASAT_ERR_INVALID_ARGUMENT, "null clause arrays");
+    int n = 0, m = 0, k = 0;
+    size_t off = 0;
+    for (int j = 0; j < nclauses; j++) {
+        require(lengths[j] >= 0, ASAT_ERR_INVALID_ARGUMENT, "negative clause length");
+        int len1 = 0;
+        for (int i = 0; i < lengths[j]; i++) {
+            int l = literals[off + i];
+            if (l == 0) continue;
+            require(l != INT32_MIN, ASAT_ERR_INVALID_ARGUMENT, "literal out of range");
+            n = std::max(n, std::abs(l));
+            len1++;
+        }
+        if (len1 > 0) { m++; k = std::max(k, len1); }
+        off += (size_t)lengths[j];
+    }
+    require(m > 0 && k > 0, ASAT_ERR_INVALID_ARGUMENT, "empty SAT problem");
+    require(k <= GEN_KMAX, ASAT_ERR_UNSUPPORTED, "clauses longer than 32 literals are not supported");
+    require((long long)m * k < (1LL << 31), ASAT_ERR_UNSUPPORTED, "more than 2^31 literal slots");
+    P.n = n; P.m = m; P.k = k;
+    P.lit.assign((size_t)m * k, 0);
+    off = 0;
+    int row = 0;
+    for (int j = 0; j < nclauses; j++) {
+        int q = 0;
+        for (int i = 0; i < lengths[j]; i++) {
+            int l = literals[off + i];
+            if (l == 0) continue;
+            P.lit[(size_t)row * k + q++] = l;
+        }
+        if (q > 0) row++;
+        off += (size_t)lengths[j];
+    }
+    // counting sort of slots by variable; slots visited in ascending order
+    P.csr_ptr.assign((size_t)n + 2, 0);
+    for (size_t s = 0; s < P.lit.size(); s++) {
+        int l = P.lit[s];
+        if (l) P.csr_ptr[(size_t)std::abs(l) + 1]++;
+    }
+    for (int i = 0; i <= n; i++) P.csr_ptr[i + 1] += P.csr_ptr[i];
+    P.csr_slot.assign((size_t)P.csr_ptr[n + 1], 0);
+    std::vector<int> fill(P.csr_ptr.begin(), P.csr_ptr.end() - 1);
+    for (size_t s = 0; s < P.lit.size(); s++) {
+        int l = P.lit[s];
+        if (l) P.csr_slot[(size_t)fill[std::abs(l)]++] = (int)s;
+    }
+}
+
+// ===========================================================================
+// device kernels
+// ===========================================================================
+template <typename T>
+struct StageArgs {
+    const int* lit;
+    const int* csr_ptr;
+    const int* csr_slot;
+    T* contrib;
+    T* xa;       // state buffer 0
+    T* xb;       // state buffer 1 (x = parity ? xb : xa; x_new is the other)
+    T* k1; T* k2; T* k3; T* k4; T* k5;
+    T* tmp;      // spin stage input (n+1)
+    double* partial;   // per-block error maxima: clause blocks first, then var blocks
+    const Ctrl* ctrl;  // null: standalone RHS (x = xa, output to k1, no tmp)
+    int n, m, k, aoff;
+    int nblk_clause;
+    T knorm;
+};
+
+template <typename T>
+__device__ __forceinline__ T spin_factor(int l, const T* sp)
+{
+    T s = l != 0 ? sp[l < 0 ? -l : l] : (T)-1.0;
+    return l < 0 ? (T)1.0 + s : (T)1.0 - s;
+}
+
+// product of f[0..K-1] except position Q, left to right (reference KmiProd order)
+template <typename T, int K, int Q>
+__device__ __forceinline__ T kmi_prod(const T (&f)[K])
+{
+    T p = (T)1.0;
+    bool first = true;
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+        if (q == Q) continue;
+        p = first ? f[q] : p * f[q];
+        first = false;
+    }
+    return p;
+}
+
+template <typename T, int K, int Q>
+__device__ __forceinline__ void write_contribs(const int (&l)[K], const T (&f)[K], T r, T* out)
+{
+    if constexpr (Q < K) {
+        if (l[Q] != 0) {
+            T rr = r * kmi_prod<T, K, Q>(f);
+            rr *= l[Q] < 0 ? (T)-2.0 : (T)2.0;
+            out[Q] = rr;
+            write_contribs<T, K, Q + 1>(l, f, r, out);
+        }
+    }
+}
+
+// ---- clause pass, compile-time K (Ex order, K <= 10) ---------------------
+template <typename T, int K, int J>
+__global__ __launch_bounds__(BLK) void k_clause(StageArgs<T> a)
+{
+    __shared__ double sh[BLK / 64];
+    const Ctrl* c = a.ctrl;
+    int parity = 0;
+    double h = 0.0, hnext = 0.0;
+    if (c) {
+        if (!c->active) return;
+        if (J == 1 && !c->step_start) return;
+        parity = c->parity; h = c->h; hnext = c->hnext;
+    }
+    const T hs = (T)h;
+    const T* x = parity ? a.xb : a.xa;
+    T* xnew = parity ? a.xa : a.xb;
+    const T* sp = (J == 1) ? x : a.tmp;
+    const int m = blockIdx.x * BLK + threadIdx.x;
+    double err = 0.0;
+    if (m < a.m) {
+        int l[K];
+        T f[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) l[q] = a.lit[(size_t)m * K + q];
+        T km = a.knorm;
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            f[q] = spin_factor<T>(l[q], sp);
+            km *= f[q];
+        }
+        const size_t ai = (size_t)a.aoff + m;
+        T am, k1 = 0, k3 = 0, k4 = 0, k5 = 0;
+        if constexpr (J == 1) {
+            am = x[ai];
+        } else {
+            k1 = a.k1[ai];
+            T k2 = J > 2 ? a.k2[ai] : (T)0;
+            k3 = J > 3 ? a.k3[ai] : (T)0;
+            k4 = J > 4 ? a.k4[ai] : (T)0;
+            k5 = J > 5 ? a.k5[ai] : (T)0;
+            am = stage_input<T, J>(x[ai], k1, k2, k3, k4, k5, hs);
+        }
+        T kj = am * km * km;
+        if constexpr (J == 1) a.k1[ai] = kj;
+        else if constexpr (J == 2) a.k2[ai] = kj;
+        else if constexpr (J == 3) a.k3[ai] = kj;
+        else if constexpr (J == 4) a.k4[ai] = kj;
+        else if constexpr (J == 5) a.k5[ai] = kj;
+        else {
+            T xn;
+            final_combination<T>(x[ai], k1, k3, k4, k5, kj, hs, h, hnext, xn, err);
+            xnew[ai] = xn;
+        }
+        const T r = km * a.knorm * am;
+        T out[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) out[q] = (T)0;
+        write_contribs<T, K, 0>(l, f, r, out);
+#pragma unroll
+        for (int q = 0; q < K; q++)
+            if (l[q] != 0) a.contrib[(size_t)m * K + q] = out[q];
+    }
+    if constexpr (J == 6) {
+        double bm = block_max<BLK>(err, sh);
+        if (threadIdx.x == 0) a.partial[blockIdx.x] = bm;
+    }
+}
+
+// ---- clause pass, runtime K (any K <= 32; K > 10 uses CalculateAll order) --
+template <typename T, int J>
+__global__ __launch_bounds__(BLK) void k_clause_gen(StageArgs<T> a)
+{
+    __shared__ double sh[BLK / 64];
+    __shared__ T fsh[GEN_KMAX * BLK];
+    const Ctrl* c = a.ctrl;
+    int parity = 0;
+    double h = 0.0, hnext = 0.0;
+    if (c) {
+        if (!c->active) return;
+        if (J == 1 && !c->step_start) return;
+        parity = c->parity; h = c->h; hnext = c->hnext;
+    }
+    const T hs = (T)h;
+    const T* x = parity ? a.xb : a.xa;
+    T* xnew = parity ? a.xa : a.xb;
+    const T* sp = (J == 1) ? x : a.tmp;
+    const int K = a.k;
+    const int m = blockIdx.x * BLK + threadIdx.x;
+    T* f = fsh + threadIdx.x;   // f[q*BLK]
+    double err = 0.0;
+    if (m < a.m) {
+        const int* lm = a.lit + (size_t)m * K;
+        T km = a.knorm;
+        for (int q = 0; q < K; q++) {
+            T fq = spin_factor<T>(lm[q], sp);
+            km *= fq;
+            f[q * BLK] = fq;
+        }
+        const size_t ai = (size_t)a.aoff + m;
+        T am, k1 = 0, k3 = 0, k4 = 0, k5 = 0;
+        if constexpr (J == 1) {
+            am = x[ai];
+        } else {
+            k1 = a.k1[ai];
+            T k2 = J > 2 ? a.k2[ai] : (T)0;
+            k3 = J > 3 ? a.k3[ai] : (T)0;
+            k4 = J > 4 ? a.k4[ai] : (T)0;
+            k5 = J > 5 ? a.k5[ai] : (T)0;
+            am = stage_input<T, J>(x[ai], k1, k2, k3, k4, k5, hs);
+        }
+        T kj = am * km * km;
+        if constexpr (J == 1) a.k1[ai] = kj;
+        else if constexpr (J == 2) a.k2[ai] = kj;
+        else if constexpr (J == 3) a.k3[ai] = kj;
+        else if constexpr (J == 4) a.k4[ai] = kj;
+        else if constexpr (J == 5) a.k5[ai] = kj;
+        else {
+            T xn;
+            final_combination<T>(x[ai], k1, k3, k4, k5, kj, hs, h, hnext, xn, err);
+            xnew[ai] = xn;
+        }
+        T* out = a.contrib + (size_t)m * K;
+        if (K <= 10) {
+            const T r = km * a.knorm * am;
+            for (int q = 0; q < K; q++) {
+                int l = lm[q];
+                if (l == 0) break;
+                T p = (T)1.0;
+                bool first = true;
+                for (int qq = 0; qq < K; qq++) {
+                    if (qq == q) continue;
+                    p = first ? f[qq * BLK] : p * f[qq * BLK];
+                    first = false;
+                }
+                T rr = r * p;
+                rr *= l < 0 ? (T)-2.0 : (T)2.0;
+                out[q] = rr;
+            }
+        } else {
+            for (int q = 0; q < K; q++) {
+                int l = lm[q];
+                if (l == 0) continue;
+                T temp = km;
+                for (int qq = 0; qq < K; qq++) temp *= (qq == q) ? (T)1.0 : f[qq * BLK];
+                out[q] = (T)2.0 * temp * a.knorm * am * (l < 0 ? (T)-1.0 : (T)1.0);
+            }
+        }
+    }
+    if constexpr (J == 6) {
+        double bm = block_max<BLK>(err, sh);
+        if (threadIdx.x == 0) a.partial[blockIdx.x] = bm;
+    }
+}
+
+// ---- var pass: deterministic gather-sum + spin stage combination ---------
+template <typename T, int J>
+__global__ __launch_bounds__(BLK) void k_var(StageArgs<T> a)
+{
+    __shared__ double sh[BLK / 64];
+    const Ctrl* c = a.ctrl;
+    int parity = 0;
+    double h = 0.0, hnext = 0.0;
+    bool fresh = true;
+    if (c) {
+        if (!c->active) return;
+        parity = c->parity; h = c->h; hnext = c->hnext;
+        fresh = (J != 1) || c->step_start;
+    }
+    const T hs = (T)h;
+    const T* x = parity ? a.xb : a.xa;
+    T* xnew = parity ? a.xa : a.xb;
+    const int i = blockIdx.x * BLK + threadIdx.x + 1;
+    double err = 0.0;
+    if (i <= a.n) {
+        T kj = (T)0;
+        if (fresh) {
+            const int e0 = a.csr_ptr[i], e1 = a.csr_ptr[i + 1];
+            T sum = (T)0;
+            for (int e = e0; e < e1; e++) sum += a.contrib[a.csr_slot[e]];
+            kj = sum;
+        }
+        if constexpr (J == 1) {
+            if (fresh) a.k1[i] = kj;
+            else kj = a.k1[i];
+            if (a.tmp) a.tmp[i] = stage_input<T, 2>(x[i], kj, 0, 0, 0, 0, hs);
+        } else if constexpr (J == 2) {
+            a.k2[i] = kj;
+            a.tmp[i] = stage_input<T, 3>(x[i], a.k1[i], kj, 0, 0, 0, hs);
+        } else if constexpr (J == 3) {
+            a.k3[i] = kj;
+            a.tmp[i] = stage_input<T, 4>(x[i], a.k1[i], a.k2[i], kj, 0, 0, hs);
+        } else if constexpr (J == 4) {
+            a.k4[i] = kj;
+            a.tmp[i] = stage_input<T, 5>(x[i], a.k1[i], a.k2[i], a.k3[i], kj, 0, hs);
+        } else if constexpr (J == 5) {
+            a.k5[i] = kj;
+            a.tmp[i] = stage_input<T, 6>(x[i], a.k1[i], a.k2[i], a.k3[i], a.k4[i], kj, hs);
+        } else {
+            T xn;
+            final_combination<T>(x[i], a.k1[i], a.k3[i], a.k4[i], a.k5[i], kj, hs, h, hnext, xn, err);
+            xnew[i] = xn;
+        }
+    }
+    if constexpr (J == 6) {
+        double bm = block_max<BLK>(err, sh);
+        if (threadIdx.x == 0) a.partial[a.nblk_clause + blockIdx.x] = bm;
+    }
+}
+
+// ---- control: max-norm + Cash-Karp step control (CpuRungeKutta.cpp:217-267)
+__global__ __launch_bounds__(CTRL_BLK) void k_control(Ctrl* c, const double* partial, int np)
+{
+    __shared__ double sh[CTRL_BLK / 64];
+    if (!c->active) return;
+    double v = 0.0;
+    for (int i = threadIdx.x; i < np; i += CTRL_BLK) v = fmax(v, partial[i]);
+    v = block_max<CTRL_BLK>(v, sh);
+    if (threadIdx.x != 0) return;
+    const double errmax = v / c->eps;
+    const double h = c->h;
+    c->errmax = errmax;
+    c->attempts += 1;
+    c->rhscount += c->step_start ? 6 : 5;
+    if (errmax <= 1.0) {
+        double hn = errmax > TabD::ERRCON ? TabD::SAFETY * h * pow_ctrl(errmax, 5) : 5.0 * h;
+        c->hdid = h;
+        c->time += h;
+        c->steps += 1;
+        c->parity ^= 1;
+        c->step_start = 1;
+        c->hnext = hn;
+        c->h = hn;
+        if (c->steps >= c->target) c->active = 0;
+    } else {
+        double htemp = TabD::SAFETY * h * pow_ctrl(errmax, 4);
+        double hn = h >= 0.0 ? fmax(htemp, TabD::HFACTOR * h) : fmin(htemp, TabD::HFACTOR * h);
+        c->h = hn;
+        c->step_start = 0;
+        if (c->time + hn == c->time) {
+            c->underflow = 1;
+            c->active = 0;
+        }
+    }
+}
+
+__global__ void k_ctrl_begin(Ctrl* c, long long count, int parity)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        c->target = c->steps + count;
+        c->parity = parity;
+        c->active = (c->underflow || count <= 0) ? 0 : 1;
+    }
+}
+
+__global__ void k_ctrl_reset(Ctrl* c, double eps)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        c->h = 1e-9; c->hnext = 1e-9; c->time = 0.0; c->hdid = 0.0; c->errmax = 0.0; c->eps = eps;
+        c->steps = 0; c->attempts = 0; c->rhscount = 0; c->target = 0;
+        c->parity = 0; c->step_start = 1; c->active = 0; c->underflow = 0; c->violations = 0;
+    }
+}
+
+__global__ void k_ctrl_set_eps(Ctrl* c, double eps)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) c->eps = eps;
+}
+
+// ---- violated-clause count (CpuSat::CalculateClauses semantics) ----------
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_violations(const int* lit, int m, int k, const T* x, int* out)
+{
+    __shared__ int sh[BLK / 64];
+    const int j = blockIdx.x * BLK + threadIdx.x;
+    int viol = 0;
+    if (j < m) {
+        int sat = 0;
+        for (int q = 0; q < k && !sat; q++) {
+            int l = lit[(size_t)j * k + q];
+            if (l == 0) continue;            // padding literal is constant FALSE
+            int t = x[l < 0 ? -l : l] > (T)0;
+            sat = l < 0 ? !t : t;
+        }
+        viol = !sat;
+    }
+    int s = block_sum<BLK>(viol, sh);
+    if (threadIdx.x == 0 && s) atomicAdd(out, s);
+}
+
+// ---- random numbers: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) ----
+__device__ __forceinline__ void philox4x32_10(uint32_t ctr[4], uint32_t key0, uint32_t key1)
+{
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)M0 * ctr[0];
+        uint64_t p1 = (uint64_t)M1 * ctr[2];
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ ctr[1] ^ key0;
+        uint32_t n2 = hi0 ^ ctr[3] ^ key1;
+        ctr[0] = n0; ctr[1] = lo1; ctr[2] = n2; ctr[3] = lo0;
+        key0 += W0; key1 += W1;
+    }
+}
+
+template <typename T>
+__global__ void k_uniform(T* out, int len, unsigned long long seed, unsigned long long offset)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    unsigned long long idx = offset + (unsigned long long)i;
+    uint32_t ctr[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), 0x5EEDu, 0u};
+    philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
+    if constexpr (sizeof(T) == 8) {
+        uint64_t v = ((uint64_t)ctr[0] << 32) | ctr[1];
+        out[i] = (double)(v >> 11) * 0x1.0p-53;
+    } else {
+        out[i] = (float)(ctr[0] >> 8) * 0x1.0p-24f;
+    }
+}
+
+// s = 2u - 1 for spins, a = u + 1 for aux (CpuSat.cpp:127-132); input uniforms
+// are laid out in the public order [s, a] inside the state buffer's spin area
+template <typename T>
+__global__ void k_set_initial(T* x, int n, int m, int aoff, const T* u)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[1 + i] = u[i] * (T)2.0 - (T)1.0;
+    else if (i < (long long)n + m) x[aoff + (i - n)] = u[i] + (T)1.0;
+}
+
+template <typename T>
+__global__ void k_fill_const(T* x)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) x[0] = (T)-1.0;
+}
+
+// ===========================================================================
+// host objects
+// ===========================================================================
+template <typename T>
+static void dfree(T*& p)
+{
+    if (p) { (void)hipFree((void*)p); p = nullptr; }
+}
+
+struct Sys;
+
+struct State {
+    Sys* sys = nullptr;
+    void* d = nullptr;     // device buffer of the system's precision
+    int alloc = 0;
+};
+
+struct Sys {
+    int prec = ASAT_F64;
+    int esize = 8;
+    double b = 0.0;
+    int n = 0, m = 0, k = 0, aoff = 0, alloc = 0;
+    long long nslots = 0;
+    hipStream_t stream = nullptr;
+    int* d_lit = nullptr;
+    int* d_csr_ptr = nullptr;
+    int* d_csr_slot = nullptr;
+    void* d_contrib = nullptr;
+    int* d_count = nullptr;
+    HostProblem hp;
+    bool ready = false;
+
+    ~Sys()
+    {
+        dfree(d_lit); dfree(d_csr_ptr); dfree(d_csr_slot); dfree(d_contrib); dfree(d_count);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void require_ready() const { require(ready, ASAT_ERR_NOT_CONFIGURED, "SAT problem has not been set"); }
+};
+
+struct Rng {
+    unsigned long long seed = 0, offset = 0;
+};
+
+static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclauses)
+{
+    HostProblem P;
+    build_problem(lits, lens, nclauses, P);
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    dfree(s->d_lit); dfree(s->d_csr_ptr); dfree(s->d_csr_slot); dfree(s->d_contrib);
+    s->ready = false;
+    s->n = P.n; s->m = P.m; s->k = P.k;
+    s->aoff = (int)round_up((long long)P.n + 1, 64);
+    s->alloc = (int)(s->aoff + round_up(P.m, 64));
+    s->nslots = (long long)P.m * P.k;
+    HIP_CHECK(hipMalloc((void**)&s->d_lit, sizeof(int) * P.lit.size()));
+    HIP_CHECK(hipMalloc((void**)&s->d_csr_ptr, sizeof(int) * P.csr_ptr.size()));
+    HIP_CHECK(hipMalloc((void**)&s->d_csr_slot, sizeof(int) * std::max<size_t>(P.csr_slot.size(), 1)));
+    HIP_CHECK(hipMalloc(&s->d_contrib, (size_t)s->esize * P.lit.size()));
+    HIP_CHECK(hipMemcpy(s->d_lit, P.lit.data(), sizeof(int) * P.lit.size(), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(s->d_csr_ptr, P.csr_ptr.data(), sizeof(int) * P.csr_ptr.size(), hipMemcpyHostToDevice));
+    if (!P.csr_slot.empty())
+        HIP_CHECK(hipMemcpy(s->d_csr_slot, P.csr_slot.data(), sizeof(int) * P.csr_slot.size(),
+                            hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemset(s->d_contrib, 0, (size_t)s->esize * P.lit.size()));
+    s->hp = std::move(P);
+    s->ready = true;
+}
+
+static State* state_new(Sys* s)
+{
+    s->require_ready();
+    std::unique_ptr<State> st(new State());
+    st->sys = s;
+    st->alloc = s->alloc;
+    HIP_CHECK(hipMalloc(&st->d, (size_t)s->esize * s->alloc));
+    HIP_CHECK(hipMemsetAsync(st->d, 0, (size_t)s->esize * s->alloc, s->stream));
+    if (s->prec == ASAT_F64) hipLaunchKernelGGL(k_fill_const<double>, dim3(1), dim3(64), 0, s->stream, (double*)st->d);
+    else hipLaunchKernelGGL(k_fill_const<float>, dim3(1), dim3(64), 0, s->stream, (float*)st->d);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    return st.release();
+}
+
+static void state_check(const State* st)
+{
+    require(st && st->sys, ASAT_ERR_INVALID_ARGUMENT, "null state");
+    require(st->alloc == st->sys->alloc && st->sys->ready, ASAT_ERR_INVALID_ARGUMENT,
+            "state does not belong to the current problem");
+}
+
+static void state_set_public(State* st, const void* pub)
+{
+    state_check(st);
+    Sys* s = st->sys;
+    const char* p = (const char*)pub;
+    char* d = (char*)st->d;
+    const size_t e = (size_t)s->esize;
+    HIP_CHECK(hipMemcpyAsync(d + e, p, e * s->n, hipMemcpyHostToDevice, s->stream));
+    HIP_CHECK(hipMemcpyAsync(d + e * s->aoff, p + e * s->n, e * s->m, hipMemcpyHostToDevice, s->stream));
+    if (s->prec == ASAT_F64) hipLaunchKernelGGL(k_fill_const<double>, dim3(1), dim3(64), 0, s->stream, (double*)st->d);
+    else hipLaunchKernelGGL(k_fill_const<float>, dim3(1), dim3(64), 0, s->stream, (float*)st->d);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+}
+
+static void state_get_public(const State* st, void* pub)
+{
+    state_check(st);
+    Sys* s = st->sys;
+    char* p = (char*)pub;
+    const char* d = (const char*)st->d;
+    const size_t e = (size_t)s->esize;
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    HIP_CHECK(hipMemcpy(p, d + e, e * s->n, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(p + e * s->n, d + e * s->aoff, e * s->m, hipMemcpyDeviceToHost));
+}
+
+template <typename T>
+static StageArgs<T> stage_args(Sys* s, T* xa, T* xb, T* const* kk, T* tmp, double* partial, const Ctrl* c)
+{
+    StageArgs<T> a;
+    a.lit = s->d_lit; a.csr_ptr = s->d_csr_ptr; a.csr_slot = s->d_csr_slot;
+    a.contrib = (T*)s->d_contrib;
+    a.xa = xa; a.xb = xb;
+    a.k1 = kk[0]; a.k2 = kk[1]; a.k3 = kk[2]; a.k4 = kk[3]; a.k5 = kk[4];
+    a.tmp = tmp; a.partial = partial; a.ctrl = c;
+    a.n = s->n; a.m = s->m; a.k = s->k; a.aoff = s->aoff;
+    a.nblk_clause = ceil_div(s->m, BLK);
+    a.knorm = (T)std::pow(2.0, -s->k);
+    return a;
+}
+
+template <typename T, int J>
+static void launch_clause(const StageArgs<T>& a, hipStream_t st)
+{
+    dim3 g(ceil_div(a.m, BLK)), b(BLK);
+    switch (a.k) {
+    case 3: hipLaunchKernelGGL((k_clause<T, 3, J>), g, b, 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_clause<T, 4, J>), g, b, 0, st, a); break;
+    default: hipLaunchKernelGGL((k_clause_gen<T, J>), g, b, 0, st, a); break;
+    }
+}
+
+template <typename T, int J>
+static void launch_var(const StageArgs<T>& a, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_var<T, J>), dim3(std::max(1, ceil_div(a.n, BLK))), dim3(BLK), 0, st, a);
+}
+
+template <typename T>
+static void launch_attempt(const StageArgs<T>& a, Ctrl* ctrl, const double* partial, int np, hipStream_t st)
+{
+    launch_clause<T, 1>(a, st); launch_var<T, 1>(a, st);
+    launch_clause<T, 2>(a, st); launch_var<T, 2>(a, st);
+    launch_clause<T, 3>(a, st); launch_var<T, 3>(a, st);
+    launch_clause<T, 4>(a, st); launch_var<T, 4>(a, st);
+    launch_clause<T, 5>(a, st); launch_var<T, 5>(a, st);
+    launch_clause<T, 6>(a, st); launch_var<T, 6>(a, st);
+    hipLaunchKernelGGL(k_control, dim3(1), dim3(CTRL_BLK), 0, st, ctrl, partial, np);
+}
+
+template <typename T>
+static void sys_rhs(Sys* s, State* dxdt, const State* x)
+{
+    s->require_ready();
+    state_check(dxdt); state_check(x);
+    T* kk[5] = {(T*)dxdt->d, nullptr, nullptr, nullptr, nullptr};
+    StageArgs<T> a = stage_args<T>(s, (T*)x->d, nullptr, kk, nullptr, nullptr, nullptr);
+    // dxdt.SetZero(): padding and the constant slot read as 0
+    HIP_CHECK(hipMemsetAsync(dxdt->d, 0, (size_t)s->esize * s->alloc, s->stream));
+    launch_clause<T, 1>(a, s->stream);
+    launch_var<T, 1>(a, s->stream);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+}
+
+template <typename T>
+static int sys_violations(Sys* s, const State* x)
+{
+    s->require_ready();
+    state_check(x);
+    HIP_CHECK(hipMemsetAsync(s->d_count, 0, sizeof(int), s->stream));
+    hipLaunchKernelGGL(k_violations<T>, dim3(ceil_div(s->m, BLK)), dim3(BLK), 0, s->stream, s->d_lit, s->m, s->k,
+                       (const T*)x->d, s->d_count);
+    HIP_CHECK(hipGetLastError());
+    int v = 0;
+    HIP_CHECK(hipMemcpyAsync(&v, s->d_count, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// integrator
+// ---------------------------------------------------------------------------
+struct RK {
+    Sys* sys = nullptr;
+    double eps = 1e-4;      // CudaRungeKutta() default (CudaRungeKutta.cu:175)
+    int alloc = 0, prec = -1;
+    void* k[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    void* tmp = nullptr;
+    double* partial = nullptr;
+    int npartial = 0;
+    Ctrl* d_ctrl = nullptr;
+    Ctrl* h_ctrl = nullptr;   // pinned mirror
+    void* pair[2] = {nullptr, nullptr};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = -1.f;
+    bool configured = false;
+
+    void free_all()
+    {
+        for (auto& p : k) dfree(p);
+        dfree(tmp); dfree(partial); dfree(d_ctrl);
+        if (h_ctrl) { (void)hipHostFree(h_ctrl); h_ctrl = nullptr; }
+    }
+    ~RK()
+    {
+        free_all();
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+    }
+    void require_configured() const
+    {
+        require(configured && sys && sys->ready, ASAT_ERR_NOT_CONFIGURED,
+                "the RK4 solver has not been configured with an ODE yet");
+    }
+};
+
+static void rk_reset(RK* r)
+{
+    r->require_configured();
+    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, r->sys->stream, r->d_ctrl, r->eps);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(r->h_ctrl, r->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, r->sys->stream));
+    HIP_CHECK(hipStreamSynchronize(r->sys->stream));
+    r->pair[0] = r->pair[1] = nullptr;
+}
+
+static void rk_configure(RK* r, Sys* s)
+{
+    s->require_ready();
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    r->free_all();
+    r->configured = false;
+    r->sys = s;
+    r->alloc = s->alloc;
+    r->prec = s->prec;
+    const size_t bytes = (size_t)s->esize * s->alloc;
+    for (auto& p : r->k) {
+        HIP_CHECK(hipMalloc(&p, bytes));
+        HIP_CHECK(hipMemset(p, 0, bytes));
+    }
+    HIP_CHECK(hipMalloc(&r->tmp, (size_t)s->esize * (s->n + 1)));
+    HIP_CHECK(hipMemset(r->tmp, 0, (size_t)s->esize * (s->n + 1)));
+    r->npartial = ceil_div(s->m, BLK) + std::max(1, ceil_div(s->n, BLK));
+    HIP_CHECK(hipMalloc((void**)&r->partial, sizeof(double) * r->npartial));
+    HIP_CHECK(hipMemset(r->partial, 0, sizeof(double) * r->npartial));
+    HIP_CHECK(hipMalloc((void**)&r->d_ctrl, sizeof(Ctrl)));
+    HIP_CHECK(hipHostMalloc((void**)&r->h_ctrl, sizeof(Ctrl), hipHostMallocDefault));
+    memset(r->h_ctrl, 0, sizeof(Ctrl));
+    if (!r->ev0) HIP_CHECK(hipEventCreate(&r->ev0));
+    if (!r->ev1) HIP_CHECK(hipEventCreate(&r->ev1));
+    r->configured = true;
+    rk_reset(r);
+}
+
+template <typename T>
+static int rk_step_impl(RK* r, State* olds, State* news, int count)
+{
+    Sys* s = r->sys;
+    hipStream_t st = s->stream;
+    // canonical ping-pong pair: parity 0 means x lives in pair[0]
+    if (!((r->pair[0] == olds->d && r->pair[1] == news->d) || (r->pair[0] == news->d && r->pair[1] == olds->d))) {
+        r->pair[0] = olds->d;
+        r->pair[1] = news->d;
+    }
+    const int parity = (r->pair[0] == olds->d) ? 0 : 1;
+    T* kk[5] = {(T*)r->k[0], (T*)r->k[1], (T*)r->k[2], (T*)r->k[3], (T*)r->k[4]};
+    StageArgs<T> a = stage_args<T>(s, (T*)r->pair[0], (T*)r->pair[1], kk, (T*)r->tmp, r->partial, r->d_ctrl);
+    HIP_CHECK(hipEventRecord(r->ev0, st));
+    hipLaunchKernelGGL(k_ctrl_begin, dim3(1), dim3(64), 0, st, r->d_ctrl, (long long)count, parity);
+    long long remaining = count;
+    const long long target = r->h_ctrl->steps + count;
+    long long last_attempts = r->h_ctrl->attempts;
+    while (remaining > 0) {
+        for (long long i = 0; i < remaining; i++) launch_attempt<T>(a, r->d_ctrl, r->partial, r->npartial, st);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(r->h_ctrl, r->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        if (r->h_ctrl->underflow) break;
+        remaining = target - r->h_ctrl->steps;
+        // every enqueued attempt either accepts, rejects or underflows; no
+        // progress here means the control block was left inactive
+        require(r->h_ctrl->attempts > last_attempts || remaining <= 0, ASAT_ERR_HIP,
+                "integrator made no progress (control block inactive)");
+        last_attempts = r->h_ctrl->attempts;
+    }
+    HIP_CHECK(hipEventRecord(r->ev1, st));
+    HIP_CHECK(hipEventSynchronize(r->ev1));
+    HIP_CHECK(hipEventElapsedTime(&r->last_ms, r->ev0, r->ev1));
+    // hand the newest state to `olds` (the reference swaps the device pointers)
+    const int par = r->h_ctrl->parity;
+    void* xcur = r->pair[par];
+    void* xother = r->pair[par ^ 1];
+    olds->d = xcur;
+    news->d = xother;
+    return r->h_ctrl->underflow ? ASAT_UNDERFLOW : ASAT_OK;
+}
+
+static int rk_step(RK* r, State* olds, State* news, int count)
+{
+    r->require_configured();
+    state_check(olds); state_check(news);
+    require(olds != news && olds->d != news->d, ASAT_ERR_INVALID_ARGUMENT, "old and new state must differ");
+    require(olds->sys == r->sys && news->sys == r->sys, ASAT_ERR_INVALID_ARGUMENT,
+            "states belong to another system");
+    require(r->sys->b == 0.0, ASAT_ERR_UNSUPPORTED, "the bias term (b > 0) is not supported on the GPU path");
+    require(count >= 0, ASAT_ERR_INVALID_ARGUMENT, "negative step count");
+    if (r->h_ctrl->underflow) return ASAT_UNDERFLOW;
+    if (count == 0) return ASAT_OK;
+    return r->prec == ASAT_F64 ? rk_step_impl<double>(r, olds, news, count)
+                               : rk_step_impl<float>(r, olds, news, count);
+}
+
+// Times each kernel kind of one attempt over `reps` launches with HIP events
+// on the integrator's stream (order: clause J=1..6, var J=1..6, control).
+// The control block is saved and restored, so the integrator state is unchanged;
+// the scratch buffers (k1..k5, tmp, x_new) are overwritten, which is harmless
+// because the next attempt starts a fresh step (step_start == 1 after Step).
+template <typename T>
+static void rk_profile_impl(RK* r, State* olds, State* news, int reps, float* us)
+{
+    Sys* s = r->sys;
+    hipStream_t st = s->stream;
+    if (!((r->pair[0] == olds->d && r->pair[1] == news->d) || (r->pair[0] == news->d && r->pair[1] == olds->d))) {
+        r->pair[0] = olds->d;
+        r->pair[1] = news->d;
+    }
+    const int parity = (r->pair[0] == olds->d) ? 0 : 1;
+    T* kk[5] = {(T*)r->k[0], (T*)r->k[1], (T*)r->k[2], (T*)r->k[3], (T*)r->k[4]};
+    StageArgs<T> a = stage_args<T>(s, (T*)r->pair[0], (T*)r->pair[1], kk, (T*)r->tmp, r->partial, r->d_ctrl);
+    Ctrl* backup = nullptr;
+    HIP_CHECK(hipMalloc((void**)&backup, sizeof(Ctrl)));
+    HIP_CHECK(hipMemcpyAsync(backup, r->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_ctrl_begin, dim3(1), dim3(64), 0, st, r->d_ctrl, (long long)1 << 40, parity);
+    auto timeit = [&](int slot, auto&& launch) {
+        launch();   // warm
+        HIP_CHECK(hipEventRecord(r->ev0, st));
+        for (int i = 0; i < reps; i++) launch();
+        HIP_CHECK(hipEventRecord(r->ev1, st));
+        HIP_CHECK(hipEventSynchronize(r->ev1));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+        us[slot] = ms * 1000.f / (float)reps;
+    };
+    timeit(0, [&] { launch_clause<T, 1>(a, st); });
+    timeit(1, [&] { launch_clause<T, 2>(a, st); });
+    timeit(2, [&] { launch_clause<T, 3>(a, st); });
+    timeit(3, [&] { launch_clause<T, 4>(a, st); });
+    timeit(4, [&] { launch_clause<T, 5>(a, st); });
+    timeit(5, [&] { launch_clause<T, 6>(a, st); });
+    timeit(6, [&] { launch_var<T, 1>(a, st); });
+    timeit(7, [&] { launch_var<T, 2>(a, st); });
+    timeit(8, [&] { launch_var<T, 3>(a, st); });
+    timeit(9, [&] { launch_var<T, 4>(a, st); });
+    timeit(10, [&] { launch_var<T, 5>(a, st); });
+    timeit(11, [&] { launch_var<T, 6>(a, st); });
+    timeit(12, [&] { hipLaunchKernelGGL(k_control, dim3(1), dim3(CTRL_BLK), 0, st, r->d_ctrl, r->partial, r->npartial); });
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(r->d_ctrl, backup, sizeof(Ctrl), hipMemcpyDeviceToDevice, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    (void)hipFree(backup);
+}
+
+}  // namespace asat
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace asat;
+
+struct asat_sys { Sys s; };
+struct asat_state { State st; };
+struct asat_rk { RK r; };
+struct asat_rng { Rng g; };
+
+#define API_BEGIN try {
+#define API_END                                                  \
+    }                                                            \
+    catch (const asat::Error& e) {                               \
+        asat::g_last_error = e.what();                           \
+        return e.code;                                           \
+    }                                                            \
+    catch (const std::bad_alloc&) {                              \
+        asat::g_last_error = "host allocation failed";           \
+        return ASAT_ERR_HIP;                                     \
+    }                                                            \
+    catch (const std::exception& e) {                            \
+        asat::g_last_error = e.what();                           \
+        return ASAT_ERR_HIP;                                     \
+    }                                                            \
+    return ASAT_OK;
+
+#define CHECK_PTR(p) require((p) != nullptr, ASAT_ERR_INVALID_ARGUMENT, "null argument: " #p)
+
+extern "C" {
+
+const char* asat_last_error(void) { return asat::g_last_error.c_str(); }
+
+const char* asat_version(void) { return "analogsat-mi355x 0.1 (gfx950)"; }
+
+int asat_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int asat_sys_create(int precision, asat_sys_t** out)
+{
+    API_BEGIN
+    CHECK_PTR(out);
+    require(precision == ASAT_F64 || precision == ASAT_F32, ASAT_ERR_INVALID_ARGUMENT, "unknown precision");
+    std::unique_ptr<asat_sys> p(new asat_sys());
+    p->s.prec = precision;
+    p->s.esize = precision == ASAT_F64 ? 8 : 4;
+    HIP_CHECK(hipStreamCreateWithFlags(&p->s.stream, hipStreamNonBlocking));
+    HIP_CHECK(hipMalloc((void**)&p->s.d_count, sizeof(int)));
+    *out = p.release();
+    API_END
+}
+
+int asat_sys_destroy(asat_sys_t* sys)
+{
+    API_BEGIN
+    delete sys;
+    API_END
+}
+
+int asat_sys_set_problem(asat_sys_t* sys, const int* literals, const int* lengths, int nclauses)
+{
+    API_BEGIN
+    CHECK_PTR(sys);
+    sys_set_problem(&sys->s, literals, lengths, nclauses);
+    API_END
+}
+
+int asat_sys_get_nmk(const asat_sys_t* sys, int* n, int* m, int* k)
+{
+    API_BEGIN
+    CHECK_PTR(sys);
+    if (n) *n = sys->s.n;
+    if (m) *m = sys->s.m;
+    if (k) *k = sys->s.k;
+    API_END
+}
+
+int asat_sys_set_b(asat_sys_t* sys, double b)
+{
+    API_BEGIN
+    CHECK_PTR(sys);
+    sys->s.b = b;
+    API_END
+}
+
+int asat_sys_get_b(const asat_sys_t* sys, double* b)
+{
+    API_BEGIN
+    CHECK_PTR(sys); CHECK_PTR(b);
+    *b = sys->s.b;
+    API_END
+}
+
+int asat_sys_rhs(asat_sys_t* sys, asat_state_t* dxdt, const asat_state_t* x, double t)
+{
+    (void)t;   // autonomous system (the reference ignores time too)
+    API_BEGIN
+    CHECK_PTR(sys); CHECK_PTR(dxdt); CHECK_PTR(x);
+    require(sys->s.b == 0.0, ASAT_ERR_UNSUPPORTED, "the bias term (b > 0) is not supported on the GPU path");
+    if (sys->s.prec == ASAT_F64) sys_rhs<double>(&sys->s, &dxdt->st, &x->st);
+    else sys_rhs<float>(&sys->s, &dxdt->st, &x->st);
+    API_END
+}
+
+int asat_sys_violations(asat_sys_t* sys, const asat_state_t* x, int* count)
+{
+    API_BEGIN
+    CHECK_PTR(sys); CHECK_PTR(x); CHECK_PTR(count);
+    *count = sys->s.prec == ASAT_F64 ? sys_violations<double>(&sys->s, &x->st)
+                                     : sys_violations<float>(&sys->s, &x->st);
+    API_END
+}
+
+int asat_state_create(asat_sys_t* sys, asat_state_t** out)
+{
+    API_BEGIN
+    CHECK_PTR(sys); CHECK_PTR(out);
+    std::unique_ptr<State> st(state_new(&sys->s));
+    asat_state* w = new asat_state();
+    w->st = *st;
+    st->d = nullptr;
+    *out = w;
+    API_END
+}
+
+int asat_state_destroy(asat_state_t* st)
+{
+    API_BEGIN
+    if (st) {
+        if (st->st.d) (void)hipFree(st->st.d);
+        delete st;
+    }
+    API_END
+}
+
+int asat_state_set_public(asat_state_t* st, const void* host_pub)
+{
+    API_BEGIN
+    CHECK_PTR(st); CHECK_PTR(host_pub);
+    state_set_public(&st->st, host_pub);
+    API_END
+}
+
+int asat_state_get_public(const asat_state_t* st, void* host_pub)
+{
+    API_BEGIN
+    CHECK_PTR(st); CHECK_PTR(host_pub);
+    state_get_public(&st->st, host_pub);
+    API_END
+}
+
+int asat_state_get_boolean(const asat_state_t* st, unsigned char* out)
+{
+    API_BEGIN
+    CHECK_PTR(st); CHECK_PTR(out);
+    const Sys* s = st->st.sys;
+    std::vector<char> buf((size_t)s->esize * (s->n + s->m));
+    state_get_public(&st->st, buf.data());
+    for (int i = 0; i < s->n; i++)
+        out[i] = s->prec == ASAT_F64 ? (((double*)buf.data())[i] > 0) : (((float*)buf.data())[i] > 0);
+    API_END
+}
+
+int asat_state_copy(asat_state_t* dst, const asat_state_t* src)
+{
+    API_BEGIN
+    CHECK_PTR(dst); CHECK_PTR(src);
+    state_check(&dst->st); state_check(&src->st);
+    require(dst->st.sys == src->st.sys, ASAT_ERR_INVALID_ARGUMENT, "states belong to different systems");
+    Sys* s = dst->st.sys;
+    HIP_CHECK(hipMemcpyAsync(dst->st.d, src->st.d, (size_t)s->esize * s->alloc, hipMemcpyDeviceToDevice, s->stream));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    API_END
+}
+
+int asat_state_set_zero(asat_state_t* st)
+{
+    API_BEGIN
+    CHECK_PTR(st);
+    state_check(&st->st);
+    Sys* s = st->st.sys;
+    HIP_CHECK(hipMemsetAsync(st->st.d, 0, (size_t)s->esize * s->alloc, s->stream));
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    API_END
+}
+
+int asat_state_alloc_size(const asat_state_t* st, int* size)
+{
+    API_BEGIN
+    CHECK_PTR(st); CHECK_PTR(size);
+    *size = st->st.alloc;
+    API_END
+}
+
+int asat_state_device_ptr(asat_state_t* st, void** dev_ptr, int* aux_offset)
+{
+    API_BEGIN
+    CHECK_PTR(st); CHECK_PTR(dev_ptr);
+    *dev_ptr = st->st.d;
+    if (aux_offset) *aux_offset = st->st.sys->aoff;
+    API_END
+}
+
+int asat_sys_set_random_state(asat_sys_t* sys, asat_state_t* x, asat_rng_t* rng)
+{
+    API_BEGIN
+    CHECK_PTR(sys); CHECK_PTR(x); CHECK_PTR(rng);
+    Sys* s = &sys->s;
+    s->require_ready();
+    state_check(&x->st);
+    const int len = s->n + s->m;
+    void* u = nullptr;
+    HIP_CHECK(hipMalloc(&u, (size_t)s->esize * len));
+    dim3 g(ceil_div(len, 256)), b(256);
+    if (s->prec == ASAT_F64) {
+        hipLaunchKernelGGL(k_uniform<double>, g, b, 0, s->stream, (double*)u, len, rng->g.seed, rng->g.offset);
+        hipLaunchKernelGGL(k_set_initial<double>, g, b, 0, s->stream, (double*)x->st.d, s->n, s->m, s->aoff,
+                           (const double*)u);
+    } else {
+        hipLaunchKernelGGL(k_uniform<float>, g, b, 0, s->stream, (float*)u, len, rng->g.seed, rng->g.offset);
+        hipLaunchKernelGGL(k_set_initial<float>, g, b, 0, s->stream, (float*)x->st.d, s->n, s->m, s->aoff,
+                           (const float*)u);
+    }
+    rng->g.offset += (unsigned long long)len;
+    hipError_t e = hipGetLastError();
+    hipError_t e2 = hipStreamSynchronize(s->stream);
+    (void)hipFree(u);
+    HIP_CHECK(e);
+    HIP_CHECK(e2);
+    API_END
+}
+
+int asat_rk_create(asat_rk_t** out)
+{
+    API_BEGIN
+    CHECK_PTR(out);
+    *out = new asat_rk();
+    API_END
+}
+
+int asat_rk_destroy(asat_rk_t* rk)
+{
+    API_BEGIN
+    delete rk;
+    API_END
+}
+
+int asat_rk_set_eps(asat_rk_t* rk, double eps)
+{
+    API_BEGIN
+    CHECK_PTR(rk);
+    // CpuRungeKutta::SetEpsilon (AnalogSAT/cpu_sat/CpuRungeKutta.cpp:126-131)
+    require(!(eps < 1e-16 || eps > 1.0), ASAT_ERR_INVALID_ARGUMENT, "epsilon must be between 1e-16 and 1");
+    rk->r.eps = eps;
+    if (rk->r.configured) {
+        hipLaunchKernelGGL(k_ctrl_set_eps, dim3(1), dim3(64), 0, rk->r.sys->stream, rk->r.d_ctrl, eps);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(rk->r.sys->stream));
+    }
+    API_END
+}
+
+int asat_rk_get_eps(const asat_rk_t* rk, double* eps)
+{
+    API_BEGIN
+    CHECK_PTR(rk); CHECK_PTR(eps);
+    *eps = rk->r.eps;
+    API_END
+}
+
+int asat_rk_configure(asat_rk_t* rk, asat_sys_t* sys)
+{
+    API_BEGIN
+    CHECK_PTR(rk); CHECK_PTR(sys);
+    rk_configure(&rk->r, &sys->s);
+    API_END
+}
+
+int asat_rk_step(asat_rk_t* rk, asat_state_t* old_state, asat_state_t* new_state, int count)
+{
+    API_BEGIN
+    CHECK_PTR(rk); CHECK_PTR(old_state); CHECK_PTR(new_state);
+    int rc = rk_step(&rk->r, &old_state->st, &new_state->st, count);
+    if (rc == ASAT_UNDERFLOW) {
+        asat::g_last_error = "step size underflow";
+        return rc;
+    }
+    API_END
+}
+
+int asat_rk_reset(asat_rk_t* rk)
+{
+    API_BEGIN
+    CHECK_PTR(rk);
+    rk_reset(&rk->r);
+    API_END
+}
+
+int asat_rk_stats(const asat_rk_t* rk, asat_rk_stats_t* out)
+{
+    API_BEGIN
+    CHECK_PTR(rk); CHECK_PTR(out);
+    rk->r.require_configured();
+    const Ctrl* c = rk->r.h_ctrl;
+    out->time = c->time;
+    out->hdid = c->hdid;
+    out->hnext = c->hnext;
+    out->errmax = c->errmax;
+    out->steps = c->steps;
+    out->attempts = c->attempts;
+    out->rhscount = c->rhscount;
+    API_END
+}
+
+int asat_rk_last_step_ms(const asat_rk_t* rk, float* ms)
+{
+    API_BEGIN
+    CHECK_PTR(rk); CHECK_PTR(ms);
+    *ms = rk->r.last_ms;
+    API_END
+}
+
+int asat_rk_profile(asat_rk_t* rk, asat_state_t* old_state, asat_state_t* new_state, int reps, float* us_out)
+{
+    API_BEGIN
+    CHECK_PTR(rk); CHECK_PTR(old_state); CHECK_PTR(new_state); CHECK_PTR(us_out);
+    RK* r = &rk->r;
+    r->require_configured();
+    state_check(&old_state->st); state_check(&new_state->st);
+    require(reps > 0, ASAT_ERR_INVALID_ARGUMENT, "reps must be positive");
+    require(old_state->st.d != new_state->st.d, ASAT_ERR_INVALID_ARGUMENT, "old and new state must differ");
+    if (r->prec == ASAT_F64) rk_profile_impl<double>(r, &old_state->st, &new_state->st, reps, us_out);
+    else rk_profile_impl<float>(r, &old_state->st, &new_state->st, reps, us_out);
+    API_END
+}
+
+int asat_rng_create(unsigned long long seed, asat_rng_t** out)
+{
+    API_BEGIN
+    CHECK_PTR(out);
+    asat_rng* r = new asat_rng();
+    r->g.seed = seed;
+    r->g.offset = 0;
+    *out = r;
+    API_END
+}
+
+int asat_rng_destroy(asat_rng_t* rng)
+{
+    API_BEGIN
+    delete rng;
+    API_END
+}
+
+int asat_rng_uniform(asat_rng_t* rng, void* dev_ptr, int len, int precision)
+{
+    API_BEGIN
+    CHECK_PTR(rng); CHECK_PTR(dev_ptr);
+    require(len >= 0, ASAT_ERR_INVALID_ARGUMENT, "negative length");
+    require(precision == ASAT_F64 || precision == ASAT_F32, ASAT_ERR_INVALID_ARGUMENT, "unknown precision");
+    if (len == 0) return ASAT_OK;
+    dim3 g(ceil_div(len, 256)), b(256);
+    if (precision == ASAT_F64)
+        hipLaunchKernelGGL(k_uniform<double>, g, b, 0, 0, (double*)dev_ptr, len, rng->g.seed, rng->g.offset);
+    else
+        hipLaunchKernelGGL(k_uniform<float>, g, b, 0, 0, (float*)dev_ptr, len, rng->g.seed, rng->g.offset);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipDeviceSynchronize());
+    rng->g.offset += (unsigned long long)len;
+    API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,195 @@
+"""Benchmark: accepted RK45 trajectory-steps/s of the CTDS integrator on MI355X.
+
+Workload (default): one uniform random 3-SAT instance, N = 1,000,000 variables,
+M = 4,200,000 clauses (alpha = 4.2), FastSatMaker stream with seed 0; s(0)
+uniform in [-1, 1) (numpy seed 0), a(0) = 1; FP32 state with FP64 error norm,
+step control and time; eps = 1e-6; no SAT exit.  A "step" is one accepted
+Cash-Karp step of the whole trajectory (all N + M state variables).
+
+Timed region: exactly K accepted steps (one Step(K) call), inputs resident in
+HBM, the device synchronised on both sides.  value = K / wall seconds.
+
+Roofline: each kernel kind of one attempt is timed separately with HIP events
+on the integrator's stream (asat_rk_profile); the dominant kernel's
+algorithmic bytes per launch / its average duration gives `achieved`.
+Algorithmic bytes (FP32 state, int32 indices; L = K*M literal slots):
+  clause pass J : 8 L + 4 M A_J,        A = (2, 3, 4, 5, 6, 7)
+  var pass J    : 8 L + 4 (N+1) + 4 N B_J, B = (3, 4, 5, 6, 7, 6)
+(8 B per slot = literal read + contribution write, resp. CSR read +
+contribution gather; A_J/B_J = state-length arrays read or written).
+
+CPU baseline: the reference's own CPU solver (oracle/_ref, CpuSat<double> +
+CpuRungeKutta<double>, single thread) on the same instance and start for a
+bounded sample of accepted steps, when that build is present; otherwise the
+oracle's C restatement.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(n, m, k, nslots):
+    L = nslots
+    A = (2, 3, 4, 5, 6, 7)
+    B = (3, 4, 5, 6, 7, 6)
+    out = {}
+    for j in range(6):
+        out["clause%d" % (j + 1)] = 8 * L + 4 * m * A[j]
+        out["var%d" % (j + 1)] = 8 * L + 4 * (n + 1) + 4 * n * B[j]
+    out["control"] = 8 * (-(-m // 256) + -(-n // 256))
+    return out
+
+
+def attempt_bytes(n, m, k, nslots):
+    """Algorithmic bytes of one attempt: 96 L + 136 D (FP32 state)."""
+    return 96 * nslots + 136 * (n + m)
+
+
+def cpu_baseline(lits, lens, pub0, eps, steps):
+    import oracle as O
+    t_total = 0.0
+    if O.ref_available():
+        R = O.ref()
+        h = R.ref_rk_create(lits, lens, len(lens), eps)
+        R.ref_rk_set_state(h, pub0)
+        t0 = time.perf_counter()
+        R.ref_rk_step(h, steps)
+        t_total = time.perf_counter() - t0
+        R.ref_rk_destroy(h)
+        kind = "reference"
+    else:
+        p = O.Problem(lits, lens)
+        rk = O.RK(p, eps=eps)
+        rk.set_state(pub0)
+        t0 = time.perf_counter()
+        rk.step(steps)
+        t_total = time.perf_counter() - t0
+        kind = "port"
+    return {"value": steps / t_total, "unit": "trajectory-steps/s", "cores": 1, "kind": kind,
+            "sample": "%d accepted steps of the same instance and start, FP64 CpuSat/CpuRungeKutta, "
+                      "single thread, %.1f s" % (steps, t_total)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1000000)
+    ap.add_argument("--alpha", type=float, default=4.2)
+    ap.add_argument("--k", type=int, default=3)
+    ap.add_argument("--eps", type=float, default=1e-6)
+    ap.add_argument("--precision", default="f32")
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-reps", type=int, default=20)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (--gpus %d requested); "
+              "multi-GPU sharding is described in DESIGN.md but not implemented" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import analogsat_amd as A
+
+    n, k = args.n, args.k
+    m = int(round(args.alpha * n))
+    log = lambda s: print("[bench] " + s, file=sys.stderr, flush=True)
+    log("instance N=%d M=%d k=%d (FastSatMaker seed 0)" % (n, m, k))
+    prob = A.fastsat(n, m, k, 0)
+    pub0 = np.concatenate([np.random.default_rng(0).uniform(-1.0, 1.0, n), np.ones(m)])
+
+    sat = A.HipSat(args.precision)
+    sat.SetProblem(prob)
+    rk = A.HipRungeKutta()
+    rk.SetEpsilon(args.eps)
+    rk.Configure(sat)
+    old, new = sat.MakeState(), sat.MakeState()
+    old.CopyFrom(pub0.astype(sat.dtype))
+
+    log("warmup %d steps" % args.warmup)
+    rk.Step(old, new, args.warmup)
+    st0 = rk.stats()
+    log("timed %d steps" % args.steps)
+    t0 = time.perf_counter()
+    ok = rk.Step(old, new, args.steps)
+    wall = time.perf_counter() - t0
+    st1 = rk.stats()
+    dev_ms = rk.last_step_ms()
+    if not ok:
+        log("step size underflow during the timed region")
+    steps = int(st1.steps - st0.steps)
+    attempts = int(st1.attempts - st0.attempts)
+    viol = sat.GetClauseViolationCount(old)
+    log("done: %d steps, %d attempts, %.3f s wall, %.3f s device, t=%.4g, h=%.3g, violations=%d"
+        % (steps, attempts, wall, dev_ms / 1e3, st1.time, st1.hdid, viol))
+
+    log("profiling kernels (%d reps each)" % args.profile_reps)
+    us = rk.profile_kernels(old, new, args.profile_reps)
+    names = ["clause%d" % j for j in range(1, 7)] + ["var%d" % j for j in range(1, 7)] + ["control"]
+    nslots = int(np.count_nonzero(prob.literals)) if len(prob.literals) else 0
+    esz = 4 if args.precision == "f32" else 8
+    kb = kernel_bytes(n, m, k, m * k)
+    if esz == 8:
+        kb = {key: v * 1.5 for key, v in kb.items()}   # rough: state arrays double, indices stay int32
+    kernels = {nm: {"us": round(float(us[i]), 2), "GBps": round(float(kb[nm] / (us[i] * 1e-6) / 1e9), 1)}
+               for i, nm in enumerate(names)}
+    log("kernels: " + ", ".join("%s %.1fus %.0fGB/s" % (nm, v["us"], v["GBps"]) for nm, v in kernels.items()))
+    dom = max(names[:-1], key=lambda nm: us[names.index(nm)])
+    dom_us = float(us[names.index(dom)])
+    achieved = float(kb[dom] / (dom_us * 1e-6) / 1e9)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_c2.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get(dom)
+    attempt_us = float(us[:13].sum())
+    att_bytes = attempt_bytes(n, m, k, m * k)
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline: %d steps" % args.cpu_steps)
+        cpu = cpu_baseline(prob.literals, prob.lengths, pub0, args.eps, args.cpu_steps)
+        log("cpu baseline %.3f steps/s (%s)" % (cpu["value"], cpu["kind"]))
+
+    value = steps / wall
+    line = {
+        "metric": "RK45 accepted trajectory-steps/s (single large instance)",
+        "value": round(value, 3),
+        "unit": "trajectory-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / max(steps, 1), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-state/fp64-control" if args.precision == "f32" else "f64",
+        "data": "synthetic (FastSatMaker seed 0, s0 numpy seed 0, a0=1)",
+        "config": {"workload": "single random 3-SAT instance, plain solver", "n": n, "m": m, "k": k,
+                   "alpha": args.alpha, "eps": args.eps, "trajectories": 1},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
+                     "kernel_us": round(dom_us, 2), "bytes_per_launch": int(kb[dom])},
+        "attempt": {"attempts_per_s": round(attempts / wall, 2), "reject_ratio": round(1 - steps / max(attempts, 1), 4),
+                    "kernel_us_sum": round(attempt_us, 1), "B_alg": int(att_bytes),
+                    "B_alg_GBps_at_kernel_sum": round(att_bytes / (attempt_us * 1e-6) / 1e9, 1),
+                    "B_alg_GBps_wall": round(att_bytes * attempts / wall / 1e9, 1)},
+        "kernels": kernels,
+        "device_ms": round(dev_ms, 3),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

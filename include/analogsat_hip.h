@@ -1,0 +1,160 @@
+/*
+ * analogsat_hip.h -- C ABI of the MI355X (gfx950) AnalogSat CTDS integrator.
+ *
+ * Plain pointers and sizes only.  Every call returns an int status
+ * (ASAT_OK == 0); on failure asat_last_error() returns a thread-local message.
+ * The C++ headers under include/asat/ wrap these calls in classes with the
+ * reference's method names (ISat / IODEInt / SatSolver) and re-throw the
+ * reference's exception types.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   asat_sys_*     ISat<TFloat,TState> : IODE  (AnalogSAT/solver/ISat.h:31-87,
+ *                  AnalogSAT/solver/IODE.h:26-41), implemented by CudaSat1/2/3
+ *                  (AnalogSAT/cuda_sat/CudaSat3.h:33-118)
+ *   asat_state_*   ISatState / IODEState  (AnalogSAT/solver/ISatState.h:31-161,
+ *                  AnalogSAT/solver/IODEState.h:26-66), implemented by
+ *                  CudaSatState2 (AnalogSAT/cuda_base/CudaSatState2.h:31-106)
+ *   asat_rk_*      IODEInt<TFloat,TState> (AnalogSAT/solver/IODEInt.h:28-60),
+ *                  implemented by CudaRungeKutta (AnalogSAT/cuda_sat/CudaRungeKutta.h:30-136)
+ *   asat_rng_*     ISatRandom<TFloat> (AnalogSAT/solver/ISatRandom.h:25-33),
+ *                  implemented by CudaRandom (AnalogSAT/cuda_base/CudaRandom.h:29-46)
+ *   asat_problem_* FastSatMaker (AnalogSAT/problem/FastSatMaker.h:29-61) and
+ *                  SatProblem::AddClauses (AnalogSAT/problem/SatProblem.h:49)
+ *   asat_batch_*   no reference counterpart: batched ragged solver (many
+ *                  trajectories / instances in one persistent launch)
+ *   asat_adj_*     no reference counterpart: discrete adjoint of RK45
+ */
+#ifndef ANALOGSAT_HIP_H
+#define ANALOGSAT_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define ASAT_OK 0
+#define ASAT_ERR_INVALID_ARGUMENT 1 /* reference: std::invalid_argument */
+#define ASAT_ERR_NOT_CONFIGURED 2   /* reference: runtime_error("SAT problem has not been set") */
+#define ASAT_ERR_HIP 3              /* reference: CudaSafe -> runtime_error (AnalogSAT/cuda_base/CudaUtils.cu:27-44) */
+#define ASAT_UNDERFLOW 4            /* reference: IODEInt::Step returns false (step-size underflow) */
+#define ASAT_ERR_UNSUPPORTED 5
+
+/* precision of the state vectors: FP64 state (reference's <double>) or FP32
+   state with FP64 step control, error norm and time */
+#define ASAT_F64 0
+#define ASAT_F32 1
+
+typedef struct asat_sys asat_sys_t;
+typedef struct asat_state asat_state_t;
+typedef struct asat_rk asat_rk_t;
+typedef struct asat_rng asat_rng_t;
+typedef struct asat_batch asat_batch_t;
+
+/* last error message of the calling thread ("" if none) */
+const char* asat_last_error(void);
+/* library version string */
+const char* asat_version(void);
+/* number of visible GPUs (0 without a GPU); never fails */
+int asat_device_count(void);
+
+/* ---------------- problem helpers (host only) ------------------------------ */
+
+/* FastSatMaker(seed) with Set_N(n), Set_M(m), Set_K(k), MakeSatProblem:
+   writes m*k literals, clause-major (AnalogSAT/problem/FastSatMaker.cpp:59-107).
+   Same libstdc++ minstd_rand0 stream, so the same seed gives the same instance. */
+int asat_problem_fastsat(int n, int m, int k, unsigned long seed, int* literals_out);
+/* The same generator as a continuing stream (one FastSatMaker object making
+   several problems): seed the engine once, then each call advances *engine. */
+int asat_problem_seed_engine(unsigned long seed, unsigned long long* engine);
+int asat_problem_fastsat_stream(int n, int m, int k, unsigned long long* engine, int* literals_out);
+
+/* ---------------- CTDS system (ISat) --------------------------------------- */
+
+/* new HipSat<T>; precision = ASAT_F64 or ASAT_F32 (CudaSat3<TFloat>() ctor) */
+int asat_sys_create(int precision, asat_sys_t** out);
+int asat_sys_destroy(asat_sys_t* sys);
+/* ISat::SetProblem(SatProblem): literals/lengths as SatProblem::AddClauses
+   (zero literals and empty clauses dropped; N = max |lit|, K = max length) */
+int asat_sys_set_problem(asat_sys_t* sys, const int* literals, const int* lengths, int nclauses);
+/* ISat::Get_N / Get_M / Get_K */
+int asat_sys_get_nmk(const asat_sys_t* sys, int* n, int* m, int* k);
+/* ISat::Get_B / Set_B (sine bias prefactor; only b == 0 is supported on the GPU) */
+int asat_sys_set_b(asat_sys_t* sys, double b);
+int asat_sys_get_b(const asat_sys_t* sys, double* b);
+/* IODE::GetDerivatives(dxdt, x, t) */
+int asat_sys_rhs(asat_sys_t* sys, asat_state_t* dxdt, const asat_state_t* x, double t);
+/* ISat::GetClauseViolationCount(state) */
+int asat_sys_violations(asat_sys_t* sys, const asat_state_t* x, int* count);
+/* ISat::SetRandomState(state, random): s = 2u-1, a = u+1 */
+int asat_sys_set_random_state(asat_sys_t* sys, asat_state_t* x, asat_rng_t* rng);
+
+/* ---------------- states (ISatState) -------------------------------------- */
+
+/* ISat::MakeState() */
+int asat_state_create(asat_sys_t* sys, asat_state_t** out);
+int asat_state_destroy(asat_state_t* st);
+/* ISatState::CopyFrom(vector<TFloat>) / CopyTo(vector<TFloat>&):
+   host array [s_1..s_N, a_1..a_M] of the system's precision (double or float) */
+int asat_state_set_public(asat_state_t* st, const void* host_pub);
+int asat_state_get_public(const asat_state_t* st, void* host_pub);
+/* ISatState::CopyBooleanTo: out[i] = s_i > 0 */
+int asat_state_get_boolean(const asat_state_t* st, unsigned char* out);
+/* IODEState::CopyFrom(const IODEState&) */
+int asat_state_copy(asat_state_t* dst, const asat_state_t* src);
+/* IODEState::SetZero */
+int asat_state_set_zero(asat_state_t* st);
+/* IODEState::GetAllocSize */
+int asat_state_alloc_size(const asat_state_t* st, int* size);
+/* IODEState::GetState(): device pointer of the internal layout
+   [-1, s_1..s_N, pad, a_1..a_M, pad]; aux values start at *aux_offset */
+int asat_state_device_ptr(asat_state_t* st, void** dev_ptr, int* aux_offset);
+
+/* ---------------- integrator (IODEInt / CudaRungeKutta) -------------------- */
+
+typedef struct {
+    double time;      /* IODEInt::GetTime */
+    double hdid;      /* IODEInt::GetLastStepSize */
+    double hnext;     /* proposed next step */
+    double errmax;    /* last error norm divided by eps */
+    long long steps;     /* IODEInt::GetStepCount */
+    long long attempts;  /* IODEInt::GetAttemptCount */
+    long long rhscount;  /* IODEInt::GetRHSCount */
+} asat_rk_stats_t;
+
+int asat_rk_create(asat_rk_t** out);
+int asat_rk_destroy(asat_rk_t* rk);
+/* CudaRungeKutta::SetEpsilon (range [1e-16, 1]) / GetEpsilon */
+int asat_rk_set_eps(asat_rk_t* rk, double eps);
+int asat_rk_get_eps(const asat_rk_t* rk, double* eps);
+/* IODEInt::Configure(ode): allocates scratch, resets h = 1e-9 */
+int asat_rk_configure(asat_rk_t* rk, asat_sys_t* sys);
+/* IODEInt::Step(old, new, count): count accepted steps; on return `old`
+   holds the newest state (the two device buffers are swapped as needed).
+   Returns ASAT_UNDERFLOW if the step size underflowed. */
+int asat_rk_step(asat_rk_t* rk, asat_state_t* old_state, asat_state_t* new_state, int count);
+/* IODEInt::Reset */
+int asat_rk_reset(asat_rk_t* rk);
+int asat_rk_stats(const asat_rk_t* rk, asat_rk_stats_t* out);
+/* Device time of the last asat_rk_step call in milliseconds (HIP events on
+   the integrator's stream); *ms = -1 if none */
+int asat_rk_last_step_ms(const asat_rk_t* rk, float* ms);
+/* Average device time in microseconds of each kernel of one attempt, measured
+   over `reps` launches with HIP events on the integrator's stream.
+   us_out[13]: clause pass J=1..6, var pass J=1..6, control.  No reference
+   counterpart (measurement hook); the integrator state is left unchanged. */
+int asat_rk_profile(asat_rk_t* rk, asat_state_t* old_state, asat_state_t* new_state, int reps, float* us_out);
+
+/* ---------------- random numbers (ISatRandom / CudaRandom) ----------------- */
+
+/* own counter-based generator (Philox4x32-10, Salmon et al., SC'11) */
+int asat_rng_create(unsigned long long seed, asat_rng_t** out);
+int asat_rng_destroy(asat_rng_t* rng);
+/* ISatRandom::GenerateUniform(devptr, len): len uniforms in [0,1) written to
+   device memory of the given precision (ASAT_F64 double / ASAT_F32 float) */
+int asat_rng_uniform(asat_rng_t* rng, void* dev_ptr, int len, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ANALOGSAT_HIP_H */

@@ -87,6 +87,10 @@ def lib():
         L.orc_uniform_f64.argtypes = [ctypes.c_uint, _f64p, ctypes.c_int]
         L.orc_random_state_f64.argtypes = [ctypes.c_uint, ctypes.c_int, ctypes.c_int, _f64p]
         L.orc_fastsat.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulong, _i32p]
+        L.orc_set_pow_mode.argtypes = [ctypes.c_int]
+        L.orc_get_pow_mode.restype = ctypes.c_int
+        L.orc_pow_cr.argtypes = [ctypes.c_double, ctypes.c_int]
+        L.orc_pow_cr.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -181,6 +185,26 @@ def random_state_f64(seed, n, m):
     out = np.zeros(n + m, dtype=np.float64)
     lib().orc_random_state_f64(seed, n, m, out)
     return out
+
+
+class pow_mode:
+    """Context manager: 'glibc' (reference behaviour, default) or 'cr'
+    (correctly rounded controller pow, the GPU's definition)."""
+
+    def __init__(self, mode):
+        self.mode = {"glibc": 0, "cr": 1}[mode]
+
+    def __enter__(self):
+        self.prev = lib().orc_get_pow_mode()
+        lib().orc_set_pow_mode(self.mode)
+        return self
+
+    def __exit__(self, *a):
+        lib().orc_set_pow_mode(self.prev)
+
+
+def pow_cr(x, q):
+    return lib().orc_pow_cr(float(x), int(q))
 
 
 def rhs(problem, x, mode="f64", b=0.0):

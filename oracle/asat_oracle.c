@@ -29,6 +29,45 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Step-controller pow.  Mode 0 (default): glibc pow, as the reference calls it
+   (AnalogSAT/cpu_sat/CpuRungeKutta.cpp:243,254).  Mode 1: correctly rounded
+   x^-0.25 / x^-0.2 (one double-double Newton step on x r^q = 1 plus the
+   first-order factor for -0.2 != -1/5), the definition the GPU controller
+   uses; glibc's pow differs from it in about 0.1% of arguments. */
+static int g_pow_mode = 0;
+void orc_set_pow_mode(int mode) { g_pow_mode = mode; }
+int orc_get_pow_mode(void) { return g_pow_mode; }
+
+static void dd_mul_d(double *hi, double *lo, double b)
+{
+    double p = *hi * b;
+    double e = fma(*hi, b, -p);
+    e = fma(*lo, b, e);
+    double s = p + e;
+    *lo = e - (s - p);
+    *hi = s;
+}
+
+double orc_pow_cr(double x, int q)
+{
+    const double y = q == 4 ? -0.25 : -0.2;
+    if (!(x > 0.0) || !(x < 1.0e300) || !(x > 1.0e-300)) return pow(x, y);
+    double r = pow(x, y);
+    double hi = r, lo = 0.0;
+    for (int i = 1; i < q; i++) dd_mul_d(&hi, &lo, r);
+    dd_mul_d(&hi, &lo, x);
+    double e = (1.0 - hi) - lo;
+    double corr = r * e / (double)q;
+    if (q == 5) corr -= r * (1.1102230246251565404e-17 * log(x));
+    return r + corr;
+}
+
+static double ctrl_pow(double x, double y)
+{
+    if (g_pow_mode == 1 && (y == -0.25 || y == -0.2)) return orc_pow_cr(x, y == -0.25 ? 4 : 5);
+    return pow(x, y);
+}
+
 #define ST double
 #define CT double
 #define SFX _f64

@@ -245,12 +245,12 @@ static int FN(rkqs)(FN(orc_rk_t) *r)
         ermax /= r->eps;
         r->last_errmax = ermax;
         if (ermax <= (CT)1.0) break;
-        CT htemp = SAFETY * h * pow(ermax, PSHRNK);
+        CT htemp = SAFETY * h * (CT)ctrl_pow((double)ermax, (double)PSHRNK);
         h = h >= (CT)0 ? fmax(htemp, HFACTOR * h) : fmin(htemp, HFACTOR * h);
         CT tnew = r->time + h;
         if (tnew == r->time) return 0;
     }
-    if (ermax > ERRCON) r->hnext = SAFETY * h * pow(ermax, PGROW);
+    if (ermax > ERRCON) r->hnext = SAFETY * h * (CT)ctrl_pow((double)ermax, (double)PGROW);
     else r->hnext = (CT)5.0 * h;
     r->hdid = h;
     r->time += r->hdid;
