@@ -57,6 +57,20 @@ def GetSatResultMessage(result):
     return _RESULT_MSG.get(int(result), "unknown result")
 
 
+class BatchResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("steps", ctypes.c_longlong), ("attempts", ctypes.c_longlong),
+                ("rhscount", ctypes.c_longlong), ("time", ctypes.c_double), ("hdid", ctypes.c_double),
+                ("hnext", ctypes.c_double), ("min_violations", ctypes.c_int), ("violations", ctypes.c_int),
+                ("sat_step", ctypes.c_int), ("sat_time", ctypes.c_double), ("solved_wall_ms", ctypes.c_double)]
+
+
+class BatchBucket(ctypes.Structure):
+    _fields_ = [("trajectories", ctypes.c_int), ("max_n", ctypes.c_int), ("max_m", ctypes.c_int),
+                ("max_slots", ctypes.c_int), ("block", ctypes.c_int), ("lds", ctypes.c_int),
+                ("grid", ctypes.c_int), ("lds_bytes", ctypes.c_longlong),
+                ("tuned_us_per_attempt", ctypes.c_double)]
+
+
 class RkStats(ctypes.Structure):
     _fields_ = [("time", ctypes.c_double), ("hdid", ctypes.c_double), ("hnext", ctypes.c_double),
                 ("errmax", ctypes.c_double), ("steps", ctypes.c_longlong), ("attempts", ctypes.c_longlong),
@@ -116,6 +130,19 @@ def lib():
             "asat_rng_create": ([ctypes.c_ulonglong, pvp], i),
             "asat_rng_destroy": ([vp], i),
             "asat_rng_uniform": ([vp, vp, i, i], i),
+            "asat_batch_create": ([pvp], i),
+            "asat_batch_destroy": ([vp], i),
+            "asat_batch_add": ([vp, vp, vp, i, vp, ip], i),
+            "asat_batch_set_params": ([vp, ctypes.c_double, ctypes.c_longlong, ctypes.c_double, i, i], i),
+            "asat_batch_set_tiling": ([vp, i, i, i], i),
+            "asat_batch_prepare": ([vp, i], i),
+            "asat_batch_run": ([vp, i, i], i),
+            "asat_batch_reset": ([vp], i),
+            "asat_batch_size": ([vp, ip], i),
+            "asat_batch_result": ([vp, i, ctypes.POINTER(BatchResult)], i),
+            "asat_batch_get_state": ([vp, i, vp], i),
+            "asat_batch_stats": ([vp, ctypes.POINTER(ctypes.c_double), ip, ip], i),
+            "asat_batch_bucket_info": ([vp, i, ctypes.POINTER(BatchBucket)], i),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -574,3 +601,83 @@ class SatSolver:
         self.lastStepSize = st.hdid
         self.running = False
         return result
+
+
+# ----------------------------------------------------------------------------
+# batched ragged solver (no reference counterpart)
+# ----------------------------------------------------------------------------
+BATCH_STATUS = {0: "running", 1: "solved", 2: "max steps", 3: "underflow", 4: "max time"}
+
+
+class HipBatch:
+    """Many independent trajectories in persistent launches (asat_batch_*).
+
+    Every trajectory runs the FP32-state / FP64-control Cash-Karp integrator
+    of an ``HipSat('f32')`` system; SAT is checked every ``check_every``
+    accepted steps, the minimum violated-clause count is tracked.
+    """
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        _check(lib().asat_batch_create(ctypes.byref(self._h)))
+        self._sizes = []
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value and _lib is not None:
+            _lib.asat_batch_destroy(h)
+
+    def add(self, problem, pub0):
+        lits = np.ascontiguousarray(problem.literals, np.int32)
+        lens = np.ascontiguousarray(problem.lengths, np.int32)
+        x0 = np.ascontiguousarray(pub0, np.float32)
+        idx = ctypes.c_int()
+        _check(lib().asat_batch_add(self._h, lits.ctypes.data, lens.ctypes.data, len(lens), x0.ctypes.data,
+                                    ctypes.byref(idx)))
+        self._sizes.append(len(x0))
+        return idx.value
+
+    def set_params(self, eps=1e-6, max_steps=10000, max_time=1e300, chunk_steps=256, check_every=1):
+        _check(lib().asat_batch_set_params(self._h, float(eps), int(max_steps), float(max_time), int(chunk_steps),
+                                           int(check_every)))
+
+    def set_tiling(self, block=0, lds_mode=-1, nbuckets=4):
+        _check(lib().asat_batch_set_tiling(self._h, int(block), int(lds_mode), int(nbuckets)))
+
+    def prepare(self, autotune=True):
+        _check(lib().asat_batch_prepare(self._h, 1 if autotune else 0))
+
+    def run(self, max_chunks=1 << 30, graph_chunks=4):
+        _check(lib().asat_batch_run(self._h, int(max_chunks), int(graph_chunks)))
+
+    def reset(self):
+        _check(lib().asat_batch_reset(self._h))
+
+    def __len__(self):
+        return len(self._sizes)
+
+    def result(self, i):
+        r = BatchResult()
+        _check(lib().asat_batch_result(self._h, int(i), ctypes.byref(r)))
+        return {f: getattr(r, f) for f, _ in BatchResult._fields_}
+
+    def results(self):
+        return [self.result(i) for i in range(len(self))]
+
+    def state(self, i):
+        out = np.zeros(self._sizes[i], np.float32)
+        _check(lib().asat_batch_get_state(self._h, int(i), out.ctypes.data))
+        return out
+
+    def stats(self):
+        ms, ch, nb = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().asat_batch_stats(self._h, ctypes.byref(ms), ctypes.byref(ch), ctypes.byref(nb)))
+        return {"run_ms": ms.value, "chunks": ch.value, "buckets": nb.value}
+
+    def buckets(self):
+        out = []
+        for b in range(self.stats()["buckets"]):
+            r = BatchBucket()
+            _check(lib().asat_batch_bucket_info(self._h, b, ctypes.byref(r)))
+            out.append({f: getattr(r, f) for f, _ in BatchBucket._fields_})
+        return out

@@ -194,6 +194,36 @@ __device__ __forceinline__ int block_sum(int v, int* sh)
     return r;
 }
 
+// block reductions whose result every thread receives (all threads of the
+// block must call them; no thread may have exited)
+template <int BLOCK>
+__device__ __forceinline__ double block_max_all(double v, double* sh)
+{
+    v = wave_max(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double r = sh[0];
+#pragma unroll
+    for (int i = 1; i < BLOCK / 64; i++) r = fmax(r, sh[i]);
+    __syncthreads();
+    return r;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ int block_sum_all(int v, int* sh)
+{
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    int r = 0;
+#pragma unroll
+    for (int i = 0; i < BLOCK / 64; i++) r += sh[i];
+    __syncthreads();
+    return r;
+}
+
 inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 }  // namespace asat

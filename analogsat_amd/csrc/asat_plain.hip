@@ -4,9 +4,12 @@
 //
 // Data layout in HBM (T = double for ASAT_F64, float for ASAT_F32):
 //   lit[m*K + q]      int32 signed literal (+v / -v, 0 = padding), clause-major
-//   csr_ptr[0..N+1]   var-major CSR row pointers over literal slots
-//   csr_slot[e]       slot index m*K+q, ascending within each variable
-//   contrib[m*K + q]  per-slot contribution 2 c a K K_mi of the current stage
+//   pos[m*K + q]      where the slot's contribution is stored (-1: padding)
+//   contrib[p]        per-slot contribution 2 c a K K_mi of the current stage,
+//                     grouped by variable group (8192 float / 4096 double
+//                     variables -> ~400 KB regions), clause order inside a region
+//   csr_ptr[0..N+1]   var-major CSR row pointers
+//   csr_pos[e]        contribution positions of each variable, in slot order
 //   state buffer      [ -1, s_1..s_N, pad | a_1..a_M, pad ], aux at aoff
 //   k1..k5            same layout as the state (stage derivatives)
 //   tmp               spin part of the next stage's input (aux stage inputs are
@@ -17,7 +20,8 @@
 //                  k_J(a_m) = a K^2, per-slot contributions (coalesced writes)
 //   var pass J:    deterministic per-variable sum over the CSR (slot order ==
 //                  the reference's accumulation order), k_J(s_i), and the
-//                  spin part of stage J+1's input
+//                  spin part of stage J+1's input; the blocks of a variable
+//                  group run on one XCD, so the gathers hit that XCD's L2
 //   J = 6 passes also form x_new and |xerr/xscal| and a per-block FP64 max;
 //   the control kernel finishes the max-norm and runs the Cash-Karp
 //   accept/reject/step-size logic in FP64 on the device.
@@ -48,16 +52,19 @@ static inline long long round_up(long long a, long long b) { return (a + b - 1) 
 // ===========================================================================
 struct HostProblem {
     int n = 0, m = 0, k = 0;
+    int vg = 0;                 // variables per group (one contiguous contribution region each)
+    int ngroups = 0;
     std::vector<int> lit;       // m*k, zero padded
+    std::vector<int> pos;       // m*k: position of the slot's contribution, -1 for padding
     std::vector<int> csr_ptr;   // n+2
-    std::vector<int> csr_slot;  // number of non-padding slots
+    std::vector<int> csr_pos;   // contribution positions of each variable, in slot order
 };
 
 // SatProblem::AddClauses semantics (AnalogSAT/problem/SatProblem.cpp:64-95) +
 // CpuSat::InitProblem clause matrix (AnalogSAT/cpu_sat/CpuSat.cpp:352-377) +
 // the transposed var-major slot list (role of CudaSat3::InitProblem,
 // AnalogSAT/cuda_sat/CudaSat3.cu:266-307, without the pad-to-32).
-static void build_problem(const int* literals, const int* lengths, int nclauses, HostProblem& P)
+static void build_problem(const int* literals, const int* lengths, int nclauses, int vg, HostProblem& P)
 {
     require(nclauses >= 0, ASAT_ERR_INVALID_ARGUMENT, "negative clause count");
     require(nclauses == 0 || (literals && lengths), ASAT_ERR_INVALID_ARGUMENT, "null clause arrays");
@@ -93,18 +100,36 @@ static void build_problem(const int* literals, const int* lengths, int nclauses,
         if (q > 0) row++;
         off += (size_t)lengths[j];
     }
-    // counting sort of slots by variable; slots visited in ascending order
+    // Contribution layout: variables are split into groups of vg; the
+    // contributions of a group's variables form one contiguous region, in
+    // slot (clause) order.  pos[slot] is the slot's place in that layout.
+    P.vg = vg;
+    P.ngroups = (n + vg - 1) / vg;
+    std::vector<long long> gcount((size_t)P.ngroups + 1, 0);
+    for (size_t s = 0; s < P.lit.size(); s++) {
+        int l = P.lit[s];
+        if (l) gcount[(size_t)(std::abs(l) - 1) / vg + 1]++;
+    }
+    for (int g = 0; g < P.ngroups; g++) gcount[g + 1] += gcount[g];
+    P.pos.assign(P.lit.size(), -1);
+    std::vector<long long> gfill(gcount.begin(), gcount.end() - 1);
+    for (size_t s = 0; s < P.lit.size(); s++) {
+        int l = P.lit[s];
+        if (l) P.pos[s] = (int)gfill[(size_t)(std::abs(l) - 1) / vg]++;
+    }
+    // per-variable lists of contribution positions; slots visited in
+    // ascending order, which is the reference's accumulation order
     P.csr_ptr.assign((size_t)n + 2, 0);
     for (size_t s = 0; s < P.lit.size(); s++) {
         int l = P.lit[s];
         if (l) P.csr_ptr[(size_t)std::abs(l) + 1]++;
     }
     for (int i = 0; i <= n; i++) P.csr_ptr[i + 1] += P.csr_ptr[i];
-    P.csr_slot.assign((size_t)P.csr_ptr[n + 1], 0);
+    P.csr_pos.assign((size_t)P.csr_ptr[n + 1], 0);
     std::vector<int> fill(P.csr_ptr.begin(), P.csr_ptr.end() - 1);
     for (size_t s = 0; s < P.lit.size(); s++) {
         int l = P.lit[s];
-        if (l) P.csr_slot[(size_t)fill[std::abs(l)]++] = (int)s;
+        if (l) P.csr_pos[(size_t)fill[std::abs(l)]++] = P.pos[s];
     }
 }
 
@@ -114,8 +139,9 @@ static void build_problem(const int* literals, const int* lengths, int nclauses,
 template <typename T>
 struct StageArgs {
     const int* lit;
+    const int* pos;        // contribution position of each slot (-1: padding)
     const int* csr_ptr;
-    const int* csr_slot;
+    const int* csr_pos;    // per-variable contribution positions
     T* contrib;
     T* xa;       // state buffer 0
     T* xb;       // state buffer 1 (x = parity ? xb : xa; x_new is the other)
@@ -125,8 +151,50 @@ struct StageArgs {
     const Ctrl* ctrl;  // null: standalone RHS (x = xa, output to k1, no tmp)
     int n, m, k, aoff;
     int nblk_clause;
+    int vg_blocks;     // var-pass blocks per variable group
+    int ngroups;
+    int clause_per_xcd;   // clause-pass chunks (of BLK clauses) per XCD
     T knorm;
 };
+
+// XCD-contiguous clause-pass mapping: XCD x (blocks b with b % 8 == x) walks
+// the clause chunks [x*per, (x+1)*per) in order.  Contribution positions are
+// assigned in clause order inside each region, so every XCD fills its own
+// contiguous stretch of each region and its L2 merges the partial lines
+// before they are written back.
+__device__ __forceinline__ int clause_of_thread(int per)
+{
+    const int b = blockIdx.x;
+    return ((b & 7) * per + (b >> 3)) * BLK + threadIdx.x;
+}
+
+// XCD-affine var-pass mapping: workgroups are dispatched to the 8 XCDs
+// round-robin, so block b runs on XCD b % 8.  All vg_blocks blocks of one
+// variable group are given to the same XCD, and each XCD walks its groups in
+// order, so a group's contribution region stays in that XCD's L2 while its
+// variables gather from it.  Returns the variable index (1-based) or 0.
+__device__ __forceinline__ int var_of_block(int vg_blocks, int ngroups, int n)
+{
+    const int b = blockIdx.x;
+    const int xcd = b & 7, t = b >> 3;
+    const int grp = (t / vg_blocks) * 8 + xcd;
+    if (grp >= ngroups) return 0;
+    const long long i = (long long)grp * vg_blocks * BLK + (long long)(t % vg_blocks) * BLK + threadIdx.x + 1;
+    return i <= n ? (int)i : 0;
+}
+
+// streaming loads/stores bypass the caches' normal retention so that the
+// gathered arrays (spin stage input, contribution regions) keep the L2
+template <typename V>
+__device__ __forceinline__ V ld_nt(const V* p)
+{
+    return __builtin_nontemporal_load(p);
+}
+template <typename V>
+__device__ __forceinline__ void st_nt(V* p, V v)
+{
+    __builtin_nontemporal_store(v, p);
+}
 
 template <typename T>
 __device__ __forceinline__ T spin_factor(int l, const T* sp)
@@ -180,13 +248,16 @@ __global__ __launch_bounds__(BLK) void k_clause(StageArgs<T> a)
     const T* x = parity ? a.xb : a.xa;
     T* xnew = parity ? a.xa : a.xb;
     const T* sp = (J == 1) ? x : a.tmp;
-    const int m = blockIdx.x * BLK + threadIdx.x;
+    const int m = clause_of_thread(a.clause_per_xcd);
     double err = 0.0;
     if (m < a.m) {
         int l[K];
         T f[K];
+        int ps[K];
 #pragma unroll
-        for (int q = 0; q < K; q++) l[q] = a.lit[(size_t)m * K + q];
+        for (int q = 0; q < K; q++) l[q] = ld_nt(a.lit + (size_t)m * K + q);
+#pragma unroll
+        for (int q = 0; q < K; q++) ps[q] = ld_nt(a.pos + (size_t)m * K + q);
         T km = a.knorm;
 #pragma unroll
         for (int q = 0; q < K; q++) {
@@ -194,27 +265,27 @@ __global__ __launch_bounds__(BLK) void k_clause(StageArgs<T> a)
             km *= f[q];
         }
         const size_t ai = (size_t)a.aoff + m;
-        T am, k1 = 0, k3 = 0, k4 = 0, k5 = 0;
+        T am, xa = ld_nt(x + ai), k1 = 0, k3 = 0, k4 = 0, k5 = 0;
         if constexpr (J == 1) {
-            am = x[ai];
+            am = xa;
         } else {
-            k1 = a.k1[ai];
-            T k2 = J > 2 ? a.k2[ai] : (T)0;
-            k3 = J > 3 ? a.k3[ai] : (T)0;
-            k4 = J > 4 ? a.k4[ai] : (T)0;
-            k5 = J > 5 ? a.k5[ai] : (T)0;
-            am = stage_input<T, J>(x[ai], k1, k2, k3, k4, k5, hs);
+            k1 = ld_nt(a.k1 + ai);
+            T k2 = J > 2 ? ld_nt(a.k2 + ai) : (T)0;
+            k3 = J > 3 ? ld_nt(a.k3 + ai) : (T)0;
+            k4 = J > 4 ? ld_nt(a.k4 + ai) : (T)0;
+            k5 = J > 5 ? ld_nt(a.k5 + ai) : (T)0;
+            am = stage_input<T, J>(xa, k1, k2, k3, k4, k5, hs);
         }
         T kj = am * km * km;
-        if constexpr (J == 1) a.k1[ai] = kj;
-        else if constexpr (J == 2) a.k2[ai] = kj;
-        else if constexpr (J == 3) a.k3[ai] = kj;
-        else if constexpr (J == 4) a.k4[ai] = kj;
-        else if constexpr (J == 5) a.k5[ai] = kj;
+        if constexpr (J == 1) st_nt(a.k1 + ai, kj);
+        else if constexpr (J == 2) st_nt(a.k2 + ai, kj);
+        else if constexpr (J == 3) st_nt(a.k3 + ai, kj);
+        else if constexpr (J == 4) st_nt(a.k4 + ai, kj);
+        else if constexpr (J == 5) st_nt(a.k5 + ai, kj);
         else {
             T xn;
-            final_combination<T>(x[ai], k1, k3, k4, k5, kj, hs, h, hnext, xn, err);
-            xnew[ai] = xn;
+            final_combination<T>(xa, k1, k3, k4, k5, kj, hs, h, hnext, xn, err);
+            st_nt(xnew + ai, xn);
         }
         const T r = km * a.knorm * am;
         T out[K];
@@ -223,7 +294,7 @@ __global__ __launch_bounds__(BLK) void k_clause(StageArgs<T> a)
         write_contribs<T, K, 0>(l, f, r, out);
 #pragma unroll
         for (int q = 0; q < K; q++)
-            if (l[q] != 0) a.contrib[(size_t)m * K + q] = out[q];
+            if (l[q] != 0) a.contrib[ps[q]] = out[q];
     }
     if constexpr (J == 6) {
         double bm = block_max<BLK>(err, sh);
@@ -250,7 +321,7 @@ __global__ __launch_bounds__(BLK) void k_clause_gen(StageArgs<T> a)
     T* xnew = parity ? a.xa : a.xb;
     const T* sp = (J == 1) ? x : a.tmp;
     const int K = a.k;
-    const int m = blockIdx.x * BLK + threadIdx.x;
+    const int m = clause_of_thread(a.clause_per_xcd);
     T* f = fsh + threadIdx.x;   // f[q*BLK]
     double err = 0.0;
     if (m < a.m) {
@@ -284,7 +355,8 @@ __global__ __launch_bounds__(BLK) void k_clause_gen(StageArgs<T> a)
             final_combination<T>(x[ai], k1, k3, k4, k5, kj, hs, h, hnext, xn, err);
             xnew[ai] = xn;
         }
-        T* out = a.contrib + (size_t)m * K;
+        T* out = a.contrib;
+        const int* pm = a.pos + (size_t)m * K;
         if (K <= 10) {
             const T r = km * a.knorm * am;
             for (int q = 0; q < K; q++) {
@@ -299,7 +371,7 @@ __global__ __launch_bounds__(BLK) void k_clause_gen(StageArgs<T> a)
                 }
                 T rr = r * p;
                 rr *= l < 0 ? (T)-2.0 : (T)2.0;
-                out[q] = rr;
+                out[pm[q]] = rr;
             }
         } else {
             for (int q = 0; q < K; q++) {
@@ -307,7 +379,7 @@ __global__ __launch_bounds__(BLK) void k_clause_gen(StageArgs<T> a)
                 if (l == 0) continue;
                 T temp = km;
                 for (int qq = 0; qq < K; qq++) temp *= (qq == q) ? (T)1.0 : f[qq * BLK];
-                out[q] = (T)2.0 * temp * a.knorm * am * (l < 0 ? (T)-1.0 : (T)1.0);
+                out[pm[q]] = (T)2.0 * temp * a.knorm * am * (l < 0 ? (T)-1.0 : (T)1.0);
             }
         }
     }
@@ -334,36 +406,39 @@ __global__ __launch_bounds__(BLK) void k_var(StageArgs<T> a)
     const T hs = (T)h;
     const T* x = parity ? a.xb : a.xa;
     T* xnew = parity ? a.xa : a.xb;
-    const int i = blockIdx.x * BLK + threadIdx.x + 1;
+    const int i = var_of_block(a.vg_blocks, a.ngroups, a.n);
     double err = 0.0;
-    if (i <= a.n) {
+    if (i != 0) {
         T kj = (T)0;
         if (fresh) {
-            const int e0 = a.csr_ptr[i], e1 = a.csr_ptr[i + 1];
+            const int e0 = ld_nt(a.csr_ptr + i), e1 = ld_nt(a.csr_ptr + i + 1);
             T sum = (T)0;
-            for (int e = e0; e < e1; e++) sum += a.contrib[a.csr_slot[e]];
+            for (int e = e0; e < e1; e++) sum += a.contrib[ld_nt(a.csr_pos + e)];
             kj = sum;
         }
+        const T xi = ld_nt(x + i);
         if constexpr (J == 1) {
-            if (fresh) a.k1[i] = kj;
-            else kj = a.k1[i];
-            if (a.tmp) a.tmp[i] = stage_input<T, 2>(x[i], kj, 0, 0, 0, 0, hs);
+            if (fresh) st_nt(a.k1 + i, kj);
+            else kj = ld_nt(a.k1 + i);
+            if (a.tmp) a.tmp[i] = stage_input<T, 2>(xi, kj, 0, 0, 0, 0, hs);
         } else if constexpr (J == 2) {
-            a.k2[i] = kj;
-            a.tmp[i] = stage_input<T, 3>(x[i], a.k1[i], kj, 0, 0, 0, hs);
+            st_nt(a.k2 + i, kj);
+            a.tmp[i] = stage_input<T, 3>(xi, ld_nt(a.k1 + i), kj, 0, 0, 0, hs);
         } else if constexpr (J == 3) {
-            a.k3[i] = kj;
-            a.tmp[i] = stage_input<T, 4>(x[i], a.k1[i], a.k2[i], kj, 0, 0, hs);
+            st_nt(a.k3 + i, kj);
+            a.tmp[i] = stage_input<T, 4>(xi, ld_nt(a.k1 + i), ld_nt(a.k2 + i), kj, 0, 0, hs);
         } else if constexpr (J == 4) {
-            a.k4[i] = kj;
-            a.tmp[i] = stage_input<T, 5>(x[i], a.k1[i], a.k2[i], a.k3[i], kj, 0, hs);
+            st_nt(a.k4 + i, kj);
+            a.tmp[i] = stage_input<T, 5>(xi, ld_nt(a.k1 + i), ld_nt(a.k2 + i), ld_nt(a.k3 + i), kj, 0, hs);
         } else if constexpr (J == 5) {
-            a.k5[i] = kj;
-            a.tmp[i] = stage_input<T, 6>(x[i], a.k1[i], a.k2[i], a.k3[i], a.k4[i], kj, hs);
+            st_nt(a.k5 + i, kj);
+            a.tmp[i] = stage_input<T, 6>(xi, ld_nt(a.k1 + i), ld_nt(a.k2 + i), ld_nt(a.k3 + i), ld_nt(a.k4 + i), kj,
+                                         hs);
         } else {
             T xn;
-            final_combination<T>(x[i], a.k1[i], a.k3[i], a.k4[i], a.k5[i], kj, hs, h, hnext, xn, err);
-            xnew[i] = xn;
+            final_combination<T>(xi, ld_nt(a.k1 + i), ld_nt(a.k3 + i), ld_nt(a.k4 + i), ld_nt(a.k5 + i), kj, hs, h,
+                                 hnext, xn, err);
+            st_nt(xnew + i, xn);
         }
     }
     if constexpr (J == 6) {
@@ -526,8 +601,10 @@ struct Sys {
     long long nslots = 0;
     hipStream_t stream = nullptr;
     int* d_lit = nullptr;
+    int* d_pos = nullptr;
     int* d_csr_ptr = nullptr;
-    int* d_csr_slot = nullptr;
+    int* d_csr_pos = nullptr;
+    int vg = 0, ngroups = 0, vg_blocks = 0;
     void* d_contrib = nullptr;
     int* d_count = nullptr;
     HostProblem hp;
@@ -535,7 +612,7 @@ struct Sys {
 
     ~Sys()
     {
-        dfree(d_lit); dfree(d_csr_ptr); dfree(d_csr_slot); dfree(d_contrib); dfree(d_count);
+        dfree(d_lit); dfree(d_pos); dfree(d_csr_ptr); dfree(d_csr_pos); dfree(d_contrib); dfree(d_count);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void require_ready() const { require(ready, ASAT_ERR_NOT_CONFIGURED, "SAT problem has not been set"); }
@@ -548,24 +625,32 @@ struct Rng {
 static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclauses)
 {
     HostProblem P;
-    build_problem(lits, lens, nclauses, P);
+    // ~400 KB contribution region per variable group (8192 float / 4096 double vars)
+    const int vg = s->esize == 4 ? 8192 : 4096;
+    build_problem(lits, lens, nclauses, vg, P);
     HIP_CHECK(hipStreamSynchronize(s->stream));
-    dfree(s->d_lit); dfree(s->d_csr_ptr); dfree(s->d_csr_slot); dfree(s->d_contrib);
+    dfree(s->d_lit); dfree(s->d_pos); dfree(s->d_csr_ptr); dfree(s->d_csr_pos); dfree(s->d_contrib);
     s->ready = false;
     s->n = P.n; s->m = P.m; s->k = P.k;
     s->aoff = (int)round_up((long long)P.n + 1, 64);
     s->alloc = (int)(s->aoff + round_up(P.m, 64));
     s->nslots = (long long)P.m * P.k;
+    s->vg = P.vg;
+    s->ngroups = P.ngroups;
+    s->vg_blocks = P.vg / BLK;
+    const size_t ncontrib = std::max<size_t>(P.csr_pos.size(), 1);
     HIP_CHECK(hipMalloc((void**)&s->d_lit, sizeof(int) * P.lit.size()));
+    HIP_CHECK(hipMalloc((void**)&s->d_pos, sizeof(int) * P.pos.size()));
     HIP_CHECK(hipMalloc((void**)&s->d_csr_ptr, sizeof(int) * P.csr_ptr.size()));
-    HIP_CHECK(hipMalloc((void**)&s->d_csr_slot, sizeof(int) * std::max<size_t>(P.csr_slot.size(), 1)));
-    HIP_CHECK(hipMalloc(&s->d_contrib, (size_t)s->esize * P.lit.size()));
+    HIP_CHECK(hipMalloc((void**)&s->d_csr_pos, sizeof(int) * ncontrib));
+    HIP_CHECK(hipMalloc(&s->d_contrib, (size_t)s->esize * ncontrib));
     HIP_CHECK(hipMemcpy(s->d_lit, P.lit.data(), sizeof(int) * P.lit.size(), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(s->d_pos, P.pos.data(), sizeof(int) * P.pos.size(), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(s->d_csr_ptr, P.csr_ptr.data(), sizeof(int) * P.csr_ptr.size(), hipMemcpyHostToDevice));
-    if (!P.csr_slot.empty())
-        HIP_CHECK(hipMemcpy(s->d_csr_slot, P.csr_slot.data(), sizeof(int) * P.csr_slot.size(),
+    if (!P.csr_pos.empty())
+        HIP_CHECK(hipMemcpy(s->d_csr_pos, P.csr_pos.data(), sizeof(int) * P.csr_pos.size(),
                             hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemset(s->d_contrib, 0, (size_t)s->esize * P.lit.size()));
+    HIP_CHECK(hipMemset(s->d_contrib, 0, (size_t)s->esize * ncontrib));
     s->hp = std::move(P);
     s->ready = true;
 }
@@ -623,13 +708,16 @@ template <typename T>
 static StageArgs<T> stage_args(Sys* s, T* xa, T* xb, T* const* kk, T* tmp, double* partial, const Ctrl* c)
 {
     StageArgs<T> a;
-    a.lit = s->d_lit; a.csr_ptr = s->d_csr_ptr; a.csr_slot = s->d_csr_slot;
+    a.lit = s->d_lit; a.pos = s->d_pos; a.csr_ptr = s->d_csr_ptr; a.csr_pos = s->d_csr_pos;
     a.contrib = (T*)s->d_contrib;
     a.xa = xa; a.xb = xb;
     a.k1 = kk[0]; a.k2 = kk[1]; a.k3 = kk[2]; a.k4 = kk[3]; a.k5 = kk[4];
     a.tmp = tmp; a.partial = partial; a.ctrl = c;
     a.n = s->n; a.m = s->m; a.k = s->k; a.aoff = s->aoff;
-    a.nblk_clause = ceil_div(s->m, BLK);
+    a.clause_per_xcd = ceil_div(ceil_div(s->m, BLK), 8);
+    a.nblk_clause = 8 * a.clause_per_xcd;
+    a.vg_blocks = s->vg_blocks;
+    a.ngroups = s->ngroups;
     a.knorm = (T)std::pow(2.0, -s->k);
     return a;
 }
@@ -637,7 +725,7 @@ static StageArgs<T> stage_args(Sys* s, T* xa, T* xb, T* const* kk, T* tmp, doubl
 template <typename T, int J>
 static void launch_clause(const StageArgs<T>& a, hipStream_t st)
 {
-    dim3 g(ceil_div(a.m, BLK)), b(BLK);
+    dim3 g(a.nblk_clause), b(BLK);
     switch (a.k) {
     case 3: hipLaunchKernelGGL((k_clause<T, 3, J>), g, b, 0, st, a); break;
     case 4: hipLaunchKernelGGL((k_clause<T, 4, J>), g, b, 0, st, a); break;
@@ -645,10 +733,12 @@ static void launch_clause(const StageArgs<T>& a, hipStream_t st)
     }
 }
 
+static int var_grid(int ngroups, int vg_blocks) { return 8 * vg_blocks * ceil_div(ngroups, 8); }
+
 template <typename T, int J>
 static void launch_var(const StageArgs<T>& a, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_var<T, J>), dim3(std::max(1, ceil_div(a.n, BLK))), dim3(BLK), 0, st, a);
+    hipLaunchKernelGGL((k_var<T, J>), dim3(var_grid(a.ngroups, a.vg_blocks)), dim3(BLK), 0, st, a);
 }
 
 template <typename T>
@@ -756,7 +846,7 @@ static void rk_configure(RK* r, Sys* s)
     }
     HIP_CHECK(hipMalloc(&r->tmp, (size_t)s->esize * (s->n + 1)));
     HIP_CHECK(hipMemset(r->tmp, 0, (size_t)s->esize * (s->n + 1)));
-    r->npartial = ceil_div(s->m, BLK) + std::max(1, ceil_div(s->n, BLK));
+    r->npartial = 8 * ceil_div(ceil_div(s->m, BLK), 8) + var_grid(s->ngroups, s->vg_blocks);
     HIP_CHECK(hipMalloc((void**)&r->partial, sizeof(double) * r->npartial));
     HIP_CHECK(hipMemset(r->partial, 0, sizeof(double) * r->npartial));
     HIP_CHECK(hipMalloc((void**)&r->d_ctrl, sizeof(Ctrl)));

@@ -153,6 +153,65 @@ int asat_rng_destroy(asat_rng_t* rng);
    device memory of the given precision (ASAT_F64 double / ASAT_F32 float) */
 int asat_rng_uniform(asat_rng_t* rng, void* dev_ptr, int len, int precision);
 
+/* ---------------- batched ragged solver (no reference counterpart) ---------
+ * Many independent trajectories (each its own instance with N <= 8191,
+ * clauses of <= 4 literals, K*M <= 262143; own step size, own early exit on
+ * SAT) advanced together by persistent launches, one workgroup per
+ * trajectory.  Each trajectory follows the same FP32-state / FP64-control
+ * Cash-Karp algorithm as asat_rk_step on an ASAT_F32 system, with SAT checked
+ * after every `check_every` accepted steps (SatSolver batch size). */
+
+#define ASAT_BATCH_RUNNING 0
+#define ASAT_BATCH_SOLVED 1    /* reference: SAT_SOLUTION_FOUND */
+#define ASAT_BATCH_MAXSTEPS 2  /* reference: SAT_MAXITER_REACHED */
+#define ASAT_BATCH_UNDERFLOW 3 /* reference: SAT_UNDERFLOW */
+#define ASAT_BATCH_MAXTIME 4   /* reference: SAT_MAXTIME_REACHED */
+
+typedef struct {
+    int status;              /* ASAT_BATCH_* */
+    long long steps, attempts, rhscount;
+    double time, hdid, hnext;
+    int min_violations;      /* minimum violated-clause count over the checked steps (MAXSAT) */
+    int violations;          /* count at the last check */
+    int sat_step;            /* accepted step at which SAT was found, -1 if not */
+    double sat_time;         /* analog time at SAT, -1 if not */
+    double solved_wall_ms;   /* host wall time since asat_batch_run started when SAT was seen */
+} asat_batch_result_t;
+
+typedef struct {
+    int trajectories, max_n, max_m, max_slots;
+    int block;               /* workgroup size chosen (256 / 512 / 1024) */
+    int lds;                 /* 1: contributions in LDS, 0: in global scratch */
+    int grid;                /* resident workgroups */
+    long long lds_bytes;
+    double tuned_us_per_attempt;   /* autotuner's measurement, -1 if not tuned */
+} asat_batch_bucket_t;
+
+int asat_batch_create(asat_batch_t** out);
+int asat_batch_destroy(asat_batch_t* b);
+/* add one trajectory: clauses as SatProblem::AddClauses, FP32 start [s, a] */
+int asat_batch_add(asat_batch_t* b, const int* literals, const int* lengths, int nclauses, const float* pub0,
+                   int* index_out);
+/* eps as CudaRungeKutta::SetEpsilon; max_steps / max_time as SatSolver;
+   chunk_steps: accepted steps per trajectory per launch; check_every: SAT check cadence */
+int asat_batch_set_params(asat_batch_t* b, double eps, long long max_steps, double max_time, int chunk_steps,
+                          int check_every);
+/* block: 0 autotune or 256/512/1024; lds_mode: -1 auto, 0 global, 1 LDS;
+   nbuckets: size buckets (trajectories split by work, one launch each) */
+int asat_batch_set_tiling(asat_batch_t* b, int block, int lds_mode, int nbuckets);
+/* upload and (autotune != 0) pick the tile mapping per size bucket */
+int asat_batch_prepare(asat_batch_t* b, int autotune);
+/* launch graphs of graph_chunks chunks until all trajectories finished or
+   max_chunks chunks were launched */
+int asat_batch_run(asat_batch_t* b, int max_chunks, int graph_chunks);
+/* restore every trajectory to its start */
+int asat_batch_reset(asat_batch_t* b);
+int asat_batch_size(const asat_batch_t* b, int* count);
+int asat_batch_result(const asat_batch_t* b, int index, asat_batch_result_t* out);
+int asat_batch_get_state(const asat_batch_t* b, int index, float* pub);
+int asat_batch_stats(const asat_batch_t* b, double* run_ms, int* chunks, int* nbuckets);
+int asat_batch_bucket_info(const asat_batch_t* b, int bucket, asat_batch_bucket_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,6 +78,7 @@ def lib():
                 g = getattr(L, "orc_rk_" + nm + sfx)
                 g.argtypes = [_vp]
                 g.restype = ctypes.c_long
+            getattr(L, "orc_rk_err_vector" + sfx).argtypes = [_vp, _f64p]
             f = getattr(L, "orc_solve" + sfx)
             f.argtypes = [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_int,
                           ctypes.POINTER(ctypes.c_int)]
@@ -267,6 +268,12 @@ class RK:
     steps = property(lambda s: s._call("orc_rk_steps"))
     attempts = property(lambda s: s._call("orc_rk_attempts"))
     rhscount = property(lambda s: s._call("orc_rk_rhscount"))
+
+    def err_vector(self):
+        """|xerr/xscal| per element of the last attempt, internal layout [0, s, a]."""
+        out = np.zeros(self.problem.n + self.problem.m + 1)
+        self._call("orc_rk_err_vector", out)
+        return out
 
     def solve(self, max_time=100.0, max_steps=100000, batch=20, max_batches=100000):
         viol = np.zeros(max_batches, dtype=np.int32)

@@ -309,6 +309,12 @@ int FN(orc_solve)(FN(orc_rk_t) *r, CT max_time, int max_steps, int batch,
     return result;
 }
 
+/* per-element |xerr/xscal| of the last attempt (length n+m+1, internal layout) */
+void FN(orc_rk_err_vector)(const FN(orc_rk_t) *r, double *out)
+{
+    for (int i = 0; i < r->len; i++) out[i] = (double)r->xerr[i];
+}
+
 /* accessors for the Python side */
 double FN(orc_rk_time)(const FN(orc_rk_t) *r) { return (double)r->time; }
 double FN(orc_rk_hdid)(const FN(orc_rk_t) *r) { return (double)r->hdid; }

@@ -1,0 +1,141 @@
+"""GPU parity of the batched ragged solver.
+
+Every trajectory of a batch must follow exactly the algorithm of the plain
+FP32 integrator: states, h, attempts, SAT step and the minimum violated-clause
+count are compared bit for bit / exactly with the oracle's mixed mode
+(correctly rounded controller pow), stepping one accepted step at a time and
+stopping where the batch stops (SAT or step budget).  Results must not depend
+on the tile mapping (workgroup size, contributions in LDS or global scratch,
+bucket count).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import analogsat_amd as A
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print("[gpu-batch]", msg, flush=True)
+    sys.stdout.flush()
+
+
+def make_instances(count, seed, nlo=20, nhi=300, scale=(0.6, 1.2)):
+    """Random 3-/4-SAT instances with alpha = threshold_k * U(scale)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.integers(nlo, nhi + 1))
+        k = int(rng.choice([3, 4]))
+        a = (4.26 if k == 3 else 9.93) * rng.uniform(*scale)
+        m = max(1, int(round(a * n)))
+        p = O.fastsat(n, m, k, 1000 * seed + i)
+        pub = np.concatenate([rng.uniform(-1, 1, n), np.ones(m)]).astype(np.float32)
+        out.append((p, pub))
+    return out
+
+
+def oracle_run(p, pub, max_steps, eps=1e-6):
+    """Step the mixed-mode oracle like the batch: SAT check after every step."""
+    with O.pow_mode("cr"):
+        rk = O.RK(p, eps=eps, mode="mix")
+        rk.set_state(pub)
+        minv = None
+        status, sat_step = 2, -1
+        for _ in range(max_steps):
+            if not rk.step(1):
+                status = 3
+                break
+            v = rk.violations()
+            minv = v if minv is None else min(minv, v)
+            if v == 0:
+                status, sat_step = 1, rk.steps
+                break
+        return rk, status, sat_step, minv
+
+
+def check_against_oracle(batch, insts, max_steps):
+    for i, (p, pub) in enumerate(insts):
+        r = batch.result(i)
+        rk, status, sat_step, minv = oracle_run(p, pub, max_steps)
+        assert r["status"] == status, (i, r, status)
+        assert r["steps"] == rk.steps and r["attempts"] == rk.attempts, (i, r["steps"], rk.steps)
+        assert r["rhscount"] == rk.rhscount
+        assert r["time"] == rk.time and r["hdid"] == rk.hdid, i
+        assert r["sat_step"] == sat_step and r["min_violations"] == minv, (i, r, sat_step, minv)
+        assert np.array_equal(batch.state(i), rk.get_state()), i
+
+
+@pytest.mark.parametrize("block,lds", [(256, -1), (1024, 0), (512, 1)])
+def test_batch_matches_oracle_per_trajectory(block, lds):
+    say("batch 24 trajectories block=%d lds=%d" % (block, lds))
+    insts = make_instances(24, 1)
+    b = A.HipBatch()
+    for p, pub in insts:
+        b.add(A.SatProblem(p.literals, p.lengths), pub)
+    b.set_params(eps=1e-6, max_steps=150, chunk_steps=7, check_every=1)
+    b.set_tiling(block=block, lds_mode=lds, nbuckets=3)
+    b.prepare(autotune=False)
+    b.run(graph_chunks=2)
+    check_against_oracle(b, insts, 150)
+
+
+def test_batch_autotune_and_buckets_do_not_change_results():
+    say("batch autotune vs fixed tiling")
+    insts = make_instances(40, 2, nlo=30, nhi=600)
+    res = []
+    for tiling in ((0, -1, 4), (1024, 0, 1)):
+        b = A.HipBatch()
+        for p, pub in insts:
+            b.add(A.SatProblem(p.literals, p.lengths), pub)
+        b.set_params(eps=1e-6, max_steps=120, chunk_steps=16)
+        b.set_tiling(*tiling)
+        b.prepare(autotune=tiling[0] == 0)
+        b.run(graph_chunks=3)
+        res.append(([b.result(i) for i in range(len(insts))], [b.state(i) for i in range(len(insts))]))
+        if tiling[0] == 0:
+            bk = b.buckets()
+            say("buckets: %s" % bk)
+            assert all(x["block"] in (256, 512, 1024) for x in bk)
+    (r0, s0), (r1, s1) = res
+    for i in range(len(insts)):
+        for key in ("status", "steps", "attempts", "time", "min_violations", "sat_step"):
+            assert r0[i][key] == r1[i][key], (i, key)
+        assert np.array_equal(s0[i], s1[i])
+
+
+def test_batch_maxsat_budget_and_min_violations():
+    say("over-constrained batch (MAXSAT)")
+    insts = make_instances(12, 3, nlo=40, nhi=120, scale=(2.0, 2.5))
+    b = A.HipBatch()
+    for p, pub in insts:
+        b.add(A.SatProblem(p.literals, p.lengths), pub)
+    b.set_params(eps=1e-6, max_steps=80, chunk_steps=25)
+    b.set_tiling(block=256, lds_mode=-1, nbuckets=2)
+    b.prepare(autotune=False)
+    b.run(graph_chunks=1)
+    res = b.results()
+    assert sum(r["status"] == 2 and r["steps"] == 80 for r in res) >= 8
+    assert all(r["min_violations"] >= 0 for r in res)
+    check_against_oracle(b, insts, 80)
+
+
+def test_batch_reset_reproduces():
+    say("batch reset")
+    insts = make_instances(8, 4)
+    b = A.HipBatch()
+    for p, pub in insts:
+        b.add(A.SatProblem(p.literals, p.lengths), pub)
+    b.set_params(eps=1e-6, max_steps=40, chunk_steps=40)
+    b.set_tiling(block=256, lds_mode=-1, nbuckets=1)
+    b.prepare(autotune=False)
+    b.run()
+    first = [b.state(i).copy() for i in range(len(insts))]
+    b.reset()
+    b.run()
+    for i in range(len(insts)):
+        assert np.array_equal(first[i], b.state(i))

@@ -215,3 +215,31 @@ def test_large_instance_rhs_property():
     x = np.concatenate([[-1.0], u]).astype(np.float32)
     want = O.rhs(p, x, mode="mix")[1:]
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_trajectory_multi_group_bitexact(precision):
+    """N = 30000 spans several variable groups (XCD-affine contribution
+    regions): 60 steps bit-identical to the oracle."""
+    say("trajectory %s N=30000 60 steps" % precision)
+    n, m = 30000, 127800
+    p = O.fastsat(n, m, 3, 77)
+    mode = "f64" if precision == "f64" else "mix"
+    dt = np.float64 if precision == "f64" else np.float32
+    pub = np.concatenate([np.random.default_rng(3).uniform(-1, 1, n), np.ones(m)]).astype(dt)
+    sat = _sys_with(p.literals, p.lengths, precision)
+    rk = A.HipRungeKutta()
+    rk.SetEpsilon(1e-6)
+    rk.Configure(sat)
+    old, new = sat.MakeState(), sat.MakeState()
+    old.CopyFrom(pub)
+    with O.pow_mode("cr"):
+        orc = O.RK(p, eps=1e-6, mode=mode)
+        orc.set_state(pub)
+        for blk in range(6):
+            rk.Step(old, new, 10)
+            orc.step(10)
+            assert np.array_equal(old.CopyTo(), orc.get_state()), blk
+            st = rk.stats()
+            assert (st.hdid, st.attempts, st.rhscount) == (orc.hdid, orc.attempts, orc.rhscount)
+    assert sat.GetClauseViolationCount(old) == orc.violations()
