@@ -121,12 +121,15 @@ def lib():
             "asat_rk_destroy": ([vp], i),
             "asat_rk_set_eps": ([vp, ctypes.c_double], i),
             "asat_rk_get_eps": ([vp, ctypes.POINTER(ctypes.c_double)], i),
+            "asat_rk_set_xscal_floor": ([vp, ctypes.c_double], i),
+            "asat_batch_set_xscal_floor": ([vp, ctypes.c_double], i),
             "asat_rk_configure": ([vp, vp], i),
             "asat_rk_step": ([vp, vp, vp, i], i),
             "asat_rk_reset": ([vp], i),
             "asat_rk_stats": ([vp, ctypes.POINTER(RkStats)], i),
             "asat_rk_last_step_ms": ([vp, ctypes.POINTER(ctypes.c_float)], i),
             "asat_rk_profile": ([vp, vp, vp, i, vp], i),
+            "asat_util_copy_bench": ([ctypes.c_longlong, i, ctypes.POINTER(ctypes.c_float)], i),
             "asat_rng_create": ([ctypes.c_ulonglong, pvp], i),
             "asat_rng_destroy": ([vp], i),
             "asat_rng_uniform": ([vp, vp, i, i], i),
@@ -150,6 +153,13 @@ def lib():
             f.restype = res
         _lib = L
     return _lib
+
+
+def copy_bench(nbytes, reps=10):
+    """Microseconds per plain device copy of nbytes (HBM counter calibration)."""
+    us = ctypes.c_float()
+    _check(lib().asat_util_copy_bench(int(nbytes), int(reps), ctypes.byref(us)))
+    return us.value
 
 
 def _check(rc):
@@ -447,6 +457,10 @@ class HipRungeKutta:
     def SetEpsilon(self, eps):
         _check(lib().asat_rk_set_eps(self._h, float(eps)))
 
+    def SetErrorFloor(self, floor):
+        """xscal = |x| + |h dxdt| + floor (reference: floor = TINY = 1e-20, the default)."""
+        _check(lib().asat_rk_set_xscal_floor(self._h, float(floor)))
+
     def GetEpsilon(self):
         e = ctypes.c_double()
         _check(lib().asat_rk_get_eps(self._h, ctypes.byref(e)))
@@ -640,6 +654,9 @@ class HipBatch:
     def set_params(self, eps=1e-6, max_steps=10000, max_time=1e300, chunk_steps=256, check_every=1):
         _check(lib().asat_batch_set_params(self._h, float(eps), int(max_steps), float(max_time), int(chunk_steps),
                                            int(check_every)))
+
+    def set_error_floor(self, floor):
+        _check(lib().asat_batch_set_xscal_floor(self._h, float(floor)))
 
     def set_tiling(self, block=0, lds_mode=-1, nbuckets=4):
         _check(lib().asat_batch_set_tiling(self._h, int(block), int(lds_mode), int(nbuckets)))

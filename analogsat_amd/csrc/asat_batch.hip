@@ -81,6 +81,7 @@ struct Params {
     int ntraj;
     int* queue;              // atomic work counter (reset by the host before a launch)
     double eps;
+    double floor;            // constant term of xscal (1e-20 = reference TINY)
     double max_time;
     long long max_steps;
     int chunk_steps;
@@ -193,7 +194,7 @@ __global__ __launch_bounds__(BS) void k_batch(Params P)
                         default: {
                             float xv;
                             double e;
-                            final_combination<float>(xa, k1, k3, k4, k5, kj, hs, h, hnext, xv, e);
+                            final_combination<float>(xa, k1, k3, k4, k5, kj, hs, h, hnext, P.floor, xv, e);
                             xn[ai] = xv;
                             err_local = fmax(err_local, e);
                         }
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(BS) void k_batch(Params P)
                             float xv;
                             double e;
                             final_combination<float>(xi, kbuf1[i], kbuf3[i], kbuf4[i], kbuf5[i], kj, hs, h, hnext,
-                                                     xv, e);
+                                                     P.floor, xv, e);
                             xn[i] = xv;
                             err_local = fmax(err_local, e);
                         }
@@ -373,6 +374,7 @@ struct Bucket {
 struct Batch {
     std::vector<Instance> inst;
     double eps = 1e-6;
+    double floor = 1e-20;
     double max_time = 1e300;
     long long max_steps = 10000;
     int chunk_steps = 256;
@@ -509,7 +511,7 @@ static Params params_of(const Batch& B, const Bucket& b)
     P.desc = B.d_desc; P.code = B.d_code; P.rowptr = B.d_rowptr; P.xa = B.d_xa; P.xb = B.d_xb;
     P.ctrl = B.d_ctrl; P.scratch = b.d_scratch; P.scratch_stride = b.scratch_stride;
     P.scratch_per_wg = b.scratch_per_wg; P.order = b.d_order; P.ntraj = (int)b.ids.size();
-    P.queue = b.d_queue; P.eps = B.eps; P.max_time = B.max_time; P.max_steps = B.max_steps;
+    P.queue = b.d_queue; P.eps = B.eps; P.floor = B.floor; P.max_time = B.max_time; P.max_steps = B.max_steps;
     P.chunk_steps = B.chunk_steps; P.check_every = std::max(1, B.check_every);
     P.sp_stride = b.sp_stride;
     return P;
@@ -807,6 +809,15 @@ int asat_batch_set_params(asat_batch_t* b, double eps, long long max_steps, doub
     require(chunk_steps >= 1 && check_every >= 1, ASAT_ERR_INVALID_ARGUMENT, "chunk/check sizes must be positive");
     b->b.eps = eps; b->b.max_steps = max_steps; b->b.max_time = max_time;
     b->b.chunk_steps = chunk_steps; b->b.check_every = check_every;
+    BAPI_END
+}
+
+int asat_batch_set_xscal_floor(asat_batch_t* b, double floor)
+{
+    BAPI_BEGIN
+    require(b != nullptr, ASAT_ERR_INVALID_ARGUMENT, "null batch");
+    require(floor > 0.0 && floor <= 1.0, ASAT_ERR_INVALID_ARGUMENT, "xscal floor must be in (0, 1]");
+    b->b.floor = floor;
     BAPI_END
 }
 

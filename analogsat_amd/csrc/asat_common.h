@@ -49,6 +49,7 @@ struct alignas(64) Ctrl {
     double hdid;       // last accepted step
     double errmax;     // last error norm / eps
     double eps;
+    double floor;      // constant term of xscal (1e-20 = reference TINY)
     long long steps;      // accepted steps
     long long attempts;   // attempts incl. rejected
     long long rhscount;   // RHS evaluations
@@ -132,15 +133,17 @@ __device__ __forceinline__ T stage_input(T x, T k1, T k2, T k3, T k4, T k5, T hs
 }
 
 // final combination (xnew) and error term |xerr / xscal| of one element
+// floor: constant term of xscal, TINY = 1e-20 in the reference
+// (AnalogSAT/cpu_sat/CpuRungeKutta.cpp:84-85)
 template <typename T>
 __device__ __forceinline__ void final_combination(T x, T k1, T k3, T k4, T k5, T k6, T hs, double h, double hnext,
-                                                  T& xnew, double& err)
+                                                  double floor, T& xnew, double& err)
 {
     using B = Tab<T>;
     xnew = x + hs * (B::c1 * k1 + B::c3 * k3 + B::c4 * k4 + B::c6 * k6);
     double xerr = h * (TabD::dc1 * (double)k1 + TabD::dc3 * (double)k3 + TabD::dc4 * (double)k4 +
                        TabD::dc5 * (double)k5 + TabD::dc6 * (double)k6);
-    double xscal = fabs((double)x) + fabs((double)k1 * hnext) + TabD::TINY;
+    double xscal = fabs((double)x) + fabs((double)k1 * hnext) + floor;
     err = fabs(xerr / xscal);
 }
 

@@ -238,26 +238,43 @@ __global__ __launch_bounds__(BLK) void k_clause(StageArgs<T> a)
     __shared__ double sh[BLK / 64];
     const Ctrl* c = a.ctrl;
     int parity = 0;
-    double h = 0.0, hnext = 0.0;
+    double h = 0.0, hnext = 0.0, floor = TabD::TINY;
     if (c) {
         if (!c->active) return;
         if (J == 1 && !c->step_start) return;
-        parity = c->parity; h = c->h; hnext = c->hnext;
+        parity = c->parity; h = c->h; hnext = c->hnext; floor = c->floor;
     }
     const T hs = (T)h;
     const T* x = parity ? a.xb : a.xa;
     T* xnew = parity ? a.xa : a.xb;
     const T* sp = (J == 1) ? x : a.tmp;
     const int m = clause_of_thread(a.clause_per_xcd);
+    // this block's tile of BLK clauses: literals and positions, read once with
+    // coalesced 16-byte streaming loads (arrays are padded to whole tiles)
+    __shared__ int s_lit[BLK * K], s_pos[BLK * K];
+    {
+        const int tile0 = m - (int)threadIdx.x;
+        if (tile0 < a.m) {
+            typedef int v4i __attribute__((ext_vector_type(4)));
+            const v4i* L4 = (const v4i*)(a.lit + (size_t)tile0 * K);
+            const v4i* P4 = (const v4i*)(a.pos + (size_t)tile0 * K);
+            for (int v = threadIdx.x; v < BLK * K / 4; v += BLK) {
+                v4i w = ld_nt(L4 + v), z = ld_nt(P4 + v);
+                s_lit[4 * v] = w.x; s_lit[4 * v + 1] = w.y; s_lit[4 * v + 2] = w.z; s_lit[4 * v + 3] = w.w;
+                s_pos[4 * v] = z.x; s_pos[4 * v + 1] = z.y; s_pos[4 * v + 2] = z.z; s_pos[4 * v + 3] = z.w;
+            }
+        }
+        __syncthreads();
+    }
     double err = 0.0;
     if (m < a.m) {
         int l[K];
         T f[K];
         int ps[K];
 #pragma unroll
-        for (int q = 0; q < K; q++) l[q] = ld_nt(a.lit + (size_t)m * K + q);
+        for (int q = 0; q < K; q++) l[q] = s_lit[threadIdx.x * K + q];
 #pragma unroll
-        for (int q = 0; q < K; q++) ps[q] = ld_nt(a.pos + (size_t)m * K + q);
+        for (int q = 0; q < K; q++) ps[q] = s_pos[threadIdx.x * K + q];
         T km = a.knorm;
 #pragma unroll
         for (int q = 0; q < K; q++) {
@@ -284,7 +301,7 @@ __global__ __launch_bounds__(BLK) void k_clause(StageArgs<T> a)
         else if constexpr (J == 5) st_nt(a.k5 + ai, kj);
         else {
             T xn;
-            final_combination<T>(xa, k1, k3, k4, k5, kj, hs, h, hnext, xn, err);
+            final_combination<T>(xa, k1, k3, k4, k5, kj, hs, h, hnext, floor, xn, err);
             st_nt(xnew + ai, xn);
         }
         const T r = km * a.knorm * am;
@@ -310,11 +327,11 @@ __global__ __launch_bounds__(BLK) void k_clause_gen(StageArgs<T> a)
     __shared__ T fsh[GEN_KMAX * BLK];
     const Ctrl* c = a.ctrl;
     int parity = 0;
-    double h = 0.0, hnext = 0.0;
+    double h = 0.0, hnext = 0.0, floor = TabD::TINY;
     if (c) {
         if (!c->active) return;
         if (J == 1 && !c->step_start) return;
-        parity = c->parity; h = c->h; hnext = c->hnext;
+        parity = c->parity; h = c->h; hnext = c->hnext; floor = c->floor;
     }
     const T hs = (T)h;
     const T* x = parity ? a.xb : a.xa;
@@ -352,7 +369,7 @@ __global__ __launch_bounds__(BLK) void k_clause_gen(StageArgs<T> a)
         else if constexpr (J == 5) a.k5[ai] = kj;
         else {
             T xn;
-            final_combination<T>(x[ai], k1, k3, k4, k5, kj, hs, h, hnext, xn, err);
+            final_combination<T>(x[ai], k1, k3, k4, k5, kj, hs, h, hnext, floor, xn, err);
             xnew[ai] = xn;
         }
         T* out = a.contrib;
@@ -390,17 +407,25 @@ __global__ __launch_bounds__(BLK) void k_clause_gen(StageArgs<T> a)
 }
 
 // ---- var pass: deterministic gather-sum + spin stage combination ---------
+// A block owns BLK consecutive variables of one group.  Their CSR entries form
+// one contiguous range; the block first gathers that whole range into LDS
+// with coalesced csr_pos reads and many independent contribution loads in
+// flight, then every thread adds its own variable's run from LDS left to
+// right (slot order, as the reference accumulates).
+constexpr int VAR_STAGE = 6144;   // staged contributions per round (LDS)
+
 template <typename T, int J>
 __global__ __launch_bounds__(BLK) void k_var(StageArgs<T> a)
 {
     __shared__ double sh[BLK / 64];
+    __shared__ T vals[VAR_STAGE];
     const Ctrl* c = a.ctrl;
     int parity = 0;
-    double h = 0.0, hnext = 0.0;
+    double h = 0.0, hnext = 0.0, floor = TabD::TINY;
     bool fresh = true;
     if (c) {
         if (!c->active) return;
-        parity = c->parity; h = c->h; hnext = c->hnext;
+        parity = c->parity; h = c->h; hnext = c->hnext; floor = c->floor;
         fresh = (J != 1) || c->step_start;
     }
     const T hs = (T)h;
@@ -408,14 +433,30 @@ __global__ __launch_bounds__(BLK) void k_var(StageArgs<T> a)
     T* xnew = parity ? a.xa : a.xb;
     const int i = var_of_block(a.vg_blocks, a.ngroups, a.n);
     double err = 0.0;
-    if (i != 0) {
-        T kj = (T)0;
-        if (fresh) {
-            const int e0 = ld_nt(a.csr_ptr + i), e1 = ld_nt(a.csr_ptr + i + 1);
+    T kj = (T)0;
+    if (fresh) {
+        // variables of this block: [i0, i1]; thread's own range [e0, e1)
+        const int b = blockIdx.x, xcd = b & 7, t = b >> 3;
+        const int grp = (t / a.vg_blocks) * 8 + xcd;
+        const long long i0ll = (long long)grp * a.vg_blocks * BLK + (long long)(t % a.vg_blocks) * BLK + 1;
+        const int i0 = grp < a.ngroups && i0ll <= a.n ? (int)i0ll : 0;
+        if (i0 != 0) {
+            const int i1 = (int)std::min<long long>(i0ll + BLK - 1, a.n);
+            const int base = a.csr_ptr[i0], end = a.csr_ptr[i1 + 1];
+            const int e0 = i ? a.csr_ptr[i] : 0, e1 = i ? a.csr_ptr[i + 1] : 0;
             T sum = (T)0;
-            for (int e = e0; e < e1; e++) sum += a.contrib[ld_nt(a.csr_pos + e)];
+            for (int r0 = base; r0 < end; r0 += VAR_STAGE) {
+                const int r1 = std::min(end, r0 + VAR_STAGE);
+                for (int e = r0 + threadIdx.x; e < r1; e += BLK) vals[e - r0] = a.contrib[a.csr_pos[e]];
+                __syncthreads();
+                const int lo = std::max(e0, r0), hi = std::min(e1, r1);
+                for (int e = lo; e < hi; e++) sum += vals[e - r0];
+                __syncthreads();
+            }
             kj = sum;
         }
+    }
+    if (i != 0) {
         const T xi = ld_nt(x + i);
         if constexpr (J == 1) {
             if (fresh) st_nt(a.k1 + i, kj);
@@ -437,7 +478,7 @@ __global__ __launch_bounds__(BLK) void k_var(StageArgs<T> a)
         } else {
             T xn;
             final_combination<T>(xi, ld_nt(a.k1 + i), ld_nt(a.k3 + i), ld_nt(a.k4 + i), ld_nt(a.k5 + i), kj, hs, h,
-                                 hnext, xn, err);
+                                 hnext, floor, xn, err);
             st_nt(xnew + i, xn);
         }
     }
@@ -492,18 +533,22 @@ __global__ void k_ctrl_begin(Ctrl* c, long long count, int parity)
     }
 }
 
-__global__ void k_ctrl_reset(Ctrl* c, double eps)
+__global__ void k_ctrl_reset(Ctrl* c, double eps, double floor)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         c->h = 1e-9; c->hnext = 1e-9; c->time = 0.0; c->hdid = 0.0; c->errmax = 0.0; c->eps = eps;
+        c->floor = floor;
         c->steps = 0; c->attempts = 0; c->rhscount = 0; c->target = 0;
         c->parity = 0; c->step_start = 1; c->active = 0; c->underflow = 0; c->violations = 0;
     }
 }
 
-__global__ void k_ctrl_set_eps(Ctrl* c, double eps)
+__global__ void k_ctrl_set_eps(Ctrl* c, double eps, double floor)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) c->eps = eps;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        c->eps = eps;
+        c->floor = floor;
+    }
 }
 
 // ---- violated-clause count (CpuSat::CalculateClauses semantics) ----------
@@ -568,6 +613,13 @@ __global__ void k_set_initial(T* x, int n, int m, int aoff, const T* u)
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) x[1 + i] = u[i] * (T)2.0 - (T)1.0;
     else if (i < (long long)n + m) x[aoff + (i - n)] = u[i] + (T)1.0;
+}
+
+// plain streaming copy (16 B per lane): the reference point for HBM byte counters
+__global__ void k_copy16(const int4* __restrict__ src, int4* __restrict__ dst, long long n16)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 template <typename T>
@@ -638,6 +690,9 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
     s->vg = P.vg;
     s->ngroups = P.ngroups;
     s->vg_blocks = P.vg / BLK;
+    // clause arrays padded to whole BLK-clause tiles (padding literal 0, position -1)
+    P.lit.resize((size_t)round_up(P.m, BLK) * P.k, 0);
+    P.pos.resize((size_t)round_up(P.m, BLK) * P.k, -1);
     const size_t ncontrib = std::max<size_t>(P.csr_pos.size(), 1);
     HIP_CHECK(hipMalloc((void**)&s->d_lit, sizeof(int) * P.lit.size()));
     HIP_CHECK(hipMalloc((void**)&s->d_pos, sizeof(int) * P.pos.size()));
@@ -789,6 +844,7 @@ static int sys_violations(Sys* s, const State* x)
 struct RK {
     Sys* sys = nullptr;
     double eps = 1e-4;      // CudaRungeKutta() default (CudaRungeKutta.cu:175)
+    double floor = 1e-20;   // xscal constant term (reference TINY)
     int alloc = 0, prec = -1;
     void* k[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     void* tmp = nullptr;
@@ -823,7 +879,7 @@ struct RK {
 static void rk_reset(RK* r)
 {
     r->require_configured();
-    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, r->sys->stream, r->d_ctrl, r->eps);
+    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, r->sys->stream, r->d_ctrl, r->eps, r->floor);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(r->h_ctrl, r->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, r->sys->stream));
     HIP_CHECK(hipStreamSynchronize(r->sys->stream));
@@ -1227,7 +1283,21 @@ int asat_rk_set_eps(asat_rk_t* rk, double eps)
     require(!(eps < 1e-16 || eps > 1.0), ASAT_ERR_INVALID_ARGUMENT, "epsilon must be between 1e-16 and 1");
     rk->r.eps = eps;
     if (rk->r.configured) {
-        hipLaunchKernelGGL(k_ctrl_set_eps, dim3(1), dim3(64), 0, rk->r.sys->stream, rk->r.d_ctrl, eps);
+        hipLaunchKernelGGL(k_ctrl_set_eps, dim3(1), dim3(64), 0, rk->r.sys->stream, rk->r.d_ctrl, eps, rk->r.floor);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(rk->r.sys->stream));
+    }
+    API_END
+}
+
+int asat_rk_set_xscal_floor(asat_rk_t* rk, double floor)
+{
+    API_BEGIN
+    CHECK_PTR(rk);
+    require(floor > 0.0 && floor <= 1.0, ASAT_ERR_INVALID_ARGUMENT, "xscal floor must be in (0, 1]");
+    rk->r.floor = floor;
+    if (rk->r.configured) {
+        hipLaunchKernelGGL(k_ctrl_set_eps, dim3(1), dim3(64), 0, rk->r.sys->stream, rk->r.d_ctrl, rk->r.eps, floor);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(rk->r.sys->stream));
     }
@@ -1305,6 +1375,34 @@ int asat_rk_profile(asat_rk_t* rk, asat_state_t* old_state, asat_state_t* new_st
     require(old_state->st.d != new_state->st.d, ASAT_ERR_INVALID_ARGUMENT, "old and new state must differ");
     if (r->prec == ASAT_F64) rk_profile_impl<double>(r, &old_state->st, &new_state->st, reps, us_out);
     else rk_profile_impl<float>(r, &old_state->st, &new_state->st, reps, us_out);
+    API_END
+}
+
+int asat_util_copy_bench(long long bytes, int reps, float* us_per_copy)
+{
+    API_BEGIN
+    CHECK_PTR(us_per_copy);
+    require(bytes >= 16 && bytes % 16 == 0 && reps > 0, ASAT_ERR_INVALID_ARGUMENT,
+            "bytes must be a positive multiple of 16 and reps positive");
+    void *a = nullptr, *b = nullptr;
+    HIP_CHECK(hipMalloc(&a, (size_t)bytes));
+    HIP_CHECK(hipMalloc(&b, (size_t)bytes));
+    HIP_CHECK(hipMemset(a, 1, (size_t)bytes));
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    const long long n16 = bytes / 16;
+    hipLaunchKernelGGL(k_copy16, dim3(4096), dim3(256), 0, 0, (const int4*)a, (int4*)b, n16);
+    HIP_CHECK(hipEventRecord(e0, 0));
+    for (int r = 0; r < reps; r++)
+        hipLaunchKernelGGL(k_copy16, dim3(4096), dim3(256), 0, 0, (const int4*)a, (int4*)b, n16);
+    HIP_CHECK(hipEventRecord(e1, 0));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    *us_per_copy = ms * 1000.f / reps;
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    (void)hipFree(a); (void)hipFree(b);
     API_END
 }
 

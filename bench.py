@@ -79,6 +79,104 @@ def cpu_baseline(lits, lens, pub0, eps, steps):
                       "single thread, %.1f s" % (steps, t_total)}
 
 
+def make_c3(count, n=2000, alpha=4.25):
+    """C3: count independent 3-SAT instances, instance seed i, s(0) seed i, a(0) = 1."""
+    import analogsat_amd as A
+    m = int(alpha * n)
+    out = []
+    for i in range(count):
+        p = A.fastsat(n, m, 3, i)
+        pub = np.concatenate([np.random.default_rng(i).uniform(-1.0, 1.0, n), np.ones(m)]).astype(np.float32)
+        out.append((p, pub))
+    return out
+
+
+def make_c4(count, seed=0):
+    """C4: ragged batch, N ~ U{50..5000}, k ~ U{3,4}, alpha ~ U[0.9,1.3] x threshold_k, M = round(alpha N)."""
+    import analogsat_amd as A
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.integers(50, 5001))
+        k = int(rng.integers(3, 5))
+        alpha = rng.uniform(0.9, 1.3) * (4.26 if k == 3 else 9.93)
+        m = int(round(alpha * n))
+        p = A.fastsat(n, m, k, 100000 + i)
+        pub = np.concatenate([rng.uniform(-1.0, 1.0, n), np.ones(m)]).astype(np.float32)
+        out.append((p, pub))
+    return out
+
+
+def run_batch(insts, max_steps, eps, chunk_steps=64, graph_chunks=4, log=None):
+    """Batched solver over insts; returns (batch, wall_s, accepted steps, attempts)."""
+    import analogsat_amd as A
+    b = A.HipBatch()
+    for p, pub in insts:
+        b.add(p, pub)
+    b.set_params(eps=eps, max_steps=max_steps, chunk_steps=chunk_steps, check_every=1)
+    b.set_tiling(block=0, lds_mode=-1, nbuckets=4)
+    t0 = time.perf_counter()
+    b.prepare(autotune=True)
+    if log:
+        log("batch prepared+tuned in %.2f s: %s" % (time.perf_counter() - t0,
+            [(x["trajectories"], x["max_slots"], x["block"], x["lds"]) for x in b.buckets()]))
+    t0 = time.perf_counter()
+    b.run(graph_chunks=graph_chunks)
+    wall = time.perf_counter() - t0
+    res = b.results()
+    steps = sum(r["steps"] for r in res)
+    attempts = sum(r["attempts"] for r in res)
+    return b, wall, steps, attempts, res
+
+
+def run_plain_per_instance(insts, max_steps, eps, batch=1):
+    """The plain solver launched once per instance (SatSolver loop, SAT check every `batch` steps)."""
+    import analogsat_amd as A
+    total_steps, total_wall, tts = 0, 0.0, []
+    for p, pub in insts:
+        sat = A.HipSat("f32")
+        rk = A.HipRungeKutta()
+        rk.SetEpsilon(eps)
+        solver = A.SatSolver(sat, rk)
+        solver.SetProblem(p)
+        solver.SetMaxTime(1e300)
+        solver.SetMaxSteps(max_steps)
+        solver.SetBatchSize(batch)
+        solver.SetSatState(pub)
+        t0 = time.perf_counter()
+        res = solver.Solve()
+        w = time.perf_counter() - t0
+        total_wall += w
+        total_steps += solver.GetStepCount()
+        if res == A.SatResult.SAT_SOLUTION_FOUND:
+            tts.append(w)
+    return total_steps / total_wall, tts
+
+
+def batched_section(workload, count, max_steps, eps, plain_sample, log):
+    insts = make_c3(count) if workload == "c3" else make_c4(count)
+    b, wall, steps, attempts, res = run_batch(insts, max_steps, eps, log=log)
+    solved = [r for r in res if r["status"] == 1]
+    wall_tts = sorted(r["solved_wall_ms"] for r in solved)
+    p50 = wall_tts[len(wall_tts) // 2] / 1e3 if len(wall_tts) * 2 >= len(res) else None
+    plain_rate, plain_tts = run_plain_per_instance(insts[:plain_sample], max_steps, eps)
+    out = {
+        "workload": workload, "trajectories": len(res), "max_steps": max_steps,
+        "value": round(steps / wall, 1), "unit": "trajectory-steps/s",
+        "attempts_per_s": round(attempts / wall, 1), "wall_s": round(wall, 3),
+        "solved": len(solved), "p50_time_to_solution_s": p50,
+        "p50_analog_time_to_solution": (sorted(r["sat_time"] for r in solved)[len(solved) // 2] if solved else None),
+        "plain_per_instance": {"value": round(plain_rate, 1), "unit": "trajectory-steps/s", "sample": plain_sample,
+                               "p50_time_to_solution_s": (sorted(plain_tts)[len(plain_tts) // 2]
+                                                          if plain_tts else None)},
+        "batched_over_plain": round((steps / wall) / plain_rate, 1),
+        "buckets": [{k: v for k, v in x.items()} for x in b.buckets()],
+    }
+    if workload == "c4":
+        out["min_violations"] = [r["min_violations"] for r in res[:64]]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +190,13 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=20)
+    ap.add_argument("--batch", choices=["none", "c3", "c4"], default="c3",
+                    help="also measure a batched workload next to the plain solver")
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--batch-steps", type=int, default=10000)
+    ap.add_argument("--plain-sample", type=int, default=4)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run the plain copy kernel on this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -100,6 +205,12 @@ def main():
         sys.exit(2)
 
     import analogsat_amd as A
+
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        us = A.copy_bench(nbytes, reps=5)
+        print(json.dumps({"copy_bytes": nbytes, "us_per_copy": us, "GBps": 2 * nbytes / (us * 1e-6) / 1e9}))
+        return
 
     n, k = args.n, args.k
     m = int(round(args.alpha * n))
@@ -161,6 +272,13 @@ def main():
         cpu = cpu_baseline(prob.literals, prob.lengths, pub0, args.eps, args.cpu_steps)
         log("cpu baseline %.3f steps/s (%s)" % (cpu["value"], cpu["kind"]))
 
+    batched = None
+    if args.batch != "none":
+        log("batched %s: %d trajectories, step cap %d" % (args.batch, args.batch_count, args.batch_steps))
+        batched = batched_section(args.batch, args.batch_count, args.batch_steps, args.eps, args.plain_sample, log)
+        log("batched %.0f traj-steps/s vs plain per instance %.0f (x%.1f)" % (
+            batched["value"], batched["plain_per_instance"]["value"], batched["batched_over_plain"]))
+
     value = steps / wall
     line = {
         "metric": "RK45 accepted trajectory-steps/s (single large instance)",
@@ -187,6 +305,7 @@ def main():
         "kernels": kernels,
         "device_ms": round(dev_ms, 3),
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
 

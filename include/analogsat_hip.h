@@ -126,6 +126,13 @@ int asat_rk_destroy(asat_rk_t* rk);
 /* CudaRungeKutta::SetEpsilon (range [1e-16, 1]) / GetEpsilon */
 int asat_rk_set_eps(asat_rk_t* rk, double eps);
 int asat_rk_get_eps(const asat_rk_t* rk, double* eps);
+/* Constant term of the error scale xscal = |x| + |h dxdt| + floor.  The
+   reference fixes it at TINY = 1e-20 (AnalogSAT/cpu_sat/CpuRungeKutta.cpp:84-85),
+   which is the default here and reproduces the reference exactly.  A larger
+   floor (e.g. 1e-3) adds an absolute-error floor near zero crossings; the FP32
+   state needs it at eps ~ 1e-6, where float rounding of the stage inputs
+   otherwise drives h towards zero whenever a spin passes through s = 0. */
+int asat_rk_set_xscal_floor(asat_rk_t* rk, double floor);
 /* IODEInt::Configure(ode): allocates scratch, resets h = 1e-9 */
 int asat_rk_configure(asat_rk_t* rk, asat_sys_t* sys);
 /* IODEInt::Step(old, new, count): count accepted steps; on return `old`
@@ -143,6 +150,12 @@ int asat_rk_last_step_ms(const asat_rk_t* rk, float* ms);
    us_out[13]: clause pass J=1..6, var pass J=1..6, control.  No reference
    counterpart (measurement hook); the integrator state is left unchanged. */
 int asat_rk_profile(asat_rk_t* rk, asat_state_t* old_state, asat_state_t* new_state, int reps, float* us_out);
+
+/* Measurement hook, no reference counterpart: copies `bytes` between two
+   device buffers `reps` times with a plain 16-byte-per-lane copy kernel
+   (k_copy16) and reports the average microseconds per copy.  Used to
+   calibrate the HBM byte counters of rocprofv3 against a known traffic. */
+int asat_util_copy_bench(long long bytes, int reps, float* us_per_copy);
 
 /* ---------------- random numbers (ISatRandom / CudaRandom) ----------------- */
 
@@ -196,6 +209,8 @@ int asat_batch_add(asat_batch_t* b, const int* literals, const int* lengths, int
    chunk_steps: accepted steps per trajectory per launch; check_every: SAT check cadence */
 int asat_batch_set_params(asat_batch_t* b, double eps, long long max_steps, double max_time, int chunk_steps,
                           int check_every);
+/* xscal floor as asat_rk_set_xscal_floor (default 1e-20) */
+int asat_batch_set_xscal_floor(asat_batch_t* b, double floor);
 /* block: 0 autotune or 256/512/1024; lds_mode: -1 auto, 0 global, 1 LDS;
    nbuckets: size buckets (trajectories split by work, one launch each) */
 int asat_batch_set_tiling(asat_batch_t* b, int block, int lds_mode, int nbuckets);

@@ -64,6 +64,7 @@ def lib():
             getattr(L, "orc_rk_destroy" + sfx).argtypes = [_vp]
             getattr(L, "orc_rk_reset" + sfx).argtypes = [_vp]
             getattr(L, "orc_rk_set_b" + sfx).argtypes = [_vp, ctypes.c_double]
+            getattr(L, "orc_rk_set_floor" + sfx).argtypes = [_vp, ctypes.c_double]
             getattr(L, "orc_rk_set_state" + sfx).argtypes = [_vp, fp]
             getattr(L, "orc_rk_get_state" + sfx).argtypes = [_vp, fp]
             getattr(L, "orc_rk_step" + sfx).argtypes = [_vp, ctypes.c_int]
@@ -226,7 +227,7 @@ def violations(problem, pub, mode="f64"):
 class RK:
     """Restated CpuRungeKutta bound to one trajectory (mode 'f64' or 'mix')."""
 
-    def __init__(self, problem, eps=1e-4, mode="f64", b=0.0):
+    def __init__(self, problem, eps=1e-4, mode="f64", b=0.0, floor=None):
         self.problem = problem
         self.mode = mode
         self.dt = np.float64 if mode == "f64" else np.float32
@@ -235,6 +236,8 @@ class RK:
         self._h = getattr(self._L, "orc_rk_create" + self._sfx)(problem.n, problem.m, problem.k, problem.C, eps)
         if b:
             self._call("orc_rk_set_b", b)
+        if floor is not None:
+            self._call("orc_rk_set_floor", floor)
 
     def _call(self, name, *args):
         return getattr(self._L, name + self._sfx)(self._h, *args)

@@ -39,6 +39,7 @@ typedef struct {
     ST *x, *xnew, *temp, *dxdt, *ak2, *ak3, *ak4, *ak5, *ak6;
     CT *xscal, *xerr;
     CT time, hdid, hnext, eps;
+    CT floor;            /* constant term of xscal: TINY = 1e-20 in the reference */
     long steps, attempts, rhscount;
     CT last_errmax;
 } FN(orc_rk_t);
@@ -150,6 +151,7 @@ FN(orc_rk_t) *FN(orc_rk_create)(int n, int m, int k, const int *C, CT eps)
     r->xscal = (CT *)calloc(L, sizeof(CT)); r->xerr = (CT *)calloc(L, sizeof(CT));
     r->x[0] = (ST)-1.0; r->xnew[0] = (ST)-1.0;
     r->eps = eps;
+    r->floor = (CT)1e-20;
     r->b = 0;
     /* Reset(): CpuRungeKutta.cpp:168-177 */
     r->hnext = (CT)1e-9; r->steps = 0; r->attempts = 0; r->time = 0; r->rhscount = 0;
@@ -172,6 +174,9 @@ void FN(orc_rk_reset)(FN(orc_rk_t) *r)
 }
 
 void FN(orc_rk_set_b)(FN(orc_rk_t) *r, double b) { r->b = (ST)b; }
+
+/* xscal = |x| + |h k1| + floor; floor = TINY (1e-20) reproduces the reference */
+void FN(orc_rk_set_floor)(FN(orc_rk_t) *r, double floor) { r->floor = (CT)floor; }
 
 /* public [s, a] -> internal [-1, s, a] (CpuSatStateImpl::CopyPartsFrom) */
 void FN(orc_rk_set_state)(FN(orc_rk_t) *r, const ST *pub)
@@ -260,7 +265,7 @@ static int FN(rkqs)(FN(orc_rk_t) *r)
 /* Step(count): CpuRungeKutta.cpp:65-100.  Returns 1, or 0 on underflow. */
 int FN(orc_rk_step)(FN(orc_rk_t) *r, int count)
 {
-    const CT TINY = (CT)1e-20;
+    const CT TINY = r->floor;
     const int n = r->len;
     for (int c = 0; c < count; c++) {
         FN(rhs_call)(r, r->dxdt, r->x);
