@@ -80,6 +80,15 @@ def lib():
                 g.argtypes = [_vp]
                 g.restype = ctypes.c_long
             getattr(L, "orc_rk_err_vector" + sfx).argtypes = [_vp, _f64p]
+            getattr(L, "orc_vjp" + sfx).argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, fp, _f64p, _f64p]
+            getattr(L, "orc_energy_grad" + sfx).argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, fp, _f64p]
+            getattr(L, "orc_energy_grad" + sfx).restype = ctypes.c_double
+            getattr(L, "orc_frozen_energy" + sfx).argtypes = [_vp, fp, _f64p, ctypes.c_int]
+            getattr(L, "orc_frozen_energy" + sfx).restype = ctypes.c_double
+            getattr(L, "orc_adjoint" + sfx).argtypes = [_vp, fp, ctypes.c_int, _f64p, _f64p]
+            getattr(L, "orc_adjoint" + sfx).restype = ctypes.c_double
+            getattr(L, "orc_adjoint_frozen" + sfx).argtypes = [_vp, fp, _f64p, ctypes.c_int, _f64p]
+            getattr(L, "orc_adjoint_frozen" + sfx).restype = ctypes.c_double
             f = getattr(L, "orc_solve" + sfx)
             f.argtypes = [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_int,
                           ctypes.POINTER(ctypes.c_int)]
@@ -218,6 +227,25 @@ def rhs(problem, x, mode="f64", b=0.0):
     return out
 
 
+def vjp(problem, Y, w, mode="f64"):
+    """u = J(Y)^T w on the internal layout (Y in the state type, w/u double)."""
+    dt = np.float64 if mode == "f64" else np.float32
+    Y = np.ascontiguousarray(Y, dtype=dt)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    u = np.zeros(len(Y))
+    getattr(lib(), "orc_vjp_" + mode)(problem.n, problem.m, problem.k, problem.C, Y, w, u)
+    return u
+
+
+def energy_grad(problem, x, mode="f64"):
+    """(V, grad V) at the internal-layout state x; V = sum_m a_m K_m^2."""
+    dt = np.float64 if mode == "f64" else np.float32
+    x = np.ascontiguousarray(x, dtype=dt)
+    g = np.zeros(len(x))
+    V = getattr(lib(), "orc_energy_grad_" + mode)(problem.n, problem.m, problem.k, problem.C, x, g)
+    return V, g
+
+
 def violations(problem, pub, mode="f64"):
     dt = np.float64 if mode == "f64" else np.float32
     x = np.concatenate([[-1.0], np.asarray(pub, dtype=np.float64)]).astype(dt)
@@ -277,6 +305,29 @@ class RK:
         out = np.zeros(self.problem.n + self.problem.m + 1)
         self._call("orc_rk_err_vector", out)
         return out
+
+    def adjoint(self, pub, T):
+        """Adaptive forward of T accepted steps from pub, then the discrete
+        adjoint: returns (V(T), dV/d pub (double), h sequence)."""
+        pub = np.ascontiguousarray(pub, dtype=self.dt)
+        h = np.zeros(T)
+        g = np.zeros(self.problem.n + self.problem.m)
+        V = self._call("orc_adjoint", pub, T, h, g)
+        return V, g, h
+
+    def adjoint_frozen(self, pub, h):
+        """(V(T), dV/d pub) through len(h) Cash-Karp steps with step sizes h."""
+        pub = np.ascontiguousarray(pub, dtype=self.dt)
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        g = np.zeros(self.problem.n + self.problem.m)
+        V = self._call("orc_adjoint_frozen", pub, h, len(h), g)
+        return V, g
+
+    def frozen_energy(self, pub, h):
+        """V after len(h) Cash-Karp steps with the given step sizes (no control)."""
+        pub = np.ascontiguousarray(pub, dtype=self.dt)
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        return self._call("orc_frozen_energy", pub, h, len(h))
 
     def solve(self, max_time=100.0, max_steps=100000, batch=20, max_batches=100000):
         viol = np.zeros(max_batches, dtype=np.int32)

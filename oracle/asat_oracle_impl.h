@@ -329,6 +329,216 @@ long FN(orc_rk_steps)(const FN(orc_rk_t) *r) { return r->steps; }
 long FN(orc_rk_attempts)(const FN(orc_rk_t) *r) { return r->attempts; }
 long FN(orc_rk_rhscount)(const FN(orc_rk_t) *r) { return r->rhscount; }
 
+
+/* ===========================================================================
+ * Adjoint (no reference counterpart; the derivation is this build's):
+ * gradient of V(T) = sum_m a_m K_m^2 at the state after T accepted steps,
+ * with respect to the start state, through the realised Cash-Karp steps
+ * (step sizes frozen).  Per step x' = x + h sum_l c_l k_l, k_l = f(Y_l),
+ * Y_l = x + h sum_{j<l} b_lj k_j.  Reverse: mu_l = h c_l lam' +
+ * sum_{j>l} h b_jl nu_j, nu_l = J(Y_l)^T mu_l, lam = lam' + sum_l nu_l.
+ * VJP of one clause (f_q = 1 - c_q s_q, K = 2^-k prod f, K_q / K_qp the
+ * products without q / without q,p; padding factors 2 are constants):
+ *   u_a    = w_a K^2 + sum_q w_q 2 c_q K K_q
+ *   u_s[p] = w_a (-2 a c_p K K_p) + w_p (-2 a K_p^2) + sum_{q!=p} w_q (-4 a c_q c_p K K_qp)
+ * All adjoint arithmetic is double; the state type ST only sets how the
+ * forward states (and so the stage inputs) are rounded.
+ * =========================================================================*/
+
+/* product over q' of fd[q'] skipping up to two positions, left to right, times kn */
+static double FN(adj_prod)(const double *fd, int k, int skip1, int skip2, double kn)
+{
+    double p = kn;
+    for (int q = 0; q < k; q++) {
+        if (q == skip1 || q == skip2) continue;
+        p = p * fd[q];
+    }
+    return p;
+}
+
+/* u = J(Y)^T w on the internal layout [0, s_1..s_N, a_1..a_M] (index 0 unused) */
+void FN(orc_vjp)(int n, int m, int k, const int *C, const ST *Y, const double *w, double *u)
+{
+    const int D = n + m + 1;
+    const double kn = pow(2.0, -k);
+    double fd[64], cs[64];
+    int v[64];
+    for (int i = 0; i < D; i++) u[i] = 0.0;
+    for (int j = 0; j < m; j++) {
+        for (int q = 0; q < k; q++) {
+            int l = C[(size_t)j * k + q];
+            v[q] = l < 0 ? -l : l;
+            cs[q] = l < 0 ? -1.0 : 1.0;
+            ST s = l != 0 ? Y[v[q]] : (ST)-1.0;
+            ST f = l < 0 ? (ST)1.0 + s : (ST)1.0 - s;
+            fd[q] = (double)f;
+        }
+        const double K = FN(adj_prod)(fd, k, -1, -1, kn);
+        const double a = (double)Y[n + 1 + j];
+        const double wa = w[n + 1 + j];
+        double ua = wa * K * K;
+        for (int q = 0; q < k; q++) {
+            if (v[q] == 0) continue;
+            ua += w[v[q]] * (2.0 * cs[q] * K * FN(adj_prod)(fd, k, q, -1, kn));
+        }
+        u[n + 1 + j] = ua;
+        for (int p = 0; p < k; p++) {
+            if (v[p] == 0) continue;
+            const double Kp = FN(adj_prod)(fd, k, p, -1, kn);
+            double t = wa * (-2.0 * a * cs[p] * K * Kp) + w[v[p]] * (-2.0 * a * Kp * Kp);
+            for (int q = 0; q < k; q++) {
+                if (q == p || v[q] == 0) continue;
+                t += w[v[q]] * (-4.0 * a * cs[q] * cs[p] * K * FN(adj_prod)(fd, k, q, p, kn));
+            }
+            u[v[p]] += t;
+        }
+    }
+}
+
+/* grad V at x: dV/da_m = K_m^2, dV/ds_i = sum_m -2 a_m c_mi K_m K_mi (double) */
+double FN(orc_energy_grad)(int n, int m, int k, const int *C, const ST *x, double *g)
+{
+    const int D = n + m + 1;
+    const double kn = pow(2.0, -k);
+    double fd[64], cs[64];
+    int v[64];
+    double V = 0.0;
+    for (int i = 0; i < D; i++) g[i] = 0.0;
+    for (int j = 0; j < m; j++) {
+        for (int q = 0; q < k; q++) {
+            int l = C[(size_t)j * k + q];
+            v[q] = l < 0 ? -l : l;
+            cs[q] = l < 0 ? -1.0 : 1.0;
+            ST s = l != 0 ? x[v[q]] : (ST)-1.0;
+            ST f = l < 0 ? (ST)1.0 + s : (ST)1.0 - s;
+            fd[q] = (double)f;
+        }
+        const double K = FN(adj_prod)(fd, k, -1, -1, kn);
+        const double a = (double)x[n + 1 + j];
+        V += a * K * K;
+        g[n + 1 + j] = K * K;
+        for (int p = 0; p < k; p++) {
+            if (v[p] == 0) continue;
+            g[v[p]] += -2.0 * a * cs[p] * K * FN(adj_prod)(fd, k, p, -1, kn);
+        }
+    }
+    return V;
+}
+
+/* one Cash-Karp step with a given h (no control), the realised map; also
+   returns the stage inputs Y[0..5] (each length len) when Ys != NULL */
+static void FN(frozen_step)(FN(orc_rk_t) *r, const ST *x, ST *xout, double h, ST *Ys)
+{
+    const int n = r->len;
+    const ST hs = (ST)h;
+    const ST b21 = (ST)0.2, b31 = (ST)(3.0 / 40.0), b32 = (ST)(9.0 / 40.0),
+             b41 = (ST)0.3, b42 = (ST)-0.9, b43 = (ST)1.2,
+             b51 = (ST)(-11.0 / 54.0), b52 = (ST)2.5, b53 = (ST)(-70.0 / 27.0), b54 = (ST)(35.0 / 27.0),
+             b61 = (ST)(1631.0 / 55296.0), b62 = (ST)(175.0 / 512.0), b63 = (ST)(575.0 / 13824.0),
+             b64 = (ST)(44275.0 / 110592.0), b65 = (ST)(253.0 / 4096.0),
+             c1 = (ST)(37.0 / 378.0), c3 = (ST)(250.0 / 621.0), c4 = (ST)(125.0 / 594.0),
+             c6 = (ST)(512.0 / 1771.0);
+    ST *k1 = r->dxdt, *k2 = r->ak2, *k3 = r->ak3, *k4 = r->ak4, *k5 = r->ak5, *k6 = r->ak6, *t = r->temp;
+    FN(orc_rhs)(r->n, r->m, r->k, r->C, r->b, x, k1);
+    if (Ys) memcpy(Ys, x, sizeof(ST) * (size_t)n);
+    for (int i = 0; i < n; i++) t[i] = x[i] + b21 * hs * k1[i];
+    if (Ys) memcpy(Ys + (size_t)n, t, sizeof(ST) * (size_t)n);
+    FN(orc_rhs)(r->n, r->m, r->k, r->C, r->b, t, k2);
+    for (int i = 0; i < n; i++) t[i] = x[i] + hs * (b31 * k1[i] + b32 * k2[i]);
+    if (Ys) memcpy(Ys + 2 * (size_t)n, t, sizeof(ST) * (size_t)n);
+    FN(orc_rhs)(r->n, r->m, r->k, r->C, r->b, t, k3);
+    for (int i = 0; i < n; i++) t[i] = x[i] + hs * (b41 * k1[i] + b42 * k2[i] + b43 * k3[i]);
+    if (Ys) memcpy(Ys + 3 * (size_t)n, t, sizeof(ST) * (size_t)n);
+    FN(orc_rhs)(r->n, r->m, r->k, r->C, r->b, t, k4);
+    for (int i = 0; i < n; i++) t[i] = x[i] + hs * (b51 * k1[i] + b52 * k2[i] + b53 * k3[i] + b54 * k4[i]);
+    if (Ys) memcpy(Ys + 4 * (size_t)n, t, sizeof(ST) * (size_t)n);
+    FN(orc_rhs)(r->n, r->m, r->k, r->C, r->b, t, k5);
+    for (int i = 0; i < n; i++) t[i] = x[i] + hs * (b61 * k1[i] + b62 * k2[i] + b63 * k3[i] + b64 * k4[i] + b65 * k5[i]);
+    if (Ys) memcpy(Ys + 5 * (size_t)n, t, sizeof(ST) * (size_t)n);
+    FN(orc_rhs)(r->n, r->m, r->k, r->C, r->b, t, k6);
+    for (int i = 0; i < n; i++) xout[i] = x[i] + hs * (c1 * k1[i] + c3 * k3[i] + c4 * k4[i] + c6 * k6[i]);
+    xout[0] = (ST)-1.0;
+}
+
+/* V after T frozen steps from the public start pub (h[0..T-1] given) */
+double FN(orc_frozen_energy)(FN(orc_rk_t) *r, const ST *pub, const double *h, int T)
+{
+    const int n = r->len;
+    ST *x = (ST *)malloc(sizeof(ST) * (size_t)n), *y = (ST *)malloc(sizeof(ST) * (size_t)n);
+    double *g = (double *)malloc(sizeof(double) * (size_t)n);
+    x[0] = (ST)-1.0;
+    memcpy(x + 1, pub, sizeof(ST) * (size_t)(n - 1));
+    for (int s = 0; s < T; s++) {
+        FN(frozen_step)(r, x, y, h[s], NULL);
+        ST *t = x; x = y; y = t;
+    }
+    double V = FN(orc_energy_grad)(r->n, r->m, r->k, r->C, x, g);
+    free(x); free(y); free(g);
+    return V;
+}
+
+/* Discrete adjoint of T Cash-Karp steps with the given step sizes h[0..T-1]
+   (the realised map of an adaptive run, or any frozen sequence): returns V(T)
+   and its gradient (public layout [s, a], double) with respect to pub. */
+double FN(orc_adjoint_frozen)(FN(orc_rk_t) *r, const ST *pub, const double *h, int T, double *grad)
+{
+    const int n = r->len;
+    ST *X = (ST *)malloc(sizeof(ST) * (size_t)n * (size_t)(T + 1));
+    ST *Ys = (ST *)malloc(sizeof(ST) * (size_t)n * 6);
+    ST *y = (ST *)malloc(sizeof(ST) * (size_t)n);
+    double *lam = (double *)malloc(sizeof(double) * (size_t)n);
+    double *mu = (double *)malloc(sizeof(double) * (size_t)n);
+    double *nu = (double *)malloc(sizeof(double) * (size_t)n * 6);
+    X[0] = (ST)-1.0;
+    memcpy(X + 1, pub, sizeof(ST) * (size_t)(n - 1));
+    for (int s = 0; s < T; s++) FN(frozen_step)(r, X + (size_t)s * n, X + (size_t)(s + 1) * n, h[s], NULL);
+    const double V = FN(orc_energy_grad)(r->n, r->m, r->k, r->C, X + (size_t)T * n, lam);
+    lam[0] = 0.0;
+    /* Butcher coefficients as doubles of the state-type values used forward */
+    const double B[7][6] = {
+        {0}, {0},
+        {(double)(ST)0.2},
+        {(double)(ST)(3.0 / 40.0), (double)(ST)(9.0 / 40.0)},
+        {(double)(ST)0.3, (double)(ST)-0.9, (double)(ST)1.2},
+        {(double)(ST)(-11.0 / 54.0), (double)(ST)2.5, (double)(ST)(-70.0 / 27.0), (double)(ST)(35.0 / 27.0)},
+        {(double)(ST)(1631.0 / 55296.0), (double)(ST)(175.0 / 512.0), (double)(ST)(575.0 / 13824.0),
+         (double)(ST)(44275.0 / 110592.0), (double)(ST)(253.0 / 4096.0)}};
+    const double Cw[7] = {0, (double)(ST)(37.0 / 378.0), 0, (double)(ST)(250.0 / 621.0),
+                          (double)(ST)(125.0 / 594.0), 0, (double)(ST)(512.0 / 1771.0)};
+    for (int s = T - 1; s >= 0; s--) {
+        const double hh = (double)(ST)h[s];
+        FN(frozen_step)(r, X + (size_t)s * n, y, h[s], Ys);
+        for (int l = 6; l >= 1; l--) {
+            for (int i = 0; i < n; i++) {
+                double acc = Cw[l] * lam[i];
+                for (int j = l + 1; j <= 6; j++) acc += B[j][l - 1] * nu[(size_t)(j - 1) * n + i];
+                mu[i] = hh * acc;
+            }
+            mu[0] = 0.0;
+            FN(orc_vjp)(r->n, r->m, r->k, r->C, Ys + (size_t)(l - 1) * n, mu, nu + (size_t)(l - 1) * n);
+        }
+        for (int i = 1; i < n; i++)
+            lam[i] = lam[i] + nu[i] + nu[(size_t)n + i] + nu[2 * (size_t)n + i] + nu[3 * (size_t)n + i] +
+                     nu[4 * (size_t)n + i] + nu[5 * (size_t)n + i];
+    }
+    memcpy(grad, lam + 1, sizeof(double) * (size_t)(n - 1));
+    free(X); free(Ys); free(y); free(lam); free(mu); free(nu);
+    return V;
+}
+
+/* Adaptive forward of T accepted steps from pub (records h[0..T-1]), then the
+   discrete adjoint of those realised steps.  Returns V(T), or NaN on underflow. */
+double FN(orc_adjoint)(FN(orc_rk_t) *r, const ST *pub, int T, double *h_out, double *grad)
+{
+    FN(orc_rk_set_state)(r, pub);
+    FN(orc_rk_reset)(r);
+    for (int s = 0; s < T; s++) {
+        if (!FN(orc_rk_step)(r, 1)) return NAN;
+        h_out[s] = (double)r->hdid;
+    }
+    return FN(orc_adjoint_frozen)(r, pub, h_out, T, grad);
+}
+
 #undef FN
 #undef CAT
 #undef CAT2
