@@ -130,6 +130,16 @@ def lib():
             "asat_rk_last_step_ms": ([vp, ctypes.POINTER(ctypes.c_float)], i),
             "asat_rk_profile": ([vp, vp, vp, i, vp], i),
             "asat_util_copy_bench": ([ctypes.c_longlong, i, ctypes.POINTER(ctypes.c_float)], i),
+            "asat_rk_set_step_log": ([vp, vp, ctypes.c_longlong], i),
+            "asat_adj_create": ([pvp], i),
+            "asat_adj_destroy": ([vp], i),
+            "asat_adj_set_problem": ([vp, vp, vp, i], i),
+            "asat_adj_set_params": ([vp, ctypes.c_double, i, i], i),
+            "asat_adj_forward": ([vp, i, vp, vp], i),
+            "asat_adj_get_steps": ([vp, vp], i),
+            "asat_adj_backward": ([vp, vp], i),
+            "asat_adj_replay": ([vp, i, vp, vp, vp, vp], i),
+            "asat_adj_timing": ([vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], i),
             "asat_rng_create": ([ctypes.c_ulonglong, pvp], i),
             "asat_rng_destroy": ([vp], i),
             "asat_rng_uniform": ([vp, vp, i, i], i),
@@ -698,3 +708,92 @@ class HipBatch:
             _check(lib().asat_batch_bucket_info(self._h, b, ctypes.byref(r)))
             out.append({f: getattr(r, f) for f, _ in BatchBucket._fields_})
         return out
+
+
+# ----------------------------------------------------------------------------
+# discrete adjoint (no reference counterpart)
+# ----------------------------------------------------------------------------
+class HipAdjoint:
+    """Gradient of V(T) = sum_m a_m K_m^2 after T accepted FP32 steps with
+    respect to [s(0), a(0)] for a batch of trajectories of one instance
+    (asat_adj_*).  Forward adaptive with logged steps and checkpoints every
+    ``checkpoint_every`` steps; backward in FP64."""
+
+    def __init__(self, problem, eps=1e-6, steps=500, checkpoint_every=32):
+        self._h = ctypes.c_void_p()
+        _check(lib().asat_adj_create(ctypes.byref(self._h)))
+        lits = np.ascontiguousarray(problem.literals, np.int32)
+        lens = np.ascontiguousarray(problem.lengths, np.int32)
+        _check(lib().asat_adj_set_problem(self._h, lits.ctypes.data, lens.ctypes.data, len(lens)))
+        _check(lib().asat_adj_set_params(self._h, float(eps), int(steps), int(checkpoint_every)))
+        self.T = int(steps)
+        self.nvar = problem.Get_N() + problem.Get_M()
+        self.ntraj = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value and _lib is not None:
+            _lib.asat_adj_destroy(h)
+
+    def forward(self, pub0):
+        """pub0: (ntraj, n+m) start states; returns V(T) per trajectory (float64)."""
+        x = np.ascontiguousarray(pub0, np.float32).reshape(-1, self.nvar)
+        self.ntraj = x.shape[0]
+        V = np.zeros(self.ntraj)
+        _check(lib().asat_adj_forward(self._h, self.ntraj, x.ctypes.data, V.ctypes.data))
+        return V
+
+    def steps(self):
+        h = np.zeros((self.ntraj, self.T))
+        _check(lib().asat_adj_get_steps(self._h, h.ctypes.data))
+        return h
+
+    def backward(self):
+        """dV(T)/d[s(0), a(0)] of the last forward run, shape (ntraj, n+m), float64."""
+        g = np.zeros((self.ntraj, self.nvar))
+        _check(lib().asat_adj_backward(self._h, g.ctypes.data))
+        return g
+
+    def replay(self, pub0, h):
+        """V(T) and x(T) of T FP32 steps with the given step sizes (ntraj, T)."""
+        x = np.ascontiguousarray(pub0, np.float32).reshape(-1, self.nvar)
+        h = np.ascontiguousarray(h, np.float64).reshape(x.shape[0], self.T)
+        V = np.zeros(x.shape[0])
+        xT = np.zeros_like(x)
+        _check(lib().asat_adj_replay(self._h, x.shape[0], x.ctypes.data, h.ctypes.data, V.ctypes.data, xT.ctypes.data))
+        self.ntraj = 0
+        return V, xT
+
+    def timing(self):
+        f, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().asat_adj_timing(self._h, ctypes.byref(f), ctypes.byref(b)))
+        return f.value, b.value
+
+
+def energy_function():
+    """torch.autograd.Function computing V(T) per trajectory from (s0, a0).
+
+    Usage: V = energy_function().apply(s0, a0, adjoint) with s0 (ntraj, n),
+    a0 (ntraj, m) tensors on any device; backward returns FP64 gradients
+    produced by asat_adj_backward (cast to the inputs' dtype/device).
+    """
+    import torch
+
+    class CTDSEnergy(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, s0, a0, adjoint):
+            pub = torch.cat([s0, a0], dim=1).detach().to("cpu", torch.float32).numpy()
+            V = adjoint.forward(pub)
+            ctx.adjoint = adjoint
+            ctx.n = s0.shape[1]
+            ctx.meta = (s0.dtype, s0.device, a0.dtype, a0.device)
+            return torch.from_numpy(V).to(s0.device)
+
+        @staticmethod
+        def backward(ctx, gV):
+            g = torch.from_numpy(ctx.adjoint.backward())
+            g = g * gV.detach().to("cpu", torch.float64)[:, None]
+            sd, sdev, ad, adev = ctx.meta
+            return g[:, :ctx.n].to(sdev, sd), g[:, ctx.n:].to(adev, ad), None
+
+    return CTDSEnergy

@@ -1,0 +1,730 @@
+// Discrete adjoint of the FP32 Cash-Karp integrator on MI355X.  No reference
+// counterpart: the reference only integrates forward.
+//
+// Gradient of V(T) = sum_m a_m K_m(x_T)^2 after T accepted steps, with
+// respect to the start state, for a batch of trajectories of one instance.
+// Forward: the plain FP32 integrator (FP64 control) runs each trajectory
+// adaptively, logging every accepted step size on the device and keeping a
+// checkpoint every C steps.  Reverse, all trajectories in lockstep (one per
+// blockIdx.y), segment by segment from the last checkpoint:
+//   recompute the segment's states with the logged (frozen) step sizes, then
+//   for each step backwards recompute its six stage inputs Y_l and run
+//     mu_l = h (c_l lam' + sum_{j>l} b_jl nu_j),  nu_l = J(Y_l)^T mu_l,
+//     lam  = lam' + sum_l nu_l
+// The recomputation uses the forward's FP32 arithmetic (same operation
+// order), so it reproduces the realised forward states exactly; everything
+// adjoint (mu, nu, lam, the VJP) is FP64.  VJP of one clause (f_q = 1 - c_q
+// s_q, K = 2^-k prod f, K_q / K_qp the products without q / without q and p):
+//   u_a    = w_a K^2 + sum_q w_q 2 c_q K K_q
+//   u_s[p] = w_a (-2 a c_p K K_p) + w_p (-2 a K_p^2) + sum_{q!=p} w_q (-4 a c_q c_p K K_qp)
+// Per-variable sums use the var-major slot list in slot order (deterministic).
+// Internal layout of every vector: [0 | s_1..s_N | a_1..a_M], stride D.
+#include "asat_common.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+namespace asat {
+extern thread_local std::string g_last_error;
+
+namespace adj {
+
+constexpr int BLK = 256;
+
+struct Dev {
+    const int* lit;       // m*k (zero padded)
+    const int* csr_ptr;   // n+2, var-major over slots
+    const int* csr_slot;  // slot indices, ascending per variable
+    int n, m, k;
+    long long D;          // vector stride (>= n+m+1)
+    long long L;          // m*k
+};
+
+// ---- FP32 recompute (forward arithmetic) ----------------------------------
+template <int K>
+__global__ __launch_bounds__(BLK) void k_rhs_clause(Dev d, const float* Y, float* out, float* contrib)
+{
+    const int m = blockIdx.x * BLK + threadIdx.x;
+    const long long tr = blockIdx.y;
+    if (m >= d.m) return;
+    const float* y = Y + tr * d.D;
+    int l[K];
+    float f[K];
+#pragma unroll
+    for (int q = 0; q < K; q++) l[q] = d.lit[(size_t)m * K + q];
+    float km = (float)(1.0 / (1 << K));
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+        f[q] = spin_factor<float>(l[q], y);
+        km *= f[q];
+    }
+    const long long ai = d.n + 1 + m;
+    const float am = y[ai];
+    out[tr * d.D + ai] = am * km * km;
+    const float r = km * (float)(1.0 / (1 << K)) * am;
+    float o[K];
+#pragma unroll
+    for (int q = 0; q < K; q++) o[q] = 0.f;
+    write_contribs<float, K, 0>(l, f, r, o);
+    float* c = contrib + tr * d.L + (long long)m * K;
+#pragma unroll
+    for (int q = 0; q < K; q++)
+        if (l[q] != 0) c[q] = o[q];
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_rhs_var(Dev d, const T* contrib, T* out)
+{
+    const int i = blockIdx.x * BLK + threadIdx.x + 1;
+    const long long tr = blockIdx.y;
+    if (i > d.n) return;
+    const T* c = contrib + tr * d.L;
+    T sum = (T)0;
+    for (int e = d.csr_ptr[i]; e < d.csr_ptr[i + 1]; e++) sum += c[d.csr_slot[e]];
+    out[tr * d.D + i] = sum;
+}
+
+// stage input J of every element (spins and aux) for every trajectory
+template <int J>
+__global__ __launch_bounds__(BLK) void k_stage(Dev d, const float* X, const float* k1, const float* k2,
+                                                const float* k3, const float* k4, const float* k5,
+                                                const double* h, float* Y)
+{
+    const long long i = (long long)blockIdx.x * BLK + threadIdx.x + 1;
+    const long long tr = blockIdx.y;
+    if (i > (long long)d.n + d.m) return;
+    const long long o = tr * d.D + i;
+    const float hs = (float)h[tr];
+    Y[o] = stage_input<float, J>(X[o], k1[o], J > 2 ? k2[o] : 0.f, J > 3 ? k3[o] : 0.f, J > 4 ? k4[o] : 0.f,
+                                 J > 5 ? k5[o] : 0.f, hs);
+}
+
+__global__ __launch_bounds__(BLK) void k_final(Dev d, const float* X, const float* k1, const float* k3,
+                                               const float* k4, const float* k6, const double* h, float* Xn)
+{
+    const long long i = (long long)blockIdx.x * BLK + threadIdx.x;
+    const long long tr = blockIdx.y;
+    if (i > (long long)d.n + d.m) return;
+    const long long o = tr * d.D + i;
+    if (i == 0) {
+        Xn[o] = -1.0f;
+        return;
+    }
+    using B = Tab<float>;
+    const float hs = (float)h[tr];
+    Xn[o] = X[o] + hs * (B::c1 * k1[o] + B::c3 * k3[o] + B::c4 * k4[o] + B::c6 * k6[o]);
+}
+
+// ---- FP64 adjoint ------------------------------------------------------------
+template <int K>
+__device__ __forceinline__ double prod_skip(const double (&fd)[K], int s1, int s2, double kn)
+{
+    double p = kn;
+#pragma unroll
+    for (int q = 0; q < K; q++)
+        if (q != s1 && q != s2) p = p * fd[q];
+    return p;
+}
+
+// u_a directly, per-slot spin contributions into contrib64
+template <int K>
+__global__ __launch_bounds__(BLK) void k_vjp_clause(Dev d, const float* Y, const double* w, double* u,
+                                                    double* contrib)
+{
+    const int m = blockIdx.x * BLK + threadIdx.x;
+    const long long tr = blockIdx.y;
+    if (m >= d.m) return;
+    const float* y = Y + tr * d.D;
+    const double* wt = w + tr * d.D;
+    const double kn = 1.0 / (1 << K);
+    int v[K];
+    double fd[K], cs[K], wv[K];
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+        const int l = d.lit[(size_t)m * K + q];
+        v[q] = l < 0 ? -l : l;
+        cs[q] = l < 0 ? -1.0 : 1.0;
+        fd[q] = (double)spin_factor<float>(l, y);
+        wv[q] = v[q] ? wt[v[q]] : 0.0;
+    }
+    const double Kc = prod_skip<K>(fd, -1, -1, kn);
+    const long long ai = d.n + 1 + m;
+    const double a = (double)y[ai];
+    const double wa = wt[ai];
+    double ua = wa * Kc * Kc;
+#pragma unroll
+    for (int q = 0; q < K; q++)
+        if (v[q]) ua += wv[q] * (2.0 * cs[q] * Kc * prod_skip<K>(fd, q, -1, kn));
+    u[tr * d.D + ai] = ua;
+    double* c = contrib + tr * d.L + (long long)m * K;
+#pragma unroll
+    for (int p = 0; p < K; p++) {
+        if (!v[p]) continue;
+        const double Kp = prod_skip<K>(fd, p, -1, kn);
+        double t = wa * (-2.0 * a * cs[p] * Kc * Kp) + wv[p] * (-2.0 * a * Kp * Kp);
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            if (q == p || !v[q]) continue;
+            t += wv[q] * (-4.0 * a * cs[q] * cs[p] * Kc * prod_skip<K>(fd, q, p, kn));
+        }
+        c[p] = t;
+    }
+}
+
+// grad V: aux K^2, spin contributions -2 a c K K_p; per-block partial sums of V
+template <int K>
+__global__ __launch_bounds__(BLK) void k_gradv_clause(Dev d, const float* X, double* g, double* contrib,
+                                                      double* vpart, int nblk)
+{
+    __shared__ double sh[BLK / 64];
+    const int m = blockIdx.x * BLK + threadIdx.x;
+    const long long tr = blockIdx.y;
+    double vloc = 0.0;
+    if (m < d.m) {
+        const float* x = X + tr * d.D;
+        const double kn = 1.0 / (1 << K);
+        int v[K];
+        double fd[K], cs[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const int l = d.lit[(size_t)m * K + q];
+            v[q] = l < 0 ? -l : l;
+            cs[q] = l < 0 ? -1.0 : 1.0;
+            fd[q] = (double)spin_factor<float>(l, x);
+        }
+        const double Kc = prod_skip<K>(fd, -1, -1, kn);
+        const long long ai = d.n + 1 + m;
+        const double a = (double)x[ai];
+        vloc = a * Kc * Kc;
+        g[tr * d.D + ai] = Kc * Kc;
+        double* c = contrib + tr * d.L + (long long)m * K;
+#pragma unroll
+        for (int p = 0; p < K; p++)
+            if (v[p]) c[p] = -2.0 * a * cs[p] * Kc * prod_skip<K>(fd, p, -1, kn);
+    }
+    // deterministic block sum (fixed tree), summed in block order on the host side
+    double s = vloc;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < BLK / 64; i++) t += sh[i];
+        vpart[tr * nblk + blockIdx.x] = t;
+    }
+}
+
+__global__ void k_sum_parts(const double* vpart, int nblk, double* V)
+{
+    const int tr = blockIdx.x;
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int b = 0; b < nblk; b++) t += vpart[(long long)tr * nblk + b];
+        V[tr] = t;
+    }
+}
+
+// mu = h * (Cw_l lam + sum_{j>l} B_jl nu_j)   (j descending order of terms as written)
+__global__ __launch_bounds__(BLK) void k_mu(Dev d, int l, const double* lam, const double* nu, const double* hh,
+                                            double* mu)
+{
+    const double Bj[7][6] = {
+        {0}, {0},
+        {(double)Tab<float>::b21},
+        {(double)Tab<float>::b31, (double)Tab<float>::b32},
+        {(double)Tab<float>::b41, (double)Tab<float>::b42, (double)Tab<float>::b43},
+        {(double)Tab<float>::b51, (double)Tab<float>::b52, (double)Tab<float>::b53, (double)Tab<float>::b54},
+        {(double)Tab<float>::b61, (double)Tab<float>::b62, (double)Tab<float>::b63, (double)Tab<float>::b64,
+         (double)Tab<float>::b65}};
+    const double Cw[7] = {0, (double)Tab<float>::c1, 0, (double)Tab<float>::c3, (double)Tab<float>::c4, 0,
+                          (double)Tab<float>::c6};
+    const long long i = (long long)blockIdx.x * BLK + threadIdx.x;
+    const long long tr = blockIdx.y;
+    if (i > (long long)d.n + d.m) return;
+    const long long o = tr * d.D + i;
+    if (i == 0) {
+        mu[o] = 0.0;
+        return;
+    }
+    const long long vs = (long long)gridDim.y * d.D;   // stride between nu_j vectors
+    double acc = Cw[l] * lam[o];
+    for (int j = l + 1; j <= 6; j++) acc += Bj[j][l - 1] * nu[(long long)(j - 1) * vs + o];
+    mu[o] = hh[tr] * acc;
+}
+
+__global__ __launch_bounds__(BLK) void k_lam(Dev d, double* lam, const double* nu)
+{
+    const long long i = (long long)blockIdx.x * BLK + threadIdx.x + 1;
+    const long long tr = blockIdx.y;
+    if (i > (long long)d.n + d.m) return;
+    const long long o = tr * d.D + i;
+    const long long vs = (long long)gridDim.y * d.D;
+    lam[o] = lam[o] + nu[o] + nu[vs + o] + nu[2 * vs + o] + nu[3 * vs + o] + nu[4 * vs + o] + nu[5 * vs + o];
+}
+
+__global__ void k_zero_slot0(double* v, long long D, int ntraj)
+{
+    const int tr = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tr < ntraj) v[(long long)tr * D] = 0.0;
+}
+
+// plain-solver layout (spins 1..N, aux at aoff) -> internal layout (aux at N+1)
+__global__ __launch_bounds__(BLK) void k_from_plain(const float* src, int n, int m, int aoff, float* dst)
+{
+    const long long i = (long long)blockIdx.x * BLK + threadIdx.x;
+    if (i == 0) dst[0] = -1.0f;
+    else if (i <= n) dst[i] = src[i];
+    else if (i <= (long long)n + m) dst[i] = src[aoff + (i - n - 1)];
+}
+
+__global__ __launch_bounds__(BLK) void k_h_round(const double* h, double* hh, int n)
+{
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i < n) hh[i] = (double)(float)h[i];
+}
+
+// ---------------------------------------------------------------------------
+struct Adj {
+    int n = 0, m = 0, k = 0;
+    long long D = 0, L = 0;
+    double eps = 1e-6;
+    int T = 0, C = 32;
+    int ntraj = 0;
+    bool have_problem = false, have_forward = false;
+    std::vector<int> lits_in, lens_in;
+    hipStream_t stream = nullptr;
+    int *d_lit = nullptr, *d_csr_ptr = nullptr, *d_csr_slot = nullptr;
+    // per batch
+    float* ckpt = nullptr;       // [ncheck][ntraj][D]
+    float* seg = nullptr;        // [C+1][ntraj][D]
+    float* Y = nullptr;          // [6][ntraj][D]
+    float* kk = nullptr;         // [6][ntraj][D]
+    float* c32 = nullptr;        // [ntraj][L]
+    double* c64 = nullptr;       // [ntraj][L]
+    double* lam = nullptr;       // [ntraj][D]
+    double* mu = nullptr;        // [ntraj][D]
+    double* nu = nullptr;        // [6][ntraj][D]
+    double* hlog = nullptr;      // [ntraj][T]
+    double* hcol = nullptr;      // [T][ntraj] step sizes by step
+    double* hcolr = nullptr;     // [T][ntraj] FP32-rounded step sizes (adjoint)
+    const double* hcur = nullptr;   // current step's column
+    const double* hcurr = nullptr;
+    double* vpart = nullptr;
+    double* dV = nullptr;
+    int ncheck = 0;
+    double fwd_ms = 0, bwd_ms = 0;
+
+    void free_batch()
+    {
+        for (void* p : {(void*)ckpt, (void*)seg, (void*)Y, (void*)kk, (void*)c32, (void*)c64, (void*)lam,
+                        (void*)mu, (void*)nu, (void*)hlog, (void*)hcol, (void*)hcolr, (void*)vpart, (void*)dV})
+            (void)hipFree(p);
+        ckpt = seg = Y = kk = c32 = nullptr;
+        c64 = lam = mu = nu = hlog = hcol = hcolr = vpart = dV = nullptr;
+    }
+    ~Adj()
+    {
+        free_batch();
+        (void)hipFree(d_lit); (void)hipFree(d_csr_ptr); (void)hipFree(d_csr_slot);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    Dev dev() const { return Dev{d_lit, d_csr_ptr, d_csr_slot, n, m, k, D, L}; }
+};
+
+static void set_problem(Adj& A, const int* lits, const int* lens, int nclauses)
+{
+    require(nclauses > 0 && lits && lens, ASAT_ERR_INVALID_ARGUMENT, "empty problem");
+    int n = 0, m = 0, k = 0;
+    size_t off = 0;
+    for (int j = 0; j < nclauses; j++) {
+        int len1 = 0;
+        for (int i = 0; i < lens[j]; i++)
+            if (lits[off + i]) { n = std::max(n, std::abs(lits[off + i])); len1++; }
+        if (len1) { m++; k = std::max(k, len1); }
+        off += (size_t)lens[j];
+    }
+    require(m > 0, ASAT_ERR_INVALID_ARGUMENT, "empty problem");
+    require(k == 3 || k == 4, ASAT_ERR_UNSUPPORTED, "the adjoint supports k = 3 or 4 (longest clause)");
+    std::vector<int> C((size_t)m * k, 0);
+    off = 0;
+    int row = 0;
+    for (int j = 0; j < nclauses; j++) {
+        int q = 0;
+        for (int i = 0; i < lens[j]; i++)
+            if (lits[off + i]) C[(size_t)row * k + q++] = lits[off + i];
+        if (q) row++;
+        off += (size_t)lens[j];
+    }
+    std::vector<int> ptr((size_t)n + 2, 0);
+    for (int l : C)
+        if (l) ptr[(size_t)std::abs(l) + 1]++;
+    for (int i = 0; i <= n; i++) ptr[i + 1] += ptr[i];
+    std::vector<int> slot(std::max(ptr[n + 1], 1));
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (size_t s = 0; s < C.size(); s++)
+        if (C[s]) slot[(size_t)fill[std::abs(C[s])]++] = (int)s;
+    if (!A.stream) HIP_CHECK(hipStreamCreateWithFlags(&A.stream, hipStreamNonBlocking));
+    HIP_CHECK(hipStreamSynchronize(A.stream));
+    A.free_batch();
+    (void)hipFree(A.d_lit); (void)hipFree(A.d_csr_ptr); (void)hipFree(A.d_csr_slot);
+    HIP_CHECK(hipMalloc((void**)&A.d_lit, sizeof(int) * C.size()));
+    HIP_CHECK(hipMalloc((void**)&A.d_csr_ptr, sizeof(int) * ptr.size()));
+    HIP_CHECK(hipMalloc((void**)&A.d_csr_slot, sizeof(int) * slot.size()));
+    HIP_CHECK(hipMemcpy(A.d_lit, C.data(), sizeof(int) * C.size(), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(A.d_csr_ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(A.d_csr_slot, slot.data(), sizeof(int) * slot.size(), hipMemcpyHostToDevice));
+    A.n = n; A.m = m; A.k = k;
+    A.D = (long long)((n + m + 1 + 63) / 64) * 64;
+    A.L = (long long)m * k;
+    A.lits_in.assign(lits, lits + off);
+    A.lens_in.assign(lens, lens + nclauses);
+    A.have_problem = true;
+    A.have_forward = false;
+}
+
+static void alloc_batch(Adj& A, int ntraj)
+{
+    A.free_batch();
+    A.ntraj = ntraj;
+    A.ncheck = (A.T + A.C - 1) / A.C + 1;
+    const size_t V = (size_t)ntraj * A.D;
+    HIP_CHECK(hipMalloc((void**)&A.ckpt, sizeof(float) * V * A.ncheck));
+    HIP_CHECK(hipMalloc((void**)&A.seg, sizeof(float) * V * (A.C + 1)));
+    HIP_CHECK(hipMalloc((void**)&A.Y, sizeof(float) * V * 6));
+    HIP_CHECK(hipMalloc((void**)&A.kk, sizeof(float) * V * 6));
+    HIP_CHECK(hipMalloc((void**)&A.c32, sizeof(float) * (size_t)ntraj * A.L));
+    HIP_CHECK(hipMalloc((void**)&A.c64, sizeof(double) * (size_t)ntraj * A.L));
+    HIP_CHECK(hipMalloc((void**)&A.lam, sizeof(double) * V));
+    HIP_CHECK(hipMalloc((void**)&A.mu, sizeof(double) * V));
+    HIP_CHECK(hipMalloc((void**)&A.nu, sizeof(double) * V * 6));
+    HIP_CHECK(hipMalloc((void**)&A.hlog, sizeof(double) * (size_t)ntraj * std::max(A.T, 1)));
+    HIP_CHECK(hipMalloc((void**)&A.hcol, sizeof(double) * (size_t)ntraj * std::max(A.T, 1)));
+    HIP_CHECK(hipMalloc((void**)&A.hcolr, sizeof(double) * (size_t)ntraj * std::max(A.T, 1)));
+    const int nblk = ceil_div(A.m, BLK);
+    HIP_CHECK(hipMalloc((void**)&A.vpart, sizeof(double) * (size_t)ntraj * nblk));
+    HIP_CHECK(hipMalloc((void**)&A.dV, sizeof(double) * ntraj));
+    HIP_CHECK(hipMemset(A.seg, 0, sizeof(float) * V * (A.C + 1)));
+    HIP_CHECK(hipMemset(A.Y, 0, sizeof(float) * V * 6));
+    HIP_CHECK(hipMemset(A.kk, 0, sizeof(float) * V * 6));
+    HIP_CHECK(hipMemset(A.ckpt, 0, sizeof(float) * V * A.ncheck));
+}
+
+// f = RHS(Y) for all trajectories (FP32, forward arithmetic)
+static void rhs_all(Adj& A, const float* Yv, float* out)
+{
+    const Dev d = A.dev();
+    dim3 gc(ceil_div(A.m, BLK), A.ntraj), gv(ceil_div(A.n, BLK), A.ntraj);
+    if (A.k == 3) hipLaunchKernelGGL(k_rhs_clause<3>, gc, dim3(BLK), 0, A.stream, d, Yv, out, A.c32);
+    else hipLaunchKernelGGL(k_rhs_clause<4>, gc, dim3(BLK), 0, A.stream, d, Yv, out, A.c32);
+    hipLaunchKernelGGL(k_rhs_var<float>, gv, dim3(BLK), 0, A.stream, d, (const float*)A.c32, out);
+}
+
+// one frozen Cash-Karp step for every trajectory with step sizes hcol;
+// fills Y[0..5] (stage inputs) and kk[0..5]; writes x_next if non-null
+static void frozen_step(Adj& A, const float* X, float* Xn)
+{
+    const Dev d = A.dev();
+    const size_t V = (size_t)A.ntraj * A.D;
+    float* Yl[6];
+    float* kl[6];
+    for (int i = 0; i < 6; i++) { Yl[i] = A.Y + i * V; kl[i] = A.kk + i * V; }
+    dim3 ge(ceil_div((long long)A.n + A.m, BLK), A.ntraj);
+    HIP_CHECK(hipMemcpyAsync(Yl[0], X, sizeof(float) * V, hipMemcpyDeviceToDevice, A.stream));
+    rhs_all(A, Yl[0], kl[0]);
+    hipLaunchKernelGGL(k_stage<2>, ge, dim3(BLK), 0, A.stream, d, X, kl[0], kl[1], kl[2], kl[3], kl[4], A.hcur, Yl[1]);
+    rhs_all(A, Yl[1], kl[1]);
+    hipLaunchKernelGGL(k_stage<3>, ge, dim3(BLK), 0, A.stream, d, X, kl[0], kl[1], kl[2], kl[3], kl[4], A.hcur, Yl[2]);
+    rhs_all(A, Yl[2], kl[2]);
+    hipLaunchKernelGGL(k_stage<4>, ge, dim3(BLK), 0, A.stream, d, X, kl[0], kl[1], kl[2], kl[3], kl[4], A.hcur, Yl[3]);
+    rhs_all(A, Yl[3], kl[3]);
+    hipLaunchKernelGGL(k_stage<5>, ge, dim3(BLK), 0, A.stream, d, X, kl[0], kl[1], kl[2], kl[3], kl[4], A.hcur, Yl[4]);
+    rhs_all(A, Yl[4], kl[4]);
+    hipLaunchKernelGGL(k_stage<6>, ge, dim3(BLK), 0, A.stream, d, X, kl[0], kl[1], kl[2], kl[3], kl[4], A.hcur, Yl[5]);
+    rhs_all(A, Yl[5], kl[5]);
+    if (Xn) {
+        dim3 gf(ceil_div((long long)A.n + A.m + 1, BLK), A.ntraj);
+        hipLaunchKernelGGL(k_final, gf, dim3(BLK), 0, A.stream, d, X, kl[0], kl[2], kl[3], kl[5], A.hcur, Xn);
+    }
+}
+
+// V (and grad V into lam if want_grad) at X for every trajectory
+static void energy(Adj& A, const float* X, bool want_grad)
+{
+    const Dev d = A.dev();
+    const int nblk = ceil_div(A.m, BLK);
+    dim3 gc(nblk, A.ntraj), gv(ceil_div(A.n, BLK), A.ntraj);
+    double* g = A.lam;
+    if (A.k == 3) hipLaunchKernelGGL(k_gradv_clause<3>, gc, dim3(BLK), 0, A.stream, d, X, g, A.c64, A.vpart, nblk);
+    else hipLaunchKernelGGL(k_gradv_clause<4>, gc, dim3(BLK), 0, A.stream, d, X, g, A.c64, A.vpart, nblk);
+    if (want_grad) {
+        hipLaunchKernelGGL(k_rhs_var<double>, gv, dim3(BLK), 0, A.stream, d, (const double*)A.c64, g);
+        hipLaunchKernelGGL(k_zero_slot0, dim3(ceil_div(A.ntraj, 64)), dim3(64), 0, A.stream, g, A.D, A.ntraj);
+    }
+    hipLaunchKernelGGL(k_sum_parts, dim3(A.ntraj), dim3(64), 0, A.stream, (const double*)A.vpart, nblk, A.dV);
+}
+
+// step sizes of step s for every trajectory: hT[s][traj] (and FP32-rounded hTr)
+__global__ void k_transpose_h(const double* hlog, int ntraj, int T, double* hT, double* hTr)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)ntraj * T) return;
+    const int t = (int)(i / T), s = (int)(i % T);
+    const double h = hlog[i];
+    hT[(long long)s * ntraj + t] = h;
+    hTr[(long long)s * ntraj + t] = (double)(float)h;
+}
+
+static void make_columns(Adj& A)
+{
+    const long long n = (long long)A.ntraj * A.T;
+    hipLaunchKernelGGL(k_transpose_h, dim3(ceil_div(n, 256)), dim3(256), 0, A.stream, (const double*)A.hlog,
+                       A.ntraj, A.T, A.hcol, A.hcolr);
+}
+
+static void load_h(Adj& A, int s)
+{
+    A.hcur = A.hcol + (size_t)s * A.ntraj;
+    A.hcurr = A.hcolr + (size_t)s * A.ntraj;
+}
+
+}  // namespace adj
+}  // namespace asat
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace asat;
+using namespace asat::adj;
+
+struct asat_adj { Adj a; };
+
+#define AAPI_BEGIN try {
+#define AAPI_END                                                 \
+    }                                                            \
+    catch (const asat::Error& e) {                               \
+        asat::g_last_error = e.what();                           \
+        return e.code;                                           \
+    }                                                            \
+    catch (const std::exception& e) {                            \
+        asat::g_last_error = e.what();                           \
+        return ASAT_ERR_HIP;                                     \
+    }                                                            \
+    return ASAT_OK;
+
+extern "C" {
+
+int asat_adj_create(asat_adj_t** out)
+{
+    AAPI_BEGIN
+    require(out != nullptr, ASAT_ERR_INVALID_ARGUMENT, "null argument: out");
+    *out = new asat_adj();
+    AAPI_END
+}
+
+int asat_adj_destroy(asat_adj_t* a)
+{
+    AAPI_BEGIN
+    delete a;
+    AAPI_END
+}
+
+int asat_adj_set_problem(asat_adj_t* a, const int* literals, const int* lengths, int nclauses)
+{
+    AAPI_BEGIN
+    require(a != nullptr, ASAT_ERR_INVALID_ARGUMENT, "null adjoint");
+    set_problem(a->a, literals, lengths, nclauses);
+    AAPI_END
+}
+
+int asat_adj_set_params(asat_adj_t* a, double eps, int steps, int checkpoint_every)
+{
+    AAPI_BEGIN
+    require(a != nullptr, ASAT_ERR_INVALID_ARGUMENT, "null adjoint");
+    require(!(eps < 1e-16 || eps > 1.0), ASAT_ERR_INVALID_ARGUMENT, "epsilon must be between 1e-16 and 1");
+    require(steps >= 1 && checkpoint_every >= 1, ASAT_ERR_INVALID_ARGUMENT, "steps and checkpoint interval must be positive");
+    a->a.eps = eps; a->a.T = steps; a->a.C = checkpoint_every;
+    a->a.have_forward = false;
+    AAPI_END
+}
+
+int asat_adj_forward(asat_adj_t* a, int ntraj, const float* pub0, double* V_out)
+{
+    AAPI_BEGIN
+    require(a && pub0 && ntraj > 0, ASAT_ERR_INVALID_ARGUMENT, "null argument or no trajectories");
+    Adj& A = a->a;
+    require(A.have_problem && A.T > 0, ASAT_ERR_NOT_CONFIGURED, "adjoint problem/parameters not set");
+    alloc_batch(A, ntraj);
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, A.stream));
+    // the plain FP32 integrator, driven through its C ABI
+    asat_sys_t* sys = nullptr;
+    asat_rk_t* rk = nullptr;
+    asat_state_t *so = nullptr, *sn = nullptr;
+    auto chk = [](int rc) {
+        if (rc != ASAT_OK) throw asat::Error(rc, asat_last_error());
+    };
+    struct Cleanup {
+        asat_sys_t*& s; asat_rk_t*& r; asat_state_t*& o; asat_state_t*& nw;
+        ~Cleanup() { asat_state_destroy(o); asat_state_destroy(nw); asat_rk_destroy(r); asat_sys_destroy(s); }
+    } cleanup{sys, rk, so, sn};
+    chk(asat_sys_create(ASAT_F32, &sys));
+    chk(asat_sys_set_problem(sys, A.lits_in.data(), A.lens_in.data(), (int)A.lens_in.size()));
+    chk(asat_rk_create(&rk));
+    chk(asat_rk_set_eps(rk, A.eps));
+    chk(asat_rk_configure(rk, sys));
+    chk(asat_state_create(sys, &so));
+    chk(asat_state_create(sys, &sn));
+    const size_t V = (size_t)A.ntraj * A.D;
+    for (int t = 0; t < ntraj; t++) {
+        chk(asat_state_set_public(so, pub0 + (size_t)t * (A.n + A.m)));
+        chk(asat_rk_set_step_log(rk, A.hlog + (size_t)t * A.T, A.T));
+        chk(asat_rk_reset(rk));
+        for (int c = 0; c * A.C <= A.T; c++) {
+            void* dp = nullptr;
+            int aoff = 0;
+            chk(asat_state_device_ptr(so, &dp, &aoff));
+            hipLaunchKernelGGL(k_from_plain, dim3(ceil_div((long long)A.n + A.m + 1, BLK)), dim3(BLK), 0, A.stream,
+                               (const float*)dp, A.n, A.m, aoff, A.ckpt + (size_t)c * V + (size_t)t * A.D);
+            HIP_CHECK(hipStreamSynchronize(A.stream));
+            const int todo = std::min(A.C, A.T - c * A.C);
+            if (todo <= 0) break;
+            int rc = asat_rk_step(rk, so, sn, todo);
+            if (rc == ASAT_UNDERFLOW) throw asat::Error(ASAT_UNDERFLOW, "step size underflow in the forward run");
+            chk(rc);
+        }
+    }
+    chk(asat_rk_set_step_log(rk, nullptr, 0));
+    make_columns(A);
+    // V(T) from the last checkpoint column (= x_T)
+    const int last = A.T / A.C + ((A.T % A.C) ? 1 : 0);
+    energy(A, A.ckpt + (size_t)last * V, false);
+    HIP_CHECK(hipEventRecord(e1, A.stream));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    A.fwd_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (V_out) HIP_CHECK(hipMemcpy(V_out, A.dV, sizeof(double) * ntraj, hipMemcpyDeviceToHost));
+    A.have_forward = true;
+    AAPI_END
+}
+
+int asat_adj_get_steps(const asat_adj_t* a, double* h_out)
+{
+    AAPI_BEGIN
+    require(a && h_out, ASAT_ERR_INVALID_ARGUMENT, "null argument");
+    require(a->a.have_forward, ASAT_ERR_NOT_CONFIGURED, "no forward run");
+    HIP_CHECK(hipMemcpy(h_out, a->a.hlog, sizeof(double) * (size_t)a->a.ntraj * a->a.T, hipMemcpyDeviceToHost));
+    AAPI_END
+}
+
+int asat_adj_replay(asat_adj_t* a, int ntraj, const float* pub0, const double* h, double* V_out, float* xT_out)
+{
+    AAPI_BEGIN
+    require(a && pub0 && h && ntraj > 0, ASAT_ERR_INVALID_ARGUMENT, "null argument or no trajectories");
+    Adj& A = a->a;
+    require(A.have_problem && A.T > 0, ASAT_ERR_NOT_CONFIGURED, "adjoint problem/parameters not set");
+    alloc_batch(A, ntraj);
+    const size_t V = (size_t)ntraj * A.D;
+    HIP_CHECK(hipMemcpy(A.hlog, h, sizeof(double) * (size_t)ntraj * A.T, hipMemcpyHostToDevice));
+    make_columns(A);
+    std::vector<float> x0(V, 0.f);
+    for (int t = 0; t < ntraj; t++) {
+        x0[(size_t)t * A.D] = -1.0f;
+        std::copy(pub0 + (size_t)t * (A.n + A.m), pub0 + (size_t)(t + 1) * (A.n + A.m), x0.begin() + (size_t)t * A.D + 1);
+    }
+    HIP_CHECK(hipMemcpy(A.seg, x0.data(), sizeof(float) * V, hipMemcpyHostToDevice));
+    float* cur = A.seg;
+    float* nxt = A.seg + V;
+    for (int s = 0; s < A.T; s++) {
+        load_h(A, s);
+        frozen_step(A, cur, nxt);
+        std::swap(cur, nxt);
+    }
+    energy(A, cur, false);
+    HIP_CHECK(hipStreamSynchronize(A.stream));
+    if (V_out) HIP_CHECK(hipMemcpy(V_out, A.dV, sizeof(double) * ntraj, hipMemcpyDeviceToHost));
+    if (xT_out) {
+        std::vector<float> xt(V);
+        HIP_CHECK(hipMemcpy(xt.data(), cur, sizeof(float) * V, hipMemcpyDeviceToHost));
+        for (int t = 0; t < ntraj; t++)
+            std::copy(xt.begin() + (size_t)t * A.D + 1, xt.begin() + (size_t)t * A.D + 1 + A.n + A.m,
+                      xT_out + (size_t)t * (A.n + A.m));
+    }
+    A.have_forward = false;
+    AAPI_END
+}
+
+int asat_adj_backward(asat_adj_t* a, double* grad_out)
+{
+    AAPI_BEGIN
+    require(a && grad_out, ASAT_ERR_INVALID_ARGUMENT, "null argument");
+    Adj& A = a->a;
+    require(A.have_forward, ASAT_ERR_NOT_CONFIGURED, "asat_adj_forward must run first");
+    const Dev d = A.dev();
+    const size_t V = (size_t)A.ntraj * A.D;
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, A.stream));
+    const int nseg = (A.T + A.C - 1) / A.C;
+    // lam = grad V(x_T)
+    energy(A, A.ckpt + (size_t)nseg * V, true);
+    dim3 ge(ceil_div((long long)A.n + A.m + 1, BLK), A.ntraj), ge1(ceil_div((long long)A.n + A.m, BLK), A.ntraj);
+    dim3 gc(ceil_div(A.m, BLK), A.ntraj), gv(ceil_div(A.n, BLK), A.ntraj);
+    for (int sg = nseg - 1; sg >= 0; sg--) {
+        const int s0 = sg * A.C, s1 = std::min(A.T, s0 + A.C);
+        // recompute the segment's states from its checkpoint
+        HIP_CHECK(hipMemcpyAsync(A.seg, A.ckpt + (size_t)sg * V, sizeof(float) * V, hipMemcpyDeviceToDevice, A.stream));
+        for (int s = s0; s < s1 - 1; s++) {
+            load_h(A, s);
+            frozen_step(A, A.seg + (size_t)(s - s0) * V, A.seg + (size_t)(s - s0 + 1) * V);
+        }
+        for (int s = s1 - 1; s >= s0; s--) {
+            load_h(A, s);
+            frozen_step(A, A.seg + (size_t)(s - s0) * V, nullptr);
+            for (int l = 6; l >= 1; l--) {
+                hipLaunchKernelGGL(k_mu, ge, dim3(BLK), 0, A.stream, d, l, (const double*)A.lam,
+                                   (const double*)A.nu, A.hcurr, A.mu);
+                double* nul = A.nu + (size_t)(l - 1) * V;
+                const float* Yl = A.Y + (size_t)(l - 1) * V;
+                if (A.k == 3) hipLaunchKernelGGL(k_vjp_clause<3>, gc, dim3(BLK), 0, A.stream, d, Yl, (const double*)A.mu, nul, A.c64);
+                else hipLaunchKernelGGL(k_vjp_clause<4>, gc, dim3(BLK), 0, A.stream, d, Yl, (const double*)A.mu, nul, A.c64);
+                hipLaunchKernelGGL(k_rhs_var<double>, gv, dim3(BLK), 0, A.stream, d, (const double*)A.c64, nul);
+            }
+            hipLaunchKernelGGL(k_lam, ge1, dim3(BLK), 0, A.stream, d, A.lam, (const double*)A.nu);
+        }
+    }
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(e1, A.stream));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    A.bwd_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::vector<double> lam(V);
+    HIP_CHECK(hipMemcpy(lam.data(), A.lam, sizeof(double) * V, hipMemcpyDeviceToHost));
+    for (int t = 0; t < A.ntraj; t++)
+        std::copy(lam.begin() + (size_t)t * A.D + 1, lam.begin() + (size_t)t * A.D + 1 + A.n + A.m,
+                  grad_out + (size_t)t * (A.n + A.m));
+    AAPI_END
+}
+
+int asat_adj_timing(const asat_adj_t* a, double* forward_ms, double* backward_ms)
+{
+    AAPI_BEGIN
+    require(a != nullptr, ASAT_ERR_INVALID_ARGUMENT, "null adjoint");
+    if (forward_ms) *forward_ms = a->a.fwd_ms;
+    if (backward_ms) *backward_ms = a->a.bwd_ms;
+    AAPI_END
+}
+
+}  // extern "C"

@@ -50,6 +50,8 @@ struct alignas(64) Ctrl {
     double errmax;     // last error norm / eps
     double eps;
     double floor;      // constant term of xscal (1e-20 = reference TINY)
+    double* hlog;      // optional: accepted step sizes, hlog[steps-1] = hdid
+    long long hlog_len;
     long long steps;      // accepted steps
     long long attempts;   // attempts incl. rejected
     long long rhscount;   // RHS evaluations
@@ -145,6 +147,45 @@ __device__ __forceinline__ void final_combination(T x, T k1, T k3, T k4, T k5, T
                        TabD::dc5 * (double)k5 + TabD::dc6 * (double)k6);
     double xscal = fabs((double)x) + fabs((double)k1 * hnext) + floor;
     err = fabs(xerr / xscal);
+}
+
+// ---------------------------------------------------------------------------
+// per-clause arithmetic shared by every path (reference order:
+// AnalogSAT/cpu_sat/CpuSat.cpp:206-229, AnalogSAT/cpu_sat/cpu_helper.h:23-113)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T spin_factor(int l, const T* sp)
+{
+    T s = l != 0 ? sp[l < 0 ? -l : l] : (T)-1.0;
+    return l < 0 ? (T)1.0 + s : (T)1.0 - s;
+}
+
+// product of f[0..K-1] except position Q, left to right (reference KmiProd order)
+template <typename T, int K, int Q>
+__device__ __forceinline__ T kmi_prod(const T (&f)[K])
+{
+    T p = (T)1.0;
+    bool first = true;
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+        if (q == Q) continue;
+        p = first ? f[q] : p * f[q];
+        first = false;
+    }
+    return p;
+}
+
+template <typename T, int K, int Q>
+__device__ __forceinline__ void write_contribs(const int (&l)[K], const T (&f)[K], T r, T* out)
+{
+    if constexpr (Q < K) {
+        if (l[Q] != 0) {
+            T rr = r * kmi_prod<T, K, Q>(f);
+            rr *= l[Q] < 0 ? (T)-2.0 : (T)2.0;
+            out[Q] = rr;
+            write_contribs<T, K, Q + 1>(l, f, r, out);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------

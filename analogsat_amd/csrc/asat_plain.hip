@@ -196,41 +196,6 @@ __device__ __forceinline__ void st_nt(V* p, V v)
     __builtin_nontemporal_store(v, p);
 }
 
-template <typename T>
-__device__ __forceinline__ T spin_factor(int l, const T* sp)
-{
-    T s = l != 0 ? sp[l < 0 ? -l : l] : (T)-1.0;
-    return l < 0 ? (T)1.0 + s : (T)1.0 - s;
-}
-
-// product of f[0..K-1] except position Q, left to right (reference KmiProd order)
-template <typename T, int K, int Q>
-__device__ __forceinline__ T kmi_prod(const T (&f)[K])
-{
-    T p = (T)1.0;
-    bool first = true;
-#pragma unroll
-    for (int q = 0; q < K; q++) {
-        if (q == Q) continue;
-        p = first ? f[q] : p * f[q];
-        first = false;
-    }
-    return p;
-}
-
-template <typename T, int K, int Q>
-__device__ __forceinline__ void write_contribs(const int (&l)[K], const T (&f)[K], T r, T* out)
-{
-    if constexpr (Q < K) {
-        if (l[Q] != 0) {
-            T rr = r * kmi_prod<T, K, Q>(f);
-            rr *= l[Q] < 0 ? (T)-2.0 : (T)2.0;
-            out[Q] = rr;
-            write_contribs<T, K, Q + 1>(l, f, r, out);
-        }
-    }
-}
-
 // ---- clause pass, compile-time K (Ex order, K <= 10) ---------------------
 template <typename T, int K, int J>
 __global__ __launch_bounds__(BLK) void k_clause(StageArgs<T> a)
@@ -506,6 +471,7 @@ __global__ __launch_bounds__(CTRL_BLK) void k_control(Ctrl* c, const double* par
         double hn = errmax > TabD::ERRCON ? TabD::SAFETY * h * pow_ctrl(errmax, 5) : 5.0 * h;
         c->hdid = h;
         c->time += h;
+        if (c->hlog && c->steps < c->hlog_len) c->hlog[c->steps] = h;
         c->steps += 1;
         c->parity ^= 1;
         c->step_start = 1;
@@ -533,11 +499,11 @@ __global__ void k_ctrl_begin(Ctrl* c, long long count, int parity)
     }
 }
 
-__global__ void k_ctrl_reset(Ctrl* c, double eps, double floor)
+__global__ void k_ctrl_reset(Ctrl* c, double eps, double floor, double* hlog, long long hlog_len)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         c->h = 1e-9; c->hnext = 1e-9; c->time = 0.0; c->hdid = 0.0; c->errmax = 0.0; c->eps = eps;
-        c->floor = floor;
+        c->floor = floor; c->hlog = hlog; c->hlog_len = hlog_len;
         c->steps = 0; c->attempts = 0; c->rhscount = 0; c->target = 0;
         c->parity = 0; c->step_start = 1; c->active = 0; c->underflow = 0; c->violations = 0;
     }
@@ -845,6 +811,8 @@ struct RK {
     Sys* sys = nullptr;
     double eps = 1e-4;      // CudaRungeKutta() default (CudaRungeKutta.cu:175)
     double floor = 1e-20;   // xscal constant term (reference TINY)
+    double* hlog = nullptr; // optional device log of accepted step sizes (set before Reset)
+    long long hlog_len = 0;
     int alloc = 0, prec = -1;
     void* k[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     void* tmp = nullptr;
@@ -879,7 +847,8 @@ struct RK {
 static void rk_reset(RK* r)
 {
     r->require_configured();
-    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, r->sys->stream, r->d_ctrl, r->eps, r->floor);
+    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, r->sys->stream, r->d_ctrl, r->eps, r->floor, r->hlog,
+                       r->hlog_len);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(r->h_ctrl, r->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, r->sys->stream));
     HIP_CHECK(hipStreamSynchronize(r->sys->stream));
@@ -1298,6 +1267,30 @@ int asat_rk_set_xscal_floor(asat_rk_t* rk, double floor)
     rk->r.floor = floor;
     if (rk->r.configured) {
         hipLaunchKernelGGL(k_ctrl_set_eps, dim3(1), dim3(64), 0, rk->r.sys->stream, rk->r.d_ctrl, rk->r.eps, floor);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(rk->r.sys->stream));
+    }
+    API_END
+}
+
+__global__ void k_ctrl_set_log(Ctrl* c, double* hlog, long long len)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        c->hlog = hlog;
+        c->hlog_len = len;
+    }
+}
+
+int asat_rk_set_step_log(asat_rk_t* rk, double* dev_hlog, long long len)
+{
+    API_BEGIN
+    CHECK_PTR(rk);
+    require(len >= 0 && (len == 0 || dev_hlog != nullptr), ASAT_ERR_INVALID_ARGUMENT, "bad step log");
+    rk->r.hlog = len ? dev_hlog : nullptr;
+    rk->r.hlog_len = len;
+    if (rk->r.configured) {
+        hipLaunchKernelGGL(k_ctrl_set_log, dim3(1), dim3(64), 0, rk->r.sys->stream, rk->r.d_ctrl, rk->r.hlog,
+                           rk->r.hlog_len);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(rk->r.sys->stream));
     }

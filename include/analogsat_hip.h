@@ -133,6 +133,10 @@ int asat_rk_get_eps(const asat_rk_t* rk, double* eps);
    state needs it at eps ~ 1e-6, where float rounding of the stage inputs
    otherwise drives h towards zero whenever a spin passes through s = 0. */
 int asat_rk_set_xscal_floor(asat_rk_t* rk, double floor);
+/* Record accepted step sizes on the device: after the k-th accepted step
+   since Reset, dev_hlog[k-1] = hdid (for k <= len).  len = 0 switches the
+   log off.  No reference counterpart (used by the adjoint's forward run). */
+int asat_rk_set_step_log(asat_rk_t* rk, double* dev_hlog, long long len);
 /* IODEInt::Configure(ode): allocates scratch, resets h = 1e-9 */
 int asat_rk_configure(asat_rk_t* rk, asat_sys_t* sys);
 /* IODEInt::Step(old, new, count): count accepted steps; on return `old`
@@ -226,6 +230,31 @@ int asat_batch_result(const asat_batch_t* b, int index, asat_batch_result_t* out
 int asat_batch_get_state(const asat_batch_t* b, int index, float* pub);
 int asat_batch_stats(const asat_batch_t* b, double* run_ms, int* chunks, int* nbuckets);
 int asat_batch_bucket_info(const asat_batch_t* b, int bucket, asat_batch_bucket_t* out);
+
+/* ---------------- discrete adjoint (no reference counterpart) --------------
+ * Gradient of V(T) = sum_m a_m K_m^2 after T accepted steps of the FP32-state
+ * integrator, with respect to the start state [s(0), a(0)], for a batch of
+ * trajectories of one instance (k = 3 or 4).  Forward: adaptive, step sizes
+ * logged, checkpoint every C steps.  Backward: recompute from checkpoints,
+ * discrete adjoint of the realised steps in FP64. */
+typedef struct asat_adj asat_adj_t;
+int asat_adj_create(asat_adj_t** out);
+int asat_adj_destroy(asat_adj_t* a);
+int asat_adj_set_problem(asat_adj_t* a, const int* literals, const int* lengths, int nclauses);
+/* eps as CudaRungeKutta::SetEpsilon; steps T; checkpoint interval C */
+int asat_adj_set_params(asat_adj_t* a, double eps, int steps, int checkpoint_every);
+/* adaptive forward of ntraj trajectories (pub0: ntraj x (n+m) floats [s, a]);
+   V_out[ntraj] = V(T) */
+int asat_adj_forward(asat_adj_t* a, int ntraj, const float* pub0, double* V_out);
+/* accepted step sizes of the last forward run, ntraj x T */
+int asat_adj_get_steps(const asat_adj_t* a, double* h_out);
+/* grad_out[ntraj x (n+m)] = dV(T)/d[s(0), a(0)] of the last forward run (FP64) */
+int asat_adj_backward(asat_adj_t* a, double* grad_out);
+/* replay T steps with given step sizes h (ntraj x T), FP32 like the forward:
+   V_out[ntraj] and optionally the end states xT_out[ntraj x (n+m)] */
+int asat_adj_replay(asat_adj_t* a, int ntraj, const float* pub0, const double* h, double* V_out, float* xT_out);
+/* device milliseconds of the last forward / backward */
+int asat_adj_timing(const asat_adj_t* a, double* forward_ms, double* backward_ms);
 
 #ifdef __cplusplus
 }

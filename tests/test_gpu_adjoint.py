@@ -1,0 +1,110 @@
+"""GPU discrete adjoint vs the oracle (C ABI through the Python mirror).
+
+* forward: logged step sizes == the oracle's mixed-mode adaptive run, bit for bit
+* replay with the logged steps reproduces V(T) and x(T) of the forward exactly
+* gradient == the oracle's FP32-forward / FP64-adjoint on the same steps
+  (max relative difference <= 1e-12; the only difference is the order of the
+  final V tree-sum, not used by the gradient)
+* gradient vs central finite differences of the FP64 oracle with the same
+  frozen step sequence (h = 1e-4): agreement within the FP32-forward error
+* the checkpoint interval does not change the gradient
+* torch.autograd.Function returns the C ABI gradient
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import analogsat_amd as A
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print("[gpu-adjoint]", msg, flush=True)
+    sys.stdout.flush()
+
+
+def setup(n=60, m=255, k=3, seed=7, ntraj=4):
+    p = O.fastsat(n, m, k, seed)
+    rng = np.random.default_rng(seed)
+    pubs = np.stack([np.concatenate([rng.uniform(-1, 1, n), rng.uniform(1, 2, m)]) for _ in range(ntraj)])
+    return p, pubs.astype(np.float32)
+
+
+def test_adjoint_matches_oracle_and_replay():
+    say("adjoint N=60 T=40 C=8, 4 trajectories")
+    p, pubs = setup()
+    T = 40
+    adj = A.HipAdjoint(A.SatProblem(p.literals, p.lengths), eps=1e-6, steps=T, checkpoint_every=8)
+    V = adj.forward(pubs)
+    h = adj.steps()
+    g = adj.backward()
+    with O.pow_mode("cr"):
+        for t in range(len(pubs)):
+            rk = O.RK(p, eps=1e-6, mode="mix")
+            Vo, go, ho = rk.adjoint(pubs[t], T)
+            assert np.array_equal(h[t], ho), t
+            assert V[t] == pytest.approx(Vo, rel=1e-13)
+            scale = np.abs(go).max()
+            assert np.abs(g[t] - go).max() <= 1e-12 * scale, (t, np.abs(g[t] - go).max() / scale)
+    Vr, xT = adj.replay(pubs, h)
+    assert np.array_equal(Vr, V)
+    for t in range(len(pubs)):
+        rk = O.RK(p, eps=1e-6, mode="mix")
+        rk.set_state(pubs[t])
+        with O.pow_mode("cr"):
+            rk.step(T)
+        assert np.array_equal(xT[t], rk.get_state())
+
+
+def test_checkpoint_interval_invariance():
+    say("checkpoint interval invariance")
+    p, pubs = setup(seed=3, ntraj=2)
+    grads = []
+    for C in (1, 7, 64):
+        adj = A.HipAdjoint(A.SatProblem(p.literals, p.lengths), eps=1e-6, steps=30, checkpoint_every=C)
+        adj.forward(pubs)
+        grads.append(adj.backward())
+    assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2])
+
+
+def test_gradient_vs_fd_of_fp64_frozen_map():
+    say("FD of the FP64 frozen map")
+    n, m, T = 50, 212, 30
+    p, pubs = setup(n, m, 3, seed=11, ntraj=1)
+    adj = A.HipAdjoint(A.SatProblem(p.literals, p.lengths), eps=1e-6, steps=T, checkpoint_every=16)
+    adj.forward(pubs)
+    h = adj.steps()[0]
+    g = adj.backward()[0]
+    rk = O.RK(p, eps=1e-6, mode="f64")
+    x0 = pubs[0].astype(np.float64)
+    rng = np.random.default_rng(0)
+    errs = []
+    for i in rng.choice(n + m, 32, replace=False):
+        e = np.zeros(n + m)
+        e[i] = 1e-4
+        fd = (rk.frozen_energy(x0 + e, h) - rk.frozen_energy(x0 - e, h)) / 2e-4
+        errs.append(abs(g[i] - fd))
+    scale = np.abs(g).max()
+    say("max |grad - FD| / max|grad| = %.2e" % (max(errs) / scale))
+    assert max(errs) <= 1e-3 * scale
+
+
+def test_torch_autograd_function():
+    torch = pytest.importorskip("torch")
+    say("torch.autograd.Function")
+    n, m = 40, 170
+    p, pubs = setup(n, m, 3, seed=5, ntraj=3)
+    adj = A.HipAdjoint(A.SatProblem(p.literals, p.lengths), eps=1e-6, steps=20, checkpoint_every=4)
+    s0 = torch.tensor(pubs[:, :n], dtype=torch.float32, requires_grad=True)
+    a0 = torch.tensor(pubs[:, n:], dtype=torch.float32, requires_grad=True)
+    f = A.energy_function()
+    V = f.apply(s0, a0, adj)
+    (V * torch.tensor([1.0, 2.0, -0.5], dtype=torch.float64)).sum().backward()
+    adj2 = A.HipAdjoint(A.SatProblem(p.literals, p.lengths), eps=1e-6, steps=20, checkpoint_every=4)
+    adj2.forward(pubs)
+    g = adj2.backward() * np.array([1.0, 2.0, -0.5])[:, None]
+    np.testing.assert_allclose(s0.grad.numpy(), g[:, :n].astype(np.float32), rtol=1e-6)
+    np.testing.assert_allclose(a0.grad.numpy(), g[:, n:].astype(np.float32), rtol=1e-6)
