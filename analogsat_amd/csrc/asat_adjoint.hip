@@ -585,15 +585,17 @@ int asat_adj_forward(asat_adj_t* a, int ntraj, const float* pub0, double* V_out)
         chk(asat_state_set_public(so, pub0 + (size_t)t * (A.n + A.m)));
         chk(asat_rk_set_step_log(rk, A.hlog + (size_t)t * A.T, A.T));
         chk(asat_rk_reset(rk));
-        for (int c = 0; c * A.C <= A.T; c++) {
+        // checkpoint c holds x at step c*C; the last one (c = ceil(T/C)) holds
+        // x_T, also when the final segment is shorter than C
+        for (int c = 0;; c++) {
             void* dp = nullptr;
             int aoff = 0;
             chk(asat_state_device_ptr(so, &dp, &aoff));
             hipLaunchKernelGGL(k_from_plain, dim3(ceil_div((long long)A.n + A.m + 1, BLK)), dim3(BLK), 0, A.stream,
                                (const float*)dp, A.n, A.m, aoff, A.ckpt + (size_t)c * V + (size_t)t * A.D);
             HIP_CHECK(hipStreamSynchronize(A.stream));
+            if ((long long)c * A.C >= A.T) break;
             const int todo = std::min(A.C, A.T - c * A.C);
-            if (todo <= 0) break;
             int rc = asat_rk_step(rk, so, sn, todo);
             if (rc == ASAT_UNDERFLOW) throw asat::Error(ASAT_UNDERFLOW, "step size underflow in the forward run");
             chk(rc);

@@ -62,12 +62,14 @@ def test_adjoint_matches_oracle_and_replay():
 def test_checkpoint_interval_invariance():
     say("checkpoint interval invariance")
     p, pubs = setup(seed=3, ntraj=2)
-    grads = []
-    for C in (1, 7, 64):
+    grads, Vs = [], []
+    for C in (1, 7, 30, 64):      # 7: a short last segment; 64 > T: a single one
         adj = A.HipAdjoint(A.SatProblem(p.literals, p.lengths), eps=1e-6, steps=30, checkpoint_every=C)
-        adj.forward(pubs)
+        Vs.append(adj.forward(pubs))
         grads.append(adj.backward())
-    assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2])
+    for i in range(1, len(grads)):
+        assert np.array_equal(Vs[0], Vs[i]), i
+        assert np.array_equal(grads[0], grads[i]), i
 
 
 def test_gradient_vs_fd_of_fp64_frozen_map():

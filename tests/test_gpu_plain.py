@@ -243,3 +243,30 @@ def test_trajectory_multi_group_bitexact(precision):
             st = rk.stats()
             assert (st.hdid, st.attempts, st.rhscount) == (orc.hdid, orc.attempts, orc.rhscount)
     assert sat.GetClauseViolationCount(old) == orc.violations()
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_trajectory_with_error_floor_bitexact(golden_dir, precision):
+    """A raised xscal floor (SetErrorFloor) changes the error norm the same
+    way on the GPU as in the oracle: 300 steps bit-identical."""
+    say("trajectory %s C1 floor=1e-3 300 steps" % precision)
+    p, g = _c1(golden_dir)
+    mode = "f64" if precision == "f64" else "mix"
+    dt = np.float64 if precision == "f64" else np.float32
+    sat = _sys_with(p.literals, p.lengths, precision)
+    rk = A.HipRungeKutta()
+    rk.SetEpsilon(1e-6)
+    rk.SetErrorFloor(1e-3)
+    rk.Configure(sat)
+    old, new = sat.MakeState(), sat.MakeState()
+    pub = g["pub0"].astype(dt)
+    old.CopyFrom(pub)
+    with O.pow_mode("cr"):
+        orc = O.RK(p, eps=1e-6, mode=mode, floor=1e-3)
+        orc.set_state(pub)
+        for blk in range(30):
+            rk.Step(old, new, 10)
+            orc.step(10)
+            assert np.array_equal(old.CopyTo(), orc.get_state()), blk
+            st = rk.stats()
+            assert (st.hdid, st.time, st.attempts) == (orc.hdid, orc.time, orc.attempts), blk

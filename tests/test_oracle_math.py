@@ -212,3 +212,47 @@ def test_pow_mode_only_changes_controller(golden_dir):
     # the first 12 steps never call pow (5x growth); afterwards a difference may
     # only come from a glibc misrounding, which is rare
     assert first is None or first > 12
+
+
+
+# Cash & Karp (1990) embedded 4(5) tableau, written out independently of the
+# oracle's C code (which follows AnalogSAT/cpu_sat/CpuRungeKutta.cpp:180-215).
+_CK_B = [[], [1 / 5], [3 / 40, 9 / 40], [3 / 10, -9 / 10, 6 / 5], [-11 / 54, 5 / 2, -70 / 27, 35 / 27],
+         [1631 / 55296, 175 / 512, 575 / 13824, 44275 / 110592, 253 / 4096]]
+_CK_C5 = np.array([37 / 378, 0, 250 / 621, 125 / 594, 0, 512 / 1771])
+_CK_C4 = np.array([2825 / 27648, 0, 18575 / 48384, 13525 / 55296, 277 / 14336, 1 / 4])
+
+
+def _ck_errmax(p, x, h, h_scale, eps, floor):
+    """max |xerr / xscal| / eps of one Cash-Karp attempt of size h from x, with
+    xscal = |x| + |h_scale k1| + floor (h_scale: the step size at step start)."""
+    ks = []
+    for row in _CK_B:
+        y = x + h * sum(b * k for b, k in zip(row, ks)) if row else x
+        ks.append(O.rhs(p, y))
+    err = h * sum((c5 - c4) * k for c5, c4, k in zip(_CK_C5, _CK_C4, ks))
+    scale = np.abs(x) + np.abs(h_scale * ks[0]) + floor
+    return np.max(np.abs(err / scale)) / eps
+
+
+@pytest.mark.parametrize("floor", [1e-20, 1e-6, 1e-3, 0.5])
+def test_error_scale_floor(floor):
+    """The constant term of xscal (TINY = 1e-20 in
+    AnalogSAT/cpu_sat/CpuRungeKutta.cpp:67-76) enters the error norm as
+    |x| + |h k1| + floor: the accepted attempt's errmax equals an independent
+    Cash-Karp evaluation, and raising the floor can only lower it."""
+    p = O.fastsat(60, 255, 3, 5)
+    pub = O.random_state_f64(2, p.n, p.m)
+    pub[: p.n : 3] = 0.0          # spins at zero: there xscal is |h k1| + floor
+    rk = O.RK(p, eps=1e-6, floor=floor)
+    rk.set_state(pub)
+    checked = 0
+    for _ in range(14):
+        x0 = np.concatenate([[-1.0], rk.get_state()])
+        h_scale, att = rk.hnext, rk.attempts
+        rk.step(1)
+        want = _ck_errmax(p, x0, rk.hdid, h_scale, 1e-6, floor)
+        assert rk.errmax == pytest.approx(want, rel=1e-9, abs=1e-300)
+        assert _ck_errmax(p, x0, rk.hdid, h_scale, 1e-6, floor * 100) <= want
+        checked += rk.attempts - att
+    assert rk.hdid > 1e-4 and checked >= 14
