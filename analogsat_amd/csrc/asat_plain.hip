@@ -3,25 +3,35 @@
 // (AnalogSAT/cuda_sat/CudaSat3.cu:33-430, AnalogSAT/cuda_sat/CudaRungeKutta.cu:56-484).
 //
 // Data layout in HBM (T = double for ASAT_F64, float for ASAT_F32):
-//   lit[m*K + q]      int32 signed literal (+v / -v, 0 = padding), clause-major
-//   pos[m*K + q]      where the slot's contribution is stored (-1: padding)
-//   contrib[p]        per-slot contribution 2 c a K K_mi of the current stage,
-//                     grouped by variable group (8192 float / 4096 double
-//                     variables -> ~400 KB regions), clause order inside a region
+//   lit[m*K + q]      int32 signed literal (+v / -v, 0 = padding), clause-major,
+//                     padded to whole tiles of TC clauses
+//   contrib[p]        per-slot contribution 2 c a K K_mi of the current stage.
+//                     Variables are split into groups (about 2048 float / 1024
+//                     double variables); a group's contributions form one
+//                     region (64-element aligned, at most 144 KB), in slot
+//                     (clause) order
+//   spos, sord        per tile: the tile's slots in ascending contribution
+//                     position (int32 position; uint16 tile-local slot with the
+//                     literal's sign in bit 15)
 //   csr_ptr[0..N+1]   var-major CSR row pointers
-//   csr_pos[e]        contribution positions of each variable, in slot order
+//   csr_loc[e]        uint16 offset of each of a variable's contributions in
+//                     its group's region, in slot order
 //   state buffer      [ -1, s_1..s_N, pad | a_1..a_M, pad ], aux at aoff
 //   k1..k5            same layout as the state (stage derivatives)
 //   tmp               spin part of the next stage's input (aux stage inputs are
 //                     recomputed inside the clause pass and never stored)
+// Problems outside the fast paths use a generic clause pass (K other than 3
+// and 4: pos[m*K + q] per slot) and a gather var pass (a group whose region
+// exceeds the LDS: csr_pos global positions, ~400 KB groups).
 //
 // One RK45 attempt = 6 x (clause pass + var pass) + 1 control kernel:
-//   clause pass J: gather the stage-J spins, K_m, aux stage input and
-//                  k_J(a_m) = a K^2, per-slot contributions (coalesced writes)
-//   var pass J:    deterministic per-variable sum over the CSR (slot order ==
-//                  the reference's accumulation order), k_J(s_i), and the
-//                  spin part of stage J+1's input; the blocks of a variable
-//                  group run on one XCD, so the gathers hit that XCD's L2
+//   clause pass J: per tile, gather the stage-J spins, K_m, aux stage input
+//                  and k_J(a_m) = a K^2, then store the tile's contributions
+//                  in position order (one run per region and tile)
+//   var pass J:    one workgroup per group reads the group's region into LDS
+//                  with coalesced loads, adds each variable's contributions
+//                  in slot order (== the reference's accumulation order),
+//                  then k_J(s_i) and the spin part of stage J+1's input
 //   J = 6 passes also form x_new and |xerr/xscal| and a per-block FP64 max;
 //   the control kernel finishes the max-norm and runs the Cash-Karp
 //   accept/reject/step-size logic in FP64 on the device.
@@ -50,21 +60,34 @@ static inline long long round_up(long long a, long long b) { return (a + b - 1) 
 // ===========================================================================
 // problem (host)
 // ===========================================================================
+constexpr int TC = 1024;                  // clauses per tile of the tiled clause pass (one workgroup)
+constexpr int TILE_BT = 1024;             // threads per tile workgroup (a clause each)
+constexpr int VBLK = 1024;                // var-pass workgroup (one variable group)
+constexpr int VAR_LDS_BYTES = 147456;     // LDS holding one group's contribution region
+constexpr int REGION_ALIGN = 64;          // region starts, in elements
+
 struct HostProblem {
     int n = 0, m = 0, k = 0;
-    int vg = 0;                 // variables per group (one contiguous contribution region each)
-    int ngroups = 0;
-    std::vector<int> lit;       // m*k, zero padded
-    std::vector<int> pos;       // m*k: position of the slot's contribution, -1 for padding
-    std::vector<int> csr_ptr;   // n+2
-    std::vector<int> csr_pos;   // contribution positions of each variable, in slot order
+    std::vector<int> lit;          // m*k, zero padded, clause-major
+    // contribution layout: variables split into groups of vg; the
+    // contributions of a group's variables form one region, in slot (clause)
+    // order; region starts are REGION_ALIGN-element aligned
+    int vg = 0, ngroups = 0;
+    long long max_region = 0;      // largest region incl. alignment padding
+    std::vector<long long> gstart; // ngroups+1
+    std::vector<int> pos;          // m*k: the slot's contribution position, -1 for padding
+    std::vector<int> csr_ptr;      // n+2
+    std::vector<int> csr_pos;      // contribution positions of each variable, in slot order
+    std::vector<uint16_t> csr_loc; // the same relative to the variable's region start
+    // tiled clause pass: per tile of TC clauses, its slots sorted by position
+    std::vector<int> spos;         // ntiles*TC*k positions (-1 past the tile's last slot)
+    std::vector<uint16_t> sord;    // tile-local slot index (clause*k + q) of each sorted entry,
+                                   // bit 15 set for a negative literal
 };
 
 // SatProblem::AddClauses semantics (AnalogSAT/problem/SatProblem.cpp:64-95) +
-// CpuSat::InitProblem clause matrix (AnalogSAT/cpu_sat/CpuSat.cpp:352-377) +
-// the transposed var-major slot list (role of CudaSat3::InitProblem,
-// AnalogSAT/cuda_sat/CudaSat3.cu:266-307, without the pad-to-32).
-static void build_problem(const int* literals, const int* lengths, int nclauses, int vg, HostProblem& P)
+// CpuSat::InitProblem clause matrix (AnalogSAT/cpu_sat/CpuSat.cpp:352-377).
+static void parse_clauses(const int* literals, const int* lengths, int nclauses, HostProblem& P)
 {
     require(nclauses >= 0, ASAT_ERR_INVALID_ARGUMENT, "negative clause count");
     require(nclauses == 0 || (literals && lengths), ASAT_ERR_INVALID_ARGUMENT, "null clause arrays");
@@ -85,7 +108,7 @@ static void build_problem(const int* literals, const int* lengths, int nclauses,
     }
     require(m > 0 && k > 0, ASAT_ERR_INVALID_ARGUMENT, "empty SAT problem");
     require(k <= GEN_KMAX, ASAT_ERR_UNSUPPORTED, "clauses longer than 32 literals are not supported");
-    require((long long)m * k < (1LL << 31), ASAT_ERR_UNSUPPORTED, "more than 2^31 literal slots");
+    require((long long)m * k < (1LL << 31) - (long long)TC * k, ASAT_ERR_UNSUPPORTED, "more than 2^31 literal slots");
     P.n = n; P.m = m; P.k = k;
     P.lit.assign((size_t)m * k, 0);
     off = 0;
@@ -100,19 +123,31 @@ static void build_problem(const int* literals, const int* lengths, int nclauses,
         if (q > 0) row++;
         off += (size_t)lengths[j];
     }
-    // Contribution layout: variables are split into groups of vg; the
-    // contributions of a group's variables form one contiguous region, in
-    // slot (clause) order.  pos[slot] is the slot's place in that layout.
+}
+
+// Contribution layout for groups of vg variables, and the transposed
+// var-major slot list (role of CudaSat3::InitProblem,
+// AnalogSAT/cuda_sat/CudaSat3.cu:266-307, without the pad-to-32).
+static void build_layout(HostProblem& P, int vg)
+{
+    const int n = P.n;
     P.vg = vg;
     P.ngroups = (n + vg - 1) / vg;
-    std::vector<long long> gcount((size_t)P.ngroups + 1, 0);
+    std::vector<long long> gcount((size_t)P.ngroups, 0);
     for (size_t s = 0; s < P.lit.size(); s++) {
         int l = P.lit[s];
-        if (l) gcount[(size_t)(std::abs(l) - 1) / vg + 1]++;
+        if (l) gcount[(size_t)(std::abs(l) - 1) / vg]++;
     }
-    for (int g = 0; g < P.ngroups; g++) gcount[g + 1] += gcount[g];
+    P.gstart.assign((size_t)P.ngroups + 1, 0);
+    P.max_region = 0;
+    for (int g = 0; g < P.ngroups; g++) {
+        const long long sz = round_up(std::max<long long>(gcount[g], 1), REGION_ALIGN);
+        P.gstart[g + 1] = P.gstart[g] + sz;
+        P.max_region = std::max(P.max_region, sz);
+    }
+    require(P.gstart[P.ngroups] < (1LL << 31), ASAT_ERR_UNSUPPORTED, "contribution array exceeds 2^31 entries");
     P.pos.assign(P.lit.size(), -1);
-    std::vector<long long> gfill(gcount.begin(), gcount.end() - 1);
+    std::vector<long long> gfill(P.gstart.begin(), P.gstart.end() - 1);
     for (size_t s = 0; s < P.lit.size(); s++) {
         int l = P.lit[s];
         if (l) P.pos[s] = (int)gfill[(size_t)(std::abs(l) - 1) / vg]++;
@@ -131,6 +166,52 @@ static void build_problem(const int* literals, const int* lengths, int nclauses,
         int l = P.lit[s];
         if (l) P.csr_pos[(size_t)fill[std::abs(l)]++] = P.pos[s];
     }
+    P.csr_loc.clear();
+    if (P.max_region <= 65536) {
+        P.csr_loc.resize(P.csr_pos.size());
+        for (int i = 1; i <= n; i++) {
+            const long long base = P.gstart[(size_t)(i - 1) / vg];
+            for (int e = P.csr_ptr[i]; e < P.csr_ptr[i + 1]; e++) P.csr_loc[e] = (uint16_t)(P.csr_pos[e] - base);
+        }
+    }
+}
+
+// Per tile of TC clauses: the tile's slots in ascending position order.
+// Positions inside a region follow slot order, so this is a stable counting
+// sort of the tile's slots by variable group; the clause pass then writes
+// each group's run of contributions with neighbouring lanes.
+static void build_tile_order(HostProblem& P)
+{
+    const int K = P.k;
+    const long long ntiles = ceil_div((long long)P.m, TC);
+    const size_t per = (size_t)TC * K;
+    P.spos.assign((size_t)ntiles * per, -1);
+    P.sord.assign((size_t)ntiles * per, 0);
+    std::vector<int> cnt((size_t)P.ngroups + 1, 0);
+    std::vector<int> touched;
+    touched.reserve(per);
+    for (long long t = 0; t < ntiles; t++) {
+        const size_t s0 = (size_t)t * per, s1 = std::min(P.lit.size(), s0 + per);
+        touched.clear();
+        for (size_t s = s0; s < s1; s++) {
+            int l = P.lit[s];
+            if (!l) continue;
+            int g = (std::abs(l) - 1) / P.vg;
+            if (cnt[g]++ == 0) touched.push_back(g);
+        }
+        std::sort(touched.begin(), touched.end());
+        int acc = 0;
+        for (int g : touched) { int c = cnt[g]; cnt[g] = acc; acc += c; }
+        for (size_t s = s0; s < s1; s++) {
+            int l = P.lit[s];
+            if (!l) continue;
+            int g = (std::abs(l) - 1) / P.vg;
+            const size_t j = s0 + (size_t)cnt[g]++;
+            P.spos[j] = P.pos[s];
+            P.sord[j] = (uint16_t)((s - s0) | (l < 0 ? 0x8000u : 0u));
+        }
+        for (int g : touched) cnt[g] = 0;
+    }
 }
 
 // ===========================================================================
@@ -139,9 +220,13 @@ static void build_problem(const int* literals, const int* lengths, int nclauses,
 template <typename T>
 struct StageArgs {
     const int* lit;
-    const int* pos;        // contribution position of each slot (-1: padding)
+    const int* pos;        // contribution position of each slot (-1: padding); generic-K pass
+    const int* spos;       // tiled pass: per tile, positions in ascending order (-1: none)
+    const uint16_t* sord;  // tiled pass: tile-local slot of each sorted entry
     const int* csr_ptr;
-    const int* csr_pos;    // per-variable contribution positions
+    const int* csr_pos;    // per-variable contribution positions (gather var pass)
+    const uint16_t* csr_loc;   // the same relative to the region start (LDS var pass)
+    const long long* gstart;   // region starts, ngroups+1
     T* contrib;
     T* xa;       // state buffer 0
     T* xb;       // state buffer 1 (x = parity ? xb : xa; x_new is the other)
@@ -150,10 +235,12 @@ struct StageArgs {
     double* partial;   // per-block error maxima: clause blocks first, then var blocks
     const Ctrl* ctrl;  // null: standalone RHS (x = xa, output to k1, no tmp)
     int n, m, k, aoff;
-    int nblk_clause;
-    int vg_blocks;     // var-pass blocks per variable group
+    int nblk_clause;   // clause-pass grid (its J = 6 error partials come first)
+    int vg;            // variables per group
+    int vg_blocks;     // gather var pass: blocks per variable group
     int ngroups;
-    int clause_per_xcd;   // clause-pass chunks (of BLK clauses) per XCD
+    int clause_per_xcd;   // clause-pass chunks (BLK clauses, or TC-clause tiles) per XCD
+    int ntiles;           // tiled pass: tiles of TC clauses
     T knorm;
 };
 
@@ -168,19 +255,25 @@ __device__ __forceinline__ int clause_of_thread(int per)
     return ((b & 7) * per + (b >> 3)) * BLK + threadIdx.x;
 }
 
-// XCD-affine var-pass mapping: workgroups are dispatched to the 8 XCDs
-// round-robin, so block b runs on XCD b % 8.  All vg_blocks blocks of one
-// variable group are given to the same XCD, and each XCD walks its groups in
-// order, so a group's contribution region stays in that XCD's L2 while its
-// variables gather from it.  Returns the variable index (1-based) or 0.
-__device__ __forceinline__ int var_of_block(int vg_blocks, int ngroups, int n)
+// XCD-affine var-pass mapping for the gather var pass: workgroups are
+// dispatched to the 8 XCDs round-robin, so block b runs on XCD b % 8.  All
+// vg_blocks blocks of one variable group are given to the same XCD, and each
+// XCD walks its groups in order, so a group's contribution region stays in
+// that XCD's L2 while its variables gather from it.  Returns the block's
+// first variable (1-based) and last variable, or 0 for an idle block.
+__device__ __forceinline__ int var_block_range(int vg, int vg_blocks, int ngroups, int n, int& last)
 {
     const int b = blockIdx.x;
     const int xcd = b & 7, t = b >> 3;
     const int grp = (t / vg_blocks) * 8 + xcd;
+    last = 0;
     if (grp >= ngroups) return 0;
-    const long long i = (long long)grp * vg_blocks * BLK + (long long)(t % vg_blocks) * BLK + threadIdx.x + 1;
-    return i <= n ? (int)i : 0;
+    const long long g0 = (long long)grp * vg + 1;
+    const long long i0 = g0 + (long long)(t % vg_blocks) * BLK;
+    const long long gl = std::min<long long>(g0 + vg - 1, n);
+    if (i0 > gl) return 0;
+    last = (int)std::min<long long>(i0 + BLK - 1, gl);
+    return (int)i0;
 }
 
 // streaming loads/stores bypass the caches' normal retention so that the
@@ -196,11 +289,22 @@ __device__ __forceinline__ void st_nt(V* p, V v)
     __builtin_nontemporal_store(v, p);
 }
 
-// ---- clause pass, compile-time K (Ex order, K <= 10) ---------------------
-template <typename T, int K, int J>
-__global__ __launch_bounds__(BLK) void k_clause(StageArgs<T> a)
+// ---- clause pass, compile-time K (Ex order), tiled -----------------------
+// One workgroup of BT threads per tile of TC clauses (XCD-contiguous tile
+// ranges).  Phase 1, a thread per clause (TC / BT clauses each): load every
+// literal, stage spin and aux value of the thread's clauses first, so that
+// the gathers are in flight together, then form K_m, the aux stage input and
+// k_J(a_m), and keep the factors and r = K_m 2^-k a_m in LDS.  Phase 2, a
+// thread per slot in ascending position order: form the slot's contribution
+// from LDS and store it, so that neighbouring lanes write neighbouring
+// positions of one region.
+template <typename T, int K, int J, int BT>
+__global__ __launch_bounds__(BT) void k_clause_tile(StageArgs<T> a)
 {
-    __shared__ double sh[BLK / 64];
+    constexpr int CPT = TC / BT;
+    __shared__ double sh[BT / 64];
+    __shared__ T s_f[TC * K];
+    __shared__ T s_r[TC];
     const Ctrl* c = a.ctrl;
     int parity = 0;
     double h = 0.0, hnext = 0.0, floor = TabD::TINY;
@@ -213,73 +317,103 @@ __global__ __launch_bounds__(BLK) void k_clause(StageArgs<T> a)
     const T* x = parity ? a.xb : a.xa;
     T* xnew = parity ? a.xa : a.xb;
     const T* sp = (J == 1) ? x : a.tmp;
-    const int m = clause_of_thread(a.clause_per_xcd);
-    // this block's tile of BLK clauses: literals and positions, read once with
-    // coalesced 16-byte streaming loads (arrays are padded to whole tiles)
-    __shared__ int s_lit[BLK * K], s_pos[BLK * K];
-    {
-        const int tile0 = m - (int)threadIdx.x;
-        if (tile0 < a.m) {
-            typedef int v4i __attribute__((ext_vector_type(4)));
-            const v4i* L4 = (const v4i*)(a.lit + (size_t)tile0 * K);
-            const v4i* P4 = (const v4i*)(a.pos + (size_t)tile0 * K);
-            for (int v = threadIdx.x; v < BLK * K / 4; v += BLK) {
-                v4i w = ld_nt(L4 + v), z = ld_nt(P4 + v);
-                s_lit[4 * v] = w.x; s_lit[4 * v + 1] = w.y; s_lit[4 * v + 2] = w.z; s_lit[4 * v + 3] = w.w;
-                s_pos[4 * v] = z.x; s_pos[4 * v + 1] = z.y; s_pos[4 * v + 2] = z.z; s_pos[4 * v + 3] = z.w;
+    const int b = blockIdx.x;
+    const int tile = (b & 7) * a.clause_per_xcd + (b >> 3);
+    double err = 0.0;
+    if (tile < a.ntiles) {
+        const long long c0 = (long long)tile * TC;
+        int l[CPT][K];
+        T sv[CPT][K];
+        T ax[CPT], a1[CPT], a2[CPT], a3[CPT], a4[CPT], a5[CPT];
+        // literal rows are padded to whole tiles, so they are read unguarded
+#pragma unroll
+        for (int u = 0; u < CPT; u++) {
+            const long long m = c0 + u * BT + threadIdx.x;
+#pragma unroll
+            for (int q = 0; q < K; q++) l[u][q] = ld_nt(a.lit + m * K + q);
+        }
+#pragma unroll
+        for (int u = 0; u < CPT; u++) {
+#pragma unroll
+            for (int q = 0; q < K; q++) {
+                const int v = l[u][q] < 0 ? -l[u][q] : l[u][q];
+                sv[u][q] = v != 0 ? sp[v] : (T)-1.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < CPT; u++) {
+            const long long m = c0 + u * BT + threadIdx.x;
+            const size_t ai = (size_t)a.aoff + m;
+            ax[u] = a1[u] = a2[u] = a3[u] = a4[u] = a5[u] = (T)0;
+            if (m < a.m) {
+                ax[u] = ld_nt(x + ai);
+                if constexpr (J > 1) a1[u] = ld_nt(a.k1 + ai);
+                if constexpr (J > 2) a2[u] = ld_nt(a.k2 + ai);
+                if constexpr (J > 3) a3[u] = ld_nt(a.k3 + ai);
+                if constexpr (J > 4) a4[u] = ld_nt(a.k4 + ai);
+                if constexpr (J > 5) a5[u] = ld_nt(a.k5 + ai);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < CPT; u++) {
+            const int cl = u * BT + threadIdx.x;
+            const long long m = c0 + cl;
+            if (m < a.m) {
+                T f[K];
+                T km = a.knorm;
+#pragma unroll
+                for (int q = 0; q < K; q++) {
+                    f[q] = l[u][q] < 0 ? (T)1.0 + sv[u][q] : (T)1.0 - sv[u][q];
+                    km *= f[q];
+                }
+                const size_t ai = (size_t)a.aoff + m;
+                T am;
+                if constexpr (J == 1) am = ax[u];
+                else am = stage_input<T, J>(ax[u], a1[u], a2[u], a3[u], a4[u], a5[u], hs);
+                T kj = am * km * km;
+                if constexpr (J == 1) st_nt(a.k1 + ai, kj);
+                else if constexpr (J == 2) st_nt(a.k2 + ai, kj);
+                else if constexpr (J == 3) st_nt(a.k3 + ai, kj);
+                else if constexpr (J == 4) st_nt(a.k4 + ai, kj);
+                else if constexpr (J == 5) st_nt(a.k5 + ai, kj);
+                else {
+                    T xn;
+                    double e;
+                    final_combination<T>(ax[u], a1[u], a3[u], a4[u], a5[u], kj, hs, h, hnext, floor, xn, e);
+                    err = fmax(err, e);
+                    st_nt(xnew + ai, xn);
+                }
+                s_r[cl] = km * a.knorm * am;
+#pragma unroll
+                for (int q = 0; q < K; q++) s_f[cl * K + q] = f[q];
             }
         }
         __syncthreads();
-    }
-    double err = 0.0;
-    if (m < a.m) {
-        int l[K];
-        T f[K];
-        int ps[K];
+        const int* sp_t = a.spos + c0 * K;
+        const uint16_t* so_t = a.sord + c0 * K;
+        for (int j = threadIdx.x; j < TC * K; j += BT) {
+            const int p = ld_nt(sp_t + j);
+            if (p < 0) break;           // sorted entries form a prefix of the tile's list
+            const unsigned w = ld_nt(so_t + j);
+            const int o = (int)(w & 0x7FFFu);
+            const int cl = o / K, q = o - cl * K;
+            // product of the clause's other factors, left to right (KmiProd order)
+            T prod = (T)1.0;
+            bool first = true;
 #pragma unroll
-        for (int q = 0; q < K; q++) l[q] = s_lit[threadIdx.x * K + q];
-#pragma unroll
-        for (int q = 0; q < K; q++) ps[q] = s_pos[threadIdx.x * K + q];
-        T km = a.knorm;
-#pragma unroll
-        for (int q = 0; q < K; q++) {
-            f[q] = spin_factor<T>(l[q], sp);
-            km *= f[q];
+            for (int qq = 0; qq < K; qq++) {
+                if (qq == q) continue;
+                const T fv = s_f[cl * K + qq];
+                prod = first ? fv : prod * fv;
+                first = false;
+            }
+            T rr = s_r[cl] * prod;
+            rr *= (w & 0x8000u) ? (T)-2.0 : (T)2.0;
+            a.contrib[p] = rr;
         }
-        const size_t ai = (size_t)a.aoff + m;
-        T am, xa = ld_nt(x + ai), k1 = 0, k3 = 0, k4 = 0, k5 = 0;
-        if constexpr (J == 1) {
-            am = xa;
-        } else {
-            k1 = ld_nt(a.k1 + ai);
-            T k2 = J > 2 ? ld_nt(a.k2 + ai) : (T)0;
-            k3 = J > 3 ? ld_nt(a.k3 + ai) : (T)0;
-            k4 = J > 4 ? ld_nt(a.k4 + ai) : (T)0;
-            k5 = J > 5 ? ld_nt(a.k5 + ai) : (T)0;
-            am = stage_input<T, J>(xa, k1, k2, k3, k4, k5, hs);
-        }
-        T kj = am * km * km;
-        if constexpr (J == 1) st_nt(a.k1 + ai, kj);
-        else if constexpr (J == 2) st_nt(a.k2 + ai, kj);
-        else if constexpr (J == 3) st_nt(a.k3 + ai, kj);
-        else if constexpr (J == 4) st_nt(a.k4 + ai, kj);
-        else if constexpr (J == 5) st_nt(a.k5 + ai, kj);
-        else {
-            T xn;
-            final_combination<T>(xa, k1, k3, k4, k5, kj, hs, h, hnext, floor, xn, err);
-            st_nt(xnew + ai, xn);
-        }
-        const T r = km * a.knorm * am;
-        T out[K];
-#pragma unroll
-        for (int q = 0; q < K; q++) out[q] = (T)0;
-        write_contribs<T, K, 0>(l, f, r, out);
-#pragma unroll
-        for (int q = 0; q < K; q++)
-            if (l[q] != 0) a.contrib[ps[q]] = out[q];
     }
     if constexpr (J == 6) {
-        double bm = block_max<BLK>(err, sh);
+        double bm = block_max<BT>(err, sh);
         if (threadIdx.x == 0) a.partial[blockIdx.x] = bm;
     }
 }
@@ -371,8 +505,94 @@ __global__ __launch_bounds__(BLK) void k_clause_gen(StageArgs<T> a)
     }
 }
 
-// ---- var pass: deterministic gather-sum + spin stage combination ---------
-// A block owns BLK consecutive variables of one group.  Their CSR entries form
+// ---- var pass: deterministic per-variable sums + spin stage combination ---
+// Stage combination of one variable, shared by both var passes.  kj is the
+// variable's summed derivative (or, on a retry's J = 1, read back from k1).
+template <typename T, int J>
+__device__ __forceinline__ void var_stage(const StageArgs<T>& a, int i, bool fresh, T kj, const T* x, T* xnew, T hs,
+                                          double h, double hnext, double floor, double& err)
+{
+    const T xi = ld_nt(x + i);
+    if constexpr (J == 1) {
+        if (fresh) st_nt(a.k1 + i, kj);
+        else kj = ld_nt(a.k1 + i);
+        if (a.tmp) a.tmp[i] = stage_input<T, 2>(xi, kj, 0, 0, 0, 0, hs);
+    } else if constexpr (J == 2) {
+        st_nt(a.k2 + i, kj);
+        a.tmp[i] = stage_input<T, 3>(xi, ld_nt(a.k1 + i), kj, 0, 0, 0, hs);
+    } else if constexpr (J == 3) {
+        st_nt(a.k3 + i, kj);
+        a.tmp[i] = stage_input<T, 4>(xi, ld_nt(a.k1 + i), ld_nt(a.k2 + i), kj, 0, 0, hs);
+    } else if constexpr (J == 4) {
+        st_nt(a.k4 + i, kj);
+        a.tmp[i] = stage_input<T, 5>(xi, ld_nt(a.k1 + i), ld_nt(a.k2 + i), ld_nt(a.k3 + i), kj, 0, hs);
+    } else if constexpr (J == 5) {
+        st_nt(a.k5 + i, kj);
+        a.tmp[i] = stage_input<T, 6>(xi, ld_nt(a.k1 + i), ld_nt(a.k2 + i), ld_nt(a.k3 + i), ld_nt(a.k4 + i), kj, hs);
+    } else {
+        T xn;
+        double e;
+        final_combination<T>(xi, ld_nt(a.k1 + i), ld_nt(a.k3 + i), ld_nt(a.k4 + i), ld_nt(a.k5 + i), kj, hs, h, hnext,
+                             floor, xn, e);
+        err = fmax(err, e);
+        st_nt(xnew + i, xn);
+    }
+}
+
+// LDS var pass: one workgroup per variable group.  The group's whole
+// contribution region is read into LDS with coalesced 16-byte loads, then
+// each thread adds its variables' runs from LDS in slot order (the
+// reference's accumulation order) through 16-bit region offsets.
+template <typename T, int J>
+__global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
+{
+    constexpr int CAP = VAR_LDS_BYTES / (int)sizeof(T);
+    constexpr int V = 16 / (int)sizeof(T);
+    typedef T vT __attribute__((ext_vector_type(V)));
+    __shared__ double sh[VBLK / 64];
+    __shared__ __attribute__((aligned(16))) T reg[CAP];
+    const Ctrl* c = a.ctrl;
+    int parity = 0;
+    double h = 0.0, hnext = 0.0, floor = TabD::TINY;
+    bool fresh = true;
+    if (c) {
+        if (!c->active) return;
+        parity = c->parity; h = c->h; hnext = c->hnext; floor = c->floor;
+        fresh = (J != 1) || c->step_start;
+    }
+    const T hs = (T)h;
+    const T* x = parity ? a.xb : a.xa;
+    T* xnew = parity ? a.xa : a.xb;
+    const int g = blockIdx.x;
+    if (fresh) {
+        // region sizes are multiples of REGION_ALIGN and at most CAP (host-checked)
+        const long long r0 = a.gstart[g];
+        const int len = (int)(a.gstart[g + 1] - r0);
+        const vT* src = (const vT*)(a.contrib + r0);
+        vT* dst = (vT*)reg;
+        for (int v = threadIdx.x; v < len / V; v += VBLK) dst[v] = src[v];
+        __syncthreads();
+    }
+    double err = 0.0;
+    const int i0 = g * a.vg + 1, i1 = (int)std::min<long long>((long long)(g + 1) * a.vg, a.n);
+    for (int i = i0 + (int)threadIdx.x; i <= i1; i += VBLK) {
+        T kj = (T)0;
+        if (fresh) {
+            const int e0 = a.csr_ptr[i], e1 = a.csr_ptr[i + 1];
+            T sum = (T)0;
+            for (int e = e0; e < e1; e++) sum += reg[a.csr_loc[e]];
+            kj = sum;
+        }
+        var_stage<T, J>(a, i, fresh, kj, x, xnew, hs, h, hnext, floor, err);
+    }
+    if constexpr (J == 6) {
+        double bm = block_max<VBLK>(err, sh);
+        if (threadIdx.x == 0) a.partial[a.nblk_clause + blockIdx.x] = bm;
+    }
+}
+
+// Gather var pass, for groups whose region does not fit in LDS.  A block
+// owns up to BLK consecutive variables of one group.  Their CSR entries form
 // one contiguous range; the block first gathers that whole range into LDS
 // with coalesced csr_pos reads and many independent contribution loads in
 // flight, then every thread adds its own variable's run from LDS left to
@@ -396,57 +616,26 @@ __global__ __launch_bounds__(BLK) void k_var(StageArgs<T> a)
     const T hs = (T)h;
     const T* x = parity ? a.xb : a.xa;
     T* xnew = parity ? a.xa : a.xb;
-    const int i = var_of_block(a.vg_blocks, a.ngroups, a.n);
+    int i1 = 0;
+    const int i0 = var_block_range(a.vg, a.vg_blocks, a.ngroups, a.n, i1);
+    const int i = (i0 != 0 && i0 + (int)threadIdx.x <= i1) ? i0 + (int)threadIdx.x : 0;
     double err = 0.0;
     T kj = (T)0;
-    if (fresh) {
-        // variables of this block: [i0, i1]; thread's own range [e0, e1)
-        const int b = blockIdx.x, xcd = b & 7, t = b >> 3;
-        const int grp = (t / a.vg_blocks) * 8 + xcd;
-        const long long i0ll = (long long)grp * a.vg_blocks * BLK + (long long)(t % a.vg_blocks) * BLK + 1;
-        const int i0 = grp < a.ngroups && i0ll <= a.n ? (int)i0ll : 0;
-        if (i0 != 0) {
-            const int i1 = (int)std::min<long long>(i0ll + BLK - 1, a.n);
-            const int base = a.csr_ptr[i0], end = a.csr_ptr[i1 + 1];
-            const int e0 = i ? a.csr_ptr[i] : 0, e1 = i ? a.csr_ptr[i + 1] : 0;
-            T sum = (T)0;
-            for (int r0 = base; r0 < end; r0 += VAR_STAGE) {
-                const int r1 = std::min(end, r0 + VAR_STAGE);
-                for (int e = r0 + threadIdx.x; e < r1; e += BLK) vals[e - r0] = a.contrib[a.csr_pos[e]];
-                __syncthreads();
-                const int lo = std::max(e0, r0), hi = std::min(e1, r1);
-                for (int e = lo; e < hi; e++) sum += vals[e - r0];
-                __syncthreads();
-            }
-            kj = sum;
+    if (fresh && i0 != 0) {
+        const int base = a.csr_ptr[i0], end = a.csr_ptr[i1 + 1];
+        const int e0 = i ? a.csr_ptr[i] : 0, e1 = i ? a.csr_ptr[i + 1] : 0;
+        T sum = (T)0;
+        for (int r0 = base; r0 < end; r0 += VAR_STAGE) {
+            const int r1 = std::min(end, r0 + VAR_STAGE);
+            for (int e = r0 + threadIdx.x; e < r1; e += BLK) vals[e - r0] = a.contrib[a.csr_pos[e]];
+            __syncthreads();
+            const int lo = std::max(e0, r0), hi = std::min(e1, r1);
+            for (int e = lo; e < hi; e++) sum += vals[e - r0];
+            __syncthreads();
         }
+        kj = sum;
     }
-    if (i != 0) {
-        const T xi = ld_nt(x + i);
-        if constexpr (J == 1) {
-            if (fresh) st_nt(a.k1 + i, kj);
-            else kj = ld_nt(a.k1 + i);
-            if (a.tmp) a.tmp[i] = stage_input<T, 2>(xi, kj, 0, 0, 0, 0, hs);
-        } else if constexpr (J == 2) {
-            st_nt(a.k2 + i, kj);
-            a.tmp[i] = stage_input<T, 3>(xi, ld_nt(a.k1 + i), kj, 0, 0, 0, hs);
-        } else if constexpr (J == 3) {
-            st_nt(a.k3 + i, kj);
-            a.tmp[i] = stage_input<T, 4>(xi, ld_nt(a.k1 + i), ld_nt(a.k2 + i), kj, 0, 0, hs);
-        } else if constexpr (J == 4) {
-            st_nt(a.k4 + i, kj);
-            a.tmp[i] = stage_input<T, 5>(xi, ld_nt(a.k1 + i), ld_nt(a.k2 + i), ld_nt(a.k3 + i), kj, 0, hs);
-        } else if constexpr (J == 5) {
-            st_nt(a.k5 + i, kj);
-            a.tmp[i] = stage_input<T, 6>(xi, ld_nt(a.k1 + i), ld_nt(a.k2 + i), ld_nt(a.k3 + i), ld_nt(a.k4 + i), kj,
-                                         hs);
-        } else {
-            T xn;
-            final_combination<T>(xi, ld_nt(a.k1 + i), ld_nt(a.k3 + i), ld_nt(a.k4 + i), ld_nt(a.k5 + i), kj, hs, h,
-                                 hnext, floor, xn, err);
-            st_nt(xnew + i, xn);
-        }
-    }
+    if (i != 0) var_stage<T, J>(a, i, fresh, kj, x, xnew, hs, h, hnext, floor, err);
     if constexpr (J == 6) {
         double bm = block_max<BLK>(err, sh);
         if (threadIdx.x == 0) a.partial[a.nblk_clause + blockIdx.x] = bm;
@@ -619,18 +808,29 @@ struct Sys {
     long long nslots = 0;
     hipStream_t stream = nullptr;
     int* d_lit = nullptr;
-    int* d_pos = nullptr;
+    int* d_pos = nullptr;        // generic-K clause pass only
+    int* d_spos = nullptr;       // tiled clause pass (K = 3, 4)
+    uint16_t* d_sord = nullptr;
     int* d_csr_ptr = nullptr;
-    int* d_csr_pos = nullptr;
+    int* d_csr_pos = nullptr;    // gather var pass only
+    uint16_t* d_csr_loc = nullptr;   // LDS var pass only
+    long long* d_gstart = nullptr;
     int vg = 0, ngroups = 0, vg_blocks = 0;
+    bool tiled = false;          // clause pass: tiled kernel (K = 3, 4) or generic
+    bool lds_var = false;        // var pass: LDS region kernel or gather kernel
     void* d_contrib = nullptr;
     int* d_count = nullptr;
-    HostProblem hp;
     bool ready = false;
 
+    void free_problem()
+    {
+        dfree(d_lit); dfree(d_pos); dfree(d_spos); dfree(d_sord); dfree(d_csr_ptr); dfree(d_csr_pos);
+        dfree(d_csr_loc); dfree(d_gstart); dfree(d_contrib);
+    }
     ~Sys()
     {
-        dfree(d_lit); dfree(d_pos); dfree(d_csr_ptr); dfree(d_csr_pos); dfree(d_contrib); dfree(d_count);
+        free_problem();
+        dfree(d_count);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void require_ready() const { require(ready, ASAT_ERR_NOT_CONFIGURED, "SAT problem has not been set"); }
@@ -640,14 +840,45 @@ struct Rng {
     unsigned long long seed = 0, offset = 0;
 };
 
+template <typename V>
+static void upload(V*& d, const std::vector<V>& h)
+{
+    HIP_CHECK(hipMalloc((void**)&d, sizeof(V) * std::max<size_t>(h.size(), 1)));
+    if (!h.empty()) HIP_CHECK(hipMemcpy(d, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice));
+}
+
+// Variable groups for the LDS var pass: about 2048 (float) / 1024 (double)
+// variables per group, a multiple of 256 groups once there are more than
+// 256 (one workgroup per CU per round), at least 256 groups while groups
+// keep 256+ variables; halved until the largest region fits in LDS.  When no
+// size fits (a variable with more occurrences than the LDS holds), the gather
+// var pass runs with ~400 KB regions.
+static bool choose_layout(HostProblem& P, int esize)
+{
+    const long long cap = VAR_LDS_BYTES / esize;
+    const long long target = esize == 4 ? 2048 : 1024;
+    long long G = std::max(ceil_div((long long)P.n, target), std::min(256, ceil_div((long long)P.n, 256)));
+    if (G > 256) G = round_up(G, 256);
+    int vg = ceil_div((long long)P.n, G);
+    for (;;) {
+        build_layout(P, vg);
+        if (P.max_region <= cap && !P.csr_loc.empty()) return true;
+        if (vg <= 64) break;
+        vg = (vg + 1) / 2;
+    }
+    build_layout(P, esize == 4 ? 8192 : 4096);
+    return false;
+}
+
 static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclauses)
 {
     HostProblem P;
-    // ~400 KB contribution region per variable group (8192 float / 4096 double vars)
-    const int vg = s->esize == 4 ? 8192 : 4096;
-    build_problem(lits, lens, nclauses, vg, P);
+    parse_clauses(lits, lens, nclauses, P);
+    const bool lds_var = choose_layout(P, s->esize);
+    const bool tiled = P.k == 3 || P.k == 4;
+    if (tiled) build_tile_order(P);
     HIP_CHECK(hipStreamSynchronize(s->stream));
-    dfree(s->d_lit); dfree(s->d_pos); dfree(s->d_csr_ptr); dfree(s->d_csr_pos); dfree(s->d_contrib);
+    s->free_problem();
     s->ready = false;
     s->n = P.n; s->m = P.m; s->k = P.k;
     s->aoff = (int)round_up((long long)P.n + 1, 64);
@@ -655,24 +886,26 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
     s->nslots = (long long)P.m * P.k;
     s->vg = P.vg;
     s->ngroups = P.ngroups;
-    s->vg_blocks = P.vg / BLK;
-    // clause arrays padded to whole BLK-clause tiles (padding literal 0, position -1)
-    P.lit.resize((size_t)round_up(P.m, BLK) * P.k, 0);
-    P.pos.resize((size_t)round_up(P.m, BLK) * P.k, -1);
-    const size_t ncontrib = std::max<size_t>(P.csr_pos.size(), 1);
-    HIP_CHECK(hipMalloc((void**)&s->d_lit, sizeof(int) * P.lit.size()));
-    HIP_CHECK(hipMalloc((void**)&s->d_pos, sizeof(int) * P.pos.size()));
-    HIP_CHECK(hipMalloc((void**)&s->d_csr_ptr, sizeof(int) * P.csr_ptr.size()));
-    HIP_CHECK(hipMalloc((void**)&s->d_csr_pos, sizeof(int) * ncontrib));
+    s->vg_blocks = ceil_div(P.vg, BLK);
+    s->tiled = tiled;
+    s->lds_var = lds_var;
+    // clause arrays padded to whole tiles (padding literal 0, position -1)
+    P.lit.resize((size_t)round_up(P.m, TC) * P.k, 0);
+    upload(s->d_lit, P.lit);
+    if (tiled) {
+        upload(s->d_spos, P.spos);
+        upload(s->d_sord, P.sord);
+    } else {
+        P.pos.resize((size_t)round_up(P.m, TC) * P.k, -1);
+        upload(s->d_pos, P.pos);
+    }
+    upload(s->d_csr_ptr, P.csr_ptr);
+    if (lds_var) upload(s->d_csr_loc, P.csr_loc);
+    else upload(s->d_csr_pos, P.csr_pos);
+    upload(s->d_gstart, P.gstart);
+    const size_t ncontrib = (size_t)P.gstart[P.ngroups] + REGION_ALIGN;
     HIP_CHECK(hipMalloc(&s->d_contrib, (size_t)s->esize * ncontrib));
-    HIP_CHECK(hipMemcpy(s->d_lit, P.lit.data(), sizeof(int) * P.lit.size(), hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemcpy(s->d_pos, P.pos.data(), sizeof(int) * P.pos.size(), hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemcpy(s->d_csr_ptr, P.csr_ptr.data(), sizeof(int) * P.csr_ptr.size(), hipMemcpyHostToDevice));
-    if (!P.csr_pos.empty())
-        HIP_CHECK(hipMemcpy(s->d_csr_pos, P.csr_pos.data(), sizeof(int) * P.csr_pos.size(),
-                            hipMemcpyHostToDevice));
     HIP_CHECK(hipMemset(s->d_contrib, 0, (size_t)s->esize * ncontrib));
-    s->hp = std::move(P);
     s->ready = true;
 }
 
@@ -729,48 +962,53 @@ template <typename T>
 static StageArgs<T> stage_args(Sys* s, T* xa, T* xb, T* const* kk, T* tmp, double* partial, const Ctrl* c)
 {
     StageArgs<T> a;
-    a.lit = s->d_lit; a.pos = s->d_pos; a.csr_ptr = s->d_csr_ptr; a.csr_pos = s->d_csr_pos;
+    a.lit = s->d_lit; a.pos = s->d_pos; a.spos = s->d_spos; a.sord = s->d_sord;
+    a.csr_ptr = s->d_csr_ptr; a.csr_pos = s->d_csr_pos; a.csr_loc = s->d_csr_loc; a.gstart = s->d_gstart;
     a.contrib = (T*)s->d_contrib;
     a.xa = xa; a.xb = xb;
     a.k1 = kk[0]; a.k2 = kk[1]; a.k3 = kk[2]; a.k4 = kk[3]; a.k5 = kk[4];
     a.tmp = tmp; a.partial = partial; a.ctrl = c;
     a.n = s->n; a.m = s->m; a.k = s->k; a.aoff = s->aoff;
-    a.clause_per_xcd = ceil_div(ceil_div(s->m, BLK), 8);
+    a.ntiles = ceil_div(s->m, TC);
+    a.clause_per_xcd = ceil_div(s->tiled ? a.ntiles : ceil_div(s->m, BLK), 8);
     a.nblk_clause = 8 * a.clause_per_xcd;
+    a.vg = s->vg;
     a.vg_blocks = s->vg_blocks;
     a.ngroups = s->ngroups;
     a.knorm = (T)std::pow(2.0, -s->k);
     return a;
 }
 
+static int clause_grid(const Sys* s) { return 8 * ceil_div(s->tiled ? ceil_div(s->m, TC) : ceil_div(s->m, BLK), 8); }
+static int var_grid(const Sys* s) { return s->lds_var ? s->ngroups : 8 * s->vg_blocks * ceil_div(s->ngroups, 8); }
+
 template <typename T, int J>
-static void launch_clause(const StageArgs<T>& a, hipStream_t st)
+static void launch_clause(const StageArgs<T>& a, const Sys* s, hipStream_t st)
 {
     dim3 g(a.nblk_clause), b(BLK);
-    switch (a.k) {
-    case 3: hipLaunchKernelGGL((k_clause<T, 3, J>), g, b, 0, st, a); break;
-    case 4: hipLaunchKernelGGL((k_clause<T, 4, J>), g, b, 0, st, a); break;
-    default: hipLaunchKernelGGL((k_clause_gen<T, J>), g, b, 0, st, a); break;
-    }
+    const bool tiled = s->tiled;
+    if (tiled && a.k == 3) hipLaunchKernelGGL((k_clause_tile<T, 3, J, TILE_BT>), g, dim3(TILE_BT), 0, st, a);
+    else if (tiled && a.k == 4) hipLaunchKernelGGL((k_clause_tile<T, 4, J, TILE_BT>), g, dim3(TILE_BT), 0, st, a);
+    else hipLaunchKernelGGL((k_clause_gen<T, J>), g, b, 0, st, a);
 }
 
-static int var_grid(int ngroups, int vg_blocks) { return 8 * vg_blocks * ceil_div(ngroups, 8); }
-
 template <typename T, int J>
-static void launch_var(const StageArgs<T>& a, hipStream_t st)
+static void launch_var(const StageArgs<T>& a, const Sys* s, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_var<T, J>), dim3(var_grid(a.ngroups, a.vg_blocks)), dim3(BLK), 0, st, a);
+    if (s->lds_var) hipLaunchKernelGGL((k_var_region<T, J>), dim3(var_grid(s)), dim3(VBLK), 0, st, a);
+    else hipLaunchKernelGGL((k_var<T, J>), dim3(var_grid(s)), dim3(BLK), 0, st, a);
 }
 
 template <typename T>
-static void launch_attempt(const StageArgs<T>& a, Ctrl* ctrl, const double* partial, int np, hipStream_t st)
+static void launch_attempt(const StageArgs<T>& a, const Sys* s, Ctrl* ctrl, const double* partial, int np,
+                           hipStream_t st)
 {
-    launch_clause<T, 1>(a, st); launch_var<T, 1>(a, st);
-    launch_clause<T, 2>(a, st); launch_var<T, 2>(a, st);
-    launch_clause<T, 3>(a, st); launch_var<T, 3>(a, st);
-    launch_clause<T, 4>(a, st); launch_var<T, 4>(a, st);
-    launch_clause<T, 5>(a, st); launch_var<T, 5>(a, st);
-    launch_clause<T, 6>(a, st); launch_var<T, 6>(a, st);
+    launch_clause<T, 1>(a, s, st); launch_var<T, 1>(a, s, st);
+    launch_clause<T, 2>(a, s, st); launch_var<T, 2>(a, s, st);
+    launch_clause<T, 3>(a, s, st); launch_var<T, 3>(a, s, st);
+    launch_clause<T, 4>(a, s, st); launch_var<T, 4>(a, s, st);
+    launch_clause<T, 5>(a, s, st); launch_var<T, 5>(a, s, st);
+    launch_clause<T, 6>(a, s, st); launch_var<T, 6>(a, s, st);
     hipLaunchKernelGGL(k_control, dim3(1), dim3(CTRL_BLK), 0, st, ctrl, partial, np);
 }
 
@@ -783,8 +1021,8 @@ static void sys_rhs(Sys* s, State* dxdt, const State* x)
     StageArgs<T> a = stage_args<T>(s, (T*)x->d, nullptr, kk, nullptr, nullptr, nullptr);
     // dxdt.SetZero(): padding and the constant slot read as 0
     HIP_CHECK(hipMemsetAsync(dxdt->d, 0, (size_t)s->esize * s->alloc, s->stream));
-    launch_clause<T, 1>(a, s->stream);
-    launch_var<T, 1>(a, s->stream);
+    launch_clause<T, 1>(a, s, s->stream);
+    launch_var<T, 1>(a, s, s->stream);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(s->stream));
 }
@@ -871,7 +1109,7 @@ static void rk_configure(RK* r, Sys* s)
     }
     HIP_CHECK(hipMalloc(&r->tmp, (size_t)s->esize * (s->n + 1)));
     HIP_CHECK(hipMemset(r->tmp, 0, (size_t)s->esize * (s->n + 1)));
-    r->npartial = 8 * ceil_div(ceil_div(s->m, BLK), 8) + var_grid(s->ngroups, s->vg_blocks);
+    r->npartial = clause_grid(s) + var_grid(s);
     HIP_CHECK(hipMalloc((void**)&r->partial, sizeof(double) * r->npartial));
     HIP_CHECK(hipMemset(r->partial, 0, sizeof(double) * r->npartial));
     HIP_CHECK(hipMalloc((void**)&r->d_ctrl, sizeof(Ctrl)));
@@ -902,7 +1140,7 @@ static int rk_step_impl(RK* r, State* olds, State* news, int count)
     const long long target = r->h_ctrl->steps + count;
     long long last_attempts = r->h_ctrl->attempts;
     while (remaining > 0) {
-        for (long long i = 0; i < remaining; i++) launch_attempt<T>(a, r->d_ctrl, r->partial, r->npartial, st);
+        for (long long i = 0; i < remaining; i++) launch_attempt<T>(a, s, r->d_ctrl, r->partial, r->npartial, st);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipMemcpyAsync(r->h_ctrl, r->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipStreamSynchronize(st));
@@ -972,18 +1210,18 @@ static void rk_profile_impl(RK* r, State* olds, State* news, int reps, float* us
         HIP_CHECK(hipEventElapsedTime(&ms, r->ev0, r->ev1));
         us[slot] = ms * 1000.f / (float)reps;
     };
-    timeit(0, [&] { launch_clause<T, 1>(a, st); });
-    timeit(1, [&] { launch_clause<T, 2>(a, st); });
-    timeit(2, [&] { launch_clause<T, 3>(a, st); });
-    timeit(3, [&] { launch_clause<T, 4>(a, st); });
-    timeit(4, [&] { launch_clause<T, 5>(a, st); });
-    timeit(5, [&] { launch_clause<T, 6>(a, st); });
-    timeit(6, [&] { launch_var<T, 1>(a, st); });
-    timeit(7, [&] { launch_var<T, 2>(a, st); });
-    timeit(8, [&] { launch_var<T, 3>(a, st); });
-    timeit(9, [&] { launch_var<T, 4>(a, st); });
-    timeit(10, [&] { launch_var<T, 5>(a, st); });
-    timeit(11, [&] { launch_var<T, 6>(a, st); });
+    timeit(0, [&] { launch_clause<T, 1>(a, s, st); });
+    timeit(1, [&] { launch_clause<T, 2>(a, s, st); });
+    timeit(2, [&] { launch_clause<T, 3>(a, s, st); });
+    timeit(3, [&] { launch_clause<T, 4>(a, s, st); });
+    timeit(4, [&] { launch_clause<T, 5>(a, s, st); });
+    timeit(5, [&] { launch_clause<T, 6>(a, s, st); });
+    timeit(6, [&] { launch_var<T, 1>(a, s, st); });
+    timeit(7, [&] { launch_var<T, 2>(a, s, st); });
+    timeit(8, [&] { launch_var<T, 3>(a, s, st); });
+    timeit(9, [&] { launch_var<T, 4>(a, s, st); });
+    timeit(10, [&] { launch_var<T, 5>(a, s, st); });
+    timeit(11, [&] { launch_var<T, 6>(a, s, st); });
     timeit(12, [&] { hipLaunchKernelGGL(k_control, dim3(1), dim3(CTRL_BLK), 0, st, r->d_ctrl, r->partial, r->npartial); });
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(r->d_ctrl, backup, sizeof(Ctrl), hipMemcpyDeviceToDevice, st));

@@ -270,3 +270,37 @@ def test_trajectory_with_error_floor_bitexact(golden_dir, precision):
             assert np.array_equal(old.CopyTo(), orc.get_state()), blk
             st = rk.stats()
             assert (st.hdid, st.time, st.attempts) == (orc.hdid, orc.time, orc.attempts), blk
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_high_degree_variable_uses_gather_pass(precision):
+    """A variable in every clause has more contributions than one LDS region
+    holds: the var pass falls back to gathering from global memory.  RHS and
+    20 steps stay bit-identical to the oracle."""
+    say("high-degree variable %s" % precision)
+    rng = np.random.default_rng(12)
+    n, m = 2000, 45000
+    lits = np.empty((m, 3), np.int32)
+    lits[:, 0] = 1
+    lits[:, 1:] = rng.integers(2, n + 1, (m, 2))
+    lits *= np.where(rng.random((m, 3)) < 0.5, -1, 1).astype(np.int32)
+    p = O.Problem(lits.ravel(), np.full(m, 3, np.int32))
+    mode = "f64" if precision == "f64" else "mix"
+    dt = np.float64 if precision == "f64" else np.float32
+    sat = _sys_with(p.literals, p.lengths, precision)
+    pub = O.random_state_f64(3, n, m).astype(dt)
+    got = _gpu_rhs(sat, pub)
+    want = O.rhs(p, np.concatenate([[-1.0], pub]).astype(dt), mode=mode)[1:]
+    assert np.array_equal(got, want)
+    rk = A.HipRungeKutta()
+    rk.SetEpsilon(1e-6)
+    rk.Configure(sat)
+    old, new = sat.MakeState(), sat.MakeState()
+    old.CopyFrom(pub)
+    with O.pow_mode("cr"):
+        orc = O.RK(p, eps=1e-6, mode=mode)
+        orc.set_state(pub)
+        rk.Step(old, new, 20)
+        orc.step(20)
+    assert np.array_equal(old.CopyTo(), orc.get_state())
+    assert rk.stats().attempts == orc.attempts
