@@ -108,6 +108,11 @@ def lib():
             "asat_sys_rhs": ([vp, vp, vp, ctypes.c_double], i),
             "asat_sys_violations": ([vp, vp, ip], i),
             "asat_sys_set_random_state": ([vp, vp, vp], i),
+            "asat_sys_clause_order": ([vp, vp], i),
+            "asat_sys_set_family": ([vp, ctypes.c_int], i),
+            "asat_sys_get_family": ([vp, ip], i),
+            "asat_sys_set_q": ([vp, ctypes.c_double], i),
+            "asat_sys_get_q": ([vp, ctypes.POINTER(ctypes.c_double)], i),
             "asat_state_create": ([vp, pvp], i),
             "asat_state_destroy": ([vp], i),
             "asat_state_set_public": ([vp, vp], i),
@@ -450,6 +455,30 @@ class HipSat:
         v = ctypes.c_int()
         _check(lib().asat_sys_violations(self._h, state._h, ctypes.byref(v)))
         return v.value
+
+    def Get_Q(self):
+        q = ctypes.c_double()
+        _check(lib().asat_sys_get_q(self._h, ctypes.byref(q)))
+        return q.value
+
+    def Set_Q(self, q):
+        _check(lib().asat_sys_set_q(self._h, float(q)))
+
+    def ClauseOrder(self):
+        """Clause index of each position of a state's device aux block."""
+        out = np.zeros(self.Get_M(), np.int32)
+        _check(lib().asat_sys_clause_order(self._h, out.ctypes.data))
+        return out
+
+
+class HipSatTanh(HipSat):
+    """CpuSatTanh / CudaSatTanh3 on the GPU (AnalogSAT/cpu_sat/CpuSatTanh.cpp:35-117):
+    a slot adds knorm a |K_mi| tanh(q f_mi) c_mi; q defaults to 1.1."""
+
+    def __init__(self, precision="f64", q=1.1):
+        super().__init__(precision)
+        _check(lib().asat_sys_set_family(self._h, 1))
+        self.Set_Q(q)
 
 
 class HipRungeKutta:

@@ -273,12 +273,20 @@ __global__ void k_zero_slot0(double* v, long long D, int ntraj)
 }
 
 // plain-solver layout (spins 1..N, aux at aoff) -> internal layout (aux at N+1)
-__global__ __launch_bounds__(BLK) void k_from_plain(const float* src, int n, int m, int aoff, float* dst)
+// plain-solver state -> [ -1 | s | a ] in clause order; order[j] is the
+// clause of the plain state's aux position j
+__global__ __launch_bounds__(BLK) void k_from_plain(const float* src, int n, int m, int aoff, const int* order,
+                                                    float* dst)
 {
     const long long i = (long long)blockIdx.x * BLK + threadIdx.x;
-    if (i == 0) dst[0] = -1.0f;
-    else if (i <= n) dst[i] = src[i];
-    else if (i <= (long long)n + m) dst[i] = src[aoff + (i - n - 1)];
+    if (i == 0) {
+        dst[0] = -1.0f;
+    } else if (i <= n) {
+        dst[i] = src[i];
+    } else if (i <= (long long)n + m) {
+        const int j = (int)(i - n - 1);
+        dst[1 + n + order[j]] = src[aoff + j];
+    }
 }
 
 __global__ __launch_bounds__(BLK) void k_h_round(const double* h, double* hh, int n)
@@ -581,6 +589,15 @@ int asat_adj_forward(asat_adj_t* a, int ntraj, const float* pub0, double* V_out)
     chk(asat_state_create(sys, &so));
     chk(asat_state_create(sys, &sn));
     const size_t V = (size_t)A.ntraj * A.D;
+    std::vector<int> order((size_t)A.m);
+    chk(asat_sys_clause_order(sys, order.data()));
+    int* d_order = nullptr;
+    HIP_CHECK(hipMalloc((void**)&d_order, sizeof(int) * order.size()));
+    struct FreeOrder {
+        int*& p;
+        ~FreeOrder() { (void)hipFree(p); }
+    } free_order{d_order};
+    HIP_CHECK(hipMemcpy(d_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
     for (int t = 0; t < ntraj; t++) {
         chk(asat_state_set_public(so, pub0 + (size_t)t * (A.n + A.m)));
         chk(asat_rk_set_step_log(rk, A.hlog + (size_t)t * A.T, A.T));
@@ -592,7 +609,8 @@ int asat_adj_forward(asat_adj_t* a, int ntraj, const float* pub0, double* V_out)
             int aoff = 0;
             chk(asat_state_device_ptr(so, &dp, &aoff));
             hipLaunchKernelGGL(k_from_plain, dim3(ceil_div((long long)A.n + A.m + 1, BLK)), dim3(BLK), 0, A.stream,
-                               (const float*)dp, A.n, A.m, aoff, A.ckpt + (size_t)c * V + (size_t)t * A.D);
+                               (const float*)dp, A.n, A.m, aoff, (const int*)d_order,
+                               A.ckpt + (size_t)c * V + (size_t)t * A.D);
             HIP_CHECK(hipStreamSynchronize(A.stream));
             if ((long long)c * A.C >= A.T) break;
             const int todo = std::min(A.C, A.T - c * A.C);

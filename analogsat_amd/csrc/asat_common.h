@@ -188,6 +188,26 @@ __device__ __forceinline__ void write_contribs(const int (&l)[K], const T (&f)[K
     }
 }
 
+// tanh-family slot term, knorm a |prod| tanh(q f) c (AnalogSAT/cpu_sat/
+// cpu_helper.h:115-125, AnalogSAT/cpu_sat/CpuSatTanh.cpp:262-275), r = knorm a.
+// For K = 3..10 the reference's templated helper calls the C library's double
+// tanh (it is declared before the file's using-directive), so in FP32 that
+// product is formed in double and rounded once; otherwise the float overload.
+template <typename T>
+__device__ __forceinline__ T tanh_term(T r, T prod, T f, T q, bool neg, bool ex)
+{
+    const T ap = prod < (T)0 ? -prod : prod;
+    T rr;
+    if constexpr (sizeof(T) == 4) {
+        if (ex) rr = (T)((double)(r * ap) * tanh((double)(q * f)));
+        else rr = r * ap * tanhf(q * f);
+    } else {
+        (void)ex;
+        rr = r * ap * tanh(q * f);
+    }
+    return neg ? -rr : rr;
+}
+
 // ---------------------------------------------------------------------------
 // wave64 / block reductions (DPP-free, via __shfl_xor on 64 lanes)
 // fmax ignores NaN like the CPU reference's std::fmax.

@@ -65,10 +65,12 @@ constexpr int TILE_BT = 1024;             // threads per tile workgroup (a claus
 constexpr int VBLK = 1024;                // var-pass workgroup (one variable group)
 constexpr int VAR_LDS_BYTES = 147456;     // LDS holding one group's contribution region
 constexpr int REGION_ALIGN = 64;          // region starts, in elements
+constexpr bool REORDER_CLAUSES = true;    // tiled pass: process clauses sorted by a primary variable
 
 struct HostProblem {
     int n = 0, m = 0, k = 0;
-    std::vector<int> lit;          // m*k, zero padded, clause-major
+    std::vector<int> lit;          // m*k, zero padded, clause-major, in processing order
+    std::vector<int> perm;         // processing position -> clause index (empty: identity)
     // contribution layout: variables split into groups of vg; the
     // contributions of a group's variables form one region, in slot (clause)
     // order; region starts are REGION_ALIGN-element aligned
@@ -125,6 +127,38 @@ static void parse_clauses(const int* literals, const int* lengths, int nclauses,
     }
 }
 
+// Processing order of the tiled clause pass.  Every clause gets one primary
+// variable (greedily its least-loaded variable, so that each variable is
+// primary for about M/N clauses) and clauses are processed sorted by it
+// (stable).  A wave then reads its clauses' primary spins from a few adjacent
+// lines, and a tile stores its primary contributions as one run.  Literal
+// order inside a clause is kept, and so is every per-variable sum order.
+static void reorder_clauses(HostProblem& P)
+{
+    const int K = P.k, m = P.m, n = P.n;
+    std::vector<int> load((size_t)n + 1, 0), prim((size_t)m, 0);
+    for (int j = 0; j < m; j++) {
+        int best = 0;
+        for (int q = 0; q < K; q++) {
+            const int l = P.lit[(size_t)j * K + q];
+            if (!l) continue;
+            const int v = std::abs(l);
+            if (best == 0 || load[v] < load[best]) best = v;
+        }
+        prim[j] = best;
+        load[best]++;
+    }
+    std::vector<int> start((size_t)n + 2, 0);
+    for (int j = 0; j < m; j++) start[(size_t)prim[j] + 1]++;
+    for (int v = 0; v <= n; v++) start[v + 1] += start[v];
+    P.perm.assign((size_t)m, 0);
+    for (int j = 0; j < m; j++) P.perm[(size_t)start[prim[j]]++] = j;
+    std::vector<int> lit2(P.lit.size());
+    for (int j = 0; j < m; j++)
+        std::copy_n(P.lit.begin() + (size_t)P.perm[j] * K, K, lit2.begin() + (size_t)j * K);
+    P.lit.swap(lit2);
+}
+
 // Contribution layout for groups of vg variables, and the transposed
 // var-major slot list (role of CudaSat3::InitProblem,
 // AnalogSAT/cuda_sat/CudaSat3.cu:266-307, without the pad-to-32).
@@ -152,8 +186,8 @@ static void build_layout(HostProblem& P, int vg)
         int l = P.lit[s];
         if (l) P.pos[s] = (int)gfill[(size_t)(std::abs(l) - 1) / vg]++;
     }
-    // per-variable lists of contribution positions; slots visited in
-    // ascending order, which is the reference's accumulation order
+    // per-variable lists of contribution positions; slots visited in the
+    // original clause order, which is the reference's accumulation order
     P.csr_ptr.assign((size_t)n + 2, 0);
     for (size_t s = 0; s < P.lit.size(); s++) {
         int l = P.lit[s];
@@ -162,10 +196,15 @@ static void build_layout(HostProblem& P, int vg)
     for (int i = 0; i <= n; i++) P.csr_ptr[i + 1] += P.csr_ptr[i];
     P.csr_pos.assign((size_t)P.csr_ptr[n + 1], 0);
     std::vector<int> fill(P.csr_ptr.begin(), P.csr_ptr.end() - 1);
-    for (size_t s = 0; s < P.lit.size(); s++) {
-        int l = P.lit[s];
-        if (l) P.csr_pos[(size_t)fill[std::abs(l)]++] = P.pos[s];
-    }
+    std::vector<int> row((size_t)P.m);
+    for (int j = 0; j < P.m; j++) row[P.perm.empty() ? j : P.perm[j]] = j;
+    const int K = P.k;
+    for (int mo = 0; mo < P.m; mo++)
+        for (int q = 0; q < K; q++) {
+            const size_t s = (size_t)row[mo] * K + q;
+            int l = P.lit[s];
+            if (l) P.csr_pos[(size_t)fill[std::abs(l)]++] = P.pos[s];
+        }
     P.csr_loc.clear();
     if (P.max_region <= 65536) {
         P.csr_loc.resize(P.csr_pos.size());
@@ -241,7 +280,9 @@ struct StageArgs {
     int ngroups;
     int clause_per_xcd;   // clause-pass chunks (BLK clauses, or TC-clause tiles) per XCD
     int ntiles;           // tiled pass: tiles of TC clauses
+    int family;           // ASAT_FAMILY_CTDS or ASAT_FAMILY_TANH
     T knorm;
+    T q;                  // tanh family parameter
 };
 
 // XCD-contiguous clause-pass mapping: XCD x (blocks b with b % 8 == x) walks
@@ -383,7 +424,7 @@ __global__ __launch_bounds__(BT) void k_clause_tile(StageArgs<T> a)
                     err = fmax(err, e);
                     st_nt(xnew + ai, xn);
                 }
-                s_r[cl] = km * a.knorm * am;
+                s_r[cl] = a.family == ASAT_FAMILY_TANH ? a.knorm * am : km * a.knorm * am;
 #pragma unroll
                 for (int q = 0; q < K; q++) s_f[cl * K + q] = f[q];
             }
@@ -407,8 +448,13 @@ __global__ __launch_bounds__(BT) void k_clause_tile(StageArgs<T> a)
                 prod = first ? fv : prod * fv;
                 first = false;
             }
-            T rr = s_r[cl] * prod;
-            rr *= (w & 0x8000u) ? (T)-2.0 : (T)2.0;
+            T rr;
+            if (a.family == ASAT_FAMILY_TANH) {
+                rr = tanh_term<T>(s_r[cl], prod, s_f[cl * K + q], a.q, (w & 0x8000u) != 0, true);
+            } else {
+                rr = s_r[cl] * prod;
+                rr *= (w & 0x8000u) ? (T)-2.0 : (T)2.0;
+            }
             a.contrib[p] = rr;
         }
     }
@@ -473,7 +519,23 @@ __global__ __launch_bounds__(BLK) void k_clause_gen(StageArgs<T> a)
         }
         T* out = a.contrib;
         const int* pm = a.pos + (size_t)m * K;
-        if (K <= 10) {
+        if (a.family == ASAT_FAMILY_TANH) {
+            // CpuSatTanh: templated helper for K = 3..10, generic loop otherwise
+            const T r = a.knorm * am;
+            const bool ex = K >= 3 && K <= 10;
+            for (int q = 0; q < K; q++) {
+                int l = lm[q];
+                if (l == 0) break;
+                T p = (T)1.0;
+                bool first = true;
+                for (int qq = 0; qq < K; qq++) {
+                    if (qq == q) continue;
+                    p = first ? f[qq * BLK] : p * f[qq * BLK];
+                    first = false;
+                }
+                out[pm[q]] = tanh_term<T>(r, p, f[q * BLK], a.q, l < 0, ex);
+            }
+        } else if (K <= 10) {
             const T r = km * a.knorm * am;
             for (int q = 0; q < K; q++) {
                 int l = lm[q];
@@ -763,11 +825,15 @@ __global__ void k_uniform(T* out, int len, unsigned long long seed, unsigned lon
 // s = 2u - 1 for spins, a = u + 1 for aux (CpuSat.cpp:127-132); input uniforms
 // are laid out in the public order [s, a] inside the state buffer's spin area
 template <typename T>
-__global__ void k_set_initial(T* x, int n, int m, int aoff, const T* u)
+__global__ void k_set_initial(T* x, int n, int m, int aoff, const T* u, const int* perm)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[1 + i] = u[i] * (T)2.0 - (T)1.0;
-    else if (i < (long long)n + m) x[aoff + (i - n)] = u[i] + (T)1.0;
+    if (i < n) {
+        x[1 + i] = u[i] * (T)2.0 - (T)1.0;
+    } else if (i < (long long)n + m) {
+        const int j = (int)(i - n);
+        x[aoff + j] = u[n + (perm ? perm[j] : j)] + (T)1.0;
+    }
 }
 
 // plain streaming copy (16 B per lane): the reference point for HBM byte counters
@@ -815,7 +881,11 @@ struct Sys {
     int* d_csr_pos = nullptr;    // gather var pass only
     uint16_t* d_csr_loc = nullptr;   // LDS var pass only
     long long* d_gstart = nullptr;
+    int* d_perm = nullptr;       // processing position -> clause index (null: identity)
+    std::vector<int> perm;       // host copy, for the public <-> internal a-block order
     int vg = 0, ngroups = 0, vg_blocks = 0;
+    int family = ASAT_FAMILY_CTDS;
+    double q = 1.1;              // tanh family parameter (CpuSatTanh.cpp:39)
     bool tiled = false;          // clause pass: tiled kernel (K = 3, 4) or generic
     bool lds_var = false;        // var pass: LDS region kernel or gather kernel
     void* d_contrib = nullptr;
@@ -825,7 +895,8 @@ struct Sys {
     void free_problem()
     {
         dfree(d_lit); dfree(d_pos); dfree(d_spos); dfree(d_sord); dfree(d_csr_ptr); dfree(d_csr_pos);
-        dfree(d_csr_loc); dfree(d_gstart); dfree(d_contrib);
+        dfree(d_csr_loc); dfree(d_gstart); dfree(d_perm); dfree(d_contrib);
+        perm.clear();
     }
     ~Sys()
     {
@@ -874,8 +945,9 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
 {
     HostProblem P;
     parse_clauses(lits, lens, nclauses, P);
-    const bool lds_var = choose_layout(P, s->esize);
     const bool tiled = P.k == 3 || P.k == 4;
+    if (tiled && REORDER_CLAUSES) reorder_clauses(P);
+    const bool lds_var = choose_layout(P, s->esize);
     if (tiled) build_tile_order(P);
     HIP_CHECK(hipStreamSynchronize(s->stream));
     s->free_problem();
@@ -903,6 +975,8 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
     if (lds_var) upload(s->d_csr_loc, P.csr_loc);
     else upload(s->d_csr_pos, P.csr_pos);
     upload(s->d_gstart, P.gstart);
+    if (!P.perm.empty()) upload(s->d_perm, P.perm);
+    s->perm = std::move(P.perm);
     const size_t ncontrib = (size_t)P.gstart[P.ngroups] + REGION_ALIGN;
     HIP_CHECK(hipMalloc(&s->d_contrib, (size_t)s->esize * ncontrib));
     HIP_CHECK(hipMemset(s->d_contrib, 0, (size_t)s->esize * ncontrib));
@@ -938,8 +1012,15 @@ static void state_set_public(State* st, const void* pub)
     const char* p = (const char*)pub;
     char* d = (char*)st->d;
     const size_t e = (size_t)s->esize;
+    std::vector<char> ab;
+    const char* pa = p + e * s->n;
+    if (!s->perm.empty()) {   // public clause order -> processing order
+        ab.resize(e * s->m);
+        for (int j = 0; j < s->m; j++) memcpy(ab.data() + e * j, pa + e * s->perm[j], e);
+        pa = ab.data();
+    }
     HIP_CHECK(hipMemcpyAsync(d + e, p, e * s->n, hipMemcpyHostToDevice, s->stream));
-    HIP_CHECK(hipMemcpyAsync(d + e * s->aoff, p + e * s->n, e * s->m, hipMemcpyHostToDevice, s->stream));
+    HIP_CHECK(hipMemcpyAsync(d + e * s->aoff, pa, e * s->m, hipMemcpyHostToDevice, s->stream));
     if (s->prec == ASAT_F64) hipLaunchKernelGGL(k_fill_const<double>, dim3(1), dim3(64), 0, s->stream, (double*)st->d);
     else hipLaunchKernelGGL(k_fill_const<float>, dim3(1), dim3(64), 0, s->stream, (float*)st->d);
     HIP_CHECK(hipGetLastError());
@@ -955,7 +1036,13 @@ static void state_get_public(const State* st, void* pub)
     const size_t e = (size_t)s->esize;
     HIP_CHECK(hipStreamSynchronize(s->stream));
     HIP_CHECK(hipMemcpy(p, d + e, e * s->n, hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(p + e * s->n, d + e * s->aoff, e * s->m, hipMemcpyDeviceToHost));
+    if (s->perm.empty()) {
+        HIP_CHECK(hipMemcpy(p + e * s->n, d + e * s->aoff, e * s->m, hipMemcpyDeviceToHost));
+    } else {   // processing order -> public clause order
+        std::vector<char> ab(e * s->m);
+        HIP_CHECK(hipMemcpy(ab.data(), d + e * s->aoff, e * s->m, hipMemcpyDeviceToHost));
+        for (int j = 0; j < s->m; j++) memcpy(p + e * (s->n + s->perm[j]), ab.data() + e * j, e);
+    }
 }
 
 template <typename T>
@@ -976,6 +1063,8 @@ static StageArgs<T> stage_args(Sys* s, T* xa, T* xb, T* const* kk, T* tmp, doubl
     a.vg_blocks = s->vg_blocks;
     a.ngroups = s->ngroups;
     a.knorm = (T)std::pow(2.0, -s->k);
+    a.family = s->family;
+    a.q = (T)s->q;
     return a;
 }
 
@@ -1438,6 +1527,53 @@ int asat_state_device_ptr(asat_state_t* st, void** dev_ptr, int* aux_offset)
     API_END
 }
 
+int asat_sys_set_family(asat_sys_t* sys, int family)
+{
+    API_BEGIN
+    CHECK_PTR(sys);
+    require(family == ASAT_FAMILY_CTDS || family == ASAT_FAMILY_TANH, ASAT_ERR_INVALID_ARGUMENT,
+            "unknown right-hand-side family");
+    HIP_CHECK(hipStreamSynchronize(sys->s.stream));
+    sys->s.family = family;
+    API_END
+}
+
+int asat_sys_get_family(const asat_sys_t* sys, int* family)
+{
+    API_BEGIN
+    CHECK_PTR(sys); CHECK_PTR(family);
+    *family = sys->s.family;
+    API_END
+}
+
+int asat_sys_set_q(asat_sys_t* sys, double q)
+{
+    API_BEGIN
+    CHECK_PTR(sys);
+    require(std::isfinite(q), ASAT_ERR_INVALID_ARGUMENT, "q must be finite");
+    HIP_CHECK(hipStreamSynchronize(sys->s.stream));
+    sys->s.q = q;
+    API_END
+}
+
+int asat_sys_get_q(const asat_sys_t* sys, double* q)
+{
+    API_BEGIN
+    CHECK_PTR(sys); CHECK_PTR(q);
+    *q = sys->s.q;
+    API_END
+}
+
+int asat_sys_clause_order(const asat_sys_t* sys, int* order)
+{
+    API_BEGIN
+    CHECK_PTR(sys); CHECK_PTR(order);
+    const Sys* s = &sys->s;
+    s->require_ready();
+    for (int j = 0; j < s->m; j++) order[j] = s->perm.empty() ? j : s->perm[j];
+    API_END
+}
+
 int asat_sys_set_random_state(asat_sys_t* sys, asat_state_t* x, asat_rng_t* rng)
 {
     API_BEGIN
@@ -1452,11 +1588,11 @@ int asat_sys_set_random_state(asat_sys_t* sys, asat_state_t* x, asat_rng_t* rng)
     if (s->prec == ASAT_F64) {
         hipLaunchKernelGGL(k_uniform<double>, g, b, 0, s->stream, (double*)u, len, rng->g.seed, rng->g.offset);
         hipLaunchKernelGGL(k_set_initial<double>, g, b, 0, s->stream, (double*)x->st.d, s->n, s->m, s->aoff,
-                           (const double*)u);
+                           (const double*)u, (const int*)s->d_perm);
     } else {
         hipLaunchKernelGGL(k_uniform<float>, g, b, 0, s->stream, (float*)u, len, rng->g.seed, rng->g.offset);
         hipLaunchKernelGGL(k_set_initial<float>, g, b, 0, s->stream, (float*)x->st.d, s->n, s->m, s->aoff,
-                           (const float*)u);
+                           (const float*)u, (const int*)s->d_perm);
     }
     rng->g.offset += (unsigned long long)len;
     hipError_t e = hipGetLastError();

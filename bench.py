@@ -24,6 +24,7 @@ bounded sample of accepted steps, when that build is present; otherwise the
 oracle's C restatement.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -37,21 +38,33 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def kernel_bytes(n, m, k, nslots):
+def kernel_bytes(n, m, k, nslots, esize=4):
+    """Algorithmic bytes per launch of each plain-solver kernel (every array
+    the kernel must touch, once; index arrays int32/uint16, state arrays of
+    esize bytes).
+
+    clause pass J (tiled, K = 3, 4): per slot a literal (4), a sorted position
+    (4) and a tile-local slot index (2) read and one contribution stored
+    (esize); per clause A_J aux values (x_a and k_1..k_{J-1} read, k_J or x_new
+    written).  Spin gathers hit the N-element stage vector, which stays in
+    cache, and are not counted.
+    var pass J (LDS regions): per slot a contribution (esize) and a 16-bit
+    region offset (2); per variable a row pointer (4) and B_J state values.
+    """
     L = nslots
     A = (2, 3, 4, 5, 6, 7)
     B = (3, 4, 5, 6, 7, 6)
     out = {}
     for j in range(6):
-        out["clause%d" % (j + 1)] = 8 * L + 4 * m * A[j]
-        out["var%d" % (j + 1)] = 8 * L + 4 * (n + 1) + 4 * n * B[j]
-    out["control"] = 8 * (-(-m // 256) + -(-n // 256))
+        out["clause%d" % (j + 1)] = (10 + esize) * L + esize * m * A[j]
+        out["var%d" % (j + 1)] = (2 + esize) * L + 4 * (n + 1) + esize * n * B[j]
+    out["control"] = 8 * (-(-m // 1024) + -(-n // 1024))
     return out
 
 
-def attempt_bytes(n, m, k, nslots):
-    """Algorithmic bytes of one attempt: 96 L + 136 D (FP32 state)."""
-    return 96 * nslots + 136 * (n + m)
+def attempt_bytes(n, m, k, nslots, esize=4):
+    """Algorithmic bytes of one attempt (sum over the 13 kernels)."""
+    return sum(kernel_bytes(n, m, k, nslots, esize).values())
 
 
 def cpu_baseline(lits, lens, pub0, eps, steps):
@@ -177,6 +190,132 @@ def batched_section(workload, count, max_steps, eps, plain_sample, log):
     return out
 
 
+def p50(xs):
+    xs = sorted(xs)
+    return xs[len(xs) // 2] if xs else None
+
+
+def workload_c1(args, log):
+    """C1 time-to-solution over seeds: N = 50, M = 213, k = 3, eps = 1e-6,
+    SAT checked after every accepted step, 1e4-step cap.  Instance seeds
+    0..S-1 x initial-condition seeds 0..9.  The batched solver (FP32 state)
+    runs all trajectories at once; the plain solver (FP64, the reference's
+    <double>) and the reference's CPU solver run a sample one at a time."""
+    import analogsat_amd as A
+    import oracle as O
+    S, n, m = args.c1_seeds, 50, 213
+    insts = []
+    for seed in range(S):
+        p = A.fastsat(n, m, 3, seed)
+        for ic in range(10):
+            s0 = np.random.default_rng(1000 * seed + ic).uniform(-1.0, 1.0, n)
+            insts.append((p, np.concatenate([s0, np.ones(m)])))
+    log("C1: %d trajectories (batched, FP32)" % len(insts))
+    b, wall, steps, attempts, res = run_batch([(p, x.astype(np.float32)) for p, x in insts], 10000, 1e-6,
+                                              chunk_steps=256, graph_chunks=8, log=log)
+    solved = [r for r in res if r["status"] == 1]
+    out = {"workload": "c1", "trajectories": len(res), "step_cap": 10000,
+           "batched": {"dtype": "fp32-state/fp64-control", "wall_s": round(wall, 3),
+                       "trajectory_steps_per_s": round(steps / wall, 1), "solved": len(solved),
+                       "p50_time_to_solution_s": p50([r["solved_wall_ms"] / 1e3 for r in solved]),
+                       "p50_steps_to_solution": p50([r["sat_step"] for r in solved])}}
+    sample = insts[:args.c1_plain_sample]
+    log("C1: plain FP64 solver, %d trajectories one at a time" % len(sample))
+    tts, steps_p, wall_p = [], 0, 0.0
+    for p, x in sample:
+        sat = A.HipSat("f64")
+        rk = A.HipRungeKutta()
+        rk.SetEpsilon(1e-6)
+        solver = A.SatSolver(sat, rk)
+        solver.SetProblem(p)
+        solver.SetMaxTime(1e300)
+        solver.SetMaxSteps(10000)
+        solver.SetBatchSize(1)
+        solver.SetSatState(x)
+        t0 = time.perf_counter()
+        r = solver.Solve()
+        w = time.perf_counter() - t0
+        wall_p += w
+        steps_p += solver.GetStepCount()
+        if r == A.SatResult.SAT_SOLUTION_FOUND:
+            tts.append(w)
+    out["plain"] = {"dtype": "f64", "sample": len(sample), "trajectory_steps_per_s": round(steps_p / wall_p, 1),
+                    "solved": len(tts), "p50_time_to_solution_s": p50(tts)}
+    if not args.no_cpu:
+        kind = "reference" if O.ref_available() else "port"
+        log("C1: CPU solver (%s), %d trajectories one at a time, 1 thread" % (kind, len(sample)))
+        tts_c, steps_c, wall_c = [], 0, 0.0
+        for p, x in sample:
+            lits = np.ascontiguousarray(p.literals, np.int32)
+            lens = np.ascontiguousarray(p.lengths, np.int32)
+            x64 = np.ascontiguousarray(x, np.float64)
+            t0 = time.perf_counter()
+            if kind == "reference":
+                R = O.ref()
+                viol = np.zeros(10001, np.int32)
+                nb, tout = ctypes.c_int(), ctypes.c_double()
+                xo = np.zeros(len(x64))
+                stats = np.zeros(3, np.int64)
+                code = R.ref_solve(lits, lens, len(lens), x64, 1e-6, 1, 10000, 1e300, viol, len(viol),
+                                   ctypes.byref(nb), xo, stats, ctypes.byref(tout))
+                st = int(stats[0])
+            else:
+                rk = O.RK(O.Problem(lits, lens), eps=1e-6)
+                rk.set_state(x64)
+                code = rk.solve(max_time=1e300, max_steps=10000, batch=1, max_batches=10001)[0]
+                st = rk.steps
+            w = time.perf_counter() - t0
+            wall_c += w
+            steps_c += st
+            if code == 0:
+                tts_c.append(w)
+        out["cpu"] = {"kind": kind, "cores": 1, "sample": len(sample),
+                      "trajectory_steps_per_s": round(steps_c / wall_c, 1), "solved": len(tts_c),
+                      "p50_time_to_solution_s": p50(tts_c)}
+    return out
+
+
+def workload_c4(args, log):
+    """C4: ragged batch, N ~ U{50..5000}, k ~ U{3,4}; budget of accepted steps
+    per instance; exact minimum violation count per instance and the
+    autotuned tile per size bucket."""
+    insts = make_c4(args.batch_count)
+    b, wall, steps, attempts, res = run_batch(insts, args.batch_steps, args.eps, log=log)
+    slots = sum(int(np.count_nonzero(p.literals)) for p, _ in insts)
+    return {"workload": "c4", "trajectories": len(res), "step_budget": args.batch_steps, "slots": slots,
+            "value": round(steps / wall, 1), "unit": "trajectory-steps/s", "wall_s": round(wall, 3),
+            "attempts_per_s": round(attempts / wall, 1),
+            "slot_attempts_per_s": round(sum(int(np.count_nonzero(p.literals)) * r["attempts"]
+                                             for (p, _), r in zip(insts, res)) / wall, 1),
+            "solved": sum(r["status"] == 1 for r in res),
+            "min_violations_first64": [r["min_violations"] for r in res[:64]],
+            "buckets": b.buckets()}
+
+
+def workload_c5(args, log):
+    """C5: discrete adjoint, N = 1e4, M = 42 500, k = 3, 64 trajectories,
+    T = 500 accepted steps, checkpoints every 32; gradient of
+    V(T) = sum a K^2 with respect to s(0) and a(0)."""
+    import analogsat_amd as A
+    n, m, ntraj, T, C = 10000, 42500, 64, 500, 32
+    p = A.fastsat(n, m, 3, 5)
+    rng = np.random.default_rng(5)
+    pubs = np.stack([np.concatenate([rng.uniform(-1, 1, n), np.ones(m)]) for _ in range(ntraj)]).astype(np.float32)
+    adj = A.HipAdjoint(p, eps=1e-6, steps=T, checkpoint_every=C)
+    log("C5: forward %d trajectories x %d steps" % (ntraj, T))
+    t0 = time.perf_counter()
+    adj.forward(pubs)
+    t1 = time.perf_counter()
+    g = adj.backward()
+    t2 = time.perf_counter()
+    fwd_ms, bwd_ms = adj.timing()
+    return {"workload": "c5", "trajectories": ntraj, "steps": T, "checkpoint_every": C,
+            "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
+            "forward_device_ms": round(fwd_ms, 2), "backward_device_ms": round(bwd_ms, 2),
+            "backward_trajectory_steps_per_s": round(ntraj * T / (t2 - t1), 1),
+            "grad_norm_first": float(np.linalg.norm(g[0]))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +334,10 @@ def main():
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=10000)
     ap.add_argument("--plain-sample", type=int, default=4)
+    ap.add_argument("--workload", choices=["c2", "c1", "c4", "c5"], default="c2",
+                    help="c2: the headline plain-solver line; c1/c4/c5: the other measured workloads")
+    ap.add_argument("--c1-seeds", type=int, default=100)
+    ap.add_argument("--c1-plain-sample", type=int, default=20)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run the plain copy kernel on this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -212,9 +355,14 @@ def main():
         print(json.dumps({"copy_bytes": nbytes, "us_per_copy": us, "GBps": 2 * nbytes / (us * 1e-6) / 1e9}))
         return
 
+    log = lambda s: print("[bench] " + s, file=sys.stderr, flush=True)
+    if args.workload != "c2":
+        fn = {"c1": workload_c1, "c4": workload_c4, "c5": workload_c5}[args.workload]
+        print(json.dumps(fn(args, log)), flush=True)
+        return
+
     n, k = args.n, args.k
     m = int(round(args.alpha * n))
-    log = lambda s: print("[bench] " + s, file=sys.stderr, flush=True)
     log("instance N=%d M=%d k=%d (FastSatMaker seed 0)" % (n, m, k))
     prob = A.fastsat(n, m, k, 0)
     pub0 = np.concatenate([np.random.default_rng(0).uniform(-1.0, 1.0, n), np.ones(m)])
@@ -249,9 +397,7 @@ def main():
     names = ["clause%d" % j for j in range(1, 7)] + ["var%d" % j for j in range(1, 7)] + ["control"]
     nslots = int(np.count_nonzero(prob.literals)) if len(prob.literals) else 0
     esz = 4 if args.precision == "f32" else 8
-    kb = kernel_bytes(n, m, k, m * k)
-    if esz == 8:
-        kb = {key: v * 1.5 for key, v in kb.items()}   # rough: state arrays double, indices stay int32
+    kb = kernel_bytes(n, m, k, nslots, esz)
     kernels = {nm: {"us": round(float(us[i]), 2), "GBps": round(float(kb[nm] / (us[i] * 1e-6) / 1e9), 1)}
                for i, nm in enumerate(names)}
     log("kernels: " + ", ".join("%s %.1fus %.0fGB/s" % (nm, v["us"], v["GBps"]) for nm, v in kernels.items()))
@@ -264,7 +410,7 @@ def main():
         with open(tfile) as f:
             traffic = json.load(f).get(dom)
     attempt_us = float(us[:13].sum())
-    att_bytes = attempt_bytes(n, m, k, m * k)
+    att_bytes = attempt_bytes(n, m, k, nslots, esz)
 
     cpu = None
     if not args.no_cpu:

@@ -44,6 +44,10 @@ extern "C" {
 #define ASAT_F64 0
 #define ASAT_F32 1
 
+/* right-hand-side families */
+#define ASAT_FAMILY_CTDS 0
+#define ASAT_FAMILY_TANH 1
+
 typedef struct asat_sys asat_sys_t;
 typedef struct asat_state asat_state_t;
 typedef struct asat_rk asat_rk_t;
@@ -87,6 +91,21 @@ int asat_sys_rhs(asat_sys_t* sys, asat_state_t* dxdt, const asat_state_t* x, dou
 int asat_sys_violations(asat_sys_t* sys, const asat_state_t* x, int* count);
 /* ISat::SetRandomState(state, random): s = 2u-1, a = u+1 */
 int asat_sys_set_random_state(asat_sys_t* sys, asat_state_t* x, asat_rng_t* rng);
+/* Right-hand-side family: ASAT_FAMILY_CTDS is CpuSat / CudaSat1-3
+   (AnalogSAT/cpu_sat/CpuSat.cpp:197-241); ASAT_FAMILY_TANH is CpuSatTanh /
+   CudaSatTanh1-3 (AnalogSAT/cpu_sat/CpuSatTanh.cpp:64-94,192-286), where a
+   slot adds knorm a |K_mi| tanh(q f_mi) c_mi.  The reference selects the
+   family by class; here it is a setting of the system (plain solver only). */
+int asat_sys_set_family(asat_sys_t* sys, int family);
+int asat_sys_get_family(const asat_sys_t* sys, int* family);
+/* CpuSatTanh::Set_Q / Get_Q (AnalogSAT/cpu_sat/CpuSatTanh.cpp:111-117), default 1.1 */
+int asat_sys_set_q(asat_sys_t* sys, double q);
+int asat_sys_get_q(const asat_sys_t* sys, double* q);
+/* No reference counterpart (the CUDA classes keep clause order on the
+   device): order[j] (M ints) = the clause whose aux value sits at position j
+   of a state's device aux block.  Copies to and from host memory
+   (asat_state_set_public / get_public) already use the public clause order. */
+int asat_sys_clause_order(const asat_sys_t* sys, int* order);
 
 /* ---------------- states (ISatState) -------------------------------------- */
 
@@ -106,7 +125,8 @@ int asat_state_set_zero(asat_state_t* st);
 /* IODEState::GetAllocSize */
 int asat_state_alloc_size(const asat_state_t* st, int* size);
 /* IODEState::GetState(): device pointer of the internal layout
-   [-1, s_1..s_N, pad, a_1..a_M, pad]; aux values start at *aux_offset */
+   [-1, s_1..s_N, pad, aux block, pad]; the aux block starts at *aux_offset and
+   holds a_c for clause c = order[j] at position j (asat_sys_clause_order) */
 int asat_state_device_ptr(asat_state_t* st, void** dev_ptr, int* aux_offset);
 
 /* ---------------- integrator (IODEInt / CudaRungeKutta) -------------------- */

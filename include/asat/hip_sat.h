@@ -1,6 +1,7 @@
 // MI355X implementations of the plugin interfaces, thin RAII wrappers over the
 // C ABI in analogsat_hip.h.  They replace the reference's CUDA classes:
 //   HipSat<T>         for CudaSat1/2/3<T>    (AnalogSAT/cuda_sat/CudaSat3.h:33-118)
+//   HipSatTanh<T>     for CudaSatTanh1/2/3<T> (AnalogSAT/cuda_sat/CudaSatTanh3.cu)
 //   HipSatState<T>    for CudaSatState2<T>   (AnalogSAT/cuda_base/CudaSatState2.h:31-106)
 //   HipRungeKutta<T>  for CudaRungeKutta<T>  (AnalogSAT/cuda_sat/CudaRungeKutta.h:30-136)
 //   HipRandom<T>      for CudaRandom<T>      (AnalogSAT/cuda_base/CudaRandom.h:29-46)
@@ -181,6 +182,21 @@ private:
         HipSafe(asat_sys_get_nmk(h, &v[0], &v[1], &v[2]));
         return v[which];
     }
+};
+
+// Tanh CTDS family, for CpuSatTanh<T> / CudaSatTanh1-3<T>
+// (AnalogSAT/cpu_sat/CpuSatTanh.h, AnalogSAT/cuda_sat/CudaSatTanh3.cu)
+template <typename TFloat>
+class HipSatTanh : public HipSat<TFloat> {
+public:
+    HipSatTanh() { HipSafe(asat_sys_set_family(this->handle(), ASAT_FAMILY_TANH)); }
+    TFloat Get_Q() const
+    {
+        double q = 0;
+        HipSafe(asat_sys_get_q(this->handle(), &q));
+        return (TFloat)q;
+    }
+    void Set_Q(TFloat q) { HipSafe(asat_sys_set_q(this->handle(), (double)q)); }
 };
 
 template <typename TFloat>

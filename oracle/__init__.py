@@ -56,6 +56,9 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         for sfx, fp, ftype in (("_f64", _f64p, ctypes.c_double), ("_mix", _f32p, ctypes.c_float)):
             getattr(L, "orc_rhs" + sfx).argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, ftype, fp, fp]
+            getattr(L, "orc_rhs_tanh" + sfx).argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, ftype, ftype,
+                                                        fp, fp]
+            getattr(L, "orc_rk_set_family" + sfx).argtypes = [_vp, ctypes.c_int, ctypes.c_double]
             getattr(L, "orc_violations" + sfx).argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, fp]
             getattr(L, "orc_violations" + sfx).restype = ctypes.c_int
             f = getattr(L, "orc_rk_create" + sfx)
@@ -117,6 +120,8 @@ def ref():
         R = ctypes.CDLL(REF_PATH)
         R.ref_problem_sizes.argtypes = [_i32p, _i32p, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
         R.ref_rhs_f64.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_double, _f64p, _f64p]
+        R.ref_rhs_tanh_f64.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_double, ctypes.c_double, _f64p, _f64p]
+        R.ref_rhs_tanh_f32.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_double, ctypes.c_double, _f32p, _f32p]
         R.ref_violations_f64.argtypes = [_i32p, _i32p, ctypes.c_int, _f64p]
         R.ref_violations_f64.restype = ctypes.c_int
         R.ref_rk_create.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_double]
@@ -218,12 +223,16 @@ def pow_cr(x, q):
     return lib().orc_pow_cr(float(x), int(q))
 
 
-def rhs(problem, x, mode="f64", b=0.0):
-    """One RHS on the internal layout x = [-1, s, a] (length n+m+1)."""
+def rhs(problem, x, mode="f64", b=0.0, family="ctds", q=1.1):
+    """One RHS on the internal layout x = [-1, s, a] (length n+m+1);
+    family 'ctds' (CpuSat) or 'tanh' (CpuSatTanh with parameter q)."""
     dt = np.float64 if mode == "f64" else np.float32
     x = np.ascontiguousarray(x, dtype=dt)
     out = np.zeros_like(x)
-    getattr(lib(), "orc_rhs_" + mode)(problem.n, problem.m, problem.k, problem.C, b, x, out)
+    if family == "tanh":
+        getattr(lib(), "orc_rhs_tanh_" + mode)(problem.n, problem.m, problem.k, problem.C, b, q, x, out)
+    else:
+        getattr(lib(), "orc_rhs_" + mode)(problem.n, problem.m, problem.k, problem.C, b, x, out)
     return out
 
 
@@ -255,7 +264,7 @@ def violations(problem, pub, mode="f64"):
 class RK:
     """Restated CpuRungeKutta bound to one trajectory (mode 'f64' or 'mix')."""
 
-    def __init__(self, problem, eps=1e-4, mode="f64", b=0.0, floor=None):
+    def __init__(self, problem, eps=1e-4, mode="f64", b=0.0, floor=None, family="ctds", q=1.1):
         self.problem = problem
         self.mode = mode
         self.dt = np.float64 if mode == "f64" else np.float32
@@ -266,6 +275,8 @@ class RK:
             self._call("orc_rk_set_b", b)
         if floor is not None:
             self._call("orc_rk_set_floor", floor)
+        if family == "tanh":
+            self._call("orc_rk_set_family", 1, q)
 
     def _call(self, name, *args):
         return getattr(self._L, name + self._sfx)(self._h, *args)

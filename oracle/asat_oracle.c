@@ -72,21 +72,25 @@ static double ctrl_pow(double x, double y)
 #define CT double
 #define SFX _f64
 #define ST_SIN sin
+#define ST_TANH tanh
 #include "asat_oracle_impl.h"
 #undef ST
 #undef CT
 #undef SFX
 #undef ST_SIN
+#undef ST_TANH
 
 #define ST float
 #define CT double
 #define SFX _mix
 #define ST_SIN sinf
+#define ST_TANH tanhf
 #include "asat_oracle_impl.h"
 #undef ST
 #undef CT
 #undef SFX
 #undef ST_SIN
+#undef ST_TANH
 
 /* ---------------------------------------------------------------------------
  * Problem container: SatProblem::AddClauses (AnalogSAT/problem/SatProblem.cpp:64-95)

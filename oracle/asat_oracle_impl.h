@@ -35,6 +35,8 @@ typedef struct {
     const int *C;        /* clause matrix, clause-major: C[j*k + q], 0 = padding */
     ST knorm;
     ST b;                /* bias prefactor; 0 disables the sine term */
+    int family;          /* 0: CpuSat, 1: CpuSatTanh */
+    ST q;                /* CpuSatTanh's q (default 1.1) */
     int len;             /* n + m + 1 */
     ST *x, *xnew, *temp, *dxdt, *ak2, *ak3, *ak4, *ak5, *ak6;
     CT *xscal, *xerr;
@@ -114,6 +116,62 @@ void FN(orc_rhs)(int n, int m, int k, const int *C, ST b, const ST *state, ST *r
     rhs[0] = zero;
 }
 
+/* ---- tanh family right-hand side (CpuSatTanh::GetDerivatives) ----
+   AnalogSAT/cpu_sat/CpuSatTanh.cpp:64-94, CalculateAllEx<K> :192-232 (K = 3..10),
+   CalculateAll :235-286 (other K), KmiFuncTanh AnalogSAT/cpu_sat/cpu_helper.h:115-125.
+   Per slot: knorm * a * |prod of the other factors| * tanh(q * f) * c; the aux
+   derivative is a K^2 as in CpuSat.  The two reference branches multiply the
+   sign (+-1) in at different places, which gives identical results; they
+   differ in the tanh they call (see below).  The bias term is added
+   unconditionally as there. */
+void FN(orc_rhs_tanh)(int n, int m, int k, const int *C, ST b, ST qq, const ST *state, ST *rhs)
+{
+    const int D = n + m + 1;
+    const ST one = (ST)1.0, zero = (ST)0.0;
+    const ST knorm = (ST)pow(2.0, -k);   /* CpuSatTanh.cpp:318 */
+    const ST bb = b * knorm * knorm;      /* CpuSatTanh.cpp:77 */
+    const ST alpha = (ST)m / (ST)n;       /* CpuSatTanh.cpp:56 */
+    ST s[64];
+    int index[64];
+    ST sum = 0;
+
+    memset(rhs, 0, sizeof(ST) * (size_t)D);
+    for (int j = 0; j < m; j++) {
+        ST km = knorm;
+        for (int q = 0; q < k; q++) index[q] = C[(size_t)j * k + q];
+        for (int q = 0; q < k; q++) {
+            s[q] = index[q] != 0 ? state[abs(index[q])] : (ST)-1.0;
+            s[q] = index[q] < 0 ? one + s[q] : one - s[q];
+            km *= s[q];
+        }
+        int p = n + 1 + j;
+        ST am = state[p];
+        rhs[p] = am * km * km;
+        sum += am;
+        ST r = knorm * am;
+        for (int q = 0; q < k; q++) {
+            if (index[q] == 0) break;   /* padding: the generic branch adds into rhs[0] only */
+            ST kmi = FN(kmi_prod)(s, k, q);
+            ST rr;
+            if (k >= 3 && k <= 10)
+                /* KmiFuncTanh calls the C library's double tanh (cpu_helper.h is
+                   included before CpuSatTanh.cpp's using-directive), so in float
+                   the product is formed in double and rounded once */
+                rr = (ST)((double)(r * (kmi < zero ? -kmi : kmi)) * tanh((double)(qq * s[q])));
+            else
+                rr = r * (kmi < zero ? -kmi : kmi) * ST_TANH(qq * s[q]);
+            rr *= index[q] < 0 ? (ST)-1.0 : one;
+            rhs[abs(index[q])] += rr;
+        }
+    }
+    {
+        ST a = sum / (ST)m;
+        for (int i = 1; i < n + 1; i++)
+            rhs[i] += a * alpha * bb * (ST)1.570796326794897 * ST_SIN(state[i] * (ST)3.141592653589793);
+    }
+    rhs[0] = zero;
+}
+
 /* ---- violated-clause count (CpuSat::CalculateClauses) ---- */
 int FN(orc_violations)(int n, int m, int k, const int *C, const ST *state)
 {
@@ -153,6 +211,8 @@ FN(orc_rk_t) *FN(orc_rk_create)(int n, int m, int k, const int *C, CT eps)
     r->eps = eps;
     r->floor = (CT)1e-20;
     r->b = 0;
+    r->family = 0;
+    r->q = (ST)1.1;       /* CpuSatTanh.cpp:39 */
     /* Reset(): CpuRungeKutta.cpp:168-177 */
     r->hnext = (CT)1e-9; r->steps = 0; r->attempts = 0; r->time = 0; r->rhscount = 0;
     r->hdid = 0;
@@ -175,6 +235,13 @@ void FN(orc_rk_reset)(FN(orc_rk_t) *r)
 
 void FN(orc_rk_set_b)(FN(orc_rk_t) *r, double b) { r->b = (ST)b; }
 
+/* right-hand side family: 0 = CpuSat, 1 = CpuSatTanh with parameter q */
+void FN(orc_rk_set_family)(FN(orc_rk_t) *r, int family, double q)
+{
+    r->family = family;
+    r->q = (ST)q;
+}
+
 /* xscal = |x| + |h k1| + floor; floor = TINY (1e-20) reproduces the reference */
 void FN(orc_rk_set_floor)(FN(orc_rk_t) *r, double floor) { r->floor = (CT)floor; }
 
@@ -192,7 +259,8 @@ void FN(orc_rk_get_state)(const FN(orc_rk_t) *r, ST *pub)
 
 static void FN(rhs_call)(FN(orc_rk_t) *r, ST *out, const ST *in)
 {
-    FN(orc_rhs)(r->n, r->m, r->k, r->C, r->b, in, out);
+    if (r->family == 1) FN(orc_rhs_tanh)(r->n, r->m, r->k, r->C, r->b, r->q, in, out);
+    else FN(orc_rhs)(r->n, r->m, r->k, r->C, r->b, in, out);
     r->rhscount++;
 }
 

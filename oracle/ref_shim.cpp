@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "cpu_sat/CpuSat.h"
+#include "cpu_sat/CpuSatTanh.h"
 #include "cpu_sat/CpuRungeKutta.h"
 #include "cpu_sat/CpuRandom.h"
 #include "solver/SatSolver.h"
@@ -41,6 +42,23 @@ struct RefTraj {
 
 }  // namespace
 
+// CpuSatTanh<T>::GetDerivatives on the internal layout [-1, s, a]
+template <typename T>
+static int rhs_tanh(const int* lits, const int* lens, int nclauses, double b, double q, const T* x, T* dxdt)
+{
+    SatProblem p = make_problem(lits, lens, nclauses);
+    CpuSatTanh<T> sat;
+    sat.Set_B((T)b);
+    sat.Set_Q((T)q);
+    sat.SetProblem(p);
+    std::unique_ptr<CpuSatStateImpl<T>> xs(sat.MakeState()), ds(sat.MakeState());
+    int D = p.Get_N() + p.Get_M() + 1;
+    xs->GetState().assign(x, x + D);
+    sat.GetDerivatives(*ds, *xs, (T)0);
+    for (int i = 0; i < D; i++) dxdt[i] = ds->GetState()[i];
+    return 0;
+}
+
 extern "C" {
 
 int ref_problem_sizes(const int* lits, const int* lens, int nclauses, int* n, int* m, int* k)
@@ -63,6 +81,16 @@ int ref_rhs_f64(const int* lits, const int* lens, int nclauses, double b, const 
     sat.GetDerivatives(*ds, *xs, 0.0);
     for (int i = 0; i < D; i++) dxdt[i] = ds->GetState()[i];
     return 0;
+}
+
+int ref_rhs_tanh_f64(const int* lits, const int* lens, int nclauses, double b, double q, const double* x, double* dxdt)
+{
+    return rhs_tanh<double>(lits, lens, nclauses, b, q, x, dxdt);
+}
+
+int ref_rhs_tanh_f32(const int* lits, const int* lens, int nclauses, double b, double q, const float* x, float* dxdt)
+{
+    return rhs_tanh<float>(lits, lens, nclauses, b, q, x, dxdt);
 }
 
 int ref_violations_f64(const int* lits, const int* lens, int nclauses, const double* pub)

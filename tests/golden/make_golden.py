@@ -16,6 +16,8 @@ Fixtures (numpy .npz, no pickles; DIMACS .cnf for instances):
   rhs_cases.npz        single RHS evaluations (k = 3, 4, mixed lengths with
                        padding, k = 12 generic path, bias b = 2.5, N = 2000)
   gen_cases.npz        FastSatMaker literal streams and CpuRandom uniforms
+  tanh_cases.npz       CpuSatTanh<double> and CpuSatTanh<float> RHS evaluations
+                       (the cases of rhs_cases.npz, each with its own q)
 """
 import ctypes
 import os
@@ -138,6 +140,22 @@ def main():
     lbig = ref_fastsat(2000, 8500, 3, 11)
     add("n2000", lbig, np.full(8500, 3, dtype=np.int32), 2000)
     np.savez(os.path.join(HERE, "rhs_cases.npz"), **cases)
+
+    # ---- tanh family: same inputs, CpuSatTanh in FP64 and FP32 ------------
+    tanh = {}
+    for name, q in (("k3", 1.1), ("k4", 2.0), ("mixed", 1.1), ("k12", 0.7), ("bias", 1.5), ("n2000", 1.1)):
+        lt, ln = cases[name + "_lits"], cases[name + "_lens"]
+        x = cases[name + "_x"]
+        b = float(cases[name + "_b"])
+        out64 = np.zeros_like(x)
+        R.ref_rhs_tanh_f64(lt, ln, len(ln), b, q, x, out64)
+        x32 = x.astype(np.float32)
+        out32 = np.zeros_like(x32)
+        R.ref_rhs_tanh_f32(lt, ln, len(ln), b, q, x32, out32)
+        tanh[name + "_q"] = np.float64(q)
+        tanh[name + "_rhs64"] = out64
+        tanh[name + "_rhs32"] = out32
+    np.savez(os.path.join(HERE, "tanh_cases.npz"), **tanh)
 
     # ---- generator / uniform streams ------------------------------------
     gen = {}

@@ -138,3 +138,33 @@ def test_live_reference_random_instances():
             assert np.array_equal(rk.get_state(), xr)
             assert rk.hdid == R.ref_rk_hdid(h)
         R.ref_rk_destroy(h)
+
+
+@pytest.mark.parametrize("case", ["k3", "k4", "mixed", "k12", "bias", "n2000"])
+def test_tanh_cases_bitexact(golden_dir, case):
+    """The tanh family (AnalogSAT/cpu_sat/CpuSatTanh.cpp:64-94,192-286) equals
+    the reference's CpuSatTanh<double> and CpuSatTanh<float> bit for bit."""
+    r = np.load(os.path.join(golden_dir, "rhs_cases.npz"), allow_pickle=False)
+    t = np.load(os.path.join(golden_dir, "tanh_cases.npz"), allow_pickle=False)
+    p = O.Problem(r[case + "_lits"], r[case + "_lens"])
+    x, b, q = r[case + "_x"], float(r[case + "_b"]), float(t[case + "_q"])
+    assert np.array_equal(O.rhs(p, x, b=b, family="tanh", q=q), t[case + "_rhs64"])
+    x32 = x.astype(np.float32)
+    assert np.array_equal(O.rhs(p, x32, mode="mix", b=b, family="tanh", q=q), t[case + "_rhs32"])
+
+
+def test_tanh_zero_spin_closed_form():
+    """At s = 0 every factor is 1: da_m = a_m 4^-k and
+    ds_i = sum_m c_mi 2^-k a_m tanh(q)."""
+    p = O.fastsat(40, 170, 3, 4)
+    a = np.random.default_rng(0).uniform(1, 2, p.m)
+    x = np.concatenate([[-1.0], np.zeros(p.n), a])
+    q = 1.7
+    got = O.rhs(p, x, family="tanh", q=q)
+    want_s = np.zeros(p.n + 1)
+    lits = p.literals.reshape(p.m, 3)
+    for j in range(p.m):
+        for l in lits[j]:
+            want_s[abs(l)] += np.sign(l) * 0.125 * a[j] * np.tanh(q)
+    np.testing.assert_allclose(got[1:p.n + 1], want_s[1:], rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(got[p.n + 1:], a / 64.0, rtol=1e-15)
