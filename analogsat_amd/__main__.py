@@ -10,10 +10,11 @@ and write ``<resultfolder>/<name>.out`` in MiniSat-compatible form
 
 Options (same names and ranges as the reference's parser,
 Frontend/src/main.cpp:46-199): -problem, -resultfolder, -eps, -tmax, -stepmax,
--batch, -bias (must be 0 on the GPU path), -trajectory (write
-<name>_traj_orig.dat), plus -float (FP32 state with FP64 control) and -seed
-(initial condition).  Options of the reference this path does not cover
-(-nogpu, -tanh, -minisat, -ramsey, bench/speedtest/make) are rejected.
+-batch, -bias, -tanh (the tanh CTDS family, Frontend/src/cnf_runner.cpp:44-54),
+-trajectory (write <name>_traj_orig.dat or <name>_traj_tanh.dat), plus -float
+(FP32 state with FP64 control) and -seed (initial condition).  Options of the
+reference this path does not cover (-nogpu, -minisat, -ramsey,
+bench/speedtest/make) are rejected.
 """
 import os
 import sys
@@ -22,10 +23,10 @@ import numpy as np
 
 
 DEFAULTS = {"eps": 1e-6, "tmax": 1e8, "bias": 0.0, "batch": 50, "stepmax": 50000000, "resultfolder": ".",
-            "problem": None, "trajectory": False, "float": False, "seed": None}
+            "problem": None, "trajectory": False, "float": False, "tanh": False, "seed": None}
 RANGES = {"eps": (1e-8, 1e-1), "tmax": (0, 1e80), "bias": (0.0, 1e6), "batch": (1, 2**31 - 1),
           "stepmax": (0, 2**31 - 1), "seed": (0, 2**63)}
-UNSUPPORTED = ("-nogpu", "-tanh", "-minisat", "-ramsey", "-ramseycircular", "-config", "-usegpu", "-k", "-alpha",
+UNSUPPORTED = ("-nogpu", "-minisat", "-ramsey", "-ramseycircular", "-config", "-usegpu", "-k", "-alpha",
                "-samplestart", "-sampleend", "-nstart", "-nend", "-nstep", "-problemfolder", "-version", "-timeout",
                "-rerun")
 
@@ -33,7 +34,7 @@ UNSUPPORTED = ("-nogpu", "-tanh", "-minisat", "-ramsey", "-ramseycircular", "-co
 def parse(argv):
     if not argv or argv[0] != "run":
         raise SystemExit("usage: python -m analogsat_amd run -problem FILE.cnf [-eps E] [-batch B] [-stepmax S] "
-                         "[-tmax T] [-resultfolder DIR] [-trajectory] [-float] [-seed S]")
+                         "[-tmax T] [-resultfolder DIR] [-bias B] [-tanh] [-trajectory] [-float] [-seed S]")
     conf = dict(DEFAULTS)
     i = 1
     while i < len(argv):
@@ -41,7 +42,7 @@ def parse(argv):
         if a in UNSUPPORTED:
             raise SystemExit("%s: not supported by the MI355X integrator path" % a)
         key = a.lstrip("-")
-        if key in ("trajectory", "float"):
+        if key in ("trajectory", "float", "tanh"):
             conf[key] = True
             i += 1
             continue
@@ -67,7 +68,7 @@ def run(conf, out=sys.stdout):
     os.makedirs(conf["resultfolder"], exist_ok=True)
     problem = A.SatProblem.read_cnf(conf["problem"])
     prec = "f32" if conf["float"] else "f64"
-    sat = A.HipSat(prec)
+    sat = A.HipSatTanh(prec) if conf["tanh"] else A.HipSat(prec)
     sat.Set_B(conf["bias"])
     rk = A.HipRungeKutta()
     rk.SetEpsilon(conf["eps"])
@@ -81,7 +82,8 @@ def run(conf, out=sys.stdout):
     name = os.path.splitext(os.path.basename(conf["problem"]))[0]
     traj = None
     if conf["trajectory"]:
-        traj = open(os.path.join(conf["resultfolder"], "%s_traj_orig.dat" % name), "w")
+        traj = open(os.path.join(conf["resultfolder"], "%s_traj_%s.dat" % (name, "tanh" if conf["tanh"] else "orig")),
+                    "w")
 
     def write_state(s):
         x = s.GetSatState().CopyTo()

@@ -274,6 +274,28 @@ __device__ __forceinline__ double block_max_all(double v, double* sh)
     return r;
 }
 
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// deterministic block sum (fixed butterfly and wave order), in every thread
+template <int BLOCK>
+__device__ __forceinline__ double block_sum_all(double v, double* sh)
+{
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double r = sh[0];
+#pragma unroll
+    for (int i = 1; i < BLOCK / 64; i++) r += sh[i];
+    __syncthreads();
+    return r;
+}
+
 template <int BLOCK>
 __device__ __forceinline__ int block_sum_all(int v, int* sh)
 {

@@ -272,6 +272,7 @@ struct StageArgs {
     T* k1; T* k2; T* k3; T* k4; T* k5;
     T* tmp;      // spin stage input (n+1)
     double* partial;   // per-block error maxima: clause blocks first, then var blocks
+    double* apart;     // bias term: per clause-block sums of the stage aux values
     const Ctrl* ctrl;  // null: standalone RHS (x = xa, output to k1, no tmp)
     int n, m, k, aoff;
     int nblk_clause;   // clause-pass grid (its J = 6 error partials come first)
@@ -281,8 +282,10 @@ struct StageArgs {
     int clause_per_xcd;   // clause-pass chunks (BLK clauses, or TC-clause tiles) per XCD
     int ntiles;           // tiled pass: tiles of TC clauses
     int family;           // ASAT_FAMILY_CTDS or ASAT_FAMILY_TANH
+    int bias;             // sine bias term on (b > 0)
     T knorm;
     T q;                  // tanh family parameter
+    T bb, alpha;          // b knorm^2 and M / N (bias term)
 };
 
 // XCD-contiguous clause-pass mapping: XCD x (blocks b with b % 8 == x) walks
@@ -360,7 +363,7 @@ __global__ __launch_bounds__(BT) void k_clause_tile(StageArgs<T> a)
     const T* sp = (J == 1) ? x : a.tmp;
     const int b = blockIdx.x;
     const int tile = (b & 7) * a.clause_per_xcd + (b >> 3);
-    double err = 0.0;
+    double err = 0.0, asum = 0.0;
     if (tile < a.ntiles) {
         const long long c0 = (long long)tile * TC;
         int l[CPT][K];
@@ -412,6 +415,7 @@ __global__ __launch_bounds__(BT) void k_clause_tile(StageArgs<T> a)
                 if constexpr (J == 1) am = ax[u];
                 else am = stage_input<T, J>(ax[u], a1[u], a2[u], a3[u], a4[u], a5[u], hs);
                 T kj = am * km * km;
+                asum += (double)am;
                 if constexpr (J == 1) st_nt(a.k1 + ai, kj);
                 else if constexpr (J == 2) st_nt(a.k2 + ai, kj);
                 else if constexpr (J == 3) st_nt(a.k3 + ai, kj);
@@ -458,6 +462,10 @@ __global__ __launch_bounds__(BT) void k_clause_tile(StageArgs<T> a)
             a.contrib[p] = rr;
         }
     }
+    if (a.bias) {
+        const double bs = block_sum_all<BT>(asum, sh);
+        if (threadIdx.x == 0) a.apart[blockIdx.x] = bs;
+    }
     if constexpr (J == 6) {
         double bm = block_max<BT>(err, sh);
         if (threadIdx.x == 0) a.partial[blockIdx.x] = bm;
@@ -485,7 +493,7 @@ __global__ __launch_bounds__(BLK) void k_clause_gen(StageArgs<T> a)
     const int K = a.k;
     const int m = clause_of_thread(a.clause_per_xcd);
     T* f = fsh + threadIdx.x;   // f[q*BLK]
-    double err = 0.0;
+    double err = 0.0, asum = 0.0;
     if (m < a.m) {
         const int* lm = a.lit + (size_t)m * K;
         T km = a.knorm;
@@ -507,6 +515,7 @@ __global__ __launch_bounds__(BLK) void k_clause_gen(StageArgs<T> a)
             am = stage_input<T, J>(x[ai], k1, k2, k3, k4, k5, hs);
         }
         T kj = am * km * km;
+        asum = (double)am;
         if constexpr (J == 1) a.k1[ai] = kj;
         else if constexpr (J == 2) a.k2[ai] = kj;
         else if constexpr (J == 3) a.k3[ai] = kj;
@@ -561,6 +570,10 @@ __global__ __launch_bounds__(BLK) void k_clause_gen(StageArgs<T> a)
             }
         }
     }
+    if (a.bias) {
+        const double bs = block_sum_all<BLK>(asum, sh);
+        if (threadIdx.x == 0) a.apart[blockIdx.x] = bs;
+    }
     if constexpr (J == 6) {
         double bm = block_max<BLK>(err, sh);
         if (threadIdx.x == 0) a.partial[blockIdx.x] = bm;
@@ -601,6 +614,27 @@ __device__ __forceinline__ void var_stage(const StageArgs<T>& a, int i, bool fre
     }
 }
 
+// Bias term (AnalogSAT/cpu_sat/CpuSat.cpp:232-239, CudaSatBase.cu:75-121):
+// mean(a) alpha b knorm^2 pi/2, with mean(a) the stage aux mean.  Every var
+// workgroup adds the clause blocks' partial sums in the same order, so all of
+// them use the same value.
+template <typename T, int BLOCK>
+__device__ __forceinline__ T bias_prefactor(const StageArgs<T>& a, double* sh)
+{
+    double v = 0.0;
+    for (int i = threadIdx.x; i < a.nblk_clause; i += BLOCK) v += a.apart[i];
+    const double tot = block_sum_all<BLOCK>(v, sh);
+    const T mean = (T)tot / (T)a.m;
+    return mean * a.alpha * a.bb * (T)1.570796326794897;
+}
+
+template <typename T>
+__device__ __forceinline__ T bias_sine(T s)
+{
+    if constexpr (sizeof(T) == 4) return sinf(s * (T)3.141592653589793);
+    else return sin(s * (T)3.141592653589793);
+}
+
 // LDS var pass: one workgroup per variable group.  The group's whole
 // contribution region is read into LDS with coalesced 16-byte loads, then
 // each thread adds its variables' runs from LDS in slot order (the
@@ -636,6 +670,7 @@ __global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
         __syncthreads();
     }
     double err = 0.0;
+    const T pref = (a.bias && fresh) ? bias_prefactor<T, VBLK>(a, sh) : (T)0;
     const int i0 = g * a.vg + 1, i1 = (int)std::min<long long>((long long)(g + 1) * a.vg, a.n);
     for (int i = i0 + (int)threadIdx.x; i <= i1; i += VBLK) {
         T kj = (T)0;
@@ -644,6 +679,7 @@ __global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
             T sum = (T)0;
             for (int e = e0; e < e1; e++) sum += reg[a.csr_loc[e]];
             kj = sum;
+            if (a.bias) kj += pref * bias_sine<T>(J == 1 ? x[i] : a.tmp[i]);
         }
         var_stage<T, J>(a, i, fresh, kj, x, xnew, hs, h, hnext, floor, err);
     }
@@ -682,6 +718,7 @@ __global__ __launch_bounds__(BLK) void k_var(StageArgs<T> a)
     const int i0 = var_block_range(a.vg, a.vg_blocks, a.ngroups, a.n, i1);
     const int i = (i0 != 0 && i0 + (int)threadIdx.x <= i1) ? i0 + (int)threadIdx.x : 0;
     double err = 0.0;
+    const T pref = (a.bias && fresh) ? bias_prefactor<T, BLK>(a, sh) : (T)0;
     T kj = (T)0;
     if (fresh && i0 != 0) {
         const int base = a.csr_ptr[i0], end = a.csr_ptr[i1 + 1];
@@ -697,6 +734,7 @@ __global__ __launch_bounds__(BLK) void k_var(StageArgs<T> a)
         }
         kj = sum;
     }
+    if (a.bias && fresh && i != 0) kj += pref * bias_sine<T>(J == 1 ? x[i] : a.tmp[i]);
     if (i != 0) var_stage<T, J>(a, i, fresh, kj, x, xnew, hs, h, hnext, floor, err);
     if constexpr (J == 6) {
         double bm = block_max<BLK>(err, sh);
@@ -889,13 +927,14 @@ struct Sys {
     bool tiled = false;          // clause pass: tiled kernel (K = 3, 4) or generic
     bool lds_var = false;        // var pass: LDS region kernel or gather kernel
     void* d_contrib = nullptr;
+    double* d_apart = nullptr;   // bias term: per clause-block aux sums
     int* d_count = nullptr;
     bool ready = false;
 
     void free_problem()
     {
         dfree(d_lit); dfree(d_pos); dfree(d_spos); dfree(d_sord); dfree(d_csr_ptr); dfree(d_csr_pos);
-        dfree(d_csr_loc); dfree(d_gstart); dfree(d_perm); dfree(d_contrib);
+        dfree(d_csr_loc); dfree(d_gstart); dfree(d_perm); dfree(d_contrib); dfree(d_apart);
         perm.clear();
     }
     ~Sys()
@@ -906,6 +945,10 @@ struct Sys {
     }
     void require_ready() const { require(ready, ASAT_ERR_NOT_CONFIGURED, "SAT problem has not been set"); }
 };
+
+// clause-pass and var-pass grid sizes of a configured system
+static int clause_grid(const Sys* s) { return 8 * ceil_div(s->tiled ? ceil_div(s->m, TC) : ceil_div(s->m, BLK), 8); }
+static int var_grid(const Sys* s) { return s->lds_var ? s->ngroups : 8 * s->vg_blocks * ceil_div(s->ngroups, 8); }
 
 struct Rng {
     unsigned long long seed = 0, offset = 0;
@@ -980,6 +1023,8 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
     const size_t ncontrib = (size_t)P.gstart[P.ngroups] + REGION_ALIGN;
     HIP_CHECK(hipMalloc(&s->d_contrib, (size_t)s->esize * ncontrib));
     HIP_CHECK(hipMemset(s->d_contrib, 0, (size_t)s->esize * ncontrib));
+    HIP_CHECK(hipMalloc((void**)&s->d_apart, sizeof(double) * clause_grid(s)));
+    HIP_CHECK(hipMemset(s->d_apart, 0, sizeof(double) * clause_grid(s)));
     s->ready = true;
 }
 
@@ -1065,11 +1110,13 @@ static StageArgs<T> stage_args(Sys* s, T* xa, T* xb, T* const* kk, T* tmp, doubl
     a.knorm = (T)std::pow(2.0, -s->k);
     a.family = s->family;
     a.q = (T)s->q;
+    a.apart = s->d_apart;
+    a.bb = (T)s->b * a.knorm * a.knorm;
+    a.bias = a.bb > (T)0;
+    a.alpha = (T)s->m / (T)s->n;
     return a;
 }
 
-static int clause_grid(const Sys* s) { return 8 * ceil_div(s->tiled ? ceil_div(s->m, TC) : ceil_div(s->m, BLK), 8); }
-static int var_grid(const Sys* s) { return s->lds_var ? s->ngroups : 8 * s->vg_blocks * ceil_div(s->ngroups, 8); }
 
 template <typename T, int J>
 static void launch_clause(const StageArgs<T>& a, const Sys* s, hipStream_t st)
@@ -1260,7 +1307,6 @@ static int rk_step(RK* r, State* olds, State* news, int count)
     require(olds != news && olds->d != news->d, ASAT_ERR_INVALID_ARGUMENT, "old and new state must differ");
     require(olds->sys == r->sys && news->sys == r->sys, ASAT_ERR_INVALID_ARGUMENT,
             "states belong to another system");
-    require(r->sys->b == 0.0, ASAT_ERR_UNSUPPORTED, "the bias term (b > 0) is not supported on the GPU path");
     require(count >= 0, ASAT_ERR_INVALID_ARGUMENT, "negative step count");
     if (r->h_ctrl->underflow) return ASAT_UNDERFLOW;
     if (count == 0) return ASAT_OK;
@@ -1422,7 +1468,6 @@ int asat_sys_rhs(asat_sys_t* sys, asat_state_t* dxdt, const asat_state_t* x, dou
     (void)t;   // autonomous system (the reference ignores time too)
     API_BEGIN
     CHECK_PTR(sys); CHECK_PTR(dxdt); CHECK_PTR(x);
-    require(sys->s.b == 0.0, ASAT_ERR_UNSUPPORTED, "the bias term (b > 0) is not supported on the GPU path");
     if (sys->s.prec == ASAT_F64) sys_rhs<double>(&sys->s, &dxdt->st, &x->st);
     else sys_rhs<float>(&sys->s, &dxdt->st, &x->st);
     API_END

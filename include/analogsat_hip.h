@@ -82,7 +82,11 @@ int asat_sys_destroy(asat_sys_t* sys);
 int asat_sys_set_problem(asat_sys_t* sys, const int* literals, const int* lengths, int nclauses);
 /* ISat::Get_N / Get_M / Get_K */
 int asat_sys_get_nmk(const asat_sys_t* sys, int* n, int* m, int* k);
-/* ISat::Get_B / Set_B (sine bias prefactor; only b == 0 is supported on the GPU) */
+/* ISat::Get_B / Set_B: sine bias prefactor b (AnalogSAT/cpu_sat/CpuSat.cpp:76,232-239);
+   b > 0 adds mean(a) (M/N) b 2^-2k (pi/2) sin(pi s_i) to ds_i.  The mean is a
+   deterministic tree sum (as CudaSat3's CudaReduceSum), not the CPU's serial
+   sum, so with b > 0 results match the CPU reference to rounding, not bits.
+   Plain solver only; the batched solver and the adjoint use b = 0. */
 int asat_sys_set_b(asat_sys_t* sys, double b);
 int asat_sys_get_b(const asat_sys_t* sys, double* b);
 /* IODE::GetDerivatives(dxdt, x, t) */

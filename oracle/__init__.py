@@ -102,6 +102,8 @@ def lib():
         L.orc_random_state_f64.argtypes = [ctypes.c_uint, ctypes.c_int, ctypes.c_int, _f64p]
         L.orc_fastsat.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulong, _i32p]
         L.orc_set_pow_mode.argtypes = [ctypes.c_int]
+        L.orc_set_mean_mode.argtypes = [ctypes.c_int]
+        L.orc_get_mean_mode.restype = ctypes.c_int
         L.orc_get_pow_mode.restype = ctypes.c_int
         L.orc_pow_cr.argtypes = [ctypes.c_double, ctypes.c_int]
         L.orc_pow_cr.restype = ctypes.c_double
@@ -217,6 +219,23 @@ class pow_mode:
 
     def __exit__(self, *a):
         lib().orc_set_pow_mode(self.prev)
+
+
+class mean_mode:
+    """Context manager for the bias term's sum of a: 'serial' (reference
+    behaviour, default) or 'double' (accumulated in double and rounded once,
+    the GPU's definition)."""
+
+    def __init__(self, mode):
+        self.mode = {"serial": 0, "double": 1}[mode]
+
+    def __enter__(self):
+        self.prev = lib().orc_get_mean_mode()
+        lib().orc_set_mean_mode(self.mode)
+        return self
+
+    def __exit__(self, *a):
+        lib().orc_set_mean_mode(self.prev)
 
 
 def pow_cr(x, q):

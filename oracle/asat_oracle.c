@@ -38,6 +38,14 @@ static int g_pow_mode = 0;
 void orc_set_pow_mode(int mode) { g_pow_mode = mode; }
 int orc_get_pow_mode(void) { return g_pow_mode; }
 
+/* Accumulation of the bias term's sum of a: 0 = serial sum in the state type
+   (the reference, CpuSat.cpp:224 / CpuSatTanh.cpp:215), 1 = sum in double,
+   rounded to the state type once (the GPU's definition; its tree order
+   changes a double sum of at most 2^31 terms only far below float rounding). */
+static int g_mean_mode = 0;
+void orc_set_mean_mode(int mode) { g_mean_mode = mode; }
+int orc_get_mean_mode(void) { return g_mean_mode; }
+
 static void dd_mul_d(double *hi, double *lo, double b)
 {
     double p = *hi * b;

@@ -73,6 +73,7 @@ void FN(orc_rhs)(int n, int m, int k, const int *C, ST b, const ST *state, ST *r
     ST s[64];
     int index[64];
     ST sum = 0;
+    double sumd = 0;
 
     memset(rhs, 0, sizeof(ST) * (size_t)D);             /* dxdt.SetZero() */
     for (int j = 0; j < m; j++) {
@@ -87,6 +88,7 @@ void FN(orc_rhs)(int n, int m, int k, const int *C, ST b, const ST *state, ST *r
         ST am = state[p];
         rhs[p] = am * km * km;
         sum += am;
+        sumd += (double)am;
         if (k <= 10) {
             ST r = km * knorm * am;
             /* CpuKmiWrapper::KmiFunc: stops at the first zero (padding) index */
@@ -109,7 +111,7 @@ void FN(orc_rhs)(int n, int m, int k, const int *C, ST b, const ST *state, ST *r
         }
     }
     if (bb > zero) {
-        ST a = sum / (ST)m;
+        ST a = (g_mean_mode ? (ST)sumd : sum) / (ST)m;
         for (int i = 1; i < n + 1; i++)
             rhs[i] += a * alpha * bb * (ST)1.570796326794897 * ST_SIN(state[i] * (ST)3.141592653589793);
     }
@@ -134,6 +136,7 @@ void FN(orc_rhs_tanh)(int n, int m, int k, const int *C, ST b, ST qq, const ST *
     ST s[64];
     int index[64];
     ST sum = 0;
+    double sumd = 0;
 
     memset(rhs, 0, sizeof(ST) * (size_t)D);
     for (int j = 0; j < m; j++) {
@@ -148,6 +151,7 @@ void FN(orc_rhs_tanh)(int n, int m, int k, const int *C, ST b, ST qq, const ST *
         ST am = state[p];
         rhs[p] = am * km * km;
         sum += am;
+        sumd += (double)am;
         ST r = knorm * am;
         for (int q = 0; q < k; q++) {
             if (index[q] == 0) break;   /* padding: the generic branch adds into rhs[0] only */
@@ -165,7 +169,7 @@ void FN(orc_rhs_tanh)(int n, int m, int k, const int *C, ST b, ST qq, const ST *
         }
     }
     {
-        ST a = sum / (ST)m;
+        ST a = (g_mean_mode ? (ST)sumd : sum) / (ST)m;
         for (int i = 1; i < n + 1; i++)
             rhs[i] += a * alpha * bb * (ST)1.570796326794897 * ST_SIN(state[i] * (ST)3.141592653589793);
     }

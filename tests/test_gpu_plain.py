@@ -304,3 +304,52 @@ def test_high_degree_variable_uses_gather_pass(precision):
         orc.step(20)
     assert np.array_equal(old.CopyTo(), orc.get_state())
     assert rk.stats().attempts == orc.attempts
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_bias_term_rhs_and_trajectory(rhs_cases, precision):
+    """b > 0 (AnalogSAT/cpu_sat/CpuSat.cpp:232-239).  The GPU sums a in double
+    over a tree and rounds once (the oracle's mean_mode('double')); the
+    reference sums serially in the state type.  RHS: within 16 ulp of the
+    largest |term| of the reference (FP64 fixture) and of the oracle in both
+    mean modes (FP32).  Trajectory, 30 steps: same attempts; the difference in
+    the mean and in sin is a persistent rounding-level change of the vector
+    field, which the flow integrates (on the CPU, switching only the FP32
+    mean's accumulation moves this trajectory by 3.6e-4), so the bound is
+    1e-9 (FP64) / 1e-3 (FP32) relative."""
+    say("bias b=2.5 %s" % precision)
+    d = rhs_cases
+    p = O.Problem(d["bias_lits"], d["bias_lens"])
+    b = float(d["bias_b"])
+    dt = np.float64 if precision == "f64" else np.float32
+    mode = "f64" if precision == "f64" else "mix"
+    sat = _sys_with(d["bias_lits"], d["bias_lens"], precision)
+    sat.Set_B(b)
+    x = d["bias_x"]
+    got = _gpu_rhs(sat, x[1:].astype(dt)).astype(np.float64)
+    ulp = 2.0 ** -52 if precision == "f64" else 2.0 ** -23
+    scale = np.abs(d["bias_rhs"]).max() + np.abs(O.rhs(p, x, b=0.0)[1:]).max()
+    wants = [d["bias_rhs"][1:]] if precision == "f64" else []
+    for mm in ("serial", "double"):
+        with O.mean_mode(mm):
+            wants.append(O.rhs(p, x.astype(dt), mode=mode, b=b)[1:].astype(np.float64))
+    for w in wants:
+        assert np.abs(got - w).max() <= 16 * ulp * scale
+    rk = A.HipRungeKutta()
+    rk.SetEpsilon(1e-6)
+    rk.Configure(sat)
+    old, new = sat.MakeState(), sat.MakeState()
+    old.CopyFrom(x[1:].astype(dt))
+    rk.Step(old, new, 30)
+    g = old.CopyTo().astype(np.float64)
+    for mm in ("serial", "double"):
+        with O.pow_mode("cr"), O.mean_mode(mm):
+            orc = O.RK(p, eps=1e-6, mode=mode, b=b)
+            orc.set_state(x[1:].astype(dt))
+            orc.step(30)
+        ref = orc.get_state().astype(np.float64)
+        rel = np.abs(g - ref).max() / np.abs(ref).max()
+        say("  30 steps vs oracle (mean %s): rel diff %.3e, attempts %d vs %d"
+            % (mm, rel, rk.stats().attempts, orc.attempts))
+        assert rk.stats().attempts == orc.attempts
+        assert rel <= (1e-9 if precision == "f64" else 1e-3)

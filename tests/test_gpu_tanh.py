@@ -4,8 +4,9 @@ against the reference's own outputs and the oracle.
 The slot term calls tanh; the device library's tanh is not glibc's bit for
 bit, so the bar is a tolerance written here, not bit equality:
   * RHS: |GPU - reference| <= 8 ulp of the largest |term| (FP64: 2^-52,
-    FP32: 2^-23) per element, against the committed CpuSatTanh fixtures;
-    the share of bit-identical elements is printed
+    FP32: 2^-23) per element, against the committed CpuSatTanh fixtures
+    (16 ulp with the bias term, whose mean is a tree sum here); the share of
+    bit-identical elements is printed
   * trajectories (50 steps from the C1 start): same accepted step count and
     attempts as the oracle; states within 1e-9 (FP64) / 1e-4 (FP32) relative
 """
@@ -50,7 +51,7 @@ def _term_scale(p, x, q):
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
-@pytest.mark.parametrize("case", ["k3", "k4", "mixed", "k12", "n2000"])
+@pytest.mark.parametrize("case", ["k3", "k4", "mixed", "k12", "n2000", "bias"])
 def test_tanh_rhs_vs_reference(golden_dir, case, precision):
     say("tanh rhs %s %s" % (precision, case))
     r = np.load(os.path.join(golden_dir, "rhs_cases.npz"), allow_pickle=False)
@@ -58,6 +59,7 @@ def test_tanh_rhs_vs_reference(golden_dir, case, precision):
     q = float(t[case + "_q"])
     sat = A.HipSatTanh(precision, q=q)
     sat.SetProblem(A.SatProblem(r[case + "_lits"], r[case + "_lens"]))
+    sat.Set_B(float(r[case + "_b"]))
     assert sat.Get_Q() == pytest.approx(q)
     dt = np.float64 if precision == "f64" else np.float32
     x = r[case + "_x"]
@@ -66,10 +68,11 @@ def test_tanh_rhs_vs_reference(golden_dir, case, precision):
     ulp = 2.0 ** -52 if precision == "f64" else 2.0 ** -23
     p = O.Problem(r[case + "_lits"], r[case + "_lens"])
     scale = max(_term_scale(p, x, q), np.abs(want).max())
+    tol = 8 if float(r[case + "_b"]) == 0.0 else 16   # the bias mean is a tree sum here
     err = np.abs(got - want).max()
     say("  max |diff| = %.3e (%.2f ulp of scale), bit-identical %.1f%%"
         % (err, err / (ulp * scale), 100.0 * np.mean(got == want)))
-    assert err <= 8 * ulp * scale
+    assert err <= tol * ulp * scale
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])

@@ -256,3 +256,31 @@ def test_error_scale_floor(floor):
         assert _ck_errmax(p, x0, rk.hdid, h_scale, 1e-6, floor * 100) <= want
         checked += rk.attempts - att
     assert rk.hdid > 1e-4 and checked >= 14
+
+
+def test_bias_term_and_mean_modes():
+    """The bias term adds mean(a) (M/N) b 4^-k (pi/2) sin(pi s_i) to ds_i
+    (AnalogSAT/cpu_sat/CpuSat.cpp:232-239): the b > 0 minus b = 0 difference
+    equals that expression.  In FP64 the two mean modes coincide; in FP32 the
+    double-accumulated mean is the float-rounded exact sum."""
+    p = O.fastsat(60, 255, 3, 8)
+    pub = O.random_state_f64(5, p.n, p.m)
+    x = np.concatenate([[-1.0], pub])
+    b = 1.75
+    d = O.rhs(p, x, b=b) - O.rhs(p, x, b=0.0)
+    a = x[p.n + 1:]
+    want = a.mean() * (p.m / p.n) * b / 64.0 * (np.pi / 2) * np.sin(np.pi * x[1:p.n + 1])
+    np.testing.assert_allclose(d[1:p.n + 1], want, rtol=1e-10, atol=1e-14)
+    assert np.all(d[p.n + 1:] == 0)
+    with O.mean_mode("double"):
+        d64 = O.rhs(p, x, b=b)
+    assert np.array_equal(d64, O.rhs(p, x, b=b))
+    x32 = x.astype(np.float32)
+    with O.mean_mode("double"):
+        r_dbl = O.rhs(p, x32, mode="mix", b=b)
+    r0 = O.rhs(p, x32, mode="mix", b=0.0)
+    f32 = np.float32
+    mean32 = f32(np.sum(x32[p.n + 1:].astype(np.float64))) / f32(p.m)
+    pref = mean32 * (f32(p.m) / f32(p.n)) * (f32(b) * f32(0.125) * f32(0.125)) * f32(1.570796326794897)
+    want32 = pref * np.sin(f32(np.pi) * x32[1:p.n + 1])
+    np.testing.assert_allclose(r_dbl[1:p.n + 1] - r0[1:p.n + 1], want32, rtol=2e-5, atol=2e-6)
