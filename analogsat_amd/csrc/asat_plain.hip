@@ -10,9 +10,9 @@
 //                     double variables); a group's contributions form one
 //                     region (64-element aligned, at most 144 KB), in slot
 //                     (clause) order
-//   spos, sord        per tile: the tile's slots in ascending contribution
-//                     position (int32 position; uint16 tile-local slot with the
-//                     literal's sign in bit 15)
+//   spos, srank       per tile: the positions of the tile's slots in ascending
+//                     order (int32), and each slot's rank in that order (uint16,
+//                     in slot order)
 //   csr_ptr[0..N+1]   var-major CSR row pointers
 //   csr_loc[e]        uint16 offset of each of a variable's contributions in
 //                     its group's region, in slot order
@@ -83,8 +83,7 @@ struct HostProblem {
     std::vector<uint16_t> csr_loc; // the same relative to the variable's region start
     // tiled clause pass: per tile of TC clauses, its slots sorted by position
     std::vector<int> spos;         // ntiles*TC*k positions (-1 past the tile's last slot)
-    std::vector<uint16_t> sord;    // tile-local slot index (clause*k + q) of each sorted entry,
-                                   // bit 15 set for a negative literal
+    std::vector<uint16_t> srank;   // per slot (slot order): its rank in the tile's sorted list
 };
 
 // SatProblem::AddClauses semantics (AnalogSAT/problem/SatProblem.cpp:64-95) +
@@ -225,7 +224,7 @@ static void build_tile_order(HostProblem& P)
     const long long ntiles = ceil_div((long long)P.m, TC);
     const size_t per = (size_t)TC * K;
     P.spos.assign((size_t)ntiles * per, -1);
-    P.sord.assign((size_t)ntiles * per, 0);
+    P.srank.assign((size_t)ntiles * per, 0xFFFF);
     std::vector<int> cnt((size_t)P.ngroups + 1, 0);
     std::vector<int> touched;
     touched.reserve(per);
@@ -247,7 +246,7 @@ static void build_tile_order(HostProblem& P)
             int g = (std::abs(l) - 1) / P.vg;
             const size_t j = s0 + (size_t)cnt[g]++;
             P.spos[j] = P.pos[s];
-            P.sord[j] = (uint16_t)((s - s0) | (l < 0 ? 0x8000u : 0u));
+            P.srank[s] = (uint16_t)(j - s0);
         }
         for (int g : touched) cnt[g] = 0;
     }
@@ -261,7 +260,7 @@ struct StageArgs {
     const int* lit;
     const int* pos;        // contribution position of each slot (-1: padding); generic-K pass
     const int* spos;       // tiled pass: per tile, positions in ascending order (-1: none)
-    const uint16_t* sord;  // tiled pass: tile-local slot of each sorted entry
+    const uint16_t* srank; // tiled pass: rank of each slot in its tile's sorted list
     const int* csr_ptr;
     const int* csr_pos;    // per-variable contribution positions (gather var pass)
     const uint16_t* csr_loc;   // the same relative to the region start (LDS var pass)
@@ -334,21 +333,43 @@ __device__ __forceinline__ void st_nt(V* p, V v)
 }
 
 // ---- clause pass, compile-time K (Ex order), tiled -----------------------
+// Slot terms of one clause, written to LDS at the slots' ranks in the tile's
+// position order.  CTDS: r = K_m 2^-k a_m times the product of the other
+// factors in KmiProd order, times +-2; tanh: tanh_term with r = 2^-k a_m.
+// Stops at the first padding literal (KmiFunc / KmiFuncTanh).
+template <typename T, int K, int Q>
+__device__ __forceinline__ void slot_terms(const int (&l)[K], const T (&f)[K], T r, bool tanh_family, T q,
+                                           const uint16_t* rank, T* s_out)
+{
+    if constexpr (Q < K) {
+        if (l[Q] != 0) {
+            const T p = kmi_prod<T, K, Q>(f);
+            T rr;
+            if (tanh_family) {
+                rr = tanh_term<T>(r, p, f[Q], q, l[Q] < 0, true);
+            } else {
+                rr = r * p;
+                rr *= l[Q] < 0 ? (T)-2.0 : (T)2.0;
+            }
+            s_out[rank[Q]] = rr;
+            slot_terms<T, K, Q + 1>(l, f, r, tanh_family, q, rank, s_out);
+        }
+    }
+}
+
 // One workgroup of BT threads per tile of TC clauses (XCD-contiguous tile
 // ranges).  Phase 1, a thread per clause (TC / BT clauses each): load every
 // literal, stage spin and aux value of the thread's clauses first, so that
-// the gathers are in flight together, then form K_m, the aux stage input and
-// k_J(a_m), and keep the factors and r = K_m 2^-k a_m in LDS.  Phase 2, a
-// thread per slot in ascending position order: form the slot's contribution
-// from LDS and store it, so that neighbouring lanes write neighbouring
-// positions of one region.
+// the gathers are in flight together; then K_m, the aux stage input, k_J(a_m)
+// and the clause's slot terms, which go to LDS at the slots' ranks in the
+// tile's position order.  Phase 2, a thread per rank: store the terms, so
+// that neighbouring lanes write neighbouring positions of one region.
 template <typename T, int K, int J, int BT>
 __global__ __launch_bounds__(BT) void k_clause_tile(StageArgs<T> a)
 {
     constexpr int CPT = TC / BT;
     __shared__ double sh[BT / 64];
-    __shared__ T s_f[TC * K];
-    __shared__ T s_r[TC];
+    __shared__ T s_out[TC * K];
     const Ctrl* c = a.ctrl;
     int parity = 0;
     double h = 0.0, hnext = 0.0, floor = TabD::TINY;
@@ -361,6 +382,7 @@ __global__ __launch_bounds__(BT) void k_clause_tile(StageArgs<T> a)
     const T* x = parity ? a.xb : a.xa;
     T* xnew = parity ? a.xa : a.xb;
     const T* sp = (J == 1) ? x : a.tmp;
+    const bool tanh_family = a.family == ASAT_FAMILY_TANH;
     const int b = blockIdx.x;
     const int tile = (b & 7) * a.clause_per_xcd + (b >> 3);
     double err = 0.0, asum = 0.0;
@@ -428,38 +450,19 @@ __global__ __launch_bounds__(BT) void k_clause_tile(StageArgs<T> a)
                     err = fmax(err, e);
                     st_nt(xnew + ai, xn);
                 }
-                s_r[cl] = a.family == ASAT_FAMILY_TANH ? a.knorm * am : km * a.knorm * am;
+                uint16_t rk[K];
 #pragma unroll
-                for (int q = 0; q < K; q++) s_f[cl * K + q] = f[q];
+                for (int q = 0; q < K; q++) rk[q] = ld_nt(a.srank + m * K + q);
+                const T r = tanh_family ? a.knorm * am : km * a.knorm * am;
+                slot_terms<T, K, 0>(l[u], f, r, tanh_family, a.q, rk, s_out);
             }
         }
         __syncthreads();
         const int* sp_t = a.spos + c0 * K;
-        const uint16_t* so_t = a.sord + c0 * K;
         for (int j = threadIdx.x; j < TC * K; j += BT) {
             const int p = ld_nt(sp_t + j);
             if (p < 0) break;           // sorted entries form a prefix of the tile's list
-            const unsigned w = ld_nt(so_t + j);
-            const int o = (int)(w & 0x7FFFu);
-            const int cl = o / K, q = o - cl * K;
-            // product of the clause's other factors, left to right (KmiProd order)
-            T prod = (T)1.0;
-            bool first = true;
-#pragma unroll
-            for (int qq = 0; qq < K; qq++) {
-                if (qq == q) continue;
-                const T fv = s_f[cl * K + qq];
-                prod = first ? fv : prod * fv;
-                first = false;
-            }
-            T rr;
-            if (a.family == ASAT_FAMILY_TANH) {
-                rr = tanh_term<T>(s_r[cl], prod, s_f[cl * K + q], a.q, (w & 0x8000u) != 0, true);
-            } else {
-                rr = s_r[cl] * prod;
-                rr *= (w & 0x8000u) ? (T)-2.0 : (T)2.0;
-            }
-            a.contrib[p] = rr;
+            a.contrib[p] = s_out[j];
         }
     }
     if (a.bias) {
@@ -914,7 +917,7 @@ struct Sys {
     int* d_lit = nullptr;
     int* d_pos = nullptr;        // generic-K clause pass only
     int* d_spos = nullptr;       // tiled clause pass (K = 3, 4)
-    uint16_t* d_sord = nullptr;
+    uint16_t* d_srank = nullptr;
     int* d_csr_ptr = nullptr;
     int* d_csr_pos = nullptr;    // gather var pass only
     uint16_t* d_csr_loc = nullptr;   // LDS var pass only
@@ -933,7 +936,7 @@ struct Sys {
 
     void free_problem()
     {
-        dfree(d_lit); dfree(d_pos); dfree(d_spos); dfree(d_sord); dfree(d_csr_ptr); dfree(d_csr_pos);
+        dfree(d_lit); dfree(d_pos); dfree(d_spos); dfree(d_srank); dfree(d_csr_ptr); dfree(d_csr_pos);
         dfree(d_csr_loc); dfree(d_gstart); dfree(d_perm); dfree(d_contrib); dfree(d_apart);
         perm.clear();
     }
@@ -1009,7 +1012,7 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
     upload(s->d_lit, P.lit);
     if (tiled) {
         upload(s->d_spos, P.spos);
-        upload(s->d_sord, P.sord);
+        upload(s->d_srank, P.srank);
     } else {
         P.pos.resize((size_t)round_up(P.m, TC) * P.k, -1);
         upload(s->d_pos, P.pos);
@@ -1094,7 +1097,7 @@ template <typename T>
 static StageArgs<T> stage_args(Sys* s, T* xa, T* xb, T* const* kk, T* tmp, double* partial, const Ctrl* c)
 {
     StageArgs<T> a;
-    a.lit = s->d_lit; a.pos = s->d_pos; a.spos = s->d_spos; a.sord = s->d_sord;
+    a.lit = s->d_lit; a.pos = s->d_pos; a.spos = s->d_spos; a.srank = s->d_srank;
     a.csr_ptr = s->d_csr_ptr; a.csr_pos = s->d_csr_pos; a.csr_loc = s->d_csr_loc; a.gstart = s->d_gstart;
     a.contrib = (T*)s->d_contrib;
     a.xa = xa; a.xb = xb;

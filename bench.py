@@ -405,7 +405,7 @@ def main():
     dom_us = float(us[names.index(dom)])
     achieved = float(kb[dom] / (dom_us * 1e-6) / 1e9)
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_c2.json")
+    tfile = os.path.join(ROOT, "profiles", "plain_c2_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get(dom)
