@@ -276,6 +276,7 @@ struct StageArgs {
     int n, m, k, aoff;
     int nblk_clause;   // clause-pass grid (its J = 6 error partials come first)
     int vg;            // variables per group
+    int var_region_cap;   // LDS var pass: elements reserved for a region
     int vg_blocks;     // gather var pass: blocks per variable group
     int ngroups;
     int clause_per_xcd;   // clause-pass chunks (BLK clauses, or TC-clause tiles) per XCD
@@ -391,13 +392,22 @@ __global__ __launch_bounds__(BT) void k_clause_tile(StageArgs<T> a)
         int l[CPT][K];
         T sv[CPT][K];
         T ax[CPT], a1[CPT], a2[CPT], a3[CPT], a4[CPT], a5[CPT];
-        // literal rows are padded to whole tiles, so they are read unguarded
+        uint16_t rk[CPT][K];
+        int ps[CPT * K];
+        // every load that does not depend on another is issued first: the
+        // literal rows and slot ranks (padded to whole tiles, read unguarded)
+        // and this thread's share of the tile's sorted positions
 #pragma unroll
         for (int u = 0; u < CPT; u++) {
             const long long m = c0 + u * BT + threadIdx.x;
 #pragma unroll
             for (int q = 0; q < K; q++) l[u][q] = ld_nt(a.lit + m * K + q);
+#pragma unroll
+            for (int q = 0; q < K; q++) rk[u][q] = ld_nt(a.srank + m * K + q);
         }
+        const int* sp_t = a.spos + c0 * K;
+#pragma unroll
+        for (int i = 0; i < CPT * K; i++) ps[i] = ld_nt(sp_t + i * BT + threadIdx.x);
 #pragma unroll
         for (int u = 0; u < CPT; u++) {
 #pragma unroll
@@ -450,20 +460,14 @@ __global__ __launch_bounds__(BT) void k_clause_tile(StageArgs<T> a)
                     err = fmax(err, e);
                     st_nt(xnew + ai, xn);
                 }
-                uint16_t rk[K];
-#pragma unroll
-                for (int q = 0; q < K; q++) rk[q] = ld_nt(a.srank + m * K + q);
                 const T r = tanh_family ? a.knorm * am : km * a.knorm * am;
-                slot_terms<T, K, 0>(l[u], f, r, tanh_family, a.q, rk, s_out);
+                slot_terms<T, K, 0>(l[u], f, r, tanh_family, a.q, rk[u], s_out);
             }
         }
         __syncthreads();
-        const int* sp_t = a.spos + c0 * K;
-        for (int j = threadIdx.x; j < TC * K; j += BT) {
-            const int p = ld_nt(sp_t + j);
-            if (p < 0) break;           // sorted entries form a prefix of the tile's list
-            a.contrib[p] = s_out[j];
-        }
+#pragma unroll
+        for (int i = 0; i < CPT * K; i++)
+            if (ps[i] >= 0) a.contrib[ps[i]] = s_out[i * BT + threadIdx.x];
     }
     if (a.bias) {
         const double bs = block_sum_all<BT>(asum, sh);
@@ -639,17 +643,20 @@ __device__ __forceinline__ T bias_sine(T s)
 }
 
 // LDS var pass: one workgroup per variable group.  The group's whole
-// contribution region is read into LDS with coalesced 16-byte loads, then
-// each thread adds its variables' runs from LDS in slot order (the
-// reference's accumulation order) through 16-bit region offsets.
+// contribution region and its variables' 16-bit region offsets (one
+// contiguous CSR range) are read into LDS with coalesced 16-byte loads, all
+// issued before the first LDS write; then each thread adds its variables'
+// runs from LDS in slot order (the reference's accumulation order).
 template <typename T, int J>
 __global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
 {
-    constexpr int CAP = VAR_LDS_BYTES / (int)sizeof(T);
     constexpr int V = 16 / (int)sizeof(T);
+    constexpr int R = 8;              // 16-byte loads in flight per thread and round
     typedef T vT __attribute__((ext_vector_type(V)));
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
     __shared__ double sh[VBLK / 64];
-    __shared__ __attribute__((aligned(16))) T reg[CAP];
+    // dynamic LDS: the region (largest region x sizeof(T)), then the offsets
+    extern __shared__ __attribute__((aligned(16))) unsigned char var_lds[];
     const Ctrl* c = a.ctrl;
     int parity = 0;
     double h = 0.0, hnext = 0.0, floor = TabD::TINY;
@@ -663,24 +670,50 @@ __global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
     const T* x = parity ? a.xb : a.xa;
     T* xnew = parity ? a.xa : a.xb;
     const int g = blockIdx.x;
+    const int i0 = g * a.vg + 1, i1 = (int)std::min<long long>((long long)(g + 1) * a.vg, a.n);
+    T* reg = (T*)var_lds;
+    const uint16_t* loc = (const uint16_t*)(var_lds + (size_t)a.var_region_cap * sizeof(T));
+    int ebase = 0;
     if (fresh) {
-        // region sizes are multiples of REGION_ALIGN and at most CAP (host-checked)
+        // region sizes are multiples of REGION_ALIGN and the offsets of a group
+        // number at most its region size; the host sized the LDS for both and
+        // padded csr_loc by 8 entries for the rounded-up 16-byte loads
         const long long r0 = a.gstart[g];
-        const int len = (int)(a.gstart[g + 1] - r0);
+        const int nv = (int)(a.gstart[g + 1] - r0) / V;
+        const int e_first = a.csr_ptr[i0], e_end = a.csr_ptr[i1 + 1];
+        ebase = e_first & ~7;
+        const int nc = (e_end - ebase + 7) >> 3;
         const vT* src = (const vT*)(a.contrib + r0);
+        const v4u* csrc = (const v4u*)(a.csr_loc + ebase);
         vT* dst = (vT*)reg;
-        for (int v = threadIdx.x; v < len / V; v += VBLK) dst[v] = src[v];
+        v4u* cdst = (v4u*)loc;
+        const int nmax = nv > nc ? nv : nc;
+        for (int base = 0; base < nmax; base += R * VBLK) {
+            vT rb[R];
+            v4u cb[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int v = base + r * VBLK + (int)threadIdx.x;
+                if (v < nv) rb[r] = src[v];
+                if (v < nc) cb[r] = csrc[v];
+            }
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int v = base + r * VBLK + (int)threadIdx.x;
+                if (v < nv) dst[v] = rb[r];
+                if (v < nc) cdst[v] = cb[r];
+            }
+        }
         __syncthreads();
     }
     double err = 0.0;
     const T pref = (a.bias && fresh) ? bias_prefactor<T, VBLK>(a, sh) : (T)0;
-    const int i0 = g * a.vg + 1, i1 = (int)std::min<long long>((long long)(g + 1) * a.vg, a.n);
     for (int i = i0 + (int)threadIdx.x; i <= i1; i += VBLK) {
         T kj = (T)0;
         if (fresh) {
-            const int e0 = a.csr_ptr[i], e1 = a.csr_ptr[i + 1];
+            const int e0 = a.csr_ptr[i] - ebase, e1 = a.csr_ptr[i + 1] - ebase;
             T sum = (T)0;
-            for (int e = e0; e < e1; e++) sum += reg[a.csr_loc[e]];
+            for (int e = e0; e < e1; e++) sum += reg[loc[e]];
             kj = sum;
             if (a.bias) kj += pref * bias_sine<T>(J == 1 ? x[i] : a.tmp[i]);
         }
@@ -929,6 +962,8 @@ struct Sys {
     double q = 1.1;              // tanh family parameter (CpuSatTanh.cpp:39)
     bool tiled = false;          // clause pass: tiled kernel (K = 3, 4) or generic
     bool lds_var = false;        // var pass: LDS region kernel or gather kernel
+    int var_lds = 0;             // LDS region kernel: dynamic LDS bytes (region + offsets)
+    int var_region_cap = 0;      // LDS region kernel: elements reserved for a region
     void* d_contrib = nullptr;
     double* d_apart = nullptr;   // bias term: per clause-block aux sums
     int* d_count = nullptr;
@@ -970,9 +1005,16 @@ static void upload(V*& d, const std::vector<V>& h)
 // keep 256+ variables; halved until the largest region fits in LDS.  When no
 // size fits (a variable with more occurrences than the LDS holds), the gather
 // var pass runs with ~400 KB regions.
+// LDS of the region var pass: the region and its variables' uint16 offsets
+// (at most one per region element), plus room for the rounded 16-byte loads
+static int var_lds_bytes(long long max_region, int esize)
+{
+    return (int)(max_region * esize + round_up(max_region * 2 + 32, 16));
+}
+
 static bool choose_layout(HostProblem& P, int esize)
 {
-    const long long cap = VAR_LDS_BYTES / esize;
+    const long long cap = (VAR_LDS_BYTES - 48) / (esize + 2);
     const long long target = esize == 4 ? 2048 : 1024;
     long long G = std::max(ceil_div((long long)P.n, target), std::min(256, ceil_div((long long)P.n, 256)));
     if (G > 256) G = round_up(G, 256);
@@ -1007,6 +1049,8 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
     s->vg_blocks = ceil_div(P.vg, BLK);
     s->tiled = tiled;
     s->lds_var = lds_var;
+    s->var_region_cap = lds_var ? (int)P.max_region : 0;
+    s->var_lds = lds_var ? var_lds_bytes(P.max_region, s->esize) : 0;
     // clause arrays padded to whole tiles (padding literal 0, position -1)
     P.lit.resize((size_t)round_up(P.m, TC) * P.k, 0);
     upload(s->d_lit, P.lit);
@@ -1018,7 +1062,10 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
         upload(s->d_pos, P.pos);
     }
     upload(s->d_csr_ptr, P.csr_ptr);
-    if (lds_var) upload(s->d_csr_loc, P.csr_loc);
+    if (lds_var) {
+        P.csr_loc.resize(P.csr_loc.size() + 8, 0);   // rounded-up 16-byte loads of the last range
+        upload(s->d_csr_loc, P.csr_loc);
+    }
     else upload(s->d_csr_pos, P.csr_pos);
     upload(s->d_gstart, P.gstart);
     if (!P.perm.empty()) upload(s->d_perm, P.perm);
@@ -1108,6 +1155,7 @@ static StageArgs<T> stage_args(Sys* s, T* xa, T* xb, T* const* kk, T* tmp, doubl
     a.clause_per_xcd = ceil_div(s->tiled ? a.ntiles : ceil_div(s->m, BLK), 8);
     a.nblk_clause = 8 * a.clause_per_xcd;
     a.vg = s->vg;
+    a.var_region_cap = s->var_region_cap;
     a.vg_blocks = s->vg_blocks;
     a.ngroups = s->ngroups;
     a.knorm = (T)std::pow(2.0, -s->k);
@@ -1134,8 +1182,16 @@ static void launch_clause(const StageArgs<T>& a, const Sys* s, hipStream_t st)
 template <typename T, int J>
 static void launch_var(const StageArgs<T>& a, const Sys* s, hipStream_t st)
 {
-    if (s->lds_var) hipLaunchKernelGGL((k_var_region<T, J>), dim3(var_grid(s)), dim3(VBLK), 0, st, a);
-    else hipLaunchKernelGGL((k_var<T, J>), dim3(var_grid(s)), dim3(BLK), 0, st, a);
+    if (s->lds_var) {
+        // allow dynamic LDS beyond the default 64 KB, once per instantiation
+        static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_var_region<T, J>),
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                           VAR_LDS_BYTES);
+        HIP_CHECK(attr);
+        hipLaunchKernelGGL((k_var_region<T, J>), dim3(var_grid(s)), dim3(VBLK), (size_t)s->var_lds, st, a);
+    } else {
+        hipLaunchKernelGGL((k_var<T, J>), dim3(var_grid(s)), dim3(BLK), 0, st, a);
+    }
 }
 
 template <typename T>
