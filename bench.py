@@ -166,6 +166,35 @@ def run_plain_per_instance(insts, max_steps, eps, batch=1):
     return total_steps / total_wall, tts
 
 
+def cpu_cores_rate(insts, eps, steps_each, threads):
+    """The reference's CPU solver (oracle/_ref, FP64, single-threaded by
+    design) on `threads` instances at once, one host thread each, for
+    `steps_each` accepted steps: trajectory-steps/s over all cores."""
+    import concurrent.futures
+    import oracle as O
+    if not O.ref_available():
+        return None
+    R = O.ref()
+
+    def one(item):
+        p, x = item
+        lits = np.ascontiguousarray(p.literals, np.int32)
+        lens = np.ascontiguousarray(p.lengths, np.int32)
+        h = R.ref_rk_create(lits, lens, len(lens), eps)
+        R.ref_rk_set_state(h, np.ascontiguousarray(x, np.float64))
+        R.ref_rk_step(h, steps_each)
+        n = R.ref_rk_steps(h)
+        R.ref_rk_destroy(h)
+        return n
+
+    t0 = time.perf_counter()
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        total = sum(ex.map(one, insts[:threads]))
+    wall = time.perf_counter() - t0
+    return {"value": round(total / wall, 1), "unit": "trajectory-steps/s", "cores": threads, "kind": "reference",
+            "sample": "%d instances, %d accepted steps each, one thread per instance (FP64)" % (threads, steps_each)}
+
+
 def batched_section(workload, count, max_steps, eps, plain_sample, log):
     insts = make_c3(count) if workload == "c3" else make_c4(count)
     b, wall, steps, attempts, res = run_batch(insts, max_steps, eps, log=log)
@@ -183,6 +212,7 @@ def batched_section(workload, count, max_steps, eps, plain_sample, log):
                                "p50_time_to_solution_s": (sorted(plain_tts)[len(plain_tts) // 2]
                                                           if plain_tts else None)},
         "batched_over_plain": round((steps / wall) / plain_rate, 1),
+        "cpu_reference_all_cores": cpu_cores_rate(insts, eps, 400, min(16, os.cpu_count() or 1)),
         "buckets": [{k: v for k, v in x.items()} for x in b.buckets()],
     }
     if workload == "c4":
