@@ -3,10 +3,11 @@
 //
 // Gradient of V(T) = sum_m a_m K_m(x_T)^2 after T accepted steps, with
 // respect to the start state, for a batch of trajectories of one instance.
-// Forward: the plain FP32 integrator (FP64 control) runs each trajectory
-// adaptively, logging every accepted step size on the device and keeping a
-// checkpoint every C steps.  Reverse, all trajectories in lockstep (one per
-// blockIdx.y), segment by segment from the last checkpoint:
+// Forward: every trajectory runs the adaptive Cash-Karp integrator (FP32
+// state, FP64 control, the plain solver's arithmetic) with its own step
+// size, all trajectories in lockstep (one per blockIdx.y); every accepted
+// step size is logged on the device and a checkpoint is kept every C steps.
+// Reverse, again in lockstep, segment by segment from the last checkpoint:
 //   recompute the segment's states with the logged (frozen) step sizes, then
 //   for each step backwards recompute its six stage inputs Y_l and run
 //     mu_l = h (c_l lam' + sum_{j>l} b_jl nu_j),  nu_l = J(Y_l)^T mu_l,
@@ -273,26 +274,107 @@ __global__ void k_zero_slot0(double* v, long long D, int ntraj)
 }
 
 // plain-solver layout (spins 1..N, aux at aoff) -> internal layout (aux at N+1)
-// plain-solver state -> [ -1 | s | a ] in clause order; order[j] is the
-// clause of the plain state's aux position j
-__global__ __launch_bounds__(BLK) void k_from_plain(const float* src, int n, int m, int aoff, const int* order,
-                                                    float* dst)
+// ---- lockstep adaptive forward (FP32 state, FP64 control) -------------------
+// Every trajectory runs CpuRungeKutta's Step/rkqs (AnalogSAT/cpu_sat/
+// CpuRungeKutta.cpp:65-100,217-267) with its own h; all trajectories take
+// their attempts together.  The arithmetic is that of the plain FP32
+// integrator, so each trajectory's accepted steps are the ones that solver
+// takes alone.
+struct TCtl {
+    double h, hnext, time, errmax;
+    int steps, attempts, active, underflow, accept, ckpt;   // ckpt: checkpoint slot to write on accept, or -1
+};
+
+// |xerr / xscal| per element, block maxima per trajectory; xerr in FP64 from
+// the FP32 stage derivatives, xscal = |x| + |k1 hnext| + TINY (as the plain
+// solver's stage-6 passes)
+__global__ __launch_bounds__(BLK) void k_err_parts(Dev d, const float* X, const float* k1, const float* k3,
+                                                   const float* k4, const float* k5, const float* k6,
+                                                   const TCtl* ctl, double* parts, int nblk)
 {
+    __shared__ double sh[BLK / 64];
     const long long i = (long long)blockIdx.x * BLK + threadIdx.x;
-    if (i == 0) {
-        dst[0] = -1.0f;
-    } else if (i <= n) {
-        dst[i] = src[i];
-    } else if (i <= (long long)n + m) {
-        const int j = (int)(i - n - 1);
-        dst[1 + n + order[j]] = src[aoff + j];
+    const long long tr = blockIdx.y;
+    double e = 0.0;
+    if (i <= (long long)d.n + d.m) {
+        const long long o = tr * d.D + i;
+        const double h = ctl[tr].h, hn = ctl[tr].hnext;
+        const double xerr = h * (TabD::dc1 * (double)k1[o] + TabD::dc3 * (double)k3[o] + TabD::dc4 * (double)k4[o] +
+                                 TabD::dc5 * (double)k5[o] + TabD::dc6 * (double)k6[o]);
+        const double xscal = fabs((double)X[o]) + fabs((double)k1[o] * hn) + TabD::TINY;
+        e = fabs(xerr / xscal);
     }
+    const double bm = block_max<BLK>(e, sh);
+    if (threadIdx.x == 0) parts[tr * nblk + blockIdx.x] = bm;
 }
 
-__global__ __launch_bounds__(BLK) void k_h_round(const double* h, double* hh, int n)
+// per-trajectory controller (the plain solver's k_control); one block each
+__global__ __launch_bounds__(BLK) void k_ctl_batch(TCtl* ctl, const double* parts, int nblk, double eps, int T,
+                                                   int C, double* hlog, double* hcur)
 {
-    const int i = blockIdx.x * BLK + threadIdx.x;
-    if (i < n) hh[i] = (double)(float)h[i];
+    __shared__ double sh[BLK / 64];
+    const int tr = blockIdx.x;
+    TCtl& c = ctl[tr];
+    if (!c.active) {
+        // finished: nothing more is accepted or checkpointed for it
+        if (threadIdx.x == 0) { c.accept = 0; c.ckpt = -1; }
+        return;
+    }
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += BLK) v = fmax(v, parts[(long long)tr * nblk + i]);
+    v = block_max<BLK>(v, sh);
+    if (threadIdx.x != 0) return;
+    const double errmax = v / eps;
+    const double h = c.h;
+    c.errmax = errmax;
+    c.attempts += 1;
+    c.accept = 0;
+    c.ckpt = -1;
+    if (errmax <= 1.0) {
+        const double hn = errmax > TabD::ERRCON ? TabD::SAFETY * h * pow_ctrl(errmax, 5) : 5.0 * h;
+        hlog[(long long)tr * T + c.steps] = h;
+        c.time += h;
+        c.steps += 1;
+        c.accept = 1;
+        if (c.steps % C == 0 || c.steps == T) c.ckpt = (c.steps + C - 1) / C;
+        c.hnext = hn;
+        c.h = hn;
+        if (c.steps >= T) c.active = 0;
+    } else {
+        const double htemp = TabD::SAFETY * h * pow_ctrl(errmax, 4);
+        const double hn = h >= 0.0 ? fmax(htemp, TabD::HFACTOR * h) : fmin(htemp, TabD::HFACTOR * h);
+        c.h = hn;
+        if (c.time + hn == c.time) {
+            c.underflow = 1;
+            c.active = 0;
+        }
+    }
+    hcur[tr] = c.h;
+}
+
+// accepted trajectories: x <- x_new, and the checkpoint when one is due
+__global__ __launch_bounds__(BLK) void k_accept(Dev d, const TCtl* ctl, const float* Xn, float* X, float* ckpt,
+                                                long long ntraj)
+{
+    const long long i = (long long)blockIdx.x * BLK + threadIdx.x;
+    const long long tr = blockIdx.y;
+    if (i > (long long)d.n + d.m || !ctl[tr].accept) return;
+    const long long o = tr * d.D + i;
+    const float v = Xn[o];
+    X[o] = v;
+    const int c = ctl[tr].ckpt;
+    if (c >= 0) ckpt[((long long)c * ntraj + tr) * d.D + i] = v;
+}
+
+__global__ void k_ctl_init(TCtl* ctl, double* hcur, int ntraj)
+{
+    const int tr = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tr >= ntraj) return;
+    TCtl c;
+    c.h = 1e-9; c.hnext = 1e-9; c.time = 0.0; c.errmax = 0.0;   // CpuRungeKutta::Reset
+    c.steps = 0; c.attempts = 0; c.active = 1; c.underflow = 0; c.accept = 0; c.ckpt = -1;
+    ctl[tr] = c;
+    hcur[tr] = 1e-9;
 }
 
 // ---------------------------------------------------------------------------
@@ -323,16 +405,21 @@ struct Adj {
     const double* hcurr = nullptr;
     double* vpart = nullptr;
     double* dV = nullptr;
+    TCtl* ctl = nullptr;         // [ntraj] lockstep forward control
+    double* hstep = nullptr;     // [ntraj] current attempt's step size
+    double* eparts = nullptr;    // [ntraj][nblk] error-norm block maxima
     int ncheck = 0;
     double fwd_ms = 0, bwd_ms = 0;
 
     void free_batch()
     {
         for (void* p : {(void*)ckpt, (void*)seg, (void*)Y, (void*)kk, (void*)c32, (void*)c64, (void*)lam,
-                        (void*)mu, (void*)nu, (void*)hlog, (void*)hcol, (void*)hcolr, (void*)vpart, (void*)dV})
+                        (void*)mu, (void*)nu, (void*)hlog, (void*)hcol, (void*)hcolr, (void*)vpart, (void*)dV,
+                        (void*)ctl, (void*)hstep, (void*)eparts})
             (void)hipFree(p);
         ckpt = seg = Y = kk = c32 = nullptr;
-        c64 = lam = mu = nu = hlog = hcol = hcolr = vpart = dV = nullptr;
+        c64 = lam = mu = nu = hlog = hcol = hcolr = vpart = dV = hstep = eparts = nullptr;
+        ctl = nullptr;
     }
     ~Adj()
     {
@@ -415,6 +502,9 @@ static void alloc_batch(Adj& A, int ntraj)
     const int nblk = ceil_div(A.m, BLK);
     HIP_CHECK(hipMalloc((void**)&A.vpart, sizeof(double) * (size_t)ntraj * nblk));
     HIP_CHECK(hipMalloc((void**)&A.dV, sizeof(double) * ntraj));
+    HIP_CHECK(hipMalloc((void**)&A.ctl, sizeof(TCtl) * ntraj));
+    HIP_CHECK(hipMalloc((void**)&A.hstep, sizeof(double) * ntraj));
+    HIP_CHECK(hipMalloc((void**)&A.eparts, sizeof(double) * (size_t)ntraj * ceil_div((long long)A.n + A.m + 1, BLK)));
     HIP_CHECK(hipMemset(A.seg, 0, sizeof(float) * V * (A.C + 1)));
     HIP_CHECK(hipMemset(A.Y, 0, sizeof(float) * V * 6));
     HIP_CHECK(hipMemset(A.kk, 0, sizeof(float) * V * 6));
@@ -473,6 +563,51 @@ static void energy(Adj& A, const float* X, bool want_grad)
         hipLaunchKernelGGL(k_zero_slot0, dim3(ceil_div(A.ntraj, 64)), dim3(64), 0, A.stream, g, A.D, A.ntraj);
     }
     hipLaunchKernelGGL(k_sum_parts, dim3(A.ntraj), dim3(64), 0, A.stream, (const double*)A.vpart, nblk, A.dV);
+}
+
+// Adaptive forward of every trajectory in lockstep: T accepted steps each,
+// step sizes logged in hlog, checkpoints c = 0..ceil(T/C) (x at step c*C,
+// the last one x_T).  X0 holds the start states in the internal layout.
+static void forward_lockstep(Adj& A, const std::vector<float>& X0)
+{
+    const Dev d = A.dev();
+    const size_t V = (size_t)A.ntraj * A.D;
+    float* X = A.seg;          // seg has C + 1 >= 2 slots
+    float* Xn = A.seg + V;
+    HIP_CHECK(hipMemcpyAsync(X, X0.data(), sizeof(float) * V, hipMemcpyHostToDevice, A.stream));
+    HIP_CHECK(hipMemcpyAsync(A.ckpt, X, sizeof(float) * V, hipMemcpyDeviceToDevice, A.stream));
+    hipLaunchKernelGGL(k_ctl_init, dim3(ceil_div(A.ntraj, 64)), dim3(64), 0, A.stream, A.ctl, A.hstep, A.ntraj);
+    A.hcur = A.hstep;
+    const int nblk = ceil_div((long long)A.n + A.m + 1, BLK);
+    const dim3 ge(nblk, A.ntraj);
+    const float* kl[6];
+    for (int i = 0; i < 6; i++) kl[i] = A.kk + i * V;
+    std::vector<TCtl> hc((size_t)A.ntraj);
+    const int per_round = 16;
+    // every trajectory needs at least T attempts; the round loop ends when none is active
+    for (long long round = 0;; round++) {
+        for (int r = 0; r < per_round; r++) {
+            frozen_step(A, X, Xn);
+            hipLaunchKernelGGL(k_err_parts, ge, dim3(BLK), 0, A.stream, d, (const float*)X, kl[0], kl[2], kl[3],
+                               kl[4], kl[5], (const TCtl*)A.ctl, A.eparts, nblk);
+            hipLaunchKernelGGL(k_ctl_batch, dim3(A.ntraj), dim3(BLK), 0, A.stream, A.ctl, (const double*)A.eparts,
+                               nblk, A.eps, A.T, A.C, A.hlog, A.hstep);
+            hipLaunchKernelGGL(k_accept, ge, dim3(BLK), 0, A.stream, d, (const TCtl*)A.ctl, (const float*)Xn, X,
+                               A.ckpt, (long long)A.ntraj);
+        }
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(hc.data(), A.ctl, sizeof(TCtl) * hc.size(), hipMemcpyDeviceToHost, A.stream));
+        HIP_CHECK(hipStreamSynchronize(A.stream));
+        int active = 0;
+        for (const TCtl& c : hc) {
+            if (c.underflow) throw asat::Error(ASAT_UNDERFLOW, "step size underflow in the forward run");
+            active += c.active;
+        }
+        if (!active) break;
+        // each round accepts or rejects every active trajectory per attempt;
+        // a trajectory that needs more than 64 attempts per step is stuck
+        require(round * per_round < 64LL * A.T + 64, ASAT_ERR_HIP, "forward run made no progress");
+    }
 }
 
 // step sizes of step s for every trajectory: hT[s][traj] (and FP32-rounded hTr)
@@ -570,59 +705,16 @@ int asat_adj_forward(asat_adj_t* a, int ntraj, const float* pub0, double* V_out)
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, A.stream));
-    // the plain FP32 integrator, driven through its C ABI
-    asat_sys_t* sys = nullptr;
-    asat_rk_t* rk = nullptr;
-    asat_state_t *so = nullptr, *sn = nullptr;
-    auto chk = [](int rc) {
-        if (rc != ASAT_OK) throw asat::Error(rc, asat_last_error());
-    };
-    struct Cleanup {
-        asat_sys_t*& s; asat_rk_t*& r; asat_state_t*& o; asat_state_t*& nw;
-        ~Cleanup() { asat_state_destroy(o); asat_state_destroy(nw); asat_rk_destroy(r); asat_sys_destroy(s); }
-    } cleanup{sys, rk, so, sn};
-    chk(asat_sys_create(ASAT_F32, &sys));
-    chk(asat_sys_set_problem(sys, A.lits_in.data(), A.lens_in.data(), (int)A.lens_in.size()));
-    chk(asat_rk_create(&rk));
-    chk(asat_rk_set_eps(rk, A.eps));
-    chk(asat_rk_configure(rk, sys));
-    chk(asat_state_create(sys, &so));
-    chk(asat_state_create(sys, &sn));
     const size_t V = (size_t)A.ntraj * A.D;
-    std::vector<int> order((size_t)A.m);
-    chk(asat_sys_clause_order(sys, order.data()));
-    int* d_order = nullptr;
-    HIP_CHECK(hipMalloc((void**)&d_order, sizeof(int) * order.size()));
-    struct FreeOrder {
-        int*& p;
-        ~FreeOrder() { (void)hipFree(p); }
-    } free_order{d_order};
-    HIP_CHECK(hipMemcpy(d_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
+    std::vector<float> x0(V, 0.f);
     for (int t = 0; t < ntraj; t++) {
-        chk(asat_state_set_public(so, pub0 + (size_t)t * (A.n + A.m)));
-        chk(asat_rk_set_step_log(rk, A.hlog + (size_t)t * A.T, A.T));
-        chk(asat_rk_reset(rk));
-        // checkpoint c holds x at step c*C; the last one (c = ceil(T/C)) holds
-        // x_T, also when the final segment is shorter than C
-        for (int c = 0;; c++) {
-            void* dp = nullptr;
-            int aoff = 0;
-            chk(asat_state_device_ptr(so, &dp, &aoff));
-            hipLaunchKernelGGL(k_from_plain, dim3(ceil_div((long long)A.n + A.m + 1, BLK)), dim3(BLK), 0, A.stream,
-                               (const float*)dp, A.n, A.m, aoff, (const int*)d_order,
-                               A.ckpt + (size_t)c * V + (size_t)t * A.D);
-            HIP_CHECK(hipStreamSynchronize(A.stream));
-            if ((long long)c * A.C >= A.T) break;
-            const int todo = std::min(A.C, A.T - c * A.C);
-            int rc = asat_rk_step(rk, so, sn, todo);
-            if (rc == ASAT_UNDERFLOW) throw asat::Error(ASAT_UNDERFLOW, "step size underflow in the forward run");
-            chk(rc);
-        }
+        x0[(size_t)t * A.D] = -1.0f;
+        std::copy(pub0 + (size_t)t * (A.n + A.m), pub0 + (size_t)(t + 1) * (A.n + A.m), x0.begin() + (size_t)t * A.D + 1);
     }
-    chk(asat_rk_set_step_log(rk, nullptr, 0));
+    forward_lockstep(A, x0);
     make_columns(A);
-    // V(T) from the last checkpoint column (= x_T)
-    const int last = A.T / A.C + ((A.T % A.C) ? 1 : 0);
+    // V(T) from the last checkpoint (= x_T)
+    const int last = (A.T + A.C - 1) / A.C;
     energy(A, A.ckpt + (size_t)last * V, false);
     HIP_CHECK(hipEventRecord(e1, A.stream));
     HIP_CHECK(hipEventSynchronize(e1));
