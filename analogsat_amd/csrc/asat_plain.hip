@@ -365,10 +365,102 @@ __device__ __forceinline__ void slot_terms(const int (&l)[K], const T (&f)[K], T
 // and the clause's slot terms, which go to LDS at the slots' ranks in the
 // tile's position order.  Phase 2, a thread per rank: store the terms, so
 // that neighbouring lanes write neighbouring positions of one region.
+// Phase 1 of one tile (see k_clause_tile): slot terms into s_out at their
+// ranks, this thread's share of the tile's sorted positions into ps.
+template <typename T, int K, int J, int BT>
+__device__ __forceinline__ void tile_phase1(const StageArgs<T>& a, long long c0, const T* x, T* xnew,
+                                            const T* sp, T hs, double h, double hnext, double floor,
+                                            bool tanh_family, T* s_out, int (&ps)[TC / BT * K], double& err,
+                                            double& asum)
+{
+    constexpr int CPT = TC / BT;
+    int l[CPT][K];
+    T sv[CPT][K];
+    T ax[CPT], a1[CPT], a2[CPT], a3[CPT], a4[CPT], a5[CPT];
+    uint16_t rk[CPT][K];
+    // every load that does not depend on another is issued first: the
+    // literal rows and slot ranks (padded to whole tiles, read unguarded)
+    // and this thread's share of the tile's sorted positions
+#pragma unroll
+    for (int u = 0; u < CPT; u++) {
+        const long long m = c0 + u * BT + threadIdx.x;
+#pragma unroll
+        for (int q = 0; q < K; q++) l[u][q] = ld_nt(a.lit + m * K + q);
+#pragma unroll
+        for (int q = 0; q < K; q++) rk[u][q] = ld_nt(a.srank + m * K + q);
+    }
+    const int* sp_t = a.spos + c0 * K;
+#pragma unroll
+    for (int i = 0; i < CPT * K; i++) ps[i] = ld_nt(sp_t + i * BT + threadIdx.x);
+#pragma unroll
+    for (int u = 0; u < CPT; u++) {
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const int v = l[u][q] < 0 ? -l[u][q] : l[u][q];
+            sv[u][q] = v != 0 ? sp[v] : (T)-1.0;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < CPT; u++) {
+        const long long m = c0 + u * BT + threadIdx.x;
+        const size_t ai = (size_t)a.aoff + m;
+        ax[u] = a1[u] = a2[u] = a3[u] = a4[u] = a5[u] = (T)0;
+        if (m < a.m) {
+            ax[u] = ld_nt(x + ai);
+            if constexpr (J > 1) a1[u] = ld_nt(a.k1 + ai);
+            if constexpr (J > 2) a2[u] = ld_nt(a.k2 + ai);
+            if constexpr (J > 3) a3[u] = ld_nt(a.k3 + ai);
+            if constexpr (J > 4) a4[u] = ld_nt(a.k4 + ai);
+            if constexpr (J > 5) a5[u] = ld_nt(a.k5 + ai);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < CPT; u++) {
+        const int cl = u * BT + threadIdx.x;
+        const long long m = c0 + cl;
+        if (m < a.m) {
+            T f[K];
+            T km = a.knorm;
+#pragma unroll
+            for (int q = 0; q < K; q++) {
+                f[q] = l[u][q] < 0 ? (T)1.0 + sv[u][q] : (T)1.0 - sv[u][q];
+                km *= f[q];
+            }
+            const size_t ai = (size_t)a.aoff + m;
+            T am;
+            if constexpr (J == 1) am = ax[u];
+            else am = stage_input<T, J>(ax[u], a1[u], a2[u], a3[u], a4[u], a5[u], hs);
+            T kj = am * km * km;
+            asum += (double)am;
+            if constexpr (J == 1) st_nt(a.k1 + ai, kj);
+            else if constexpr (J == 2) st_nt(a.k2 + ai, kj);
+            else if constexpr (J == 3) st_nt(a.k3 + ai, kj);
+            else if constexpr (J == 4) st_nt(a.k4 + ai, kj);
+            else if constexpr (J == 5) st_nt(a.k5 + ai, kj);
+            else {
+                T xn;
+                double e;
+                final_combination<T>(ax[u], a1[u], a3[u], a4[u], a5[u], kj, hs, h, hnext, floor, xn, e);
+                err = fmax(err, e);
+                st_nt(xnew + ai, xn);
+            }
+            const T r = tanh_family ? a.knorm * am : km * a.knorm * am;
+            slot_terms<T, K, 0>(l[u], f, r, tanh_family, a.q, rk[u], s_out);
+        }
+    }
+}
+
+// One workgroup of BT threads per tile of TC clauses (XCD-contiguous tile
+// ranges).  Phase 1, a thread per clause (TC / BT clauses each): load every
+// literal, stage spin and aux value of the thread's clauses first, so that
+// the gathers are in flight together; then K_m, the aux stage input, k_J(a_m)
+// and the clause's slot terms, which go to LDS at the slots' ranks in the
+// tile's position order.  Phase 2, a thread per rank: store the terms, so
+// that neighbouring lanes write neighbouring positions of one region.
 template <typename T, int K, int J, int BT>
 __global__ __launch_bounds__(BT) void k_clause_tile(StageArgs<T> a)
 {
-    constexpr int CPT = TC / BT;
+    constexpr int NP = TC / BT * K;
     __shared__ double sh[BT / 64];
     __shared__ T s_out[TC * K];
     const Ctrl* c = a.ctrl;
@@ -385,88 +477,15 @@ __global__ __launch_bounds__(BT) void k_clause_tile(StageArgs<T> a)
     const T* sp = (J == 1) ? x : a.tmp;
     const bool tanh_family = a.family == ASAT_FAMILY_TANH;
     const int b = blockIdx.x;
-    const int tile = (b & 7) * a.clause_per_xcd + (b >> 3);
     double err = 0.0, asum = 0.0;
+    const int tile = (b & 7) * a.clause_per_xcd + (b >> 3);
     if (tile < a.ntiles) {
-        const long long c0 = (long long)tile * TC;
-        int l[CPT][K];
-        T sv[CPT][K];
-        T ax[CPT], a1[CPT], a2[CPT], a3[CPT], a4[CPT], a5[CPT];
-        uint16_t rk[CPT][K];
-        int ps[CPT * K];
-        // every load that does not depend on another is issued first: the
-        // literal rows and slot ranks (padded to whole tiles, read unguarded)
-        // and this thread's share of the tile's sorted positions
-#pragma unroll
-        for (int u = 0; u < CPT; u++) {
-            const long long m = c0 + u * BT + threadIdx.x;
-#pragma unroll
-            for (int q = 0; q < K; q++) l[u][q] = ld_nt(a.lit + m * K + q);
-#pragma unroll
-            for (int q = 0; q < K; q++) rk[u][q] = ld_nt(a.srank + m * K + q);
-        }
-        const int* sp_t = a.spos + c0 * K;
-#pragma unroll
-        for (int i = 0; i < CPT * K; i++) ps[i] = ld_nt(sp_t + i * BT + threadIdx.x);
-#pragma unroll
-        for (int u = 0; u < CPT; u++) {
-#pragma unroll
-            for (int q = 0; q < K; q++) {
-                const int v = l[u][q] < 0 ? -l[u][q] : l[u][q];
-                sv[u][q] = v != 0 ? sp[v] : (T)-1.0;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < CPT; u++) {
-            const long long m = c0 + u * BT + threadIdx.x;
-            const size_t ai = (size_t)a.aoff + m;
-            ax[u] = a1[u] = a2[u] = a3[u] = a4[u] = a5[u] = (T)0;
-            if (m < a.m) {
-                ax[u] = ld_nt(x + ai);
-                if constexpr (J > 1) a1[u] = ld_nt(a.k1 + ai);
-                if constexpr (J > 2) a2[u] = ld_nt(a.k2 + ai);
-                if constexpr (J > 3) a3[u] = ld_nt(a.k3 + ai);
-                if constexpr (J > 4) a4[u] = ld_nt(a.k4 + ai);
-                if constexpr (J > 5) a5[u] = ld_nt(a.k5 + ai);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < CPT; u++) {
-            const int cl = u * BT + threadIdx.x;
-            const long long m = c0 + cl;
-            if (m < a.m) {
-                T f[K];
-                T km = a.knorm;
-#pragma unroll
-                for (int q = 0; q < K; q++) {
-                    f[q] = l[u][q] < 0 ? (T)1.0 + sv[u][q] : (T)1.0 - sv[u][q];
-                    km *= f[q];
-                }
-                const size_t ai = (size_t)a.aoff + m;
-                T am;
-                if constexpr (J == 1) am = ax[u];
-                else am = stage_input<T, J>(ax[u], a1[u], a2[u], a3[u], a4[u], a5[u], hs);
-                T kj = am * km * km;
-                asum += (double)am;
-                if constexpr (J == 1) st_nt(a.k1 + ai, kj);
-                else if constexpr (J == 2) st_nt(a.k2 + ai, kj);
-                else if constexpr (J == 3) st_nt(a.k3 + ai, kj);
-                else if constexpr (J == 4) st_nt(a.k4 + ai, kj);
-                else if constexpr (J == 5) st_nt(a.k5 + ai, kj);
-                else {
-                    T xn;
-                    double e;
-                    final_combination<T>(ax[u], a1[u], a3[u], a4[u], a5[u], kj, hs, h, hnext, floor, xn, e);
-                    err = fmax(err, e);
-                    st_nt(xnew + ai, xn);
-                }
-                const T r = tanh_family ? a.knorm * am : km * a.knorm * am;
-                slot_terms<T, K, 0>(l[u], f, r, tanh_family, a.q, rk[u], s_out);
-            }
-        }
+        int ps[NP];
+        tile_phase1<T, K, J, BT>(a, (long long)tile * TC, x, xnew, sp, hs, h, hnext, floor, tanh_family, s_out, ps,
+                                 err, asum);
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < CPT * K; i++)
+        for (int i = 0; i < NP; i++)
             if (ps[i] >= 0) a.contrib[ps[i]] = s_out[i * BT + threadIdx.x];
     }
     if (a.bias) {
@@ -1153,7 +1172,7 @@ static StageArgs<T> stage_args(Sys* s, T* xa, T* xb, T* const* kk, T* tmp, doubl
     a.n = s->n; a.m = s->m; a.k = s->k; a.aoff = s->aoff;
     a.ntiles = ceil_div(s->m, TC);
     a.clause_per_xcd = ceil_div(s->tiled ? a.ntiles : ceil_div(s->m, BLK), 8);
-    a.nblk_clause = 8 * a.clause_per_xcd;
+    a.nblk_clause = clause_grid(s);
     a.vg = s->vg;
     a.var_region_cap = s->var_region_cap;
     a.vg_blocks = s->vg_blocks;
