@@ -333,6 +333,299 @@ __global__ __launch_bounds__(BS) void k_batch(Params P)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Register-resident variant: thread tid owns clauses j = tid + u*BS (u < CPT)
+// and variables i = tid + v*BS (v < VPT) for the whole trajectory; their
+// state x and the stage derivatives k1..k5 live in registers, so a step
+// touches global memory only for the literal codes and row pointers (a few
+// hundred KB per trajectory, re-read every stage from L2).  The arithmetic
+// and its order are those of k_batch; x_new of the final stage goes to the
+// k2 registers, which the final combination does not read.
+template <int CPT, int VPT>
+struct RegState {
+    float xa[CPT], a1[CPT], a2[CPT], a3[CPT], a4[CPT], a5[CPT];
+    float xv[VPT], v1[VPT], v2[VPT], v3[VPT], v4[VPT], v5[VPT];
+    int e0[VPT], e1[VPT];
+};
+
+template <int BS, int CPT, int VPT, int J>
+__device__ __forceinline__ void reg_stage(RegState<CPT, VPT>& R, const uint32_t* code, int n, int m, int k,
+                                          float knorm, float* sp, float* contrib, float hs, double h, double hnext,
+                                          double floor, double& err)
+{
+    const int tid = threadIdx.x;
+    // ---------------- clause phase
+#pragma unroll
+    for (int u = 0; u < CPT; u++) {
+        const int j = tid + u * BS;
+        if (j < m) {
+            const uint32_t* cj = code + (long long)j * k;
+            float f[KMAX];
+            uint32_t cc[KMAX];
+            float km = knorm;
+#pragma unroll
+            for (int q = 0; q < KMAX; q++) {
+                f[q] = 1.0f;
+                cc[q] = 0;
+                if (q < k) {
+                    uint32_t c = cj[q];
+                    cc[q] = c;
+                    int v = (int)(c >> 19);
+                    float s = v ? sp[v - 1] : -1.0f;
+                    f[q] = (c >> 18) & 1u ? 1.0f + s : 1.0f - s;
+                    km *= f[q];
+                }
+            }
+            float am;
+            if constexpr (J == 1) am = R.xa[u];
+            else am = stage_input<float, J>(R.xa[u], R.a1[u], R.a2[u], R.a3[u], R.a4[u], R.a5[u], hs);
+            const float kj = am * km * km;
+            if constexpr (J == 1) R.a1[u] = kj;
+            else if constexpr (J == 2) R.a2[u] = kj;
+            else if constexpr (J == 3) R.a3[u] = kj;
+            else if constexpr (J == 4) R.a4[u] = kj;
+            else if constexpr (J == 5) R.a5[u] = kj;
+            else {
+                float xv;
+                double e;
+                final_combination<float>(R.xa[u], R.a1[u], R.a3[u], R.a4[u], R.a5[u], kj, hs, h, hnext, floor, xv, e);
+                R.a2[u] = xv;
+                err = fmax(err, e);
+            }
+            const float r = km * knorm * am;
+#pragma unroll
+            for (int q = 0; q < KMAX; q++) {
+                if (q < k && (cc[q] >> 19) != 0) {
+                    float p = 1.0f;
+                    bool first = true;
+#pragma unroll
+                    for (int qq = 0; qq < KMAX; qq++) {
+                        if (qq == q || qq >= k) continue;
+                        p = first ? f[qq] : p * f[qq];
+                        first = false;
+                    }
+                    float rr = r * p;
+                    rr *= (cc[q] >> 18) & 1u ? -2.0f : 2.0f;
+                    contrib[cc[q] & 0x3FFFFu] = rr;
+                }
+            }
+        }
+        // one clause at a time: keeps the live registers to the state plus
+        // one clause's temporaries
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();
+    // ---------------- var phase
+#pragma unroll
+    for (int v = 0; v < VPT; v++) {
+        const int i = tid + v * BS;
+        if (i < n) {
+            float sum = 0.0f;
+            for (int e = R.e0[v]; e < R.e1[v]; e++) sum += contrib[e];
+            const float kj = sum;
+            const float xi = R.xv[v];
+            if constexpr (J == 1) {
+                R.v1[v] = kj;
+                sp[i] = stage_input<float, 2>(xi, kj, 0.f, 0.f, 0.f, 0.f, hs);
+            } else if constexpr (J == 2) {
+                R.v2[v] = kj;
+                sp[i] = stage_input<float, 3>(xi, R.v1[v], kj, 0.f, 0.f, 0.f, hs);
+            } else if constexpr (J == 3) {
+                R.v3[v] = kj;
+                sp[i] = stage_input<float, 4>(xi, R.v1[v], R.v2[v], kj, 0.f, 0.f, hs);
+            } else if constexpr (J == 4) {
+                R.v4[v] = kj;
+                sp[i] = stage_input<float, 5>(xi, R.v1[v], R.v2[v], R.v3[v], kj, 0.f, hs);
+            } else if constexpr (J == 5) {
+                R.v5[v] = kj;
+                sp[i] = stage_input<float, 6>(xi, R.v1[v], R.v2[v], R.v3[v], R.v4[v], kj, hs);
+            } else {
+                float xn;
+                double e;
+                final_combination<float>(xi, R.v1[v], R.v3[v], R.v4[v], R.v5[v], kj, hs, h, hnext, floor, xn, e);
+                R.v2[v] = xn;
+                err = fmax(err, e);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <int BS, int CPT, int VPT>
+__global__ __launch_bounds__(BS) void k_batch_reg(Params P)
+{
+    extern __shared__ float lds[];          // [stage spins: sp_stride] then contributions
+    __shared__ double red[BS / 64];
+    __shared__ int redi[BS / 64];
+    __shared__ int s_traj;
+    float* sp = lds;
+    float* contrib = lds + P.sp_stride;
+    const int tid = threadIdx.x;
+
+    for (;;) {
+        if (tid == 0) s_traj = atomicAdd(P.queue, 1);
+        __syncthreads();
+        const int slot = s_traj;
+        __syncthreads();
+        if (slot >= P.ntraj) break;
+        const int tr = P.order[slot];
+        const Desc d = P.desc[tr];
+        TrajCtrl* cp = P.ctrl + tr;
+        if (cp->status != RUNNING) continue;    // uniform: every thread reads the same word
+
+        const int n = d.n, m = d.m, k = d.k;
+        const uint32_t* code = P.code + d.code_off;
+        const int* rowptr = P.rowptr + d.row_off;
+        const float knorm = d.knorm;
+        double t = cp->t, h = cp->h, hnext = cp->hnext, hdid = cp->hdid, errmax_last = cp->errmax;
+        long long steps = cp->steps, attempts = cp->attempts, rhscount = cp->rhscount;
+        int parity = cp->parity, min_viol = cp->min_viol, viol = cp->viol;
+        int status = RUNNING;
+        int sat_step = cp->sat_step;
+        double sat_time = cp->sat_time;
+        int done_in_chunk = 0;
+
+        RegState<CPT, VPT> R;
+        {
+            const float* x = (parity ? P.xb : P.xa) + d.state_off;
+#pragma unroll
+            for (int u = 0; u < CPT; u++) {
+                const int j = tid + u * BS;
+                R.xa[u] = j < m ? x[n + j] : 0.f;
+                R.a1[u] = R.a2[u] = R.a3[u] = R.a4[u] = R.a5[u] = 0.f;
+            }
+#pragma unroll
+            for (int v = 0; v < VPT; v++) {
+                const int i = tid + v * BS;
+                R.xv[v] = i < n ? x[i] : 0.f;
+                R.e0[v] = i < n ? rowptr[i] : 0;
+                R.e1[v] = i < n ? rowptr[i + 1] : 0;
+                R.v1[v] = R.v2[v] = R.v3[v] = R.v4[v] = R.v5[v] = 0.f;
+            }
+        }
+
+        while (status == RUNNING && done_in_chunk < P.chunk_steps) {
+            // ---- stage 1 at x: spins into LDS
+#pragma unroll
+            for (int v = 0; v < VPT; v++) {
+                const int i = tid + v * BS;
+                if (i < n) sp[i] = R.xv[v];
+            }
+            __syncthreads();
+            bool accepted = false;
+            bool first_attempt = true;
+            while (!accepted) {
+                const float hs = (float)h;
+                const bool had_stage1 = first_attempt;
+                double err_local = 0.0;
+                if (first_attempt)
+                    reg_stage<BS, CPT, VPT, 1>(R, code, n, m, k, knorm, sp, contrib, hs, h, hnext, P.floor, err_local);
+                first_attempt = false;
+                reg_stage<BS, CPT, VPT, 2>(R, code, n, m, k, knorm, sp, contrib, hs, h, hnext, P.floor, err_local);
+                reg_stage<BS, CPT, VPT, 3>(R, code, n, m, k, knorm, sp, contrib, hs, h, hnext, P.floor, err_local);
+                reg_stage<BS, CPT, VPT, 4>(R, code, n, m, k, knorm, sp, contrib, hs, h, hnext, P.floor, err_local);
+                reg_stage<BS, CPT, VPT, 5>(R, code, n, m, k, knorm, sp, contrib, hs, h, hnext, P.floor, err_local);
+                reg_stage<BS, CPT, VPT, 6>(R, code, n, m, k, knorm, sp, contrib, hs, h, hnext, P.floor, err_local);
+                // ---------------- error norm + control (FP64), uniform in every thread
+                const double errmax = block_max_all<BS>(err_local, red) / P.eps;
+                errmax_last = errmax;
+                attempts += 1;
+                rhscount += had_stage1 ? 6 : 5;
+                if (errmax <= 1.0) {
+                    double hn = errmax > TabD::ERRCON ? TabD::SAFETY * h * pow_ctrl(errmax, 5) : 5.0 * h;
+                    hdid = h;
+                    t += h;
+                    steps += 1;
+                    hnext = hn;
+                    h = hn;
+                    accepted = true;
+                } else {
+                    double htemp = TabD::SAFETY * h * pow_ctrl(errmax, 4);
+                    double hn = h >= 0.0 ? fmax(htemp, TabD::HFACTOR * h) : fmin(htemp, TabD::HFACTOR * h);
+                    if (t + hn == t) {
+                        h = hn;
+                        status = UNDERFLOW_;
+                        break;
+                    }
+                    h = hn;
+                    // retry: stage-2 spin input from x, k1 and the reduced h
+                    const float hs2 = (float)h;
+#pragma unroll
+                    for (int v = 0; v < VPT; v++) {
+                        const int i = tid + v * BS;
+                        if (i < n) sp[i] = stage_input<float, 2>(R.xv[v], R.v1[v], 0.f, 0.f, 0.f, 0.f, hs2);
+                    }
+                    __syncthreads();
+                }
+            }
+            if (status != RUNNING) break;
+            // accept: x <- x_new (held in the k2 registers)
+#pragma unroll
+            for (int u = 0; u < CPT; u++) R.xa[u] = R.a2[u];
+#pragma unroll
+            for (int v = 0; v < VPT; v++) R.xv[v] = R.v2[v];
+            parity ^= 1;
+            done_in_chunk += 1;
+            // ---------------- violated-clause count on the accepted state
+            if (steps % P.check_every == 0 || steps >= P.max_steps) {
+#pragma unroll
+                for (int v = 0; v < VPT; v++) {
+                    const int i = tid + v * BS;
+                    if (i < n) sp[i] = R.xv[v];
+                }
+                __syncthreads();
+                int cnt = 0;
+#pragma unroll
+                for (int u = 0; u < CPT; u++) {
+                    const int j = tid + u * BS;
+                    if (j >= m) continue;
+                    const uint32_t* cj = code + (long long)j * k;
+                    int sat = 0;
+                    for (int q = 0; q < k && !sat; q++) {
+                        uint32_t c = cj[q];
+                        int v = (int)(c >> 19);
+                        if (!v) continue;
+                        int tv = sp[v - 1] > 0.0f;
+                        sat = (c >> 18) & 1u ? !tv : tv;
+                    }
+                    cnt += !sat;
+                }
+                viol = block_sum_all<BS>(cnt, redi);
+                if (viol < min_viol) min_viol = viol;
+                if (viol == 0) {
+                    status = SOLVED;
+                    sat_step = (int)steps;
+                    sat_time = t;
+                }
+            }
+            if (status == RUNNING && steps >= P.max_steps) status = BUDGET;
+            if (status == RUNNING && t >= P.max_time) status = MAXTIME;
+            __syncthreads();
+        }
+        // the state after the last accepted step goes to the parity buffer it names
+        {
+            float* x = (parity ? P.xb : P.xa) + d.state_off;
+#pragma unroll
+            for (int u = 0; u < CPT; u++) {
+                const int j = tid + u * BS;
+                if (j < m) x[n + j] = R.xa[u];
+            }
+#pragma unroll
+            for (int v = 0; v < VPT; v++) {
+                const int i = tid + v * BS;
+                if (i < n) x[i] = R.xv[v];
+            }
+        }
+        if (tid == 0) {
+            cp->t = t; cp->h = h; cp->hnext = hnext; cp->hdid = hdid; cp->errmax = errmax_last;
+            cp->steps = steps; cp->attempts = attempts; cp->rhscount = rhscount;
+            cp->parity = parity; cp->status = status; cp->min_viol = min_viol; cp->viol = viol;
+            cp->sat_step = sat_step; cp->sat_time = sat_time;
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void k_reset_queue(int* q)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *q = 0;
@@ -353,7 +646,9 @@ struct Instance {
 struct Bucket {
     std::vector<int> ids;          // trajectory ids, largest first
     int block = 0;                 // workgroup size (256/512/1024), 0 = autotune
-    int lds = -1;                  // contributions in LDS (1) or global scratch (0), -1 = auto
+    int lds = -1;                  // contributions in LDS (1) or global scratch (0); 2: register-
+                                   // resident kernel (contributions in LDS); -1 = auto
+    int cpt = 0, vpt = 0;          // register kernel: clauses / variables per thread
     int grid = 0;
     int maxn = 0, maxm = 0, maxl = 0;
     int sp_stride = 0;
@@ -476,6 +771,44 @@ static size_t bucket_lds(const Bucket& b, bool lds)
     return sizeof(float) * ((size_t)b.sp_stride + (lds ? (size_t)round_up_ll(b.maxl, 4) : 0));
 }
 
+// register-resident variants (workgroup size, clauses and variables per
+// thread); a bucket uses the first one its largest instance fits
+struct RegVariant {
+    int bs, cpt, vpt;
+};
+static const RegVariant REG_VARIANTS[] = {{256, 1, 1}, {256, 4, 2}, {512, 8, 4}, {1024, 9, 2}, {512, 17, 4}};
+
+// first variant the bucket's largest instance fits (of workgroup size bs, or any)
+static const RegVariant* reg_variant(const Bucket& b, int bs = 0)
+{
+    for (const RegVariant& r : REG_VARIANTS)
+        if ((!bs || r.bs == bs) && b.maxm <= r.bs * r.cpt && b.maxn <= r.bs * r.vpt) return &r;
+    return nullptr;
+}
+
+#define ASAT_REG_VARIANTS(X) X(256, 1, 1) X(256, 4, 2) X(512, 8, 4) X(1024, 9, 2) X(512, 17, 4)
+
+static const void* reg_kernel(int bs, int cpt, int vpt)
+{
+#define ASAT_REG_PTR(B, C, V) \
+    if (bs == B && cpt == C && vpt == V) return (const void*)k_batch_reg<B, C, V>;
+    ASAT_REG_VARIANTS(ASAT_REG_PTR)
+#undef ASAT_REG_PTR
+    return nullptr;
+}
+
+static void launch_reg(hipStream_t st, const Params& P, int grid, int bs, int cpt, int vpt, size_t shmem)
+{
+#define ASAT_REG_LAUNCH(B, C, V)                                                               \
+    if (bs == B && cpt == C && vpt == V) {                                                     \
+        hipLaunchKernelGGL((k_batch_reg<B, C, V>), dim3(grid), dim3(B), shmem, st, P);         \
+        return;                                                                                \
+    }
+    ASAT_REG_VARIANTS(ASAT_REG_LAUNCH)
+#undef ASAT_REG_LAUNCH
+    throw Error(ASAT_ERR_HIP, "no register-resident kernel for this tiling");
+}
+
 template <int BS, bool LDS>
 static void launch_one(hipStream_t st, const Params& P, int grid, size_t shmem)
 {
@@ -492,9 +825,13 @@ static void set_lds_attr(int bs, bool lds, size_t bytes)
     HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
-static void dispatch(hipStream_t st, const Params& P, int grid, int bs, bool lds, size_t shmem)
+static void dispatch(hipStream_t st, const Params& P, const Bucket& b)
 {
-    if (lds) {
+    const int grid = b.grid, bs = b.block;
+    const size_t shmem = b.lds_bytes;
+    if (b.lds == 2) {
+        launch_reg(st, P, grid, bs, b.cpt, b.vpt, shmem);
+    } else if (b.lds == 1) {
         if (bs == 256) launch_one<256, true>(st, P, grid, shmem);
         else if (bs == 512) launch_one<512, true>(st, P, grid, shmem);
         else launch_one<1024, true>(st, P, grid, shmem);
@@ -525,19 +862,35 @@ static int per_cu(const Batch& B, const Bucket& b, int bs, bool lds)
     return std::max(1, std::min(by_lds, by_threads));
 }
 
-static void configure_bucket(Batch& B, Bucket& b, int bs, bool lds)
+// mode: 0 contributions in global scratch, 1 in LDS, 2 register-resident
+// kernel (contributions in LDS; bs comes from its variant)
+static void configure_bucket(Batch& B, Bucket& b, int bs, int mode)
 {
+    const bool lds = mode != 0;
+    if (mode == 2) {
+        const RegVariant* r = reg_variant(b, bs);
+        if (!r) r = reg_variant(b);
+        require(r != nullptr, ASAT_ERR_INVALID_ARGUMENT, "instances too large for the register-resident kernel");
+        bs = r->bs;
+        b.cpt = r->cpt;
+        b.vpt = r->vpt;
+    }
     b.block = bs;
-    b.lds = lds ? 1 : 0;
+    b.lds = mode;
     b.lds_bytes = bucket_lds(b, lds);
     b.grid = std::max(1, std::min((int)b.ids.size(), B.cus * per_cu(B, b, bs, lds)));
     b.scratch_stride = round_up_ll((long long)b.maxn + b.maxm, 64);
-    b.scratch_per_wg = 5 * b.scratch_stride + (lds ? 0 : round_up_ll(b.maxl, 64));
+    b.scratch_per_wg = mode == 2 ? 0 : 5 * b.scratch_stride + (lds ? 0 : round_up_ll(b.maxl, 64));
     (void)hipFree(b.d_scratch);
     b.d_scratch = nullptr;
-    HIP_CHECK(hipMalloc((void**)&b.d_scratch, sizeof(float) * (size_t)b.scratch_per_wg * b.grid));
-    HIP_CHECK(hipMemset(b.d_scratch, 0, sizeof(float) * (size_t)b.scratch_per_wg * b.grid));
-    set_lds_attr(bs, lds, b.lds_bytes);
+    const size_t sbytes = sizeof(float) * std::max<size_t>((size_t)b.scratch_per_wg * b.grid, 1);
+    HIP_CHECK(hipMalloc((void**)&b.d_scratch, sbytes));
+    HIP_CHECK(hipMemset(b.d_scratch, 0, sbytes));
+    if (mode == 2)
+        HIP_CHECK(hipFuncSetAttribute(reg_kernel(bs, b.cpt, b.vpt), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)b.lds_bytes));
+    else
+        set_lds_attr(bs, lds, b.lds_bytes);
 }
 
 static void reset_state(Batch& B)
@@ -561,14 +914,20 @@ static void autotune(Batch& B)
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     for (auto& b : B.buckets) {
-        std::vector<std::pair<int, bool>> cand;
+        std::vector<std::pair<int, int>> cand;   // (workgroup size, mode)
+        for (int bs : {256, 512, 1024}) {
+            const RegVariant* rv = reg_variant(b, bs);
+            if (rv && b.lds_fits && (B.force_lds == -1 || B.force_lds == 2) && (!B.force_block || B.force_block == bs))
+                cand.push_back({bs, 2});
+        }
         for (int bs : {256, 512, 1024}) {
             if (B.force_block && bs != B.force_block) continue;
-            if (b.lds_fits && B.force_lds != 0) cand.push_back({bs, true});
-            if (B.force_lds != 1) cand.push_back({bs, false});
+            if (b.lds_fits && (B.force_lds == -1 || B.force_lds == 1)) cand.push_back({bs, 1});
+            if (B.force_lds == -1 || B.force_lds == 0) cand.push_back({bs, 0});
         }
+        require(!cand.empty(), ASAT_ERR_INVALID_ARGUMENT, "no kernel configuration fits this bucket");
         double best = 1e300;
-        std::pair<int, bool> pick = cand.front();
+        std::pair<int, int> pick = cand.front();
         if (cand.size() > 1) {
             for (auto& c : cand) {
                 configure_bucket(B, b, c.first, c.second);
@@ -576,7 +935,7 @@ static void autotune(Batch& B)
                 Params P = params_of(B, b);
                 hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, B.stream, b.d_queue);
                 HIP_CHECK(hipEventRecord(e0, B.stream));
-                dispatch(B.stream, P, b.grid, b.block, b.lds == 1, b.lds_bytes);
+                dispatch(B.stream, P, b);
                 HIP_CHECK(hipEventRecord(e1, B.stream));
                 HIP_CHECK(hipEventSynchronize(e1));
                 HIP_CHECK(hipGetLastError());
@@ -683,9 +1042,13 @@ static void prepare(Batch& B, bool tune)
     }
     for (auto& b : B.buckets) {
         int bs = B.force_block ? B.force_block : 1024;
-        bool lds = B.force_lds == -1 ? b.lds_fits : B.force_lds == 1;
-        require(!lds || b.lds_fits, ASAT_ERR_UNSUPPORTED, "instances too large for LDS contributions");
-        configure_bucket(B, b, bs, lds);
+        const RegVariant* rv = reg_variant(b);
+        const bool reg_ok = rv && b.lds_fits && (!B.force_block || B.force_block == rv->bs);
+        int mode = B.force_lds == -1 ? (reg_ok ? 2 : (b.lds_fits ? 1 : 0)) : B.force_lds;
+        require(mode == 0 || b.lds_fits, ASAT_ERR_UNSUPPORTED, "instances too large for LDS contributions");
+        require(mode != 2 || reg_variant(b), ASAT_ERR_UNSUPPORTED,
+                "instances too large for the register-resident kernel");
+        configure_bucket(B, b, bs, mode);
     }
     reset_state(B);
     if (tune) autotune(B);
@@ -709,7 +1072,7 @@ static void run(Batch& B, int max_chunks, int graph_chunks)
         for (auto& b : B.buckets) {
             Params P = params_of(B, b);
             hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, B.stream, b.d_queue);
-            dispatch(B.stream, P, b.grid, b.block, b.lds == 1, b.lds_bytes);
+            dispatch(B.stream, P, b);
         }
     }
     HIP_CHECK(hipStreamEndCapture(B.stream, &graph));
@@ -827,7 +1190,7 @@ int asat_batch_set_tiling(asat_batch_t* b, int block, int lds_mode, int nbuckets
     require(b != nullptr, ASAT_ERR_INVALID_ARGUMENT, "null batch");
     require(block == 0 || block == 256 || block == 512 || block == 1024, ASAT_ERR_INVALID_ARGUMENT,
             "block must be 0 (autotune), 256, 512 or 1024");
-    require(lds_mode >= -1 && lds_mode <= 1, ASAT_ERR_INVALID_ARGUMENT, "lds_mode must be -1, 0 or 1");
+    require(lds_mode >= -1 && lds_mode <= 2, ASAT_ERR_INVALID_ARGUMENT, "lds_mode must be -1, 0, 1 or 2");
     require(nbuckets >= 1 && nbuckets <= 64, ASAT_ERR_INVALID_ARGUMENT, "nbuckets must be in [1, 64]");
     b->b.force_block = block; b->b.force_lds = lds_mode; b->b.nbuckets_req = nbuckets;
     b->b.prepared = false;

@@ -222,7 +222,7 @@ typedef struct {
 typedef struct {
     int trajectories, max_n, max_m, max_slots;
     int block;               /* workgroup size chosen (256 / 512 / 1024) */
-    int lds;                 /* 1: contributions in LDS, 0: in global scratch */
+    int lds;                 /* 1: contributions in LDS, 0: in global scratch, 2: register-resident kernel */
     int grid;                /* resident workgroups */
     long long lds_bytes;
     double tuned_us_per_attempt;   /* autotuner's measurement, -1 if not tuned */
@@ -239,8 +239,11 @@ int asat_batch_set_params(asat_batch_t* b, double eps, long long max_steps, doub
                           int check_every);
 /* xscal floor as asat_rk_set_xscal_floor (default 1e-20) */
 int asat_batch_set_xscal_floor(asat_batch_t* b, double floor);
-/* block: 0 autotune or 256/512/1024; lds_mode: -1 auto, 0 global, 1 LDS;
-   nbuckets: size buckets (trajectories split by work, one launch each) */
+/* block: 0 autotune or 256/512/1024; lds_mode: -1 auto, 0 contributions in
+   global scratch, 1 in LDS, 2 register-resident kernel (state and stage
+   derivatives in registers, contributions in LDS; its workgroup size is
+   fixed by the instance size); nbuckets: size buckets (trajectories split by
+   work, one launch each) */
 int asat_batch_set_tiling(asat_batch_t* b, int block, int lds_mode, int nbuckets);
 /* upload and (autotune != 0) pick the tile mapping per size bucket */
 int asat_batch_prepare(asat_batch_t* b, int autotune);

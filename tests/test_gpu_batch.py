@@ -139,3 +139,27 @@ def test_batch_reset_reproduces():
     b.run()
     for i in range(len(insts)):
         assert np.array_equal(first[i], b.state(i))
+
+
+def test_register_kernel_matches_oracle():
+    """The register-resident kernel (lds_mode 2: state and stage derivatives
+    in registers) on instance sizes that select its different variants,
+    including the C3 size (N = 2000, M = 8500): bit-identical to the oracle."""
+    say("register-resident kernel, mixed sizes")
+    insts = make_instances(8, 5, nlo=20, nhi=60, scale=(0.6, 1.0))        # m <= 256: 256 threads x 1 clause
+    insts += make_instances(6, 6, nlo=150, nhi=220, scale=(0.6, 1.0))      # m <= 1024 (k = 3) or 4096
+    for i in range(2):
+        p = O.fastsat(2000, 8500, 3, 7000 + i)
+        pub = np.concatenate([np.random.default_rng(i).uniform(-1, 1, 2000), np.ones(8500)]).astype(np.float32)
+        insts.append((p, pub))
+    b = A.HipBatch()
+    for p, pub in insts:
+        b.add(A.SatProblem(p.literals, p.lengths), pub)
+    b.set_params(eps=1e-6, max_steps=40, chunk_steps=9, check_every=1)
+    b.set_tiling(block=0, lds_mode=2, nbuckets=4)
+    b.prepare(autotune=False)
+    bk = b.buckets()
+    say("buckets: %s" % [(x["trajectories"], x["max_n"], x["max_m"], x["block"], x["lds"]) for x in bk])
+    assert all(x["lds"] == 2 for x in bk)
+    b.run(graph_chunks=2)
+    check_against_oracle(b, insts, 40)
