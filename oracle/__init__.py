@@ -31,6 +31,7 @@ REF_PATH = os.path.join(HERE, "_ref", "libasat_ref.so")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_ldp = np.ctypeslib.ndpointer(np.longdouble, flags="C_CONTIGUOUS")
 _vp = ctypes.c_void_p
 
 _lib = None
@@ -54,15 +55,17 @@ def lib():
         if not os.path.exists(LIB_PATH):
             build()
         L = ctypes.CDLL(LIB_PATH)
-        for sfx, fp, ftype in (("_f64", _f64p, ctypes.c_double), ("_mix", _f32p, ctypes.c_float)):
+        for sfx, fp, ftype in (("_f64", _f64p, ctypes.c_double), ("_mix", _f32p, ctypes.c_float),
+                               ("_ld", _ldp, ctypes.c_longdouble)):
             getattr(L, "orc_rhs" + sfx).argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, ftype, fp, fp]
             getattr(L, "orc_rhs_tanh" + sfx).argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, ftype, ftype,
                                                         fp, fp]
             getattr(L, "orc_rk_set_family" + sfx).argtypes = [_vp, ctypes.c_int, ctypes.c_double]
             getattr(L, "orc_violations" + sfx).argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, fp]
             getattr(L, "orc_violations" + sfx).restype = ctypes.c_int
+            ctype = ctypes.c_longdouble if sfx == "_ld" else ctypes.c_double   # the control type CT
             f = getattr(L, "orc_rk_create" + sfx)
-            f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_double]
+            f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, ctype]
             f.restype = _vp
             getattr(L, "orc_rk_destroy" + sfx).argtypes = [_vp]
             getattr(L, "orc_rk_reset" + sfx).argtypes = [_vp]
@@ -93,7 +96,7 @@ def lib():
             getattr(L, "orc_adjoint_frozen" + sfx).argtypes = [_vp, fp, _f64p, ctypes.c_int, _f64p]
             getattr(L, "orc_adjoint_frozen" + sfx).restype = ctypes.c_double
             f = getattr(L, "orc_solve" + sfx)
-            f.argtypes = [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_int,
+            f.argtypes = [_vp, ctype, ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_int,
                           ctypes.POINTER(ctypes.c_int)]
             f.restype = ctypes.c_int
         L.orc_problem_build.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
@@ -125,6 +128,8 @@ def ref():
         R.ref_rhs_tanh_f64.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_double, ctypes.c_double, _f64p, _f64p]
         R.ref_rhs_tanh_f32.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_double, ctypes.c_double, _f32p, _f32p]
         R.ref_violations_f64.argtypes = [_i32p, _i32p, ctypes.c_int, _f64p]
+        R.ref_traj_ld.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_double, _ldp, ctypes.c_int, _ldp, _ldp]
+        R.ref_traj_ld.restype = ctypes.c_int
         R.ref_violations_f64.restype = ctypes.c_int
         R.ref_rk_create.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_double]
         R.ref_rk_create.restype = _vp
@@ -242,10 +247,14 @@ def pow_cr(x, q):
     return lib().orc_pow_cr(float(x), int(q))
 
 
+def _dtype(mode):
+    return {"f64": np.float64, "mix": np.float32, "ld": np.longdouble}[mode]
+
+
 def rhs(problem, x, mode="f64", b=0.0, family="ctds", q=1.1):
     """One RHS on the internal layout x = [-1, s, a] (length n+m+1);
     family 'ctds' (CpuSat) or 'tanh' (CpuSatTanh with parameter q)."""
-    dt = np.float64 if mode == "f64" else np.float32
+    dt = _dtype(mode)
     x = np.ascontiguousarray(x, dtype=dt)
     out = np.zeros_like(x)
     if family == "tanh":
@@ -275,18 +284,20 @@ def energy_grad(problem, x, mode="f64"):
 
 
 def violations(problem, pub, mode="f64"):
-    dt = np.float64 if mode == "f64" else np.float32
+    dt = _dtype(mode)
     x = np.concatenate([[-1.0], np.asarray(pub, dtype=np.float64)]).astype(dt)
     return getattr(lib(), "orc_violations_" + mode)(problem.n, problem.m, problem.k, problem.C, x)
 
 
 class RK:
-    """Restated CpuRungeKutta bound to one trajectory (mode 'f64' or 'mix')."""
+    """Restated CpuRungeKutta bound to one trajectory: mode 'f64' (the
+    reference's <double>), 'mix' (FP32 state, FP64 control: the GPU's FP32
+    definition) or 'ld' (the reference's <long double>)."""
 
     def __init__(self, problem, eps=1e-4, mode="f64", b=0.0, floor=None, family="ctds", q=1.1):
         self.problem = problem
         self.mode = mode
-        self.dt = np.float64 if mode == "f64" else np.float32
+        self.dt = _dtype(mode)
         self._L = lib()
         self._sfx = "_" + mode
         self._h = getattr(self._L, "orc_rk_create" + self._sfx)(problem.n, problem.m, problem.k, problem.C, eps)

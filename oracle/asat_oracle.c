@@ -76,6 +76,11 @@ static double ctrl_pow(double x, double y)
     return pow(x, y);
 }
 
+#define CT_FABS fabs
+#define CT_FMAX fmax
+#define CT_FMIN fmin
+#define CT_POW(x, y) ctrl_pow((double)(x), (double)(y))
+
 #define ST double
 #define CT double
 #define SFX _f64
@@ -99,6 +104,34 @@ static double ctrl_pow(double x, double y)
 #undef SFX
 #undef ST_SIN
 #undef ST_TANH
+
+#undef CT_FABS
+#undef CT_FMAX
+#undef CT_FMIN
+#undef CT_POW
+
+/* extended precision (x87 80-bit long double): CpuSat<long double> and
+   CpuRungeKutta<long double> (AnalogSAT/cpu_sat/CpuSat.cpp:393,
+   AnalogSAT/cpu_sat/CpuRungeKutta.cpp:273), for FP32/FP64 error studies */
+#define CT_FABS fabsl
+#define CT_FMAX fmaxl
+#define CT_FMIN fminl
+#define CT_POW(x, y) powl((x), (y))
+#define ST long double
+#define CT long double
+#define SFX _ld
+#define ST_SIN sinl
+#define ST_TANH tanhl
+#include "asat_oracle_impl.h"
+#undef ST
+#undef CT
+#undef SFX
+#undef ST_SIN
+#undef ST_TANH
+#undef CT_FABS
+#undef CT_FMAX
+#undef CT_FMIN
+#undef CT_POW
 
 /* ---------------------------------------------------------------------------
  * Problem container: SatProblem::AddClauses (AnalogSAT/problem/SatProblem.cpp:64-95)

@@ -2,9 +2,13 @@
  * Type-generic body of the CPU oracle (test infrastructure only).
  *
  * Included by asat_oracle.c once per precision mode with these macros set:
- *   ST   state / right-hand-side arithmetic type (double or float)
- *   CT   step-control type: time, h, error norm, accept/reject (double)
- *   SFX  symbol suffix (_f64 for ST=double, _mix for ST=float)
+ *   ST   state / right-hand-side arithmetic type (double, float or long double)
+ *   CT   step-control type: time, h, error norm, accept/reject (double, or
+ *        long double with ST = long double)
+ *   SFX  symbol suffix (_f64, _mix for ST=float, _ld for long double)
+ *   ST_SIN, ST_TANH, CT_FABS, CT_FMAX, CT_FMIN: the math of those types;
+ *   CT_POW: the controller's pow (glibc or correctly rounded in double,
+ *   powl in long double as CpuRungeKutta<long double> calls it)
  *
  * Every routine restates a routine of the reference CPU path and follows its
  * floating-point operation order, so that with -ffp-contract=off the FP64 mode
@@ -159,9 +163,10 @@ void FN(orc_rhs_tanh)(int n, int m, int k, const int *C, ST b, ST qq, const ST *
             ST rr;
             if (k >= 3 && k <= 10)
                 /* KmiFuncTanh calls the C library's double tanh (cpu_helper.h is
-                   included before CpuSatTanh.cpp's using-directive), so in float
-                   the product is formed in double and rounded once */
-                rr = (ST)((double)(r * (kmi < zero ? -kmi : kmi)) * tanh((double)(qq * s[q])));
+                   included before CpuSatTanh.cpp's using-directive); the usual
+                   conversions then form the product in double (float state,
+                   rounded once) or in long double */
+                rr = (ST)((r * (kmi < zero ? -kmi : kmi)) * tanh((double)(qq * s[q])));
             else
                 rr = r * (kmi < zero ? -kmi : kmi) * ST_TANH(qq * s[q]);
             rr *= index[q] < 0 ? (ST)-1.0 : one;
@@ -316,18 +321,18 @@ static int FN(rkqs)(FN(orc_rk_t) *r)
     for (;;) {
         FN(rkck)(r, h);
         r->attempts++;
-        for (int i = 0; i < n; i++) r->xerr[i] = fabs(r->xerr[i] / r->xscal[i]);
+        for (int i = 0; i < n; i++) r->xerr[i] = CT_FABS(r->xerr[i] / r->xscal[i]);
         ermax = 0;
-        for (int i = 0; i < n; i++) ermax = fmax(ermax, r->xerr[i]);
+        for (int i = 0; i < n; i++) ermax = CT_FMAX(ermax, r->xerr[i]);
         ermax /= r->eps;
         r->last_errmax = ermax;
         if (ermax <= (CT)1.0) break;
-        CT htemp = SAFETY * h * (CT)ctrl_pow((double)ermax, (double)PSHRNK);
-        h = h >= (CT)0 ? fmax(htemp, HFACTOR * h) : fmin(htemp, HFACTOR * h);
+        CT htemp = SAFETY * h * CT_POW(ermax, PSHRNK);
+        h = h >= (CT)0 ? CT_FMAX(htemp, HFACTOR * h) : CT_FMIN(htemp, HFACTOR * h);
         CT tnew = r->time + h;
         if (tnew == r->time) return 0;
     }
-    if (ermax > ERRCON) r->hnext = SAFETY * h * (CT)ctrl_pow((double)ermax, (double)PGROW);
+    if (ermax > ERRCON) r->hnext = SAFETY * h * CT_POW(ermax, PGROW);
     else r->hnext = (CT)5.0 * h;
     r->hdid = h;
     r->time += r->hdid;
@@ -342,7 +347,7 @@ int FN(orc_rk_step)(FN(orc_rk_t) *r, int count)
     for (int c = 0; c < count; c++) {
         FN(rhs_call)(r, r->dxdt, r->x);
         for (int i = 0; i < n; i++)
-            r->xscal[i] = fabs((CT)r->x[i]) + fabs((CT)r->dxdt[i] * r->hnext) + TINY;
+            r->xscal[i] = CT_FABS((CT)r->x[i]) + CT_FABS((CT)r->dxdt[i] * r->hnext) + TINY;
         if (FN(rkqs)(r)) {
             r->steps++;
             ST *tmp = r->x; r->x = r->xnew; r->xnew = tmp;

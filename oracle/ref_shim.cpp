@@ -59,7 +59,38 @@ static int rhs_tanh(const int* lits, const int* lens, int nclauses, double b, do
     return 0;
 }
 
+// CpuSat<long double> + CpuRungeKutta<long double>: `steps` accepted steps one
+// at a time from pub0; the last state and every step size
+static int traj_ld(const int* lits, const int* lens, int nclauses, double eps, const long double* pub0, int steps,
+                   long double* pub_out, long double* h_out)
+{
+    SatProblem p = make_problem(lits, lens, nclauses);
+    auto sat = std::make_shared<CpuSat<long double>>();
+    sat->SetProblem(p);
+    CpuRungeKutta<long double> rk;
+    rk.SetEpsilon((long double)eps);
+    rk.Configure(sat);
+    std::unique_ptr<ISatState<long double, std::vector<long double>>> a(sat->MakeState()), b(sat->MakeState());
+    const int D = p.Get_N() + p.Get_M();
+    std::vector<long double> v(pub0, pub0 + D);
+    a->CopyFrom(v);
+    for (int s = 0; s < steps; s++) {
+        if (!rk.Step(*a, *b, 1)) return s;
+        h_out[s] = rk.GetLastStepSize();
+    }
+    a->CopyTo(v);
+    for (int i = 0; i < D; i++) pub_out[i] = v[i];
+    return steps;
+}
+
 extern "C" {
+
+int ref_traj_ld(const int* lits, const int* lens, int nclauses, double eps, const long double* pub0, int steps,
+                long double* pub_out, long double* h_out)
+{
+    return traj_ld(lits, lens, nclauses, eps, pub0, steps, pub_out, h_out);
+}
+
 
 int ref_problem_sizes(const int* lits, const int* lens, int nclauses, int* n, int* m, int* k)
 {

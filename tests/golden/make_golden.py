@@ -18,6 +18,9 @@ Fixtures (numpy .npz, no pickles; DIMACS .cnf for instances):
   gen_cases.npz        FastSatMaker literal streams and CpuRandom uniforms
   tanh_cases.npz       CpuSatTanh<double> and CpuSatTanh<float> RHS evaluations
                        (the cases of rhs_cases.npz, each with its own q)
+  c1_traj_ld.npz       CpuSat<long double> + CpuRungeKutta<long double> from the
+                       c1_traj start: states after 10, 100 and 1000 steps and the
+                       1000 step sizes (x87 extended precision)
 """
 import ctypes
 import os
@@ -156,6 +159,18 @@ def main():
         tanh[name + "_rhs64"] = out64
         tanh[name + "_rhs32"] = out32
     np.savez(os.path.join(HERE, "tanh_cases.npz"), **tanh)
+
+    # ---- extended precision trajectory -------------------------------------
+    ld = {"pub0": pub0.astype(np.longdouble)}
+    for steps in (10, 100, 1000):
+        out = np.zeros(len(pub0), np.longdouble)
+        hs = np.zeros(steps, np.longdouble)
+        done = R.ref_traj_ld(lits, lens, len(lens), 1e-6, ld["pub0"], steps, out, hs)
+        assert done == steps
+        ld["x%d" % steps] = out
+        if steps == 1000:
+            ld["hdid"] = hs
+    np.savez(os.path.join(HERE, "c1_traj_ld.npz"), **ld)
 
     # ---- generator / uniform streams ------------------------------------
     gen = {}

@@ -168,3 +168,45 @@ def test_tanh_zero_spin_closed_form():
             want_s[abs(l)] += np.sign(l) * 0.125 * a[j] * np.tanh(q)
     np.testing.assert_allclose(got[1:p.n + 1], want_s[1:], rtol=1e-13, atol=1e-15)
     np.testing.assert_allclose(got[p.n + 1:], a / 64.0, rtol=1e-15)
+
+
+def test_long_double_trajectory_bitexact(golden_dir, c1):
+    """The extended-precision mode (CpuSat<long double> +
+    CpuRungeKutta<long double>, AnalogSAT/cpu_sat/CpuSat.cpp:393,
+    AnalogSAT/cpu_sat/CpuRungeKutta.cpp:273) equals the reference's own
+    long double run: states after 10, 100 and 1000 steps and all 1000 step
+    sizes."""
+    d = np.load(os.path.join(golden_dir, "c1_traj_ld.npz"), allow_pickle=False)
+    p = c1
+    rk = O.RK(p, eps=1e-6, mode="ld")
+    rk.set_state(d["pub0"])
+    h = []
+    for s in range(1, 1001):
+        rk.step(1)
+        h.append(rk.hdid)
+        if s in (10, 100, 1000):
+            assert np.array_equal(rk.get_state(), d["x%d" % s]), s
+    assert np.array_equal(np.array(h), d["hdid"].astype(np.float64))
+
+
+def test_precision_study_against_long_double(golden_dir, c1):
+    """FP64 (the reference's <double>) and the FP32-state mode measured
+    against the extended-precision trajectory on the C1 start.  Before the
+    step sequences part, FP64 stays within a few ulp of extended precision and
+    FP32 within float rounding; both are printed for the record."""
+    d = np.load(os.path.join(golden_dir, "c1_traj_ld.npz"), allow_pickle=False)
+    p = c1
+    out = {}
+    for mode in ("f64", "mix"):
+        rk = O.RK(p, eps=1e-6, mode=mode)
+        rk.set_state(d["pub0"].astype(rk.dt))
+        for s in range(1, 1001):
+            rk.step(1)
+            if s in (10, 100, 1000):
+                ref = d["x%d" % s].astype(np.float64)
+                out[(mode, s)] = np.abs(rk.get_state().astype(np.float64) - ref).max() / np.abs(ref).max()
+    print("relative distance to long double:",
+          {"%s@%d" % k: "%.2e" % v for k, v in out.items()})
+    assert out[("f64", 10)] < 1e-14
+    assert out[("mix", 10)] < 1e-5
+    assert out[("f64", 10)] < out[("mix", 10)]
