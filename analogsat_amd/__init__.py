@@ -135,6 +135,7 @@ def lib():
             "asat_rk_last_step_ms": ([vp, ctypes.POINTER(ctypes.c_float)], i),
             "asat_rk_profile": ([vp, vp, vp, i, vp], i),
             "asat_util_copy_bench": ([ctypes.c_longlong, i, ctypes.POINTER(ctypes.c_float)], i),
+            "asat_util_gpu_name": ([ctypes.c_char_p, i], i),
             "asat_rk_set_step_log": ([vp, vp, ctypes.c_longlong], i),
             "asat_adj_create": ([pvp], i),
             "asat_adj_destroy": ([vp], i),
@@ -175,6 +176,13 @@ def copy_bench(nbytes, reps=10):
     us = ctypes.c_float()
     _check(lib().asat_util_copy_bench(int(nbytes), int(reps), ctypes.byref(us)))
     return us.value
+
+
+def GetGpuName():
+    """Name of the current device (AnalogSAT/util/gpuidtool.cpp:28-40)."""
+    buf = ctypes.create_string_buffer(256)
+    _check(lib().asat_util_gpu_name(buf, len(buf)))
+    return buf.value.decode(errors="replace")
 
 
 def _check(rc):
@@ -562,6 +570,7 @@ class SatSolver:
     def _Reset(self):
         self.rhsCount = 0
         self.walltime = 0.0
+        self.cputime = 0.0
         self.steps = 0
         self.time = 0.0
         self.lastStepSize = 0.0
@@ -577,6 +586,12 @@ class SatSolver:
         if self.running:
             self.walltime = _time.perf_counter() - self._wallstart
         return self.walltime
+
+    def GetElapsedCpuTime(self):
+        """Process CPU seconds since the start of the last Solve (SatSolver.h:107-121)."""
+        if self.running:
+            self.cputime = _time.process_time() - self._cpustart
+        return self.cputime
 
     def SetCallback(self, fn): self.callback = fn
     def ClearCallback(self): self.callback = None
@@ -624,6 +639,7 @@ class SatSolver:
         self._Reset()
         self.odeint.Configure(self.sat)
         self._wallstart = _time.perf_counter()
+        self._cpustart = _time.process_time()
         result = SatResult.SAT_UNKNOWN
         while self.time < self.maxTime and self.steps < self.maxSteps:
             before = self.odeint.GetStepCount()
@@ -648,6 +664,7 @@ class SatSolver:
             elif self.steps >= self.maxSteps:
                 result = SatResult.SAT_MAXITER_REACHED
         self.walltime = _time.perf_counter() - self._wallstart
+        self.cputime = _time.process_time() - self._cpustart
         st = self.odeint.stats()
         self.time = st.time
         self.rhsCount = int(st.rhscount)

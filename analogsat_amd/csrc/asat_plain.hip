@@ -1896,6 +1896,19 @@ int asat_util_copy_bench(long long bytes, int reps, float* us_per_copy)
     API_END
 }
 
+int asat_util_gpu_name(char* buf, int len)
+{
+    API_BEGIN
+    CHECK_PTR(buf);
+    require(len > 0, ASAT_ERR_INVALID_ARGUMENT, "len must be positive");
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+    std::snprintf(buf, (size_t)len, "%s", prop.name);
+    API_END
+}
+
 int asat_rng_create(unsigned long long seed, asat_rng_t** out)
 {
     API_BEGIN

@@ -185,6 +185,10 @@ int asat_rk_profile(asat_rk_t* rk, asat_state_t* old_state, asat_state_t* new_st
    calibrate the HBM byte counters of rocprofv3 against a known traffic. */
 int asat_util_copy_bench(long long bytes, int reps, float* us_per_copy);
 
+/* GetGpuName (AnalogSAT/util/gpuidtool.cpp:28-40): the current device's
+   name, NUL-terminated and truncated to len - 1 bytes. */
+int asat_util_gpu_name(char* buf, int len);
+
 /* ---------------- random numbers (ISatRandom / CudaRandom) ----------------- */
 
 /* own counter-based generator (Philox4x32-10, Salmon et al., SC'11) */
