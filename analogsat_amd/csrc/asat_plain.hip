@@ -9,10 +9,12 @@
 //                     Variables are split into groups (about 2048 float / 1024
 //                     double variables); a group's contributions form one
 //                     region (64-element aligned, at most 144 KB), in slot
-//                     (clause) order
+//                     (clause) order.  Between a var pass and the next clause
+//                     pass the same array holds the slot spins: each slot's
+//                     stage spin at the slot's position
 //   spos, srank       per tile: the positions of the tile's slots in ascending
 //                     order (int32), and each slot's rank in that order (uint16,
-//                     in slot order)
+//                     in slot order, the literal's sign in bit 15)
 //   csr_ptr[0..N+1]   var-major CSR row pointers
 //   csr_loc[e]        uint16 offset of each of a variable's contributions in
 //                     its group's region, in slot order
@@ -22,21 +24,24 @@
 //                     recomputed inside the clause pass and never stored)
 // Problems outside the fast paths use a generic clause pass (K other than 3
 // and 4: pos[m*K + q] per slot) and a gather var pass (a group whose region
-// exceeds the LDS: csr_pos global positions, ~400 KB groups).
+// exceeds the LDS: csr_pos global positions, ~400 KB groups); both gather the
+// stage spins by variable from x / tmp.
 //
 // One RK45 attempt = 6 x (clause pass + var pass) + 1 control kernel:
-//   clause pass J: per tile, gather the stage-J spins, K_m, aux stage input
-//                  and k_J(a_m) = a K^2, then store the tile's contributions
-//                  in position order (one run per region and tile)
+//   clause pass J: per tile, read the slot spins through the tile's sorted
+//                  positions (one run per region), K_m, aux stage input and
+//                  k_J(a_m) = a K^2, then store the tile's contributions at
+//                  the same positions
 //   var pass J:    one workgroup per group reads the group's region into LDS
 //                  with coalesced loads, adds each variable's contributions
 //                  in slot order (== the reference's accumulation order),
-//                  then k_J(s_i) and the spin part of stage J+1's input
+//                  then k_J(s_i), the spin part of stage J+1's input, and
+//                  writes the region back as stage J+1's slot spins
 //   J = 6 passes also form x_new and |xerr/xscal| and a per-block FP64 max;
 //   the control kernel finishes the max-norm and runs the Cash-Karp
 //   accept/reject/step-size logic in FP64 on the device.
-// Step(count) enqueues attempts without host round trips and reads the
-// control block back once per round of attempts.
+// Step(count) fills the slot spins of x, enqueues attempts without host
+// round trips and reads the control block back once per round of attempts.
 #include "asat_common.h"
 
 #include <algorithm>
@@ -83,7 +88,8 @@ struct HostProblem {
     std::vector<uint16_t> csr_loc; // the same relative to the variable's region start
     // tiled clause pass: per tile of TC clauses, its slots sorted by position
     std::vector<int> spos;         // ntiles*TC*k positions (-1 past the tile's last slot)
-    std::vector<uint16_t> srank;   // per slot (slot order): its rank in the tile's sorted list
+    std::vector<uint16_t> srank;   // per slot (slot order): its rank in the tile's sorted list, the
+                                   // literal's sign in bit 15; 0xFFFF for padding
 };
 
 // SatProblem::AddClauses semantics (AnalogSAT/problem/SatProblem.cpp:64-95) +
@@ -246,7 +252,7 @@ static void build_tile_order(HostProblem& P)
             int g = (std::abs(l) - 1) / P.vg;
             const size_t j = s0 + (size_t)cnt[g]++;
             P.spos[j] = P.pos[s];
-            P.srank[s] = (uint16_t)(j - s0);
+            P.srank[s] = (uint16_t)((j - s0) | (l < 0 ? 0x8000u : 0u));
         }
         for (int g : touched) cnt[g] = 0;
     }
@@ -270,6 +276,10 @@ struct StageArgs {
     T* xb;       // state buffer 1 (x = parity ? xb : xa; x_new is the other)
     T* k1; T* k2; T* k3; T* k4; T* k5;
     T* tmp;      // spin stage input (n+1)
+    // slot spins: between a var pass and the next clause pass, contrib holds
+    // the next stage's spins in contribution layout (each slot's spin at the
+    // slot's contribution position); 0: the clause pass gathers from x / tmp
+    int slot_spins;
     double* partial;   // per-block error maxima: clause blocks first, then var blocks
     double* apart;     // bias term: per clause-block sums of the stage aux values
     const Ctrl* ctrl;  // null: standalone RHS (x = xa, output to k1, no tmp)
@@ -352,22 +362,21 @@ __device__ __forceinline__ void slot_terms(const int (&l)[K], const T (&f)[K], T
                 rr = r * p;
                 rr *= l[Q] < 0 ? (T)-2.0 : (T)2.0;
             }
-            s_out[rank[Q]] = rr;
+            s_out[rank[Q] & 0x7FFF] = rr;
             slot_terms<T, K, Q + 1>(l, f, r, tanh_family, q, rank, s_out);
         }
     }
 }
 
-// One workgroup of BT threads per tile of TC clauses (XCD-contiguous tile
-// ranges).  Phase 1, a thread per clause (TC / BT clauses each): load every
-// literal, stage spin and aux value of the thread's clauses first, so that
-// the gathers are in flight together; then K_m, the aux stage input, k_J(a_m)
-// and the clause's slot terms, which go to LDS at the slots' ranks in the
-// tile's position order.  Phase 2, a thread per rank: store the terms, so
-// that neighbouring lanes write neighbouring positions of one region.
 // Phase 1 of one tile (see k_clause_tile): slot terms into s_out at their
 // ranks, this thread's share of the tile's sorted positions into ps.
-template <typename T, int K, int J, int BT>
+// SS (slot spins): sp is contrib holding the stage spins in contribution
+// layout; the tile reads them through its sorted positions (the same runs
+// its phase 2 then overwrites with the slot terms) into s_out at their
+// ranks, and each clause thread takes its slots' spins from there (the
+// literal signs come from the ranks' bit 15).  Otherwise sp is the
+// N-element stage spin vector, gathered by variable.
+template <typename T, int K, int J, int BT, bool SS>
 __device__ __forceinline__ void tile_phase1(const StageArgs<T>& a, long long c0, const T* x, T* xnew,
                                             const T* sp, T hs, double h, double hnext, double floor,
                                             bool tanh_family, T* s_out, int (&ps)[TC / BT * K], double& err,
@@ -379,27 +388,21 @@ __device__ __forceinline__ void tile_phase1(const StageArgs<T>& a, long long c0,
     T ax[CPT], a1[CPT], a2[CPT], a3[CPT], a4[CPT], a5[CPT];
     uint16_t rk[CPT][K];
     // every load that does not depend on another is issued first: the
-    // literal rows and slot ranks (padded to whole tiles, read unguarded)
-    // and this thread's share of the tile's sorted positions
+    // literal rows and slot ranks (padded to whole tiles, read unguarded),
+    // this thread's share of the tile's sorted positions and the aux values
 #pragma unroll
     for (int u = 0; u < CPT; u++) {
         const long long m = c0 + u * BT + threadIdx.x;
+        if constexpr (!SS) {
 #pragma unroll
-        for (int q = 0; q < K; q++) l[u][q] = ld_nt(a.lit + m * K + q);
+            for (int q = 0; q < K; q++) l[u][q] = ld_nt(a.lit + m * K + q);
+        }
 #pragma unroll
         for (int q = 0; q < K; q++) rk[u][q] = ld_nt(a.srank + m * K + q);
     }
     const int* sp_t = a.spos + c0 * K;
 #pragma unroll
     for (int i = 0; i < CPT * K; i++) ps[i] = ld_nt(sp_t + i * BT + threadIdx.x);
-#pragma unroll
-    for (int u = 0; u < CPT; u++) {
-#pragma unroll
-        for (int q = 0; q < K; q++) {
-            const int v = l[u][q] < 0 ? -l[u][q] : l[u][q];
-            sv[u][q] = v != 0 ? sp[v] : (T)-1.0;
-        }
-    }
 #pragma unroll
     for (int u = 0; u < CPT; u++) {
         const long long m = c0 + u * BT + threadIdx.x;
@@ -412,6 +415,34 @@ __device__ __forceinline__ void tile_phase1(const StageArgs<T>& a, long long c0,
             if constexpr (J > 3) a3[u] = ld_nt(a.k3 + ai);
             if constexpr (J > 4) a4[u] = ld_nt(a.k4 + ai);
             if constexpr (J > 5) a5[u] = ld_nt(a.k5 + ai);
+        }
+    }
+    if constexpr (SS) {
+        T so[CPT * K];
+#pragma unroll
+        for (int i = 0; i < CPT * K; i++) so[i] = ps[i] >= 0 ? sp[ps[i]] : (T)0;
+#pragma unroll
+        for (int i = 0; i < CPT * K; i++) s_out[i * BT + threadIdx.x] = so[i];
+        __syncthreads();
+        // each rank is read here and overwritten with its slot term below by
+        // the same thread (the owner of the slot), so one buffer serves both
+#pragma unroll
+        for (int u = 0; u < CPT; u++) {
+#pragma unroll
+            for (int q = 0; q < K; q++) {
+                const unsigned r = rk[u][q];
+                l[u][q] = r == 0xFFFFu ? 0 : ((r & 0x8000u) ? -1 : 1);
+                sv[u][q] = r == 0xFFFFu ? (T)-1.0 : s_out[r & 0x7FFFu];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < CPT; u++) {
+#pragma unroll
+            for (int q = 0; q < K; q++) {
+                const int v = l[u][q] < 0 ? -l[u][q] : l[u][q];
+                sv[u][q] = v != 0 ? sp[v] : (T)-1.0;
+            }
         }
     }
 #pragma unroll
@@ -452,12 +483,14 @@ __device__ __forceinline__ void tile_phase1(const StageArgs<T>& a, long long c0,
 
 // One workgroup of BT threads per tile of TC clauses (XCD-contiguous tile
 // ranges).  Phase 1, a thread per clause (TC / BT clauses each): load every
-// literal, stage spin and aux value of the thread's clauses first, so that
-// the gathers are in flight together; then K_m, the aux stage input, k_J(a_m)
-// and the clause's slot terms, which go to LDS at the slots' ranks in the
-// tile's position order.  Phase 2, a thread per rank: store the terms, so
-// that neighbouring lanes write neighbouring positions of one region.
-template <typename T, int K, int J, int BT>
+// rank, stage spin and aux value of the thread's clauses first, so that the
+// loads are in flight together (with slot spins, the spins arrive through
+// the tile's sorted positions and LDS); then K_m, the aux stage input,
+// k_J(a_m) and the clause's slot terms, which go to LDS at the slots' ranks
+// in the tile's position order.  Phase 2, a thread per rank: store the
+// terms, so that neighbouring lanes write neighbouring positions of one
+// region.
+template <typename T, int K, int J, int BT, bool SS>
 __global__ __launch_bounds__(BT) void k_clause_tile(StageArgs<T> a)
 {
     constexpr int NP = TC / BT * K;
@@ -474,14 +507,14 @@ __global__ __launch_bounds__(BT) void k_clause_tile(StageArgs<T> a)
     const T hs = (T)h;
     const T* x = parity ? a.xb : a.xa;
     T* xnew = parity ? a.xa : a.xb;
-    const T* sp = (J == 1) ? x : a.tmp;
+    const T* sp = SS ? a.contrib : ((J == 1) ? x : a.tmp);
     const bool tanh_family = a.family == ASAT_FAMILY_TANH;
     const int b = blockIdx.x;
     double err = 0.0, asum = 0.0;
     const int tile = (b & 7) * a.clause_per_xcd + (b >> 3);
     if (tile < a.ntiles) {
         int ps[NP];
-        tile_phase1<T, K, J, BT>(a, (long long)tile * TC, x, xnew, sp, hs, h, hnext, floor, tanh_family, s_out, ps,
+        tile_phase1<T, K, J, BT, SS>(a, (long long)tile * TC, x, xnew, sp, hs, h, hnext, floor, tanh_family, s_out, ps,
                                  err, asum);
         __syncthreads();
 #pragma unroll
@@ -609,35 +642,42 @@ __global__ __launch_bounds__(BLK) void k_clause_gen(StageArgs<T> a)
 // ---- var pass: deterministic per-variable sums + spin stage combination ---
 // Stage combination of one variable, shared by both var passes.  kj is the
 // variable's summed derivative (or, on a retry's J = 1, read back from k1).
+// Returns the spin the next clause pass reads: stage J+1's input, or x_new.
 template <typename T, int J>
-__device__ __forceinline__ void var_stage(const StageArgs<T>& a, int i, bool fresh, T kj, const T* x, T* xnew, T hs,
-                                          double h, double hnext, double floor, double& err)
+__device__ __forceinline__ T var_stage(const StageArgs<T>& a, int i, bool fresh, T kj, const T* x, T* xnew, T hs,
+                                       double h, double hnext, double floor, double& err)
 {
     const T xi = ld_nt(x + i);
+    T out = (T)0;
     if constexpr (J == 1) {
         if (fresh) st_nt(a.k1 + i, kj);
         else kj = ld_nt(a.k1 + i);
-        if (a.tmp) a.tmp[i] = stage_input<T, 2>(xi, kj, 0, 0, 0, 0, hs);
+        out = stage_input<T, 2>(xi, kj, 0, 0, 0, 0, hs);
+        if (a.tmp) a.tmp[i] = out;
     } else if constexpr (J == 2) {
         st_nt(a.k2 + i, kj);
-        a.tmp[i] = stage_input<T, 3>(xi, ld_nt(a.k1 + i), kj, 0, 0, 0, hs);
+        out = stage_input<T, 3>(xi, ld_nt(a.k1 + i), kj, 0, 0, 0, hs);
+        a.tmp[i] = out;
     } else if constexpr (J == 3) {
         st_nt(a.k3 + i, kj);
-        a.tmp[i] = stage_input<T, 4>(xi, ld_nt(a.k1 + i), ld_nt(a.k2 + i), kj, 0, 0, hs);
+        out = stage_input<T, 4>(xi, ld_nt(a.k1 + i), ld_nt(a.k2 + i), kj, 0, 0, hs);
+        a.tmp[i] = out;
     } else if constexpr (J == 4) {
         st_nt(a.k4 + i, kj);
-        a.tmp[i] = stage_input<T, 5>(xi, ld_nt(a.k1 + i), ld_nt(a.k2 + i), ld_nt(a.k3 + i), kj, 0, hs);
+        out = stage_input<T, 5>(xi, ld_nt(a.k1 + i), ld_nt(a.k2 + i), ld_nt(a.k3 + i), kj, 0, hs);
+        a.tmp[i] = out;
     } else if constexpr (J == 5) {
         st_nt(a.k5 + i, kj);
-        a.tmp[i] = stage_input<T, 6>(xi, ld_nt(a.k1 + i), ld_nt(a.k2 + i), ld_nt(a.k3 + i), ld_nt(a.k4 + i), kj, hs);
+        out = stage_input<T, 6>(xi, ld_nt(a.k1 + i), ld_nt(a.k2 + i), ld_nt(a.k3 + i), ld_nt(a.k4 + i), kj, hs);
+        a.tmp[i] = out;
     } else {
-        T xn;
         double e;
         final_combination<T>(xi, ld_nt(a.k1 + i), ld_nt(a.k3 + i), ld_nt(a.k4 + i), ld_nt(a.k5 + i), kj, hs, h, hnext,
-                             floor, xn, e);
+                             floor, out, e);
         err = fmax(err, e);
-        st_nt(xnew + i, xn);
+        st_nt(xnew + i, out);
     }
+    return out;
 }
 
 // Bias term (AnalogSAT/cpu_sat/CpuSat.cpp:232-239, CudaSatBase.cu:75-121):
@@ -665,12 +705,17 @@ __device__ __forceinline__ T bias_sine(T s)
 // contribution region and its variables' 16-bit region offsets (one
 // contiguous CSR range) are read into LDS with coalesced 16-byte loads, all
 // issued before the first LDS write; then each thread adds its variables'
-// runs from LDS in slot order (the reference's accumulation order).
+// runs from LDS in slot order (the reference's accumulation order).  With
+// slot spins on, each thread then writes its variables' output spins (stage
+// J+1's input, or x_new) over their own region entries, and the workgroup
+// streams the region back to contrib as the next clause pass's slot spins.
+// Each region entry belongs to one slot of one variable, so the entries a
+// thread reads and overwrites are its own.
 template <typename T, int J>
 __global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
 {
     constexpr int V = 16 / (int)sizeof(T);
-    constexpr int R = 8;              // 16-byte loads in flight per thread and round
+    constexpr int R = 4;              // 16-byte loads in flight per thread and round
     typedef T vT __attribute__((ext_vector_type(V)));
     typedef unsigned int v4u __attribute__((ext_vector_type(4)));
     __shared__ double sh[VBLK / 64];
@@ -692,13 +737,16 @@ __global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
     const int i0 = g * a.vg + 1, i1 = (int)std::min<long long>((long long)(g + 1) * a.vg, a.n);
     T* reg = (T*)var_lds;
     const uint16_t* loc = (const uint16_t*)(var_lds + (size_t)a.var_region_cap * sizeof(T));
+    T* s_next = a.slot_spins ? a.contrib : nullptr;
+    const long long r0 = a.gstart[g];
+    const int nreg = (int)(a.gstart[g + 1] - r0) / V;
     int ebase = 0;
-    if (fresh) {
+    if (fresh || s_next) {
         // region sizes are multiples of REGION_ALIGN and the offsets of a group
         // number at most its region size; the host sized the LDS for both and
-        // padded csr_loc by 8 entries for the rounded-up 16-byte loads
-        const long long r0 = a.gstart[g];
-        const int nv = (int)(a.gstart[g + 1] - r0) / V;
+        // padded csr_loc by 8 entries for the rounded-up 16-byte loads.  A
+        // retry's J = 1 pass needs only the offsets (for the slot spins).
+        const int nv = fresh ? nreg : 0;
         const int e_first = a.csr_ptr[i0], e_end = a.csr_ptr[i1 + 1];
         ebase = e_first & ~7;
         const int nc = (e_end - ebase + 7) >> 3;
@@ -713,8 +761,8 @@ __global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const int v = base + r * VBLK + (int)threadIdx.x;
-                if (v < nv) rb[r] = src[v];
-                if (v < nc) cb[r] = csrc[v];
+                if (v < nv) rb[r] = ld_nt(src + v);
+                if (v < nc) cb[r] = ld_nt(csrc + v);
             }
 #pragma unroll
             for (int r = 0; r < R; r++) {
@@ -729,14 +777,24 @@ __global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
     const T pref = (a.bias && fresh) ? bias_prefactor<T, VBLK>(a, sh) : (T)0;
     for (int i = i0 + (int)threadIdx.x; i <= i1; i += VBLK) {
         T kj = (T)0;
+        const int e0 = a.csr_ptr[i] - ebase, e1 = a.csr_ptr[i + 1] - ebase;
         if (fresh) {
-            const int e0 = a.csr_ptr[i] - ebase, e1 = a.csr_ptr[i + 1] - ebase;
             T sum = (T)0;
             for (int e = e0; e < e1; e++) sum += reg[loc[e]];
             kj = sum;
             if (a.bias) kj += pref * bias_sine<T>(J == 1 ? x[i] : a.tmp[i]);
         }
-        var_stage<T, J>(a, i, fresh, kj, x, xnew, hs, h, hnext, floor, err);
+        const T out = var_stage<T, J>(a, i, fresh, kj, x, xnew, hs, h, hnext, floor, err);
+        if (s_next)
+            for (int e = e0; e < e1; e++) reg[loc[e]] = out;
+    }
+    if (s_next) {
+        // the region's entries not owned by a variable (alignment padding)
+        // are written out too, with whatever the LDS holds; no slot reads them
+        __syncthreads();
+        const vT* src = (const vT*)reg;
+        vT* dst = (vT*)(s_next + r0);
+        for (int v = (int)threadIdx.x; v < nreg; v += VBLK) st_nt(dst + v, src[v]);
     }
     if constexpr (J == 6) {
         double bm = block_max<VBLK>(err, sh);
@@ -917,6 +975,19 @@ __global__ void k_uniform(T* out, int len, unsigned long long seed, unsigned lon
 
 // s = 2u - 1 for spins, a = u + 1 for aux (CpuSat.cpp:127-132); input uniforms
 // are laid out in the public order [s, a] inside the state buffer's spin area
+// Slot spins of a state: each variable's spin at all of its contribution
+// positions (the layout the LDS var pass writes after each stage).
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_slot_fill(const T* x, const int* csr_ptr, const uint16_t* csr_loc,
+                                                   const long long* gstart, int vg, int n, T* sl)
+{
+    const int i = blockIdx.x * BLK + (int)threadIdx.x + 1;
+    if (i > n) return;
+    const T v = x[i];
+    T* base = sl + gstart[(i - 1) / vg];
+    for (int e = csr_ptr[i]; e < csr_ptr[i + 1]; e++) base[csr_loc[e]] = v;
+}
+
 template <typename T>
 __global__ void k_set_initial(T* x, int n, int m, int aoff, const T* u, const int* perm)
 {
@@ -1169,6 +1240,7 @@ static StageArgs<T> stage_args(Sys* s, T* xa, T* xb, T* const* kk, T* tmp, doubl
     a.xa = xa; a.xb = xb;
     a.k1 = kk[0]; a.k2 = kk[1]; a.k3 = kk[2]; a.k4 = kk[3]; a.k5 = kk[4];
     a.tmp = tmp; a.partial = partial; a.ctrl = c;
+    a.slot_spins = 0;
     a.n = s->n; a.m = s->m; a.k = s->k; a.aoff = s->aoff;
     a.ntiles = ceil_div(s->m, TC);
     a.clause_per_xcd = ceil_div(s->tiled ? a.ntiles : ceil_div(s->m, BLK), 8);
@@ -1193,8 +1265,12 @@ static void launch_clause(const StageArgs<T>& a, const Sys* s, hipStream_t st)
 {
     dim3 g(a.nblk_clause), b(BLK);
     const bool tiled = s->tiled;
-    if (tiled && a.k == 3) hipLaunchKernelGGL((k_clause_tile<T, 3, J, TILE_BT>), g, dim3(TILE_BT), 0, st, a);
-    else if (tiled && a.k == 4) hipLaunchKernelGGL((k_clause_tile<T, 4, J, TILE_BT>), g, dim3(TILE_BT), 0, st, a);
+    const bool ss = a.slot_spins != 0;
+    if (tiled && a.k == 3 && ss) hipLaunchKernelGGL((k_clause_tile<T, 3, J, TILE_BT, true>), g, dim3(TILE_BT), 0, st, a);
+    else if (tiled && a.k == 4 && ss)
+        hipLaunchKernelGGL((k_clause_tile<T, 4, J, TILE_BT, true>), g, dim3(TILE_BT), 0, st, a);
+    else if (tiled && a.k == 3) hipLaunchKernelGGL((k_clause_tile<T, 3, J, TILE_BT, false>), g, dim3(TILE_BT), 0, st, a);
+    else if (tiled && a.k == 4) hipLaunchKernelGGL((k_clause_tile<T, 4, J, TILE_BT, false>), g, dim3(TILE_BT), 0, st, a);
     else hipLaunchKernelGGL((k_clause_gen<T, J>), g, b, 0, st, a);
 }
 
@@ -1335,6 +1411,28 @@ static void rk_configure(RK* r, Sys* s)
     rk_reset(r);
 }
 
+// Stage arguments of the integrator.  With slot spins (tiled clause pass and
+// LDS var pass), contrib is first filled with the slot spins of the current
+// state x, which the caller may have changed since the last Step.  Inside
+// Step the var passes keep contrib holding the next clause pass's spins: a
+// rejected attempt's J = 1 clause pass is skipped, and its J = 1 var pass
+// (which reuses k1 and reads no contributions) writes stage 2's spins.
+template <typename T>
+static StageArgs<T> rk_stage_args(RK* r, int parity)
+{
+    Sys* s = r->sys;
+    T* kk[5] = {(T*)r->k[0], (T*)r->k[1], (T*)r->k[2], (T*)r->k[3], (T*)r->k[4]};
+    StageArgs<T> a = stage_args<T>(s, (T*)r->pair[0], (T*)r->pair[1], kk, (T*)r->tmp, r->partial, r->d_ctrl);
+    if (s->tiled && s->lds_var) {
+        a.slot_spins = 1;
+        hipLaunchKernelGGL(k_slot_fill<T>, dim3(ceil_div(s->n, BLK)), dim3(BLK), 0, s->stream,
+                           (const T*)r->pair[parity], s->d_csr_ptr, s->d_csr_loc, s->d_gstart, s->vg, s->n,
+                           (T*)s->d_contrib);
+        HIP_CHECK(hipGetLastError());
+    }
+    return a;
+}
+
 template <typename T>
 static int rk_step_impl(RK* r, State* olds, State* news, int count)
 {
@@ -1346,9 +1444,8 @@ static int rk_step_impl(RK* r, State* olds, State* news, int count)
         r->pair[1] = news->d;
     }
     const int parity = (r->pair[0] == olds->d) ? 0 : 1;
-    T* kk[5] = {(T*)r->k[0], (T*)r->k[1], (T*)r->k[2], (T*)r->k[3], (T*)r->k[4]};
-    StageArgs<T> a = stage_args<T>(s, (T*)r->pair[0], (T*)r->pair[1], kk, (T*)r->tmp, r->partial, r->d_ctrl);
     HIP_CHECK(hipEventRecord(r->ev0, st));
+    StageArgs<T> a = rk_stage_args<T>(r, parity);
     hipLaunchKernelGGL(k_ctrl_begin, dim3(1), dim3(64), 0, st, r->d_ctrl, (long long)count, parity);
     long long remaining = count;
     const long long target = r->h_ctrl->steps + count;
@@ -1407,8 +1504,7 @@ static void rk_profile_impl(RK* r, State* olds, State* news, int reps, float* us
         r->pair[1] = news->d;
     }
     const int parity = (r->pair[0] == olds->d) ? 0 : 1;
-    T* kk[5] = {(T*)r->k[0], (T*)r->k[1], (T*)r->k[2], (T*)r->k[3], (T*)r->k[4]};
-    StageArgs<T> a = stage_args<T>(s, (T*)r->pair[0], (T*)r->pair[1], kk, (T*)r->tmp, r->partial, r->d_ctrl);
+    StageArgs<T> a = rk_stage_args<T>(r, parity);
     Ctrl* backup = nullptr;
     HIP_CHECK(hipMalloc((void**)&backup, sizeof(Ctrl)));
     HIP_CHECK(hipMemcpyAsync(backup, r->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToDevice, st));

@@ -43,21 +43,22 @@ def kernel_bytes(n, m, k, nslots, esize=4):
     the kernel must touch, once; index arrays int32/uint16, state arrays of
     esize bytes).
 
-    clause pass J (tiled, K = 3, 4): per slot a literal (4), a sorted position
-    (4) and a tile-local slot index (2) read and one contribution stored
-    (esize); per clause A_J aux values (x_a and k_1..k_{J-1} read, k_J or x_new
-    written).  Spin gathers hit the N-element stage vector, which stays in
-    cache, and are not counted.
+    clause pass J (tiled, K = 3, 4, slot spins): per slot a sorted position
+    (4) and a rank with the literal's sign (2) read, the slot's stage spin
+    (esize) read through the sorted position and one contribution (esize)
+    stored; per clause A_J aux values (x_a and k_1..k_{J-1} read, k_J or x_new
+    written).
     var pass J (LDS regions): per slot a contribution (esize) and a 16-bit
-    region offset (2); per variable a row pointer (4) and B_J state values.
+    region offset (2) read and the next stage's slot spin (esize) written;
+    per variable a row pointer (4) and B_J state values.
     """
     L = nslots
     A = (2, 3, 4, 5, 6, 7)
     B = (3, 4, 5, 6, 7, 6)
     out = {}
     for j in range(6):
-        out["clause%d" % (j + 1)] = (10 + esize) * L + esize * m * A[j]
-        out["var%d" % (j + 1)] = (2 + esize) * L + 4 * (n + 1) + esize * n * B[j]
+        out["clause%d" % (j + 1)] = (6 + 2 * esize) * L + esize * m * A[j]
+        out["var%d" % (j + 1)] = (2 + 2 * esize) * L + 4 * (n + 1) + esize * n * B[j]
     out["control"] = 8 * (-(-m // 1024) + -(-n // 1024))
     return out
 
