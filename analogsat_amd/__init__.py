@@ -134,6 +134,8 @@ def lib():
             "asat_rk_stats": ([vp, ctypes.POINTER(RkStats)], i),
             "asat_rk_last_step_ms": ([vp, ctypes.POINTER(ctypes.c_float)], i),
             "asat_rk_profile": ([vp, vp, vp, i, vp], i),
+            "asat_rk_set_kernel_timer": ([vp, i], i),
+            "asat_rk_kernel_timer": ([vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)], i),
             "asat_util_copy_bench": ([ctypes.c_longlong, i, ctypes.POINTER(ctypes.c_float)], i),
             "asat_util_gpu_name": ([ctypes.c_char_p, i], i),
             "asat_rk_set_step_log": ([vp, vp, ctypes.c_longlong], i),
@@ -539,6 +541,17 @@ class HipRungeKutta:
         us = np.zeros(13, np.float32)
         _check(lib().asat_rk_profile(self._h, old_state._h, new_state._h, int(reps), us.ctypes.data))
         return us
+
+    def set_kernel_timer(self, kind):
+        """Time one kernel kind (0..12, the profile_kernels order; -1: off)
+        inside every attempt of the following Step calls."""
+        _check(lib().asat_rk_set_kernel_timer(self._h, int(kind)))
+
+    def kernel_timer(self):
+        """(total ms, launches) of the kernel timer since it was set."""
+        ms, n = ctypes.c_double(), ctypes.c_longlong()
+        _check(lib().asat_rk_kernel_timer(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
 
     def last_step_ms(self):
         v = ctypes.c_float()

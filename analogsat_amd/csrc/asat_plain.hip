@@ -65,7 +65,7 @@ static inline long long round_up(long long a, long long b) { return (a + b - 1) 
 // ===========================================================================
 // problem (host)
 // ===========================================================================
-constexpr int TC = 1024;                  // clauses per tile of the tiled clause pass (one workgroup)
+constexpr int TC = 2048;                  // clauses per tile of the tiled clause pass (one workgroup)
 constexpr int TILE_BT = 1024;             // threads per tile workgroup (a clause each)
 constexpr int VBLK = 1024;                // var-pass workgroup (one variable group)
 constexpr int VAR_LDS_BYTES = 147456;     // LDS holding one group's contribution region
@@ -794,7 +794,8 @@ __global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
         __syncthreads();
         const vT* src = (const vT*)reg;
         vT* dst = (vT*)(s_next + r0);
-        for (int v = (int)threadIdx.x; v < nreg; v += VBLK) st_nt(dst + v, src[v]);
+        // ordinary stores: the next clause pass reads these lines right away
+        for (int v = (int)threadIdx.x; v < nreg; v += VBLK) dst[v] = src[v];
     }
     if constexpr (J == 6) {
         double bm = block_max<VBLK>(err, sh);
@@ -1289,17 +1290,31 @@ static void launch_var(const StageArgs<T>& a, const Sys* s, hipStream_t st)
     }
 }
 
+// One attempt.  tkind >= 0: events e0 / e1 are recorded just before and
+// after the kernel of that kind (clause J = kind + 1 for 0..5, var J =
+// kind - 5 for 6..11, control for 12).
 template <typename T>
 static void launch_attempt(const StageArgs<T>& a, const Sys* s, Ctrl* ctrl, const double* partial, int np,
-                           hipStream_t st)
+                           hipStream_t st, int tkind = -1, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
 {
-    launch_clause<T, 1>(a, s, st); launch_var<T, 1>(a, s, st);
-    launch_clause<T, 2>(a, s, st); launch_var<T, 2>(a, s, st);
-    launch_clause<T, 3>(a, s, st); launch_var<T, 3>(a, s, st);
-    launch_clause<T, 4>(a, s, st); launch_var<T, 4>(a, s, st);
-    launch_clause<T, 5>(a, s, st); launch_var<T, 5>(a, s, st);
-    launch_clause<T, 6>(a, s, st); launch_var<T, 6>(a, s, st);
+    auto mark = [&](int kind, hipEvent_t e) {
+        if (kind == tkind) HIP_CHECK(hipEventRecord(e, st));
+    };
+    mark(0, e0); launch_clause<T, 1>(a, s, st); mark(0, e1);
+    mark(6, e0); launch_var<T, 1>(a, s, st); mark(6, e1);
+    mark(1, e0); launch_clause<T, 2>(a, s, st); mark(1, e1);
+    mark(7, e0); launch_var<T, 2>(a, s, st); mark(7, e1);
+    mark(2, e0); launch_clause<T, 3>(a, s, st); mark(2, e1);
+    mark(8, e0); launch_var<T, 3>(a, s, st); mark(8, e1);
+    mark(3, e0); launch_clause<T, 4>(a, s, st); mark(3, e1);
+    mark(9, e0); launch_var<T, 4>(a, s, st); mark(9, e1);
+    mark(4, e0); launch_clause<T, 5>(a, s, st); mark(4, e1);
+    mark(10, e0); launch_var<T, 5>(a, s, st); mark(10, e1);
+    mark(5, e0); launch_clause<T, 6>(a, s, st); mark(5, e1);
+    mark(11, e0); launch_var<T, 6>(a, s, st); mark(11, e1);
+    mark(12, e0);
     hipLaunchKernelGGL(k_control, dim3(1), dim3(CTRL_BLK), 0, st, ctrl, partial, np);
+    mark(12, e1);
 }
 
 template <typename T>
@@ -1352,6 +1367,11 @@ struct RK {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = -1.f;
     bool configured = false;
+    // kernel timer (measurement hook): one event pair per attempt
+    int timer_kind = -1;
+    std::vector<hipEvent_t> tev;
+    double timer_ms = 0.0;
+    long long timer_launches = 0;
 
     void free_all()
     {
@@ -1364,6 +1384,7 @@ struct RK {
         free_all();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        for (hipEvent_t e : tev) (void)hipEventDestroy(e);
     }
     void require_configured() const
     {
@@ -1451,10 +1472,29 @@ static int rk_step_impl(RK* r, State* olds, State* news, int count)
     const long long target = r->h_ctrl->steps + count;
     long long last_attempts = r->h_ctrl->attempts;
     while (remaining > 0) {
-        for (long long i = 0; i < remaining; i++) launch_attempt<T>(a, s, r->d_ctrl, r->partial, r->npartial, st);
+        const int tk = r->timer_kind;
+        if (tk >= 0)
+            while ((long long)r->tev.size() < 2 * remaining) {
+                hipEvent_t e;
+                HIP_CHECK(hipEventCreate(&e));
+                r->tev.push_back(e);
+            }
+        for (long long i = 0; i < remaining; i++) {
+            if (tk >= 0)
+                launch_attempt<T>(a, s, r->d_ctrl, r->partial, r->npartial, st, tk, r->tev[2 * i], r->tev[2 * i + 1]);
+            else
+                launch_attempt<T>(a, s, r->d_ctrl, r->partial, r->npartial, st);
+        }
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipMemcpyAsync(r->h_ctrl, r->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipStreamSynchronize(st));
+        if (tk >= 0)
+            for (long long i = 0; i < remaining; i++) {
+                float ms = 0.f;
+                HIP_CHECK(hipEventElapsedTime(&ms, r->tev[2 * i], r->tev[2 * i + 1]));
+                r->timer_ms += ms;
+                r->timer_launches++;
+            }
         if (r->h_ctrl->underflow) break;
         remaining = target - r->h_ctrl->steps;
         // every enqueued attempt either accepts, rejects or underflows; no
@@ -1947,6 +1987,26 @@ int asat_rk_last_step_ms(const asat_rk_t* rk, float* ms)
     API_BEGIN
     CHECK_PTR(rk); CHECK_PTR(ms);
     *ms = rk->r.last_ms;
+    API_END
+}
+
+int asat_rk_set_kernel_timer(asat_rk_t* rk, int kind)
+{
+    API_BEGIN
+    CHECK_PTR(rk);
+    require(kind >= -1 && kind <= 12, ASAT_ERR_INVALID_ARGUMENT, "kernel kind must be -1 (off) or 0..12");
+    rk->r.timer_kind = kind;
+    rk->r.timer_ms = 0.0;
+    rk->r.timer_launches = 0;
+    API_END
+}
+
+int asat_rk_kernel_timer(const asat_rk_t* rk, double* total_ms, long long* launches)
+{
+    API_BEGIN
+    CHECK_PTR(rk); CHECK_PTR(total_ms); CHECK_PTR(launches);
+    *total_ms = rk->r.timer_ms;
+    *launches = rk->r.timer_launches;
     API_END
 }
 

@@ -178,6 +178,15 @@ int asat_rk_last_step_ms(const asat_rk_t* rk, float* ms);
    us_out[13]: clause pass J=1..6, var pass J=1..6, control.  No reference
    counterpart (measurement hook); the integrator state is left unchanged. */
 int asat_rk_profile(asat_rk_t* rk, asat_state_t* old_state, asat_state_t* new_state, int reps, float* us_out);
+/* Measurement hook, no reference counterpart: while kind is 0..12 (the
+   order of asat_rk_profile), every attempt that Step enqueues records a HIP
+   event pair on the integrator's stream just before and after the kernel of
+   that kind, and the elapsed times are added up; -1 turns the timer off.
+   Setting it clears the totals.  A retry's J = 1 clause pass returns at once
+   and is counted as it runs.  asat_rk_kernel_timer reads the total
+   milliseconds and the number of timed launches. */
+int asat_rk_set_kernel_timer(asat_rk_t* rk, int kind);
+int asat_rk_kernel_timer(const asat_rk_t* rk, double* total_ms, long long* launches);
 
 /* Measurement hook, no reference counterpart: copies `bytes` between two
    device buffers `reps` times with a plain 16-byte-per-lane copy kernel

@@ -245,6 +245,61 @@ def test_trajectory_multi_group_bitexact(precision):
     assert sat.GetClauseViolationCount(old) == orc.violations()
 
 
+def _mixed_problem(n, m, seed):
+    """Clauses of 1, 2 or 3 distinct literals (padding slots in the 3-wide rows)."""
+    rng = np.random.default_rng(seed)
+    lens = rng.choice([1, 2, 3], size=m, p=[0.1, 0.3, 0.6]).astype(np.int32)
+    lits = []
+    for L in lens:
+        v = rng.choice(np.arange(1, n + 1), size=L, replace=False)
+        lits.extend(np.where(rng.random(L) < 0.5, -v, v))
+    return O.Problem(np.asarray(lits, np.int32), lens)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("case", ["k4", "mixed"])
+def test_trajectory_k4_and_padding_bitexact(precision, case):
+    """The tiled clause pass with slot spins for 4-literal clauses and for
+    rows with padding slots, over several tiles and groups: 30 steps
+    bit-identical to the oracle, also after a standalone RHS (which reuses
+    the contribution array) and after the state is replaced between Step
+    calls (the slot spins are rebuilt from the new state)."""
+    say("trajectory %s %s 3 x 10 steps" % (precision, case))
+    if case == "k4":
+        n, m = 3000, 29700
+        p = O.fastsat(n, m, 4, 5)
+    else:
+        p = _mixed_problem(3000, 9000, 6)
+    n, m = p.n, p.m
+    mode = "f64" if precision == "f64" else "mix"
+    dt = np.float64 if precision == "f64" else np.float32
+    pub = O.random_state_f64(9, n, m).astype(dt)
+    sat = _sys_with(p.literals, p.lengths, precision)
+    rk = A.HipRungeKutta()
+    rk.SetEpsilon(1e-6)
+    rk.Configure(sat)
+    old, new = sat.MakeState(), sat.MakeState()
+    old.CopyFrom(pub)
+    with O.pow_mode("cr"):
+        orc = O.RK(p, eps=1e-6, mode=mode)
+        orc.set_state(pub)
+        for blk in range(3):
+            if blk == 1:
+                d = sat.MakeState()
+                sat.GetDerivatives(d, old, 0.0)
+            if blk == 2:
+                moved = old.CopyTo()
+                moved[:n] = np.clip(moved[:n] * dt(0.5), -1, 1)
+                old.CopyFrom(moved)
+                orc.set_state(moved)
+            rk.Step(old, new, 10)
+            orc.step(10)
+            assert np.array_equal(old.CopyTo(), orc.get_state()), blk
+            st = rk.stats()
+            assert (st.hdid, st.attempts) == (orc.hdid, orc.attempts), blk
+    assert sat.GetClauseViolationCount(old) == orc.violations()
+
+
 @pytest.mark.parametrize("precision", ["f64", "f32"])
 def test_trajectory_with_error_floor_bitexact(golden_dir, precision):
     """A raised xscal floor (SetErrorFloor) changes the error norm the same
