@@ -1475,8 +1475,11 @@ static int rk_step_impl(RK* r, State* olds, State* news, int count)
         const int tk = r->timer_kind;
         if (tk >= 0)
             while ((long long)r->tev.size() < 2 * remaining) {
+                // timing-only events, read after the stream synchronises:
+                // no system-scope fence, so recording them does not write
+                // back and invalidate the caches between the kernels
                 hipEvent_t e;
-                HIP_CHECK(hipEventCreate(&e));
+                HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
                 r->tev.push_back(e);
             }
         for (long long i = 0; i < remaining; i++) {
