@@ -156,6 +156,7 @@ def lib():
             "asat_batch_add": ([vp, vp, vp, i, vp, ip], i),
             "asat_batch_set_params": ([vp, ctypes.c_double, ctypes.c_longlong, ctypes.c_double, i, i], i),
             "asat_batch_set_tiling": ([vp, i, i, i], i),
+            "asat_batch_set_chunk": ([vp, i], i),
             "asat_batch_prepare": ([vp, i], i),
             "asat_batch_run": ([vp, i, i], i),
             "asat_batch_reset": ([vp], i),
@@ -729,6 +730,11 @@ class HipBatch:
 
     def set_tiling(self, block=0, lds_mode=-1, nbuckets=4):
         _check(lib().asat_batch_set_tiling(self._h, int(block), int(lds_mode), int(nbuckets)))
+
+    def set_chunk(self, clauses=0):
+        """Clause-chunked kernel (lds_mode 3): at most this many clauses per
+        chunk (0: as many as the LDS holds)."""
+        _check(lib().asat_batch_set_chunk(self._h, int(clauses)))
 
     def prepare(self, autotune=True):
         _check(lib().asat_batch_prepare(self._h, 1 if autotune else 0))

@@ -15,6 +15,11 @@
 //   var phase:    each thread takes variables i = tid, tid+BS, ...; sums its
 //     contiguous run of contributions in slot order (the reference's
 //     accumulation order), writes k_J(s_i) and the next stage's spin input
+// For instances whose contributions do not fit in LDS, the clause-chunked
+// mode keeps no contribution array: per chunk of clauses the clause phase
+// leaves r_m in LDS, and each variable forms its terms of that chunk from
+// 8-byte var-major records (the clause, the sign, the other literals) and
+// adds them in the same order.
 // Stage 6 forms x_new and |xerr/xscal|, a block FP64 max feeds the step
 // controller (thread 0, FP64, correctly rounded pow), which accepts (parity
 // flip) or rejects (retry with smaller h).  After each accepted step the
@@ -56,10 +61,17 @@ struct Desc {                 // one instance / trajectory (device)
     long long code_off;       // into code[]
     long long row_off;        // into rowptr[]
     long long state_off;      // into xa[]/xb[] (N spins then M aux)
+    long long vcode_off;      // into vcode[] (clause-chunked mode; -1: not built)
     int n, m, k, nslots;      // nslots = number of contributions (non-padding slots)
     float knorm;
     int pad_;
 };
+
+// Clause-chunked mode: one 64-bit record per contribution, in var-major
+// order: bits 0..17 the clause, bit 18 the literal's sign, then the k - 1
+// other literals of the clause in clause order, 14 bits each (variable << 1
+// | negated; variable 0 = padding).
+constexpr int VC_OTHER_SHIFT = 19;
 
 struct TrajCtrl {             // persistent per-trajectory control (device)
     double t, h, hnext, hdid, errmax, sat_time;
@@ -87,10 +99,18 @@ struct Params {
     int chunk_steps;
     int check_every;         // take the violation count every this many accepted steps
     int sp_stride;           // floats reserved for the stage spins at the start of LDS
+    const unsigned long long* vcode;   // clause-chunked mode: var-major contribution records
+    int chunk_clauses;       // clause-chunked mode: clauses per chunk (LDS for their r_m)
 };
 
 // ---------------------------------------------------------------------------
-template <int BS, bool LDS_CONTRIB>
+// MODE 0: contributions in the workgroup's global scratch; 1: in LDS; 3:
+// clause-chunked, no contribution array: per chunk of clauses, the clause
+// phase leaves r_m = K_m knorm a_m in LDS and every variable walks on
+// through its contribution records of that chunk, forming each term
+// r_m * (product of the other factors) * (+-2) and adding it in the same
+// order as the other modes (LDS: spins, running sums, record cursors, r_m).
+template <int BS, int MODE>
 __global__ __launch_bounds__(BS) void k_batch(Params P)
 {
     extern __shared__ float lds[];          // [stage spins: sp_stride] then contributions
@@ -121,7 +141,12 @@ __global__ __launch_bounds__(BS) void k_batch(Params P)
         const int n = d.n, m = d.m, k = d.k;
         const uint32_t* code = P.code + d.code_off;
         const int* rowptr = P.rowptr + d.row_off;
-        float* contrib = LDS_CONTRIB ? (lds + P.sp_stride) : (scratch + 5 * P.scratch_stride);
+        float* contrib = MODE == 1 ? (lds + P.sp_stride) : (scratch + 5 * P.scratch_stride);
+        float* vsum = lds + P.sp_stride;                          // MODE 3
+        int* vcur = (int*)(lds + 2 * P.sp_stride);                // MODE 3
+        float* rl = lds + 3 * P.sp_stride;                        // MODE 3
+        const unsigned long long* vcode = MODE == 3 ? P.vcode + d.vcode_off : nullptr;
+        const int chunk = MODE == 3 ? P.chunk_clauses : m;
         const float knorm = d.knorm;
         // scratch layout per k vector: spins at [0, n), aux at [n, n+m)
         double t = cp->t, h = cp->h, hnext = cp->hnext, hdid = cp->hdid, errmax_last = cp->errmax;
@@ -146,8 +171,10 @@ __global__ __launch_bounds__(BS) void k_batch(Params P)
                 double err_local = 0.0;
 #pragma unroll 1
                 for (int J = first_attempt ? 1 : 2; J <= 6; J++) {
-                    // ---------------- clause phase
-                    for (int j = tid; j < m; j += BS) {
+                  for (int c0 = 0; c0 < m; c0 += chunk) {
+                    const int c1 = min(m, c0 + chunk);
+                    // ---------------- clause phase (MODE 3: one chunk of clauses)
+                    for (int j = c0 + tid; j < c1; j += BS) {
                         const uint32_t* cj = code + (long long)j * k;
                         float f[KMAX];
                         uint32_t cc[KMAX];
@@ -200,29 +227,80 @@ __global__ __launch_bounds__(BS) void k_batch(Params P)
                         }
                         }
                         const float r = km * knorm * am;
+                        if constexpr (MODE == 3) {
+                            rl[j - c0] = r;
+                        } else {
 #pragma unroll
-                        for (int q = 0; q < KMAX; q++) {
-                            if (q < k && (cc[q] >> 19) != 0) {
-                                float p = 1.0f;
-                                bool first = true;
+                            for (int q = 0; q < KMAX; q++) {
+                                if (q < k && (cc[q] >> 19) != 0) {
+                                    float p = 1.0f;
+                                    bool first = true;
 #pragma unroll
-                                for (int qq = 0; qq < KMAX; qq++) {
-                                    if (qq == q || qq >= k) continue;
-                                    p = first ? f[qq] : p * f[qq];
-                                    first = false;
+                                    for (int qq = 0; qq < KMAX; qq++) {
+                                        if (qq == q || qq >= k) continue;
+                                        p = first ? f[qq] : p * f[qq];
+                                        first = false;
+                                    }
+                                    float rr = r * p;
+                                    rr *= (cc[q] >> 18) & 1u ? -2.0f : 2.0f;
+                                    contrib[cc[q] & 0x3FFFFu] = rr;
                                 }
-                                float rr = r * p;
-                                rr *= (cc[q] >> 18) & 1u ? -2.0f : 2.0f;
-                                contrib[cc[q] & 0x3FFFFu] = rr;
                             }
                         }
                     }
                     __syncthreads();
+                    if constexpr (MODE == 3) {
+                        // each variable's terms of this chunk, in record
+                        // (clause) order: the product of the other factors
+                        // as the clause phase forms it (KmiProd order)
+                        for (int i = tid; i < n; i += BS) {
+                            int e = c0 == 0 ? rowptr[i] : vcur[i];
+                            const int e1 = rowptr[i + 1];
+                            float sum = c0 == 0 ? 0.0f : vsum[i];
+                            bool more = e < e1;
+                            while (more) {
+                                // four records read ahead of the chunk-end test
+                                unsigned long long vb[4];
+#pragma unroll
+                                for (int u = 0; u < 4; u++) vb[u] = e + u < e1 ? vcode[e + u] : ~0ull;
+#pragma unroll
+                                for (int u = 0; u < 4; u++) {
+                                    if (!more) break;
+                                    const unsigned long long vc = vb[u];
+                                    const int j = (int)(vc & 0x3FFFFull);
+                                    if (e >= e1 || j >= c1) { more = false; break; }
+                                    float p = 1.0f;
+                                    bool first = true;
+                                    for (int t = 0; t < k - 1; t++) {
+                                        const unsigned o = (unsigned)(vc >> (VC_OTHER_SHIFT + 14 * t)) & 0x3FFFu;
+                                        const int v = (int)(o >> 1);
+                                        const float sv = v ? sp[v - 1] : -1.0f;
+                                        const float fo = (o & 1u) ? 1.0f + sv : 1.0f - sv;
+                                        p = first ? fo : p * fo;
+                                        first = false;
+                                    }
+                                    float rr = rl[j - c0] * p;
+                                    rr *= (vc >> 18) & 1ull ? -2.0f : 2.0f;
+                                    sum += rr;
+                                    e++;
+                                }
+                                more = more && e < e1;
+                            }
+                            vsum[i] = sum;
+                            vcur[i] = e;
+                        }
+                        __syncthreads();
+                    }
+                  }
                     // ---------------- var phase
                     for (int i = tid; i < n; i += BS) {
-                        const int e0 = rowptr[i], e1 = rowptr[i + 1];
                         float sum = 0.0f;
-                        for (int e = e0; e < e1; e++) sum += contrib[e];
+                        if constexpr (MODE == 3) {
+                            sum = vsum[i];
+                        } else {
+                            const int e0 = rowptr[i], e1 = rowptr[i + 1];
+                            for (int e = e0; e < e1; e++) sum += contrib[e];
+                        }
                         const float kj = sum;
                         const float xi = x[i];
                         switch (J) {
@@ -643,11 +721,37 @@ struct Instance {
     std::vector<float> pub0;      // n+m
 };
 
+// contribution records of the clause-chunked mode (see VC_OTHER_SHIFT), in
+// var-major (contribution) order
+static void build_vcode(const Instance& I, unsigned long long* out)
+{
+    const int k = I.k;
+    for (int j = 0; j < I.m; j++) {
+        const uint32_t* cj = I.code.data() + (size_t)j * k;
+        for (int q = 0; q < k; q++) {
+            const uint32_t c = cj[q];
+            if ((c >> 19) == 0) continue;
+            unsigned long long vc = (unsigned long long)j | ((unsigned long long)((c >> 18) & 1u) << 18);
+            int t = 0;
+            for (int qq = 0; qq < k; qq++) {
+                if (qq == q) continue;
+                const uint32_t o = ((cj[qq] >> 19) << 1) | ((cj[qq] >> 18) & 1u);
+                vc |= (unsigned long long)o << (VC_OTHER_SHIFT + 14 * t);
+                t++;
+            }
+            out[c & 0x3FFFFu] = vc;
+        }
+    }
+}
+
 struct Bucket {
     std::vector<int> ids;          // trajectory ids, largest first
     int block = 0;                 // workgroup size (256/512/1024), 0 = autotune
     int lds = -1;                  // contributions in LDS (1) or global scratch (0); 2: register-
-                                   // resident kernel (contributions in LDS); -1 = auto
+                                   // resident kernel (contributions in LDS); 3: clause-chunked
+                                   // (no contribution array); -1 = auto
+    int chunk = 0;                 // clause-chunked mode: clauses per chunk
+    bool vcode = false;            // clause-chunked records built for this bucket's instances
     int cpt = 0, vpt = 0;          // register kernel: clauses / variables per thread
     int grid = 0;
     int maxn = 0, maxm = 0, maxl = 0;
@@ -676,12 +780,14 @@ struct Batch {
     int check_every = 1;
     int force_block = 0;           // 0: autotune per bucket
     int force_lds = -1;            // -1: auto
+    int force_chunk = 0;           // clause-chunked mode: clauses per chunk, 0 = as large as LDS allows
     int nbuckets_req = 4;
     bool prepared = false;
     std::vector<Bucket> buckets;
     // device
     Desc* d_desc = nullptr;
     uint32_t* d_code = nullptr;
+    unsigned long long* d_vcode = nullptr;
     int* d_rowptr = nullptr;
     float* d_xa = nullptr;
     float* d_xb = nullptr;
@@ -700,8 +806,9 @@ struct Batch {
     void free_dev()
     {
         (void)hipFree(d_desc); (void)hipFree(d_code); (void)hipFree(d_rowptr); (void)hipFree(d_xa);
-        (void)hipFree(d_xb); (void)hipFree(d_ctrl);
+        (void)hipFree(d_xb); (void)hipFree(d_ctrl); (void)hipFree(d_vcode);
         d_desc = nullptr; d_code = nullptr; d_rowptr = nullptr; d_xa = d_xb = nullptr; d_ctrl = nullptr;
+        d_vcode = nullptr;
         for (auto& b : buckets) b.free_dev();
     }
     ~Batch()
@@ -771,6 +878,17 @@ static size_t bucket_lds(const Bucket& b, bool lds)
     return sizeof(float) * ((size_t)b.sp_stride + (lds ? (size_t)round_up_ll(b.maxl, 4) : 0));
 }
 
+// clause-chunked mode: spins, running sums and record cursors per variable,
+// then r_m of one chunk of clauses
+static size_t chunk_lds(const Bucket& b, int chunk)
+{
+    return sizeof(float) * (3 * (size_t)b.sp_stride + (size_t)round_up_ll(chunk, 4));
+}
+
+// largest chunk (a multiple of 256 clauses, at most the bucket's largest m)
+// with which `wgs` workgroups fit in the LDS of one CU; 0 if none does
+static int chunk_for(const Batch& B, const Bucket& b, int wgs);
+
 // register-resident variants (workgroup size, clauses and variables per
 // thread); a bucket uses the first one its largest instance fits
 struct RegVariant {
@@ -809,19 +927,24 @@ static void launch_reg(hipStream_t st, const Params& P, int grid, int bs, int cp
     throw Error(ASAT_ERR_HIP, "no register-resident kernel for this tiling");
 }
 
-template <int BS, bool LDS>
-static void launch_one(hipStream_t st, const Params& P, int grid, size_t shmem)
+template <int MODE>
+static void launch_mode(hipStream_t st, const Params& P, int grid, int bs, size_t shmem)
 {
-    hipLaunchKernelGGL((k_batch<BS, LDS>), dim3(grid), dim3(BS), shmem, st, P);
+    if (bs == 256) hipLaunchKernelGGL((k_batch<256, MODE>), dim3(grid), dim3(256), shmem, st, P);
+    else if (bs == 512) hipLaunchKernelGGL((k_batch<512, MODE>), dim3(grid), dim3(512), shmem, st, P);
+    else hipLaunchKernelGGL((k_batch<1024, MODE>), dim3(grid), dim3(1024), shmem, st, P);
 }
 
-static void set_lds_attr(int bs, bool lds, size_t bytes)
+template <int MODE>
+static const void* kernel_of(int bs)
 {
-    const void* f = nullptr;
-    if (lds) f = bs == 256 ? (const void*)k_batch<256, true> : bs == 512 ? (const void*)k_batch<512, true>
-                                                                          : (const void*)k_batch<1024, true>;
-    else f = bs == 256 ? (const void*)k_batch<256, false> : bs == 512 ? (const void*)k_batch<512, false>
-                                                                      : (const void*)k_batch<1024, false>;
+    return bs == 256 ? (const void*)k_batch<256, MODE> : bs == 512 ? (const void*)k_batch<512, MODE>
+                                                                   : (const void*)k_batch<1024, MODE>;
+}
+
+static void set_lds_attr(int bs, int mode, size_t bytes)
+{
+    const void* f = mode == 1 ? kernel_of<1>(bs) : mode == 3 ? kernel_of<3>(bs) : kernel_of<0>(bs);
     HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
@@ -829,17 +952,10 @@ static void dispatch(hipStream_t st, const Params& P, const Bucket& b)
 {
     const int grid = b.grid, bs = b.block;
     const size_t shmem = b.lds_bytes;
-    if (b.lds == 2) {
-        launch_reg(st, P, grid, bs, b.cpt, b.vpt, shmem);
-    } else if (b.lds == 1) {
-        if (bs == 256) launch_one<256, true>(st, P, grid, shmem);
-        else if (bs == 512) launch_one<512, true>(st, P, grid, shmem);
-        else launch_one<1024, true>(st, P, grid, shmem);
-    } else {
-        if (bs == 256) launch_one<256, false>(st, P, grid, shmem);
-        else if (bs == 512) launch_one<512, false>(st, P, grid, shmem);
-        else launch_one<1024, false>(st, P, grid, shmem);
-    }
+    if (b.lds == 2) launch_reg(st, P, grid, bs, b.cpt, b.vpt, shmem);
+    else if (b.lds == 1) launch_mode<1>(st, P, grid, bs, shmem);
+    else if (b.lds == 3) launch_mode<3>(st, P, grid, bs, shmem);
+    else launch_mode<0>(st, P, grid, bs, shmem);
 }
 
 static Params params_of(const Batch& B, const Bucket& b)
@@ -851,22 +967,35 @@ static Params params_of(const Batch& B, const Bucket& b)
     P.queue = b.d_queue; P.eps = B.eps; P.floor = B.floor; P.max_time = B.max_time; P.max_steps = B.max_steps;
     P.chunk_steps = B.chunk_steps; P.check_every = std::max(1, B.check_every);
     P.sp_stride = b.sp_stride;
+    P.vcode = B.d_vcode;
+    P.chunk_clauses = b.lds == 3 ? b.chunk : 0;
     return P;
 }
 
-// resident workgroups per CU for a configuration
-static int per_cu(const Batch& B, const Bucket& b, int bs, bool lds)
+// resident workgroups per CU for a configuration with `lds_bytes` of LDS each
+static int per_cu(const Batch& B, int bs, size_t lds_bytes)
 {
-    int by_lds = lds ? std::max(1, (int)(B.lds_limit / std::max<size_t>(bucket_lds(b, true), 1))) : 8;
+    int by_lds = std::max(1, (int)(B.lds_limit / std::max<size_t>(lds_bytes, 1)));
     int by_threads = std::max(1, 2048 / bs);
     return std::max(1, std::min(by_lds, by_threads));
 }
 
-// mode: 0 contributions in global scratch, 1 in LDS, 2 register-resident
-// kernel (contributions in LDS; bs comes from its variant)
-static void configure_bucket(Batch& B, Bucket& b, int bs, int mode)
+static int chunk_for(const Batch& B, const Bucket& b, int wgs)
 {
-    const bool lds = mode != 0;
+    // 1 KB left for the kernel's static shared arrays (block reductions)
+    const long long avail = (long long)(B.lds_limit / wgs) - (long long)chunk_lds(b, 0) - 1024;
+    long long c = std::min<long long>(avail / (long long)sizeof(float), round_up_ll(b.maxm, 256));
+    c = c / 256 * 256;
+    return c >= 256 ? (int)c : 0;
+}
+
+// mode: 0 contributions in global scratch, 1 in LDS, 2 register-resident
+// kernel (contributions in LDS; bs comes from its variant).  grid_cap > 0
+// limits the resident workgroups (global-scratch mode: the scattered
+// contribution writes of fewer workgroups stay in the Infinity Cache).
+static void configure_bucket(Batch& B, Bucket& b, int bs, int mode, int grid_cap = 0, int chunk = 0)
+{
+    const bool lds = mode == 1 || mode == 2;
     if (mode == 2) {
         const RegVariant* r = reg_variant(b, bs);
         if (!r) r = reg_variant(b);
@@ -875,12 +1004,20 @@ static void configure_bucket(Batch& B, Bucket& b, int bs, int mode)
         b.cpt = r->cpt;
         b.vpt = r->vpt;
     }
+    if (mode == 3) {
+        require(b.vcode, ASAT_ERR_INVALID_ARGUMENT, "clause-chunked records were not built for this bucket");
+        if (chunk <= 0) chunk = chunk_for(B, b, 1);
+        if (B.force_chunk > 0) chunk = std::min(chunk, B.force_chunk);
+        require(chunk > 0, ASAT_ERR_UNSUPPORTED, "instances too large for the clause-chunked kernel");
+    }
     b.block = bs;
     b.lds = mode;
-    b.lds_bytes = bucket_lds(b, lds);
-    b.grid = std::max(1, std::min((int)b.ids.size(), B.cus * per_cu(B, b, bs, lds)));
+    b.chunk = mode == 3 ? chunk : 0;
+    b.lds_bytes = mode == 3 ? chunk_lds(b, chunk) : bucket_lds(b, lds);
+    b.grid = std::max(1, std::min((int)b.ids.size(), B.cus * per_cu(B, bs, mode == 0 ? 0 : b.lds_bytes)));
+    if (grid_cap > 0) b.grid = std::min(b.grid, grid_cap);
     b.scratch_stride = round_up_ll((long long)b.maxn + b.maxm, 64);
-    b.scratch_per_wg = mode == 2 ? 0 : 5 * b.scratch_stride + (lds ? 0 : round_up_ll(b.maxl, 64));
+    b.scratch_per_wg = mode == 2 ? 0 : 5 * b.scratch_stride + (mode == 0 ? round_up_ll(b.maxl, 64) : 0);
     (void)hipFree(b.d_scratch);
     b.d_scratch = nullptr;
     const size_t sbytes = sizeof(float) * std::max<size_t>((size_t)b.scratch_per_wg * b.grid, 1);
@@ -890,7 +1027,7 @@ static void configure_bucket(Batch& B, Bucket& b, int bs, int mode)
         HIP_CHECK(hipFuncSetAttribute(reg_kernel(bs, b.cpt, b.vpt), hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)b.lds_bytes));
     else
-        set_lds_attr(bs, lds, b.lds_bytes);
+        set_lds_attr(bs, mode, b.lds_bytes);
 }
 
 static void reset_state(Batch& B)
@@ -905,7 +1042,9 @@ static void reset_state(Batch& B)
 
 // Autotuner: for each bucket, time one short chunk (2 accepted steps of every
 // trajectory in the bucket) for each workgroup size and contribution placement
-// that fits, keep the fastest per trajectory-attempt, then restore the start.
+// that fits (for global scratch also with the resident workgroups capped so
+// that their contribution scratch totals 64, 128 or 256 MB), keep the
+// fastest per trajectory-attempt, then restore the start.
 static void autotune(Batch& B)
 {
     const int saved_chunk = B.chunk_steps;
@@ -914,23 +1053,37 @@ static void autotune(Batch& B)
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     for (auto& b : B.buckets) {
-        std::vector<std::pair<int, int>> cand;   // (workgroup size, mode)
+        struct Cand { int bs, mode, cap, chunk; };
+        std::vector<Cand> cand;
         for (int bs : {256, 512, 1024}) {
             const RegVariant* rv = reg_variant(b, bs);
             if (rv && b.lds_fits && (B.force_lds == -1 || B.force_lds == 2) && (!B.force_block || B.force_block == bs))
-                cand.push_back({bs, 2});
+                cand.push_back({bs, 2, 0, 0});
         }
+        const long long contrib_bytes = 4 * round_up_ll(b.maxl, 64);
         for (int bs : {256, 512, 1024}) {
             if (B.force_block && bs != B.force_block) continue;
-            if (b.lds_fits && (B.force_lds == -1 || B.force_lds == 1)) cand.push_back({bs, 1});
-            if (B.force_lds == -1 || B.force_lds == 0) cand.push_back({bs, 0});
+            if (b.lds_fits && (B.force_lds == -1 || B.force_lds == 1)) cand.push_back({bs, 1, 0, 0});
+            if (b.vcode && (B.force_lds == -1 || B.force_lds == 3))
+                for (int wgs : {1, 2}) {
+                    const int ch = chunk_for(B, b, wgs);
+                    if (ch > 0) cand.push_back({bs, 3, 0, ch});
+                }
+            if (B.force_lds == -1 || B.force_lds == 0) {
+                cand.push_back({bs, 0, 0, 0});
+                const int full = std::max(1, std::min((int)b.ids.size(), B.cus * per_cu(B, bs, 0)));
+                for (long long mb : {256LL, 128LL, 64LL}) {
+                    const int cap = (int)std::max<long long>(8, (mb << 20) / std::max<long long>(contrib_bytes, 1));
+                    if (cap < full) cand.push_back({bs, 0, cap, 0});
+                }
+            }
         }
         require(!cand.empty(), ASAT_ERR_INVALID_ARGUMENT, "no kernel configuration fits this bucket");
         double best = 1e300;
-        std::pair<int, int> pick = cand.front();
+        Cand pick = cand.front();
         if (cand.size() > 1) {
             for (auto& c : cand) {
-                configure_bucket(B, b, c.first, c.second);
+                configure_bucket(B, b, c.bs, c.mode, c.cap, c.chunk);
                 reset_state(B);
                 Params P = params_of(B, b);
                 hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, B.stream, b.d_queue);
@@ -949,7 +1102,7 @@ static void autotune(Batch& B)
                 if (us < best) { best = us; pick = c; }
             }
         }
-        configure_bucket(B, b, pick.first, pick.second);
+        configure_bucket(B, b, pick.bs, pick.mode, pick.cap, pick.chunk);
         b.tuned_us = best < 1e300 ? best : -1.0;
     }
     (void)hipEventDestroy(e0);
@@ -985,6 +1138,7 @@ static void prepare(Batch& B, bool tune)
         desc[i].state_off = state_total;
         B.state_off[i] = state_total;
         desc[i].n = I.n; desc[i].m = I.m; desc[i].k = I.k; desc[i].nslots = I.nslots;
+        desc[i].vcode_off = -1;
         desc[i].knorm = (float)std::pow(2.0, -I.k);
         code_total += (long long)I.code.size();
         row_total += (long long)I.rowptr.size();
@@ -1023,6 +1177,24 @@ static void prepare(Batch& B, bool tune)
         HIP_CHECK(hipMalloc((void**)&b.d_queue, sizeof(int)));
         HIP_CHECK(hipMemcpy(b.d_order, b.ids.data(), sizeof(int) * b.ids.size(), hipMemcpyHostToDevice));
     }
+    // clause-chunked records for the buckets whose contributions do not fit in LDS
+    long long vc_total = 0;
+    for (auto& b : B.buckets) {
+        b.vcode = (!b.lds_fits || B.force_lds == 3) && chunk_for(B, b, 1) > 0;
+        if (!b.vcode) continue;
+        for (int id : b.ids) {
+            desc[id].vcode_off = vc_total;
+            vc_total += B.inst[id].nslots;
+        }
+    }
+    if (vc_total > 0) {
+        std::vector<unsigned long long> vc((size_t)vc_total);
+        for (int i = 0; i < T; i++)
+            if (desc[i].vcode_off >= 0) build_vcode(B.inst[i], vc.data() + desc[i].vcode_off);
+        HIP_CHECK(hipMalloc((void**)&B.d_vcode, sizeof(unsigned long long) * (size_t)vc_total));
+        HIP_CHECK(hipMemcpy(B.d_vcode, vc.data(), sizeof(unsigned long long) * (size_t)vc_total,
+                            hipMemcpyHostToDevice));
+    }
     HIP_CHECK(hipMalloc((void**)&B.d_desc, sizeof(Desc) * T));
     HIP_CHECK(hipMalloc((void**)&B.d_code, sizeof(uint32_t) * std::max<long long>(code_total, 1)));
     HIP_CHECK(hipMalloc((void**)&B.d_rowptr, sizeof(int) * std::max<long long>(row_total, 1)));
@@ -1044,8 +1216,9 @@ static void prepare(Batch& B, bool tune)
         int bs = B.force_block ? B.force_block : 1024;
         const RegVariant* rv = reg_variant(b);
         const bool reg_ok = rv && b.lds_fits && (!B.force_block || B.force_block == rv->bs);
-        int mode = B.force_lds == -1 ? (reg_ok ? 2 : (b.lds_fits ? 1 : 0)) : B.force_lds;
-        require(mode == 0 || b.lds_fits, ASAT_ERR_UNSUPPORTED, "instances too large for LDS contributions");
+        int mode = B.force_lds == -1 ? (reg_ok ? 2 : (b.lds_fits ? 1 : (b.vcode ? 3 : 0))) : B.force_lds;
+        require(mode == 0 || mode == 3 || b.lds_fits, ASAT_ERR_UNSUPPORTED, "instances too large for LDS contributions");
+        require(mode != 3 || b.vcode, ASAT_ERR_UNSUPPORTED, "instances too large for the clause-chunked kernel");
         require(mode != 2 || reg_variant(b), ASAT_ERR_UNSUPPORTED,
                 "instances too large for the register-resident kernel");
         configure_bucket(B, b, bs, mode);
@@ -1184,13 +1357,23 @@ int asat_batch_set_xscal_floor(asat_batch_t* b, double floor)
     BAPI_END
 }
 
+int asat_batch_set_chunk(asat_batch_t* b, int clauses)
+{
+    BAPI_BEGIN
+    require(b != nullptr, ASAT_ERR_INVALID_ARGUMENT, "null batch");
+    require(clauses >= 0, ASAT_ERR_INVALID_ARGUMENT, "chunk size must be >= 0");
+    b->b.force_chunk = clauses;
+    b->b.prepared = false;
+    BAPI_END
+}
+
 int asat_batch_set_tiling(asat_batch_t* b, int block, int lds_mode, int nbuckets)
 {
     BAPI_BEGIN
     require(b != nullptr, ASAT_ERR_INVALID_ARGUMENT, "null batch");
     require(block == 0 || block == 256 || block == 512 || block == 1024, ASAT_ERR_INVALID_ARGUMENT,
             "block must be 0 (autotune), 256, 512 or 1024");
-    require(lds_mode >= -1 && lds_mode <= 2, ASAT_ERR_INVALID_ARGUMENT, "lds_mode must be -1, 0, 1 or 2");
+    require(lds_mode >= -1 && lds_mode <= 3, ASAT_ERR_INVALID_ARGUMENT, "lds_mode must be -1, 0, 1, 2 or 3");
     require(nbuckets >= 1 && nbuckets <= 64, ASAT_ERR_INVALID_ARGUMENT, "nbuckets must be in [1, 64]");
     b->b.force_block = block; b->b.force_lds = lds_mode; b->b.nbuckets_req = nbuckets;
     b->b.prepared = false;

@@ -235,7 +235,8 @@ typedef struct {
 typedef struct {
     int trajectories, max_n, max_m, max_slots;
     int block;               /* workgroup size chosen (256 / 512 / 1024) */
-    int lds;                 /* 1: contributions in LDS, 0: in global scratch, 2: register-resident kernel */
+    int lds;                 /* 1: contributions in LDS, 0: in global scratch, 2: register-resident kernel,
+                                3: clause-chunked kernel (no contribution array) */
     int grid;                /* resident workgroups */
     long long lds_bytes;
     double tuned_us_per_attempt;   /* autotuner's measurement, -1 if not tuned */
@@ -255,9 +256,15 @@ int asat_batch_set_xscal_floor(asat_batch_t* b, double floor);
 /* block: 0 autotune or 256/512/1024; lds_mode: -1 auto, 0 contributions in
    global scratch, 1 in LDS, 2 register-resident kernel (state and stage
    derivatives in registers, contributions in LDS; its workgroup size is
-   fixed by the instance size); nbuckets: size buckets (trajectories split by
-   work, one launch each) */
+   fixed by the instance size), 3 clause-chunked kernel (per chunk of
+   clauses, their r_m in LDS and each variable's terms of that chunk formed
+   from 8-byte var-major records; for instances whose contributions do not
+   fit in LDS); nbuckets: size buckets (trajectories split by work, one
+   launch each) */
 int asat_batch_set_tiling(asat_batch_t* b, int block, int lds_mode, int nbuckets);
+/* clause-chunked kernel: at most `clauses` clauses per chunk (0: as many as
+   the LDS holds); results do not depend on it */
+int asat_batch_set_chunk(asat_batch_t* b, int clauses);
 /* upload and (autotune != 0) pick the tile mapping per size bucket */
 int asat_batch_prepare(asat_batch_t* b, int autotune);
 /* launch graphs of graph_chunks chunks until all trajectories finished or

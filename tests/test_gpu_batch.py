@@ -84,6 +84,51 @@ def test_batch_matches_oracle_per_trajectory(block, lds):
     check_against_oracle(b, insts, 150)
 
 
+@pytest.mark.parametrize("block,chunk", [(256, 256), (1024, 512), (512, 0)])
+def test_clause_chunked_kernel_matches_oracle(block, chunk):
+    """lds_mode 3 (no contribution array: per chunk of clauses, r_m in LDS
+    and each variable's terms formed from its var-major records), with
+    chunks small enough that every instance spans many of them: bit-identical
+    to the oracle."""
+    say("clause-chunked kernel block=%d chunk=%d" % (block, chunk))
+    insts = make_instances(12, 7, nlo=150, nhi=500)
+    b = A.HipBatch()
+    for p, pub in insts:
+        b.add(A.SatProblem(p.literals, p.lengths), pub)
+    b.set_params(eps=1e-6, max_steps=60, chunk_steps=11, check_every=1)
+    b.set_tiling(block=block, lds_mode=3, nbuckets=2)
+    b.set_chunk(chunk)
+    b.prepare(autotune=False)
+    bk = b.buckets()
+    say("buckets: %s" % [(x["trajectories"], x["max_m"], x["block"], x["lds"]) for x in bk])
+    assert all(x["lds"] == 3 for x in bk)
+    b.run(graph_chunks=2)
+    check_against_oracle(b, insts, 60)
+
+
+def test_large_instances_chunked_and_global_scratch_agree_with_oracle():
+    """Instances whose contributions do not fit in LDS (M = 40 000, k = 4):
+    the clause-chunked kernel with its default chunk (several per instance)
+    and the global-scratch kernel give the oracle's trajectory."""
+    say("large instances: clause-chunked vs global scratch")
+    insts = []
+    for i in range(3):
+        n, m = 5000, 40000
+        p = O.fastsat(n, m, 4, 8100 + i)
+        pub = np.concatenate([np.random.default_rng(40 + i).uniform(-1, 1, n), np.ones(m)]).astype(np.float32)
+        insts.append((p, pub))
+    for mode in (3, 0):
+        b = A.HipBatch()
+        for p, pub in insts:
+            b.add(A.SatProblem(p.literals, p.lengths), pub)
+        b.set_params(eps=1e-6, max_steps=8, chunk_steps=4, check_every=1)
+        b.set_tiling(block=1024, lds_mode=mode, nbuckets=1)
+        b.prepare(autotune=False)
+        assert b.buckets()[0]["lds"] == mode
+        b.run(graph_chunks=1)
+        check_against_oracle(b, insts, 8)
+
+
 def test_batch_autotune_and_buckets_do_not_change_results():
     say("batch autotune vs fixed tiling")
     insts = make_instances(40, 2, nlo=30, nhi=600)
