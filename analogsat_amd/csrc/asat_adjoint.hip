@@ -42,14 +42,35 @@ struct Dev {
     int n, m, k;
     long long D;          // vector stride (>= n+m+1)
     long long L;          // m*k
+    int ntraj;            // trajectories (grids are padded to a multiple of 8)
 };
+
+// Trajectory-batched grids are (blocks per trajectory, ntraj rounded up to
+// 8).  Workgroups are dealt to the 8 XCDs round-robin by linear id; the id is
+// remapped so that every block of trajectory tr runs on XCD tr % 8, and each
+// XCD keeps its own trajectories' gather targets (spins, contributions) in
+// its L2.  Blocks of the padding trajectories (tr >= ntraj) return at once.
+struct TB {
+    long long x, tr;
+};
+static inline int ntraj8(int ntraj) { return (ntraj + 7) / 8 * 8; }
+
+__device__ __forceinline__ TB traj_block()
+{
+    const long long per = 8LL * gridDim.x;
+    const long long L = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+    const long long r = L % per;
+    return {r >> 3, (L / per) * 8 + (r & 7)};
+}
 
 // ---- FP32 recompute (forward arithmetic) ----------------------------------
 template <int K>
 __global__ __launch_bounds__(BLK) void k_rhs_clause(Dev d, const float* Y, float* out, float* contrib)
 {
-    const int m = blockIdx.x * BLK + threadIdx.x;
-    const long long tr = blockIdx.y;
+    const TB tb = traj_block();
+    const int m = tb.x * BLK + threadIdx.x;
+    const long long tr = tb.tr;
+    if (tr >= d.ntraj) return;
     if (m >= d.m) return;
     const float* y = Y + tr * d.D;
     int l[K];
@@ -79,8 +100,10 @@ __global__ __launch_bounds__(BLK) void k_rhs_clause(Dev d, const float* Y, float
 template <typename T>
 __global__ __launch_bounds__(BLK) void k_rhs_var(Dev d, const T* contrib, T* out)
 {
-    const int i = blockIdx.x * BLK + threadIdx.x + 1;
-    const long long tr = blockIdx.y;
+    const TB tb = traj_block();
+    const int i = tb.x * BLK + threadIdx.x + 1;
+    const long long tr = tb.tr;
+    if (tr >= d.ntraj) return;
     if (i > d.n) return;
     const T* c = contrib + tr * d.L;
     T sum = (T)0;
@@ -94,8 +117,10 @@ __global__ __launch_bounds__(BLK) void k_stage(Dev d, const float* X, const floa
                                                 const float* k3, const float* k4, const float* k5,
                                                 const double* h, float* Y)
 {
-    const long long i = (long long)blockIdx.x * BLK + threadIdx.x + 1;
-    const long long tr = blockIdx.y;
+    const TB tb = traj_block();
+    const long long i = (long long)tb.x * BLK + threadIdx.x + 1;
+    const long long tr = tb.tr;
+    if (tr >= d.ntraj) return;
     if (i > (long long)d.n + d.m) return;
     const long long o = tr * d.D + i;
     const float hs = (float)h[tr];
@@ -106,8 +131,10 @@ __global__ __launch_bounds__(BLK) void k_stage(Dev d, const float* X, const floa
 __global__ __launch_bounds__(BLK) void k_final(Dev d, const float* X, const float* k1, const float* k3,
                                                const float* k4, const float* k6, const double* h, float* Xn)
 {
-    const long long i = (long long)blockIdx.x * BLK + threadIdx.x;
-    const long long tr = blockIdx.y;
+    const TB tb = traj_block();
+    const long long i = (long long)tb.x * BLK + threadIdx.x;
+    const long long tr = tb.tr;
+    if (tr >= d.ntraj) return;
     if (i > (long long)d.n + d.m) return;
     const long long o = tr * d.D + i;
     if (i == 0) {
@@ -135,8 +162,10 @@ template <int K>
 __global__ __launch_bounds__(BLK) void k_vjp_clause(Dev d, const float* Y, const double* w, double* u,
                                                     double* contrib)
 {
-    const int m = blockIdx.x * BLK + threadIdx.x;
-    const long long tr = blockIdx.y;
+    const TB tb = traj_block();
+    const int m = tb.x * BLK + threadIdx.x;
+    const long long tr = tb.tr;
+    if (tr >= d.ntraj) return;
     if (m >= d.m) return;
     const float* y = Y + tr * d.D;
     const double* wt = w + tr * d.D;
@@ -180,9 +209,11 @@ template <int K>
 __global__ __launch_bounds__(BLK) void k_gradv_clause(Dev d, const float* X, double* g, double* contrib,
                                                       double* vpart, int nblk)
 {
+    const TB tb = traj_block();
     __shared__ double sh[BLK / 64];
-    const int m = blockIdx.x * BLK + threadIdx.x;
-    const long long tr = blockIdx.y;
+    const int m = tb.x * BLK + threadIdx.x;
+    const long long tr = tb.tr;
+    if (tr >= d.ntraj) return;
     double vloc = 0.0;
     if (m < d.m) {
         const float* x = X + tr * d.D;
@@ -215,7 +246,7 @@ __global__ __launch_bounds__(BLK) void k_gradv_clause(Dev d, const float* X, dou
     if (threadIdx.x == 0) {
         double t = 0.0;
         for (int i = 0; i < BLK / 64; i++) t += sh[i];
-        vpart[tr * nblk + blockIdx.x] = t;
+        vpart[tr * nblk + tb.x] = t;
     }
 }
 
@@ -233,6 +264,7 @@ __global__ void k_sum_parts(const double* vpart, int nblk, double* V)
 __global__ __launch_bounds__(BLK) void k_mu(Dev d, int l, const double* lam, const double* nu, const double* hh,
                                             double* mu)
 {
+    const TB tb = traj_block();
     const double Bj[7][6] = {
         {0}, {0},
         {(double)Tab<float>::b21},
@@ -243,15 +275,16 @@ __global__ __launch_bounds__(BLK) void k_mu(Dev d, int l, const double* lam, con
          (double)Tab<float>::b65}};
     const double Cw[7] = {0, (double)Tab<float>::c1, 0, (double)Tab<float>::c3, (double)Tab<float>::c4, 0,
                           (double)Tab<float>::c6};
-    const long long i = (long long)blockIdx.x * BLK + threadIdx.x;
-    const long long tr = blockIdx.y;
+    const long long i = (long long)tb.x * BLK + threadIdx.x;
+    const long long tr = tb.tr;
+    if (tr >= d.ntraj) return;
     if (i > (long long)d.n + d.m) return;
     const long long o = tr * d.D + i;
     if (i == 0) {
         mu[o] = 0.0;
         return;
     }
-    const long long vs = (long long)gridDim.y * d.D;   // stride between nu_j vectors
+    const long long vs = (long long)d.ntraj * d.D;   // stride between nu_j vectors
     double acc = Cw[l] * lam[o];
     for (int j = l + 1; j <= 6; j++) acc += Bj[j][l - 1] * nu[(long long)(j - 1) * vs + o];
     mu[o] = hh[tr] * acc;
@@ -259,11 +292,13 @@ __global__ __launch_bounds__(BLK) void k_mu(Dev d, int l, const double* lam, con
 
 __global__ __launch_bounds__(BLK) void k_lam(Dev d, double* lam, const double* nu)
 {
-    const long long i = (long long)blockIdx.x * BLK + threadIdx.x + 1;
-    const long long tr = blockIdx.y;
+    const TB tb = traj_block();
+    const long long i = (long long)tb.x * BLK + threadIdx.x + 1;
+    const long long tr = tb.tr;
+    if (tr >= d.ntraj) return;
     if (i > (long long)d.n + d.m) return;
     const long long o = tr * d.D + i;
-    const long long vs = (long long)gridDim.y * d.D;
+    const long long vs = (long long)d.ntraj * d.D;
     lam[o] = lam[o] + nu[o] + nu[vs + o] + nu[2 * vs + o] + nu[3 * vs + o] + nu[4 * vs + o] + nu[5 * vs + o];
 }
 
@@ -292,9 +327,11 @@ __global__ __launch_bounds__(BLK) void k_err_parts(Dev d, const float* X, const 
                                                    const float* k4, const float* k5, const float* k6,
                                                    const TCtl* ctl, double* parts, int nblk)
 {
+    const TB tb = traj_block();
     __shared__ double sh[BLK / 64];
-    const long long i = (long long)blockIdx.x * BLK + threadIdx.x;
-    const long long tr = blockIdx.y;
+    const long long i = (long long)tb.x * BLK + threadIdx.x;
+    const long long tr = tb.tr;
+    if (tr >= d.ntraj) return;
     double e = 0.0;
     if (i <= (long long)d.n + d.m) {
         const long long o = tr * d.D + i;
@@ -305,7 +342,7 @@ __global__ __launch_bounds__(BLK) void k_err_parts(Dev d, const float* X, const 
         e = fabs(xerr / xscal);
     }
     const double bm = block_max<BLK>(e, sh);
-    if (threadIdx.x == 0) parts[tr * nblk + blockIdx.x] = bm;
+    if (threadIdx.x == 0) parts[tr * nblk + tb.x] = bm;
 }
 
 // per-trajectory controller (the plain solver's k_control); one block each
@@ -356,8 +393,10 @@ __global__ __launch_bounds__(BLK) void k_ctl_batch(TCtl* ctl, const double* part
 __global__ __launch_bounds__(BLK) void k_accept(Dev d, const TCtl* ctl, const float* Xn, float* X, float* ckpt,
                                                 long long ntraj)
 {
-    const long long i = (long long)blockIdx.x * BLK + threadIdx.x;
-    const long long tr = blockIdx.y;
+    const TB tb = traj_block();
+    const long long i = (long long)tb.x * BLK + threadIdx.x;
+    const long long tr = tb.tr;
+    if (tr >= d.ntraj) return;
     if (i > (long long)d.n + d.m || !ctl[tr].accept) return;
     const long long o = tr * d.D + i;
     const float v = Xn[o];
@@ -427,7 +466,7 @@ struct Adj {
         (void)hipFree(d_lit); (void)hipFree(d_csr_ptr); (void)hipFree(d_csr_slot);
         if (stream) (void)hipStreamDestroy(stream);
     }
-    Dev dev() const { return Dev{d_lit, d_csr_ptr, d_csr_slot, n, m, k, D, L}; }
+    Dev dev() const { return Dev{d_lit, d_csr_ptr, d_csr_slot, n, m, k, D, L, ntraj}; }
 };
 
 static void set_problem(Adj& A, const int* lits, const int* lens, int nclauses)
@@ -515,7 +554,7 @@ static void alloc_batch(Adj& A, int ntraj)
 static void rhs_all(Adj& A, const float* Yv, float* out)
 {
     const Dev d = A.dev();
-    dim3 gc(ceil_div(A.m, BLK), A.ntraj), gv(ceil_div(A.n, BLK), A.ntraj);
+    dim3 gc(ceil_div(A.m, BLK), ntraj8(A.ntraj)), gv(ceil_div(A.n, BLK), ntraj8(A.ntraj));
     if (A.k == 3) hipLaunchKernelGGL(k_rhs_clause<3>, gc, dim3(BLK), 0, A.stream, d, Yv, out, A.c32);
     else hipLaunchKernelGGL(k_rhs_clause<4>, gc, dim3(BLK), 0, A.stream, d, Yv, out, A.c32);
     hipLaunchKernelGGL(k_rhs_var<float>, gv, dim3(BLK), 0, A.stream, d, (const float*)A.c32, out);
@@ -530,7 +569,7 @@ static void frozen_step(Adj& A, const float* X, float* Xn)
     float* Yl[6];
     float* kl[6];
     for (int i = 0; i < 6; i++) { Yl[i] = A.Y + i * V; kl[i] = A.kk + i * V; }
-    dim3 ge(ceil_div((long long)A.n + A.m, BLK), A.ntraj);
+    dim3 ge(ceil_div((long long)A.n + A.m, BLK), ntraj8(A.ntraj));
     HIP_CHECK(hipMemcpyAsync(Yl[0], X, sizeof(float) * V, hipMemcpyDeviceToDevice, A.stream));
     rhs_all(A, Yl[0], kl[0]);
     hipLaunchKernelGGL(k_stage<2>, ge, dim3(BLK), 0, A.stream, d, X, kl[0], kl[1], kl[2], kl[3], kl[4], A.hcur, Yl[1]);
@@ -544,7 +583,7 @@ static void frozen_step(Adj& A, const float* X, float* Xn)
     hipLaunchKernelGGL(k_stage<6>, ge, dim3(BLK), 0, A.stream, d, X, kl[0], kl[1], kl[2], kl[3], kl[4], A.hcur, Yl[5]);
     rhs_all(A, Yl[5], kl[5]);
     if (Xn) {
-        dim3 gf(ceil_div((long long)A.n + A.m + 1, BLK), A.ntraj);
+        dim3 gf(ceil_div((long long)A.n + A.m + 1, BLK), ntraj8(A.ntraj));
         hipLaunchKernelGGL(k_final, gf, dim3(BLK), 0, A.stream, d, X, kl[0], kl[2], kl[3], kl[5], A.hcur, Xn);
     }
 }
@@ -554,7 +593,7 @@ static void energy(Adj& A, const float* X, bool want_grad)
 {
     const Dev d = A.dev();
     const int nblk = ceil_div(A.m, BLK);
-    dim3 gc(nblk, A.ntraj), gv(ceil_div(A.n, BLK), A.ntraj);
+    dim3 gc(nblk, ntraj8(A.ntraj)), gv(ceil_div(A.n, BLK), ntraj8(A.ntraj));
     double* g = A.lam;
     if (A.k == 3) hipLaunchKernelGGL(k_gradv_clause<3>, gc, dim3(BLK), 0, A.stream, d, X, g, A.c64, A.vpart, nblk);
     else hipLaunchKernelGGL(k_gradv_clause<4>, gc, dim3(BLK), 0, A.stream, d, X, g, A.c64, A.vpart, nblk);
@@ -579,7 +618,7 @@ static void forward_lockstep(Adj& A, const std::vector<float>& X0)
     hipLaunchKernelGGL(k_ctl_init, dim3(ceil_div(A.ntraj, 64)), dim3(64), 0, A.stream, A.ctl, A.hstep, A.ntraj);
     A.hcur = A.hstep;
     const int nblk = ceil_div((long long)A.n + A.m + 1, BLK);
-    const dim3 ge(nblk, A.ntraj);
+    const dim3 ge(nblk, ntraj8(A.ntraj));
     const float* kl[6];
     for (int i = 0; i < 6; i++) kl[i] = A.kk + i * V;
     std::vector<TCtl> hc((size_t)A.ntraj);
@@ -789,8 +828,8 @@ int asat_adj_backward(asat_adj_t* a, double* grad_out)
     const int nseg = (A.T + A.C - 1) / A.C;
     // lam = grad V(x_T)
     energy(A, A.ckpt + (size_t)nseg * V, true);
-    dim3 ge(ceil_div((long long)A.n + A.m + 1, BLK), A.ntraj), ge1(ceil_div((long long)A.n + A.m, BLK), A.ntraj);
-    dim3 gc(ceil_div(A.m, BLK), A.ntraj), gv(ceil_div(A.n, BLK), A.ntraj);
+    dim3 ge(ceil_div((long long)A.n + A.m + 1, BLK), ntraj8(A.ntraj)), ge1(ceil_div((long long)A.n + A.m, BLK), ntraj8(A.ntraj));
+    dim3 gc(ceil_div(A.m, BLK), ntraj8(A.ntraj)), gv(ceil_div(A.n, BLK), ntraj8(A.ntraj));
     for (int sg = nseg - 1; sg >= 0; sg--) {
         const int s0 = sg * A.C, s1 = std::min(A.T, s0 + A.C);
         // recompute the segment's states from its checkpoint
