@@ -14,7 +14,8 @@
 //                     stage spin at the slot's position
 //   spos, srank       per tile: the positions of the tile's slots in ascending
 //                     order (int32), and each slot's rank in that order (uint16,
-//                     in slot order, the literal's sign in bit 15)
+//                     the literal's sign in bit 15; literal q of the tile's
+//                     clauses at q*TC + clause, so each load is one run)
 //   csr_ptr[0..N+1]   var-major CSR row pointers
 //   csr_loc[e]        uint16 offset of each of a variable's contributions in
 //                     its group's region, in slot order
@@ -88,8 +89,9 @@ struct HostProblem {
     std::vector<uint16_t> csr_loc; // the same relative to the variable's region start
     // tiled clause pass: per tile of TC clauses, its slots sorted by position
     std::vector<int> spos;         // ntiles*TC*k positions (-1 past the tile's last slot)
-    std::vector<uint16_t> srank;   // per slot (slot order): its rank in the tile's sorted list, the
-                                   // literal's sign in bit 15; 0xFFFF for padding
+    std::vector<uint16_t> srank;   // per slot: its rank in the tile's sorted list, the literal's sign
+                                   // in bit 15, 0xFFFF for padding; per tile, literal q of every
+                                   // clause first (index tile*TC*K + q*TC + clause in the tile)
 };
 
 // SatProblem::AddClauses semantics (AnalogSAT/problem/SatProblem.cpp:64-95) +
@@ -252,7 +254,8 @@ static void build_tile_order(HostProblem& P)
             int g = (std::abs(l) - 1) / P.vg;
             const size_t j = s0 + (size_t)cnt[g]++;
             P.spos[j] = P.pos[s];
-            P.srank[s] = (uint16_t)((j - s0) | (l < 0 ? 0x8000u : 0u));
+            const size_t ls = s - s0;   // tile-local slot: clause ls / K, literal ls % K
+            P.srank[s0 + (ls % K) * TC + ls / K] = (uint16_t)((j - s0) | (l < 0 ? 0x8000u : 0u));
         }
         for (int g : touched) cnt[g] = 0;
     }
@@ -398,7 +401,7 @@ __device__ __forceinline__ void tile_phase1(const StageArgs<T>& a, long long c0,
             for (int q = 0; q < K; q++) l[u][q] = ld_nt(a.lit + m * K + q);
         }
 #pragma unroll
-        for (int q = 0; q < K; q++) rk[u][q] = ld_nt(a.srank + m * K + q);
+        for (int q = 0; q < K; q++) rk[u][q] = ld_nt(a.srank + c0 * K + q * TC + (m - c0));
     }
     const int* sp_t = a.spos + c0 * K;
 #pragma unroll
@@ -640,40 +643,59 @@ __global__ __launch_bounds__(BLK) void k_clause_gen(StageArgs<T> a)
 }
 
 // ---- var pass: deterministic per-variable sums + spin stage combination ---
+// The state values the stage combination of variable i reads (x and the
+// earlier stage derivatives it needs), loaded ahead of the sums.
+template <typename T>
+struct VarIn {
+    T x, k1, k2, k3, k4, k5;
+};
+template <typename T, int J>
+__device__ __forceinline__ VarIn<T> var_load(const StageArgs<T>& a, int i, const T* x, bool fresh)
+{
+    VarIn<T> v;
+    v.x = ld_nt(x + i);
+    v.k1 = v.k2 = v.k3 = v.k4 = v.k5 = (T)0;
+    if (J > 1 || !fresh) v.k1 = ld_nt(a.k1 + i);
+    if (J > 2 && J < 6) v.k2 = ld_nt(a.k2 + i);
+    if (J > 3) v.k3 = ld_nt(a.k3 + i);
+    if (J > 4) v.k4 = ld_nt(a.k4 + i);
+    if (J > 5) v.k5 = ld_nt(a.k5 + i);
+    return v;
+}
+
 // Stage combination of one variable, shared by both var passes.  kj is the
 // variable's summed derivative (or, on a retry's J = 1, read back from k1).
 // Returns the spin the next clause pass reads: stage J+1's input, or x_new.
 template <typename T, int J>
-__device__ __forceinline__ T var_stage(const StageArgs<T>& a, int i, bool fresh, T kj, const T* x, T* xnew, T hs,
-                                       double h, double hnext, double floor, double& err)
+__device__ __forceinline__ T var_stage(const StageArgs<T>& a, int i, bool fresh, T kj, const VarIn<T>& v, T* xnew,
+                                       T hs, double h, double hnext, double floor, double& err)
 {
-    const T xi = ld_nt(x + i);
+    const T xi = v.x;
     T out = (T)0;
     if constexpr (J == 1) {
         if (fresh) st_nt(a.k1 + i, kj);
-        else kj = ld_nt(a.k1 + i);
+        else kj = v.k1;
         out = stage_input<T, 2>(xi, kj, 0, 0, 0, 0, hs);
         if (a.tmp) a.tmp[i] = out;
     } else if constexpr (J == 2) {
         st_nt(a.k2 + i, kj);
-        out = stage_input<T, 3>(xi, ld_nt(a.k1 + i), kj, 0, 0, 0, hs);
+        out = stage_input<T, 3>(xi, v.k1, kj, 0, 0, 0, hs);
         a.tmp[i] = out;
     } else if constexpr (J == 3) {
         st_nt(a.k3 + i, kj);
-        out = stage_input<T, 4>(xi, ld_nt(a.k1 + i), ld_nt(a.k2 + i), kj, 0, 0, hs);
+        out = stage_input<T, 4>(xi, v.k1, v.k2, kj, 0, 0, hs);
         a.tmp[i] = out;
     } else if constexpr (J == 4) {
         st_nt(a.k4 + i, kj);
-        out = stage_input<T, 5>(xi, ld_nt(a.k1 + i), ld_nt(a.k2 + i), ld_nt(a.k3 + i), kj, 0, hs);
+        out = stage_input<T, 5>(xi, v.k1, v.k2, v.k3, kj, 0, hs);
         a.tmp[i] = out;
     } else if constexpr (J == 5) {
         st_nt(a.k5 + i, kj);
-        out = stage_input<T, 6>(xi, ld_nt(a.k1 + i), ld_nt(a.k2 + i), ld_nt(a.k3 + i), ld_nt(a.k4 + i), kj, hs);
+        out = stage_input<T, 6>(xi, v.k1, v.k2, v.k3, v.k4, kj, hs);
         a.tmp[i] = out;
     } else {
         double e;
-        final_combination<T>(xi, ld_nt(a.k1 + i), ld_nt(a.k3 + i), ld_nt(a.k4 + i), ld_nt(a.k5 + i), kj, hs, h, hnext,
-                             floor, out, e);
+        final_combination<T>(xi, v.k1, v.k3, v.k4, v.k5, kj, hs, h, hnext, floor, out, e);
         err = fmax(err, e);
         st_nt(xnew + i, out);
     }
@@ -740,6 +762,18 @@ __global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
     T* s_next = a.slot_spins ? a.contrib : nullptr;
     const long long r0 = a.gstart[g];
     const int nreg = (int)(a.gstart[g + 1] - r0) / V;
+    // each thread's state values, loaded before the region (groups of at most
+    // 2 * VBLK variables; larger groups load them in the loop)
+    constexpr int VP = 2;
+    const bool pre = i1 - i0 + 1 <= VP * VBLK;
+    VarIn<T> vin[VP];
+    if (pre) {
+#pragma unroll
+        for (int u = 0; u < VP; u++) {
+            const int i = i0 + (int)threadIdx.x + u * VBLK;
+            if (i <= i1) vin[u] = var_load<T, J>(a, i, x, fresh);
+        }
+    }
     int ebase = 0;
     if (fresh || s_next) {
         // region sizes are multiples of REGION_ALIGN and the offsets of a group
@@ -775,7 +809,8 @@ __global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
     }
     double err = 0.0;
     const T pref = (a.bias && fresh) ? bias_prefactor<T, VBLK>(a, sh) : (T)0;
-    for (int i = i0 + (int)threadIdx.x; i <= i1; i += VBLK) {
+    int u = 0;
+    for (int i = i0 + (int)threadIdx.x; i <= i1; i += VBLK, u++) {
         T kj = (T)0;
         const int e0 = a.csr_ptr[i] - ebase, e1 = a.csr_ptr[i + 1] - ebase;
         if (fresh) {
@@ -784,7 +819,10 @@ __global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
             kj = sum;
             if (a.bias) kj += pref * bias_sine<T>(J == 1 ? x[i] : a.tmp[i]);
         }
-        const T out = var_stage<T, J>(a, i, fresh, kj, x, xnew, hs, h, hnext, floor, err);
+        VarIn<T> in;
+        if (pre) in = u == 0 ? vin[0] : vin[1];
+        else in = var_load<T, J>(a, i, x, fresh);
+        const T out = var_stage<T, J>(a, i, fresh, kj, in, xnew, hs, h, hnext, floor, err);
         if (s_next)
             for (int e = e0; e < e1; e++) reg[loc[e]] = out;
     }
@@ -849,7 +887,7 @@ __global__ __launch_bounds__(BLK) void k_var(StageArgs<T> a)
         kj = sum;
     }
     if (a.bias && fresh && i != 0) kj += pref * bias_sine<T>(J == 1 ? x[i] : a.tmp[i]);
-    if (i != 0) var_stage<T, J>(a, i, fresh, kj, x, xnew, hs, h, hnext, floor, err);
+    if (i != 0) var_stage<T, J>(a, i, fresh, kj, var_load<T, J>(a, i, x, fresh), xnew, hs, h, hnext, floor, err);
     if constexpr (J == 6) {
         double bm = block_max<BLK>(err, sh);
         if (threadIdx.x == 0) a.partial[a.nblk_clause + blockIdx.x] = bm;
