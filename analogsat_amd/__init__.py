@@ -138,6 +138,8 @@ def lib():
             "asat_rk_kernel_timer": ([vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)], i),
             "asat_util_copy_bench": ([ctypes.c_longlong, i, ctypes.POINTER(ctypes.c_float)], i),
             "asat_util_gpu_name": ([ctypes.c_char_p, i], i),
+            "asat_util_stream_bench": ([ctypes.c_longlong, i, ctypes.POINTER(ctypes.c_float),
+                                        ctypes.POINTER(ctypes.c_float)], i),
             "asat_rk_set_step_log": ([vp, vp, ctypes.c_longlong], i),
             "asat_adj_create": ([pvp], i),
             "asat_adj_destroy": ([vp], i),
@@ -179,6 +181,13 @@ def copy_bench(nbytes, reps=10):
     us = ctypes.c_float()
     _check(lib().asat_util_copy_bench(int(nbytes), int(reps), ctypes.byref(us)))
     return us.value
+
+
+def stream_bench(nbytes=1 << 30, reps=5):
+    """Practical HBM rates (GB/s) of a well-formed copy and read-only pass."""
+    uc, ur = ctypes.c_float(), ctypes.c_float()
+    _check(lib().asat_util_stream_bench(int(nbytes), int(reps), ctypes.byref(uc), ctypes.byref(ur)))
+    return {"copy_GBps": 2 * nbytes / (uc.value * 1e-6) / 1e9, "read_GBps": nbytes / (ur.value * 1e-6) / 1e9}
 
 
 def GetGpuName():

@@ -1040,6 +1040,47 @@ __global__ void k_set_initial(T* x, int n, int m, int aoff, const T* u, const in
 }
 
 // plain streaming copy (16 B per lane): the reference point for HBM byte counters
+// Practical HBM rate probes: 16 bytes per lane, STREAM_U loads in flight per
+// thread before the first use, non-temporal.
+constexpr int STREAM_U = 4;
+typedef int stream_v4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_stream_copy(const stream_v4* __restrict__ src, stream_v4* __restrict__ dst,
+                                                     long long n16)
+{
+    const long long stride = (long long)gridDim.x * 256 * STREAM_U;
+    for (long long base = (long long)blockIdx.x * 256 * STREAM_U + threadIdx.x; base < n16; base += stride) {
+        stream_v4 r[STREAM_U];
+#pragma unroll
+        for (int u = 0; u < STREAM_U; u++) {
+            const long long i = base + u * 256;
+            if (i < n16) r[u] = ld_nt(src + i);
+        }
+#pragma unroll
+        for (int u = 0; u < STREAM_U; u++) {
+            const long long i = base + u * 256;
+            if (i < n16) st_nt(dst + i, r[u]);
+        }
+    }
+}
+// read-only pass; `out` is written only if the data hash to a value that a
+// zero-filled source never gives, which keeps the loads
+__global__ __launch_bounds__(256) void k_stream_read(const stream_v4* __restrict__ src, int* out, long long n16)
+{
+    const long long stride = (long long)gridDim.x * 256 * STREAM_U;
+    int acc = 0;
+    for (long long base = (long long)blockIdx.x * 256 * STREAM_U + threadIdx.x; base < n16; base += stride) {
+        stream_v4 r[STREAM_U];
+#pragma unroll
+        for (int u = 0; u < STREAM_U; u++) {
+            const long long i = base + u * 256;
+            r[u] = i < n16 ? ld_nt(src + i) : stream_v4{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int u = 0; u < STREAM_U; u++) acc ^= r[u].x ^ r[u].y ^ r[u].z ^ r[u].w;
+    }
+    if (acc == 0x7fffffff) out[0] = acc;
+}
+
 __global__ void k_copy16(const int4* __restrict__ src, int4* __restrict__ dst, long long n16)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (long long)gridDim.x * blockDim.x)
@@ -2090,6 +2131,43 @@ int asat_util_copy_bench(long long bytes, int reps, float* us_per_copy)
     *us_per_copy = ms * 1000.f / reps;
     (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
     (void)hipFree(a); (void)hipFree(b);
+    API_END
+}
+
+int asat_util_stream_bench(long long bytes, int reps, float* us_copy, float* us_read)
+{
+    API_BEGIN
+    CHECK_PTR(us_copy); CHECK_PTR(us_read);
+    require(bytes >= 16 && bytes % 16 == 0 && reps > 0, ASAT_ERR_INVALID_ARGUMENT,
+            "bytes must be a positive multiple of 16 and reps positive");
+    void *a = nullptr, *b = nullptr;
+    int* o = nullptr;
+    HIP_CHECK(hipMalloc(&a, (size_t)bytes));
+    HIP_CHECK(hipMalloc(&b, (size_t)bytes));
+    HIP_CHECK(hipMalloc((void**)&o, sizeof(int)));
+    HIP_CHECK(hipMemset(a, 0, (size_t)bytes));
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    const long long n16 = bytes / 16;
+    const dim3 grid(8192), blk(256);
+    float ms = 0.f;
+    hipLaunchKernelGGL(k_stream_copy, grid, blk, 0, 0, (const stream_v4*)a, (stream_v4*)b, n16);
+    HIP_CHECK(hipEventRecord(e0, 0));
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_stream_copy, grid, blk, 0, 0, (const stream_v4*)a, (stream_v4*)b, n16);
+    HIP_CHECK(hipEventRecord(e1, 0));
+    HIP_CHECK(hipEventSynchronize(e1));
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    *us_copy = ms * 1000.f / reps;
+    hipLaunchKernelGGL(k_stream_read, grid, blk, 0, 0, (const stream_v4*)a, o, n16);
+    HIP_CHECK(hipEventRecord(e0, 0));
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_stream_read, grid, blk, 0, 0, (const stream_v4*)a, o, n16);
+    HIP_CHECK(hipEventRecord(e1, 0));
+    HIP_CHECK(hipEventSynchronize(e1));
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    *us_read = ms * 1000.f / reps;
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    (void)hipFree(a); (void)hipFree(b); (void)hipFree(o);
     API_END
 }
 

@@ -452,6 +452,9 @@ def main():
     attempt_us = float(us[:13].sum())
     att_bytes = attempt_bytes(n, m, k, nslots, esz)
 
+    hbm = A.stream_bench(1 << 30, 5)
+    log("practical HBM rates: copy %.0f GB/s, read %.0f GB/s" % (hbm["copy_GBps"], hbm["read_GBps"]))
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline: %d steps" % args.cpu_steps)
@@ -484,7 +487,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                      "kernel_us": round(dom_us, 2), "kernel_launches": int(t_n),
-                     "kernel_us_alone": round(float(us[names.index(dom)]), 2), "bytes_per_launch": int(kb[dom])},
+                     "kernel_us_alone": round(float(us[names.index(dom)]), 2), "bytes_per_launch": int(kb[dom]),
+                     "practical_GBps": {"copy": round(hbm["copy_GBps"], 1), "read": round(hbm["read_GBps"], 1)}},
         "attempt": {"attempts_per_s": round(attempts / wall, 2), "reject_ratio": round(1 - steps / max(attempts, 1), 4),
                     "kernel_us_sum": round(attempt_us, 1), "B_alg": int(att_bytes),
                     "B_alg_GBps_at_kernel_sum": round(att_bytes / (attempt_us * 1e-6) / 1e9, 1),

@@ -194,6 +194,12 @@ int asat_rk_kernel_timer(const asat_rk_t* rk, double* total_ms, long long* launc
    calibrate the HBM byte counters of rocprofv3 against a known traffic. */
 int asat_util_copy_bench(long long bytes, int reps, float* us_per_copy);
 
+/* Measurement hook, no reference counterpart: the device's practical HBM
+   rates for well-formed streams (16 bytes per lane, four non-temporal loads
+   in flight per thread): average microseconds of a copy of `bytes` and of a
+   read-only pass over `bytes`, over `reps` launches each. */
+int asat_util_stream_bench(long long bytes, int reps, float* us_copy, float* us_read);
+
 /* GetGpuName (AnalogSAT/util/gpuidtool.cpp:28-40): the current device's
    name, NUL-terminated and truncated to len - 1 bytes. */
 int asat_util_gpu_name(char* buf, int len);
