@@ -204,6 +204,36 @@ def test_step_count_semantics_and_reset():
     assert st.hnext == 1e-9
 
 
+def test_kernel_timer_and_stream_probe():
+    """Measurement hooks: the kernel timer times one launch of the chosen
+    kind per enqueued attempt and does not change the trajectory; the
+    stream probe reports positive rates."""
+    say("kernel timer and stream probe")
+    p = O.fastsat(3000, 12600, 3, 21)
+    pub = O.random_state_f64(5, p.n, p.m).astype(np.float32)
+    states = []
+    for kind in (-1, 5, 11):
+        sat = _sys_with(p.literals, p.lengths, "f32")
+        rk = A.HipRungeKutta()
+        rk.SetEpsilon(1e-6)
+        rk.Configure(sat)
+        old, new = sat.MakeState(), sat.MakeState()
+        old.CopyFrom(pub)
+        rk.set_kernel_timer(kind)
+        assert rk.Step(old, new, 30)
+        ms, n = rk.kernel_timer()
+        if kind < 0:
+            assert (ms, n) == (0.0, 0)
+        else:
+            assert n == rk.GetAttemptCount() and ms > 0.0
+        states.append(old.CopyTo())
+    assert np.array_equal(states[0], states[1]) and np.array_equal(states[0], states[2])
+    with pytest.raises(ValueError):
+        A.HipRungeKutta().set_kernel_timer(13)
+    bw = A.stream_bench(64 << 20, 2)
+    assert bw["copy_GBps"] > 0 and bw["read_GBps"] > 0
+
+
 def test_large_instance_rhs_property():
     """N = 1e6-scale RHS: GPU FP32 equals the oracle's float mode bit for bit."""
     say("rhs f32 N=200000 M=852000")
