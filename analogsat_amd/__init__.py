@@ -149,6 +149,7 @@ def lib():
             "asat_adj_get_steps": ([vp, vp], i),
             "asat_adj_backward": ([vp, vp], i),
             "asat_adj_replay": ([vp, i, vp, vp, vp, vp], i),
+            "asat_adj_replay_f64": ([vp, i, vp, vp, vp], i),
             "asat_adj_timing": ([vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], i),
             "asat_rng_create": ([ctypes.c_ulonglong, pvp], i),
             "asat_rng_destroy": ([vp], i),
@@ -837,6 +838,15 @@ class HipAdjoint:
         _check(lib().asat_adj_replay(self._h, x.shape[0], x.ctypes.data, h.ctypes.data, V.ctypes.data, xT.ctypes.data))
         self.ntraj = 0
         return V, xT
+
+    def replay_f64(self, pub0, h):
+        """V(T) of T steps with the given step sizes and FP64 state."""
+        x = np.ascontiguousarray(pub0, np.float64).reshape(-1, self.nvar)
+        h = np.ascontiguousarray(h, np.float64).reshape(x.shape[0], self.T)
+        V = np.zeros(x.shape[0])
+        _check(lib().asat_adj_replay_f64(self._h, x.shape[0], x.ctypes.data, h.ctypes.data, V.ctypes.data))
+        self.ntraj = 0
+        return V
 
     def timing(self):
         f, b = ctypes.c_double(), ctypes.c_double()

@@ -63,35 +63,35 @@ __device__ __forceinline__ TB traj_block()
     return {r >> 3, (L / per) * 8 + (r & 7)};
 }
 
-// ---- FP32 recompute (forward arithmetic) ----------------------------------
-template <int K>
-__global__ __launch_bounds__(BLK) void k_rhs_clause(Dev d, const float* Y, float* out, float* contrib)
+// ---- recompute (forward arithmetic; T = float, or double for the FP64 replay)
+template <typename T, int K>
+__global__ __launch_bounds__(BLK) void k_rhs_clause(Dev d, const T* Y, T* out, T* contrib)
 {
     const TB tb = traj_block();
     const int m = tb.x * BLK + threadIdx.x;
     const long long tr = tb.tr;
     if (tr >= d.ntraj) return;
     if (m >= d.m) return;
-    const float* y = Y + tr * d.D;
+    const T* y = Y + tr * d.D;
     int l[K];
-    float f[K];
+    T f[K];
 #pragma unroll
     for (int q = 0; q < K; q++) l[q] = d.lit[(size_t)m * K + q];
-    float km = (float)(1.0 / (1 << K));
+    T km = (T)(1.0 / (1 << K));
 #pragma unroll
     for (int q = 0; q < K; q++) {
-        f[q] = spin_factor<float>(l[q], y);
+        f[q] = spin_factor<T>(l[q], y);
         km *= f[q];
     }
     const long long ai = d.n + 1 + m;
-    const float am = y[ai];
+    const T am = y[ai];
     out[tr * d.D + ai] = am * km * km;
-    const float r = km * (float)(1.0 / (1 << K)) * am;
-    float o[K];
+    const T r = km * (T)(1.0 / (1 << K)) * am;
+    T o[K];
 #pragma unroll
-    for (int q = 0; q < K; q++) o[q] = 0.f;
-    write_contribs<float, K, 0>(l, f, r, o);
-    float* c = contrib + tr * d.L + (long long)m * K;
+    for (int q = 0; q < K; q++) o[q] = (T)0;
+    write_contribs<T, K, 0>(l, f, r, o);
+    T* c = contrib + tr * d.L + (long long)m * K;
 #pragma unroll
     for (int q = 0; q < K; q++)
         if (l[q] != 0) c[q] = o[q];
@@ -112,10 +112,9 @@ __global__ __launch_bounds__(BLK) void k_rhs_var(Dev d, const T* contrib, T* out
 }
 
 // stage input J of every element (spins and aux) for every trajectory
-template <int J>
-__global__ __launch_bounds__(BLK) void k_stage(Dev d, const float* X, const float* k1, const float* k2,
-                                                const float* k3, const float* k4, const float* k5,
-                                                const double* h, float* Y)
+template <typename T, int J>
+__global__ __launch_bounds__(BLK) void k_stage(Dev d, const T* X, const T* k1, const T* k2, const T* k3, const T* k4,
+                                                const T* k5, const double* h, T* Y)
 {
     const TB tb = traj_block();
     const long long i = (long long)tb.x * BLK + threadIdx.x + 1;
@@ -123,13 +122,14 @@ __global__ __launch_bounds__(BLK) void k_stage(Dev d, const float* X, const floa
     if (tr >= d.ntraj) return;
     if (i > (long long)d.n + d.m) return;
     const long long o = tr * d.D + i;
-    const float hs = (float)h[tr];
-    Y[o] = stage_input<float, J>(X[o], k1[o], J > 2 ? k2[o] : 0.f, J > 3 ? k3[o] : 0.f, J > 4 ? k4[o] : 0.f,
-                                 J > 5 ? k5[o] : 0.f, hs);
+    const T hs = (T)h[tr];
+    Y[o] = stage_input<T, J>(X[o], k1[o], J > 2 ? k2[o] : (T)0, J > 3 ? k3[o] : (T)0, J > 4 ? k4[o] : (T)0,
+                             J > 5 ? k5[o] : (T)0, hs);
 }
 
-__global__ __launch_bounds__(BLK) void k_final(Dev d, const float* X, const float* k1, const float* k3,
-                                               const float* k4, const float* k6, const double* h, float* Xn)
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_final(Dev d, const T* X, const T* k1, const T* k3, const T* k4, const T* k6,
+                                               const double* h, T* Xn)
 {
     const TB tb = traj_block();
     const long long i = (long long)tb.x * BLK + threadIdx.x;
@@ -138,11 +138,11 @@ __global__ __launch_bounds__(BLK) void k_final(Dev d, const float* X, const floa
     if (i > (long long)d.n + d.m) return;
     const long long o = tr * d.D + i;
     if (i == 0) {
-        Xn[o] = -1.0f;
+        Xn[o] = (T)-1.0;
         return;
     }
-    using B = Tab<float>;
-    const float hs = (float)h[tr];
+    using B = Tab<T>;
+    const T hs = (T)h[tr];
     Xn[o] = X[o] + hs * (B::c1 * k1[o] + B::c3 * k3[o] + B::c4 * k4[o] + B::c6 * k6[o]);
 }
 
@@ -205,8 +205,8 @@ __global__ __launch_bounds__(BLK) void k_vjp_clause(Dev d, const float* Y, const
 }
 
 // grad V: aux K^2, spin contributions -2 a c K K_p; per-block partial sums of V
-template <int K>
-__global__ __launch_bounds__(BLK) void k_gradv_clause(Dev d, const float* X, double* g, double* contrib,
+template <typename TX, int K>
+__global__ __launch_bounds__(BLK) void k_gradv_clause(Dev d, const TX* X, double* g, double* contrib,
                                                       double* vpart, int nblk)
 {
     const TB tb = traj_block();
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(BLK) void k_gradv_clause(Dev d, const float* X, dou
     if (tr >= d.ntraj) return;
     double vloc = 0.0;
     if (m < d.m) {
-        const float* x = X + tr * d.D;
+        const TX* x = X + tr * d.D;
         const double kn = 1.0 / (1 << K);
         int v[K];
         double fd[K], cs[K];
@@ -225,7 +225,7 @@ __global__ __launch_bounds__(BLK) void k_gradv_clause(Dev d, const float* X, dou
             const int l = d.lit[(size_t)m * K + q];
             v[q] = l < 0 ? -l : l;
             cs[q] = l < 0 ? -1.0 : 1.0;
-            fd[q] = (double)spin_factor<float>(l, x);
+            fd[q] = (double)spin_factor<TX>(l, x);
         }
         const double Kc = prod_skip<K>(fd, -1, -1, kn);
         const long long ai = d.n + 1 + m;
@@ -551,52 +551,61 @@ static void alloc_batch(Adj& A, int ntraj)
 }
 
 // f = RHS(Y) for all trajectories (FP32, forward arithmetic)
-static void rhs_all(Adj& A, const float* Yv, float* out)
+template <typename T>
+static void rhs_all(Adj& A, const T* Yv, T* out, T* contrib)
 {
     const Dev d = A.dev();
     dim3 gc(ceil_div(A.m, BLK), ntraj8(A.ntraj)), gv(ceil_div(A.n, BLK), ntraj8(A.ntraj));
-    if (A.k == 3) hipLaunchKernelGGL(k_rhs_clause<3>, gc, dim3(BLK), 0, A.stream, d, Yv, out, A.c32);
-    else hipLaunchKernelGGL(k_rhs_clause<4>, gc, dim3(BLK), 0, A.stream, d, Yv, out, A.c32);
-    hipLaunchKernelGGL(k_rhs_var<float>, gv, dim3(BLK), 0, A.stream, d, (const float*)A.c32, out);
+    if (A.k == 3) hipLaunchKernelGGL((k_rhs_clause<T, 3>), gc, dim3(BLK), 0, A.stream, d, Yv, out, contrib);
+    else hipLaunchKernelGGL((k_rhs_clause<T, 4>), gc, dim3(BLK), 0, A.stream, d, Yv, out, contrib);
+    hipLaunchKernelGGL(k_rhs_var<T>, gv, dim3(BLK), 0, A.stream, d, (const T*)contrib, out);
 }
 
 // one frozen Cash-Karp step for every trajectory with step sizes hcol;
 // fills Y[0..5] (stage inputs) and kk[0..5]; writes x_next if non-null
-static void frozen_step(Adj& A, const float* X, float* Xn)
+template <typename T>
+static void frozen_step(Adj& A, const T* X, T* Xn, T* Y, T* kk, T* contrib)
 {
     const Dev d = A.dev();
     const size_t V = (size_t)A.ntraj * A.D;
-    float* Yl[6];
-    float* kl[6];
-    for (int i = 0; i < 6; i++) { Yl[i] = A.Y + i * V; kl[i] = A.kk + i * V; }
+    T* Yl[6];
+    T* kl[6];
+    for (int i = 0; i < 6; i++) { Yl[i] = Y + i * V; kl[i] = kk + i * V; }
     dim3 ge(ceil_div((long long)A.n + A.m, BLK), ntraj8(A.ntraj));
-    HIP_CHECK(hipMemcpyAsync(Yl[0], X, sizeof(float) * V, hipMemcpyDeviceToDevice, A.stream));
-    rhs_all(A, Yl[0], kl[0]);
-    hipLaunchKernelGGL(k_stage<2>, ge, dim3(BLK), 0, A.stream, d, X, kl[0], kl[1], kl[2], kl[3], kl[4], A.hcur, Yl[1]);
-    rhs_all(A, Yl[1], kl[1]);
-    hipLaunchKernelGGL(k_stage<3>, ge, dim3(BLK), 0, A.stream, d, X, kl[0], kl[1], kl[2], kl[3], kl[4], A.hcur, Yl[2]);
-    rhs_all(A, Yl[2], kl[2]);
-    hipLaunchKernelGGL(k_stage<4>, ge, dim3(BLK), 0, A.stream, d, X, kl[0], kl[1], kl[2], kl[3], kl[4], A.hcur, Yl[3]);
-    rhs_all(A, Yl[3], kl[3]);
-    hipLaunchKernelGGL(k_stage<5>, ge, dim3(BLK), 0, A.stream, d, X, kl[0], kl[1], kl[2], kl[3], kl[4], A.hcur, Yl[4]);
-    rhs_all(A, Yl[4], kl[4]);
-    hipLaunchKernelGGL(k_stage<6>, ge, dim3(BLK), 0, A.stream, d, X, kl[0], kl[1], kl[2], kl[3], kl[4], A.hcur, Yl[5]);
-    rhs_all(A, Yl[5], kl[5]);
+    HIP_CHECK(hipMemcpyAsync(Yl[0], X, sizeof(T) * V, hipMemcpyDeviceToDevice, A.stream));
+    rhs_all<T>(A, Yl[0], kl[0], contrib);
+    hipLaunchKernelGGL((k_stage<T, 2>), ge, dim3(BLK), 0, A.stream, d, X, kl[0], kl[1], kl[2], kl[3], kl[4], A.hcur, Yl[1]);
+    rhs_all<T>(A, Yl[1], kl[1], contrib);
+    hipLaunchKernelGGL((k_stage<T, 3>), ge, dim3(BLK), 0, A.stream, d, X, kl[0], kl[1], kl[2], kl[3], kl[4], A.hcur, Yl[2]);
+    rhs_all<T>(A, Yl[2], kl[2], contrib);
+    hipLaunchKernelGGL((k_stage<T, 4>), ge, dim3(BLK), 0, A.stream, d, X, kl[0], kl[1], kl[2], kl[3], kl[4], A.hcur, Yl[3]);
+    rhs_all<T>(A, Yl[3], kl[3], contrib);
+    hipLaunchKernelGGL((k_stage<T, 5>), ge, dim3(BLK), 0, A.stream, d, X, kl[0], kl[1], kl[2], kl[3], kl[4], A.hcur, Yl[4]);
+    rhs_all<T>(A, Yl[4], kl[4], contrib);
+    hipLaunchKernelGGL((k_stage<T, 6>), ge, dim3(BLK), 0, A.stream, d, X, kl[0], kl[1], kl[2], kl[3], kl[4], A.hcur, Yl[5]);
+    rhs_all<T>(A, Yl[5], kl[5], contrib);
     if (Xn) {
         dim3 gf(ceil_div((long long)A.n + A.m + 1, BLK), ntraj8(A.ntraj));
-        hipLaunchKernelGGL(k_final, gf, dim3(BLK), 0, A.stream, d, X, kl[0], kl[2], kl[3], kl[5], A.hcur, Xn);
+        hipLaunchKernelGGL(k_final<T>, gf, dim3(BLK), 0, A.stream, d, X, kl[0], kl[2], kl[3], kl[5], A.hcur, Xn);
     }
 }
 
+// the FP32 recompute of the forward, into the batch's own buffers
+static void frozen_step(Adj& A, const float* X, float* Xn)
+{
+    frozen_step<float>(A, X, Xn, A.Y, A.kk, A.c32);
+}
+
 // V (and grad V into lam if want_grad) at X for every trajectory
-static void energy(Adj& A, const float* X, bool want_grad)
+template <typename TX>
+static void energy(Adj& A, const TX* X, bool want_grad)
 {
     const Dev d = A.dev();
     const int nblk = ceil_div(A.m, BLK);
     dim3 gc(nblk, ntraj8(A.ntraj)), gv(ceil_div(A.n, BLK), ntraj8(A.ntraj));
     double* g = A.lam;
-    if (A.k == 3) hipLaunchKernelGGL(k_gradv_clause<3>, gc, dim3(BLK), 0, A.stream, d, X, g, A.c64, A.vpart, nblk);
-    else hipLaunchKernelGGL(k_gradv_clause<4>, gc, dim3(BLK), 0, A.stream, d, X, g, A.c64, A.vpart, nblk);
+    if (A.k == 3) hipLaunchKernelGGL((k_gradv_clause<TX, 3>), gc, dim3(BLK), 0, A.stream, d, X, g, A.c64, A.vpart, nblk);
+    else hipLaunchKernelGGL((k_gradv_clause<TX, 4>), gc, dim3(BLK), 0, A.stream, d, X, g, A.c64, A.vpart, nblk);
     if (want_grad) {
         hipLaunchKernelGGL(k_rhs_var<double>, gv, dim3(BLK), 0, A.stream, d, (const double*)A.c64, g);
         hipLaunchKernelGGL(k_zero_slot0, dim3(ceil_div(A.ntraj, 64)), dim3(64), 0, A.stream, g, A.D, A.ntraj);
@@ -809,6 +818,40 @@ int asat_adj_replay(asat_adj_t* a, int ntraj, const float* pub0, const double* h
             std::copy(xt.begin() + (size_t)t * A.D + 1, xt.begin() + (size_t)t * A.D + 1 + A.n + A.m,
                       xT_out + (size_t)t * (A.n + A.m));
     }
+    A.have_forward = false;
+    AAPI_END
+}
+
+int asat_adj_replay_f64(asat_adj_t* a, int ntraj, const double* pub0, const double* h, double* V_out)
+{
+    AAPI_BEGIN
+    require(a && pub0 && h && V_out && ntraj > 0, ASAT_ERR_INVALID_ARGUMENT, "null argument or no trajectories");
+    Adj& A = a->a;
+    require(A.have_problem && A.T > 0, ASAT_ERR_NOT_CONFIGURED, "adjoint problem/parameters not set");
+    alloc_batch(A, ntraj);
+    const size_t V = (size_t)ntraj * A.D;
+    HIP_CHECK(hipMemcpy(A.hlog, h, sizeof(double) * (size_t)ntraj * A.T, hipMemcpyHostToDevice));
+    make_columns(A);
+    std::vector<double> x0(V, 0.0);
+    for (int t = 0; t < ntraj; t++) {
+        x0[(size_t)t * A.D] = -1.0;
+        std::copy(pub0 + (size_t)t * (A.n + A.m), pub0 + (size_t)(t + 1) * (A.n + A.m), x0.begin() + (size_t)t * A.D + 1);
+    }
+    // FP64 state: the current and next state, six stage inputs, six stage derivatives
+    double* buf = nullptr;
+    HIP_CHECK(hipMalloc((void**)&buf, sizeof(double) * V * 14));
+    double* cur = buf;
+    double* nxt = buf + V;
+    HIP_CHECK(hipMemcpy(cur, x0.data(), sizeof(double) * V, hipMemcpyHostToDevice));
+    for (int s = 0; s < A.T; s++) {
+        load_h(A, s);
+        frozen_step<double>(A, cur, nxt, buf + 2 * V, buf + 8 * V, A.c64);
+        std::swap(cur, nxt);
+    }
+    energy<double>(A, cur, false);
+    HIP_CHECK(hipStreamSynchronize(A.stream));
+    HIP_CHECK(hipMemcpy(V_out, A.dV, sizeof(double) * ntraj, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipFree(buf));
     A.have_forward = false;
     AAPI_END
 }

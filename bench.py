@@ -344,11 +344,62 @@ def workload_c5(args, log):
     g = adj.backward()
     t2 = time.perf_counter()
     fwd_ms, bwd_ms = adj.timing()
+    # central differences of V(T) along 256 random coordinates of trajectory
+    # 0, with its step sizes frozen (FP32 replays of the perturbed starts,
+    # 64 at a time); the step is the FP32-representable difference
+    h0 = adj.steps()[0]
+    nv = n + m
+    coords = np.random.default_rng(6).choice(nv, 256, replace=False)
+    x0 = pubs[0].astype(np.float64)
+    starts, steps_fd = [], []
+    for i in coords:
+        xp, xm = x0.copy(), x0.copy()
+        xp[i] += 1e-4
+        xm[i] -= 1e-4
+        xp32, xm32 = xp.astype(np.float32), xm.astype(np.float32)
+        starts += [xp32, xm32]
+        steps_fd.append(float(xp32[i]) - float(xm32[i]))
+    starts = np.stack(starts)
+    Vs = np.concatenate([adj.replay(starts[b:b + 64], np.tile(h0, (len(starts[b:b + 64]), 1)))[0]
+                         for b in range(0, len(starts), 64)])
+    fd = (Vs[0::2] - Vs[1::2]) / np.array(steps_fd)
+    gi = g[0][coords]
+    gmax = np.abs(g[0]).max()
+    rel = np.abs(gi - fd) / gmax
+    log("C5: FD (FP32 replays, h = 1e-4) on 256 coordinates: max |grad - FD| / max|grad| = %.2e, median %.2e"
+        % (rel.max(), np.median(rel)))
+    # the same coordinates on the FP64 frozen map at h = 1e-6 and 1e-8: in
+    # its linear range the two agree, and they are the FP64 reference
+    x64 = pubs[0].astype(np.float64)
+
+    def fd_f64(step):
+        starts64 = []
+        for i in coords:
+            xp, xm = x64.copy(), x64.copy()
+            xp[i] += step
+            xm[i] -= step
+            starts64 += [xp, xm]
+        starts64 = np.stack(starts64)
+        V64 = np.concatenate([adj.replay_f64(starts64[b:b + 64], np.tile(h0, (len(starts64[b:b + 64]), 1)))
+                              for b in range(0, len(starts64), 64)])
+        return (V64[0::2] - V64[1::2]) / (2 * step)
+
+    fd64, fd64b = fd_f64(1e-6), fd_f64(1e-8)
+    rel64 = np.abs(gi - fd64) / gmax
+    conv = np.abs(fd64 - fd64b) / gmax
+    log("C5: FD (FP64 replays, h = 1e-6): max %.2e, median %.2e; h = 1e-6 vs 1e-8: max %.2e"
+        % (rel64.max(), np.median(rel64), conv.max()))
     return {"workload": "c5", "trajectories": ntraj, "steps": T, "checkpoint_every": C,
             "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
             "forward_device_ms": round(fwd_ms, 2), "backward_device_ms": round(bwd_ms, 2),
             "backward_trajectory_steps_per_s": round(ntraj * T / (t2 - t1), 1),
-            "grad_norm_first": float(np.linalg.norm(g[0]))}
+            "grad_norm_first": float(np.linalg.norm(g[0])),
+            "fd_check": {"coordinates": 256, "trajectory": 0, "frozen_steps": True,
+                         "fp32_map_h_1e-4": {"max_abs_err_over_max_grad": float(rel.max()),
+                                             "median_abs_err_over_max_grad": float(np.median(rel))},
+                         "fp64_map_h_1e-6": {"max_abs_err_over_max_grad": float(rel64.max()),
+                                             "median_abs_err_over_max_grad": float(np.median(rel64)),
+                                             "max_change_to_h_1e-8_over_max_grad": float(conv.max())}}}
 
 
 def main():

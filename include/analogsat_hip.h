@@ -306,6 +306,10 @@ int asat_adj_backward(asat_adj_t* a, double* grad_out);
 /* replay T steps with given step sizes h (ntraj x T), FP32 like the forward:
    V_out[ntraj] and optionally the end states xT_out[ntraj x (n+m)] */
 int asat_adj_replay(asat_adj_t* a, int ntraj, const float* pub0, const double* h, double* V_out, float* xT_out);
+/* the same frozen-step map with FP64 state (the reference's <double>
+   arithmetic): V_out[ntraj] = V(T) from pub0 (ntraj x (n+m) doubles); the
+   FP64 reference for finite-difference checks of the gradient */
+int asat_adj_replay_f64(asat_adj_t* a, int ntraj, const double* pub0, const double* h, double* V_out);
 /* device milliseconds of the last forward / backward */
 int asat_adj_timing(const asat_adj_t* a, double* forward_ms, double* backward_ms);
 

@@ -94,6 +94,24 @@ def test_gradient_vs_fd_of_fp64_frozen_map():
     assert max(errs) <= 1e-3 * scale
 
 
+def test_fp64_replay_matches_oracle_frozen_map():
+    """The FP64 frozen-step replay (the reference arithmetic in double)
+    equals the oracle's FP64 frozen map: V(T) to rounding of the final sum."""
+    say("FP64 frozen replay vs oracle")
+    n, m, T = 50, 212, 30
+    p, pubs = setup(n, m, 3, seed=13, ntraj=3)
+    adj = A.HipAdjoint(A.SatProblem(p.literals, p.lengths), eps=1e-6, steps=T, checkpoint_every=8)
+    adj.forward(pubs)
+    h = adj.steps()
+    x64 = pubs.astype(np.float64)
+    x64[1, 3] += 1e-6
+    V = adj.replay_f64(x64, h)
+    for t in range(3):
+        rk = O.RK(p, eps=1e-6, mode="f64")
+        want = rk.frozen_energy(x64[t], h[t])
+        assert abs(V[t] - want) <= 1e-12 * abs(want), (t, V[t], want)
+
+
 def test_torch_autograd_function():
     torch = pytest.importorskip("torch")
     say("torch.autograd.Function")
