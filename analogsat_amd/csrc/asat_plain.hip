@@ -333,8 +333,9 @@ __device__ __forceinline__ int var_block_range(int vg, int vg_blocks, int ngroup
     return (int)i0;
 }
 
-// streaming loads/stores bypass the caches' normal retention so that the
-// gathered arrays (spin stage input, contribution regions) keep the L2
+// streaming loads/stores: arrays read or written once per pass bypass the
+// caches' normal retention, so that the lines read again soon (slot-spin and
+// contribution runs, gather targets) keep the L2
 template <typename V>
 __device__ __forceinline__ V ld_nt(const V* p)
 {
