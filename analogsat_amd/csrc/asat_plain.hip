@@ -1315,6 +1315,7 @@ template <typename T>
 static StageArgs<T> stage_args(Sys* s, T* xa, T* xb, T* const* kk, T* tmp, double* partial, const Ctrl* c)
 {
     StageArgs<T> a;
+    memset(&a, 0, sizeof(a));   // padding too: the arguments are compared bytewise (captured attempts)
     a.lit = s->d_lit; a.pos = s->d_pos; a.spos = s->d_spos; a.srank = s->d_srank;
     a.csr_ptr = s->d_csr_ptr; a.csr_pos = s->d_csr_pos; a.csr_loc = s->d_csr_loc; a.gstart = s->d_gstart;
     a.contrib = (T*)s->d_contrib;
@@ -1355,15 +1356,28 @@ static void launch_clause(const StageArgs<T>& a, const Sys* s, hipStream_t st)
     else hipLaunchKernelGGL((k_clause_gen<T, J>), g, b, 0, st, a);
 }
 
+// allow the LDS var pass dynamic LDS beyond the default 64 KB, once per
+// instantiation (before the first launch, and outside any stream capture)
+template <typename T, int J>
+static void var_lds_attr()
+{
+    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_var_region<T, J>),
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, VAR_LDS_BYTES);
+    HIP_CHECK(attr);
+}
+
+template <typename T>
+static void var_lds_attrs()
+{
+    var_lds_attr<T, 1>(); var_lds_attr<T, 2>(); var_lds_attr<T, 3>();
+    var_lds_attr<T, 4>(); var_lds_attr<T, 5>(); var_lds_attr<T, 6>();
+}
+
 template <typename T, int J>
 static void launch_var(const StageArgs<T>& a, const Sys* s, hipStream_t st)
 {
     if (s->lds_var) {
-        // allow dynamic LDS beyond the default 64 KB, once per instantiation
-        static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_var_region<T, J>),
-                                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                           VAR_LDS_BYTES);
-        HIP_CHECK(attr);
+        var_lds_attr<T, J>();
         hipLaunchKernelGGL((k_var_region<T, J>), dim3(var_grid(s)), dim3(VBLK), (size_t)s->var_lds, st, a);
     } else {
         hipLaunchKernelGGL((k_var<T, J>), dim3(var_grid(s)), dim3(BLK), 0, st, a);
@@ -1452,9 +1466,20 @@ struct RK {
     std::vector<hipEvent_t> tev;
     double timer_ms = 0.0;
     long long timer_launches = 0;
+    // small problems: GRAPH_ATTEMPTS attempts captured once and replayed; the
+    // key is the exact launch arguments the graph was captured with
+    hipGraphExec_t gexec = nullptr;
+    std::vector<unsigned char> gkey;
 
+    void drop_graph()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+        gkey.clear();
+    }
     void free_all()
     {
+        drop_graph();
         for (auto& p : k) dfree(p);
         dfree(tmp); dfree(partial); dfree(d_ctrl);
         if (h_ctrl) { (void)hipHostFree(h_ctrl); h_ctrl = nullptr; }
@@ -1534,6 +1559,41 @@ static StageArgs<T> rk_stage_args(RK* r, int parity)
     return a;
 }
 
+// Launch-bound problems (few literal slots) replay a captured graph of
+// GRAPH_ATTEMPTS attempts instead of launching 13 kernels per attempt, as
+// long as a round still needs that many; the rest are launched one by one.
+constexpr long long GRAPH_MAX_SLOTS = 1LL << 20;
+constexpr int GRAPH_ATTEMPTS = 8;
+
+template <typename T>
+static void ensure_graph(RK* r, const StageArgs<T>& a)
+{
+    Sys* s = r->sys;
+    hipStream_t st = s->stream;
+    // the key: the arguments and every launch size the attempts depend on
+    std::vector<unsigned char> key(sizeof(a) + 4 * sizeof(int));
+    memcpy(key.data(), &a, sizeof(a));
+    const int dims[4] = {clause_grid(s), var_grid(s), s->var_lds, s->tiled ? 1 : 0};
+    memcpy(key.data() + sizeof(a), dims, sizeof(dims));
+    if (r->gexec && r->gkey == key) return;
+    r->drop_graph();
+    var_lds_attrs<T>();
+    hipGraph_t g = nullptr;
+    HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        for (int i = 0; i < GRAPH_ATTEMPTS; i++) launch_attempt<T>(a, s, r->d_ctrl, r->partial, r->npartial, st);
+    } catch (...) {
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    HIP_CHECK(hipStreamEndCapture(st, &g));
+    const hipError_t e = hipGraphInstantiate(&r->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIP_CHECK(e);
+    r->gkey = std::move(key);
+}
+
 template <typename T>
 static int rk_step_impl(RK* r, State* olds, State* news, int count)
 {
@@ -1551,22 +1611,33 @@ static int rk_step_impl(RK* r, State* olds, State* news, int count)
     long long remaining = count;
     const long long target = r->h_ctrl->steps + count;
     long long last_attempts = r->h_ctrl->attempts;
+    const bool graphed = r->timer_kind < 0 && s->nslots <= GRAPH_MAX_SLOTS && count >= GRAPH_ATTEMPTS;
+    if (graphed) ensure_graph<T>(r, a);
     while (remaining > 0) {
         const int tk = r->timer_kind;
-        if (tk >= 0)
-            while ((long long)r->tev.size() < 2 * remaining) {
-                // timing-only events, read after the stream synchronises:
-                // no system-scope fence, so recording them does not write
-                // back and invalidate the caches between the kernels
-                hipEvent_t e;
-                HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
-                r->tev.push_back(e);
-            }
-        for (long long i = 0; i < remaining; i++) {
-            if (tk >= 0)
-                launch_attempt<T>(a, s, r->d_ctrl, r->partial, r->npartial, st, tk, r->tev[2 * i], r->tev[2 * i + 1]);
-            else
+        if (graphed) {
+            // whole graphs while they fit, the rest one attempt at a time
+            const long long full = remaining / GRAPH_ATTEMPTS;
+            for (long long i = 0; i < full; i++) HIP_CHECK(hipGraphLaunch(r->gexec, st));
+            for (long long i = full * GRAPH_ATTEMPTS; i < remaining; i++)
                 launch_attempt<T>(a, s, r->d_ctrl, r->partial, r->npartial, st);
+        } else {
+            if (tk >= 0)
+                while ((long long)r->tev.size() < 2 * remaining) {
+                    // timing-only events, read after the stream synchronises:
+                    // no system-scope fence, so recording them does not write
+                    // back and invalidate the caches between the kernels
+                    hipEvent_t e;
+                    HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+                    r->tev.push_back(e);
+                }
+            for (long long i = 0; i < remaining; i++) {
+                if (tk >= 0)
+                    launch_attempt<T>(a, s, r->d_ctrl, r->partial, r->npartial, st, tk, r->tev[2 * i],
+                                      r->tev[2 * i + 1]);
+                else
+                    launch_attempt<T>(a, s, r->d_ctrl, r->partial, r->npartial, st);
+            }
         }
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipMemcpyAsync(r->h_ctrl, r->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
