@@ -18,7 +18,11 @@
 // s_q, K = 2^-k prod f, K_q / K_qp the products without q / without q and p):
 //   u_a    = w_a K^2 + sum_q w_q 2 c_q K K_q
 //   u_s[p] = w_a (-2 a c_p K K_p) + w_p (-2 a K_p^2) + sum_{q!=p} w_q (-4 a c_q c_p K K_qp)
-// Per-variable sums use the var-major slot list in slot order (deterministic).
+// Per-variable sums run over each variable's slots in slot order
+// (deterministic).  FP32 terms are written at their var-major positions (a
+// variable's terms are one contiguous run; 510 KB per C5 trajectory, L2-
+// resident); FP64 terms stay in clause order and are gathered through the
+// var-major slot list (scattered FP64 writes cost more than the gathers).
 // Internal layout of every vector: [0 | s_1..s_N | a_1..a_M], stride D.
 #include "asat_common.h"
 
@@ -39,6 +43,7 @@ struct Dev {
     const int* lit;       // m*k (zero padded)
     const int* csr_ptr;   // n+2, var-major over slots
     const int* csr_slot;  // slot indices, ascending per variable
+    const int* slot_pos;  // each slot's position in that var-major order, -1: padding
     int n, m, k;
     long long D;          // vector stride (>= n+m+1)
     long long L;          // m*k
@@ -91,10 +96,17 @@ __global__ __launch_bounds__(BLK) void k_rhs_clause(Dev d, const T* Y, T* out, T
 #pragma unroll
     for (int q = 0; q < K; q++) o[q] = (T)0;
     write_contribs<T, K, 0>(l, f, r, o);
-    T* c = contrib + tr * d.L + (long long)m * K;
+    if constexpr (sizeof(T) == 4) {
+        T* c = contrib + tr * d.L;
 #pragma unroll
-    for (int q = 0; q < K; q++)
-        if (l[q] != 0) c[q] = o[q];
+        for (int q = 0; q < K; q++)
+            if (l[q] != 0) c[d.slot_pos[(size_t)m * K + q]] = o[q];
+    } else {
+        T* c = contrib + tr * d.L + (long long)m * K;
+#pragma unroll
+        for (int q = 0; q < K; q++)
+            if (l[q] != 0) c[q] = o[q];
+    }
 }
 
 template <typename T>
@@ -107,7 +119,10 @@ __global__ __launch_bounds__(BLK) void k_rhs_var(Dev d, const T* contrib, T* out
     if (i > d.n) return;
     const T* c = contrib + tr * d.L;
     T sum = (T)0;
-    for (int e = d.csr_ptr[i]; e < d.csr_ptr[i + 1]; e++) sum += c[d.csr_slot[e]];
+    if constexpr (sizeof(T) == 4)
+        for (int e = d.csr_ptr[i]; e < d.csr_ptr[i + 1]; e++) sum += c[e];
+    else
+        for (int e = d.csr_ptr[i]; e < d.csr_ptr[i + 1]; e++) sum += c[d.csr_slot[e]];
     out[tr * d.D + i] = sum;
 }
 
@@ -426,7 +441,7 @@ struct Adj {
     bool have_problem = false, have_forward = false;
     std::vector<int> lits_in, lens_in;
     hipStream_t stream = nullptr;
-    int *d_lit = nullptr, *d_csr_ptr = nullptr, *d_csr_slot = nullptr;
+    int *d_lit = nullptr, *d_csr_ptr = nullptr, *d_csr_slot = nullptr, *d_slot_pos = nullptr;
     // per batch
     float* ckpt = nullptr;       // [ncheck][ntraj][D]
     float* seg = nullptr;        // [C+1][ntraj][D]
@@ -463,10 +478,10 @@ struct Adj {
     ~Adj()
     {
         free_batch();
-        (void)hipFree(d_lit); (void)hipFree(d_csr_ptr); (void)hipFree(d_csr_slot);
+        (void)hipFree(d_lit); (void)hipFree(d_csr_ptr); (void)hipFree(d_csr_slot); (void)hipFree(d_slot_pos);
         if (stream) (void)hipStreamDestroy(stream);
     }
-    Dev dev() const { return Dev{d_lit, d_csr_ptr, d_csr_slot, n, m, k, D, L, ntraj}; }
+    Dev dev() const { return Dev{d_lit, d_csr_ptr, d_csr_slot, d_slot_pos, n, m, k, D, L, ntraj}; }
 };
 
 static void set_problem(Adj& A, const int* lits, const int* lens, int nclauses)
@@ -497,20 +512,28 @@ static void set_problem(Adj& A, const int* lits, const int* lens, int nclauses)
     for (int l : C)
         if (l) ptr[(size_t)std::abs(l) + 1]++;
     for (int i = 0; i <= n; i++) ptr[i + 1] += ptr[i];
+    // var-major position of every slot: variables in order, each variable's
+    // slots ascending (the reference's accumulation order)
+    std::vector<int> spos(std::max<size_t>(C.size(), 1), -1);
     std::vector<int> slot(std::max(ptr[n + 1], 1));
     std::vector<int> fill(ptr.begin(), ptr.end() - 1);
     for (size_t s = 0; s < C.size(); s++)
-        if (C[s]) slot[(size_t)fill[std::abs(C[s])]++] = (int)s;
+        if (C[s]) {
+            spos[s] = fill[std::abs(C[s])]++;
+            slot[(size_t)spos[s]] = (int)s;
+        }
     if (!A.stream) HIP_CHECK(hipStreamCreateWithFlags(&A.stream, hipStreamNonBlocking));
     HIP_CHECK(hipStreamSynchronize(A.stream));
     A.free_batch();
-    (void)hipFree(A.d_lit); (void)hipFree(A.d_csr_ptr); (void)hipFree(A.d_csr_slot);
+    (void)hipFree(A.d_lit); (void)hipFree(A.d_csr_ptr); (void)hipFree(A.d_csr_slot); (void)hipFree(A.d_slot_pos);
     HIP_CHECK(hipMalloc((void**)&A.d_lit, sizeof(int) * C.size()));
     HIP_CHECK(hipMalloc((void**)&A.d_csr_ptr, sizeof(int) * ptr.size()));
     HIP_CHECK(hipMalloc((void**)&A.d_csr_slot, sizeof(int) * slot.size()));
+    HIP_CHECK(hipMalloc((void**)&A.d_slot_pos, sizeof(int) * spos.size()));
     HIP_CHECK(hipMemcpy(A.d_lit, C.data(), sizeof(int) * C.size(), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(A.d_csr_ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(A.d_csr_slot, slot.data(), sizeof(int) * slot.size(), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(A.d_slot_pos, spos.data(), sizeof(int) * spos.size(), hipMemcpyHostToDevice));
     A.n = n; A.m = m; A.k = k;
     A.D = (long long)((n + m + 1 + 63) / 64) * 64;
     A.L = (long long)m * k;
