@@ -159,7 +159,8 @@ int asat_rk_get_eps(const asat_rk_t* rk, double* eps);
 int asat_rk_set_xscal_floor(asat_rk_t* rk, double floor);
 /* Record accepted step sizes on the device: after the k-th accepted step
    since Reset, dev_hlog[k-1] = hdid (for k <= len).  len = 0 switches the
-   log off.  No reference counterpart (used by the adjoint's forward run). */
+   log off.  No reference counterpart (for callers that replay a run with
+   its step sizes frozen, as asat_adj_replay does). */
 int asat_rk_set_step_log(asat_rk_t* rk, double* dev_hlog, long long len);
 /* IODEInt::Configure(ode): allocates scratch, resets h = 1e-9 */
 int asat_rk_configure(asat_rk_t* rk, asat_sys_t* sys);

@@ -234,6 +234,33 @@ def test_kernel_timer_and_stream_probe():
     assert bw["copy_GBps"] > 0 and bw["read_GBps"] > 0
 
 
+def test_step_log_records_accepted_steps():
+    """asat_rk_set_step_log: after the k-th accepted step since Reset the
+    device log holds its step size, for Step calls of any size (single
+    attempts and captured groups of attempts alike)."""
+    torch = pytest.importorskip("torch")
+    import ctypes
+    say("step log")
+    p = O.fastsat(200, 852, 3, 9)
+    sat = _sys_with(p.literals, p.lengths, "f64")
+    rk = A.HipRungeKutta()
+    rk.SetEpsilon(1e-6)
+    rk.Configure(sat)
+    old, new = sat.MakeState(), sat.MakeState()
+    old.CopyFrom(O.random_state_f64(8, p.n, p.m))
+    log = torch.full((60,), -1.0, dtype=torch.float64, device="cuda")
+    assert A.lib().asat_rk_set_step_log(rk._h, ctypes.c_void_p(log.data_ptr()), 50) == 0
+    rk.Reset()
+    seen = []
+    for c in (1, 1, 3, 20, 30):
+        assert rk.Step(old, new, c)
+        seen.append(rk.stats().hdid)
+    h = log.cpu().numpy()
+    assert h[0] == seen[0] and h[1] == seen[1] and h[4] == seen[2] and h[24] == seen[3]
+    assert np.all(h[:50] > 0) and np.all(h[50:] == -1.0)   # 55 accepted steps, 50 logged
+    assert A.lib().asat_rk_set_step_log(rk._h, None, 0) == 0
+
+
 def test_large_instance_rhs_property():
     """N = 1e6-scale RHS: GPU FP32 equals the oracle's float mode bit for bit."""
     say("rhs f32 N=200000 M=852000")
