@@ -894,7 +894,7 @@ static int chunk_for(const Batch& B, const Bucket& b, int wgs);
 struct RegVariant {
     int bs, cpt, vpt;
 };
-static const RegVariant REG_VARIANTS[] = {{256, 1, 1}, {256, 4, 2}, {512, 8, 4}, {1024, 9, 2}, {512, 17, 4}};
+static const RegVariant REG_VARIANTS[] = {{256, 1, 1}, {256, 4, 2}, {512, 8, 4}, {768, 12, 3}, {1024, 9, 2}, {512, 17, 4}};
 
 // first variant the bucket's largest instance fits (of workgroup size bs, or any)
 static const RegVariant* reg_variant(const Bucket& b, int bs = 0)
@@ -904,7 +904,7 @@ static const RegVariant* reg_variant(const Bucket& b, int bs = 0)
     return nullptr;
 }
 
-#define ASAT_REG_VARIANTS(X) X(256, 1, 1) X(256, 4, 2) X(512, 8, 4) X(1024, 9, 2) X(512, 17, 4)
+#define ASAT_REG_VARIANTS(X) X(256, 1, 1) X(256, 4, 2) X(512, 8, 4) X(768, 12, 3) X(1024, 9, 2) X(512, 17, 4)
 
 static const void* reg_kernel(int bs, int cpt, int vpt)
 {
@@ -1055,7 +1055,7 @@ static void autotune(Batch& B)
     for (auto& b : B.buckets) {
         struct Cand { int bs, mode, cap, chunk; };
         std::vector<Cand> cand;
-        for (int bs : {256, 512, 1024}) {
+        for (int bs : {256, 512, 768, 1024}) {
             const RegVariant* rv = reg_variant(b, bs);
             if (rv && b.lds_fits && (B.force_lds == -1 || B.force_lds == 2) && (!B.force_block || B.force_block == bs))
                 cand.push_back({bs, 2, 0, 0});
