@@ -6,8 +6,9 @@
 //   lit[m*K + q]      int32 signed literal (+v / -v, 0 = padding), clause-major,
 //                     padded to whole tiles of TC clauses
 //   contrib[p]        per-slot contribution 2 c a K K_mi of the current stage.
-//                     Variables are split into groups (about 2048 float / 1024
-//                     double variables); a group's contributions form one
+//                     Variables are split into groups (at most about 2048 float
+//                     / 1024 double variables, halved until the largest region
+//                     fits the LDS); a group's contributions form one
 //                     region (64-element aligned, at most 144 KB), in slot
 //                     (clause) order.  Between a var pass and the next clause
 //                     pass the same array holds the slot spins: each slot's
