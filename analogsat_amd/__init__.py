@@ -82,6 +82,47 @@ def build():
     return builder.build()
 
 
+def _mapped(stem):
+    """Real paths of the mapped shared objects whose file name starts with stem."""
+    try:
+        with open("/proc/self/maps") as f:
+            paths = {line.split()[-1] for line in f if "/" + stem in line}
+    except OSError:
+        return set()
+    return {os.path.realpath(p) for p in paths if os.path.basename(p).startswith(stem)}
+
+
+def _share_hip_runtime():
+    """Keep one HIP runtime in the process.
+
+    PyTorch-ROCm wheels carry their own libamdhip64 / libhsa-runtime64 and
+    their libraries ask for them as ``libamdhip64.so`` / ``libhsa-runtime64.so``
+    with an RPATH of their own directory.  The dynamic loader reuses an object
+    only under a name it has already registered, so without this a later
+    ``import torch`` maps a second HIP and HSA runtime next to the one this
+    library was linked against, and the second can then report no GPU.
+    Registering the unversioned names for the runtime already mapped (only
+    when the name resolves to that same file) makes torch reuse it.  If torch
+    was imported first, this library has already bound to torch's runtime and
+    nothing is done."""
+    for stem in ("libamdhip64", "libhsa-runtime64"):
+        have = _mapped(stem + ".so")
+        if len(have) != 1:
+            continue
+        target = next(iter(have))
+        name = stem + ".so"
+        if os.path.realpath(os.path.join(os.path.dirname(target), name)) != target:
+            continue
+        dirs = [d for d in os.environ.get("LD_LIBRARY_PATH", "").split(":") if d]
+        first = next((os.path.join(d, name) for d in dirs if os.path.exists(os.path.join(d, name))), None)
+        if first is not None and os.path.realpath(first) != target:
+            continue
+        try:
+            ctypes.CDLL(name, mode=ctypes.RTLD_GLOBAL)
+        except OSError:
+            continue
+
+
 def lib():
     """Load the HIP library (raises if it has not been built)."""
     global _lib
@@ -90,6 +131,7 @@ def lib():
             raise RuntimeError("libanalogsat_hip.so is not built: run python -m analogsat_amd.builder "
                                "(or __graft_entry__.build())")
         L = ctypes.CDLL(LIB_PATH)
+        _share_hip_runtime()
         vp, ip, i = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int
         pvp = ctypes.POINTER(ctypes.c_void_p)
         sig = {

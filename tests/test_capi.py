@@ -2,9 +2,11 @@
 headers in include/ declare, and its host-only logic behaves like the
 reference (FastSatMaker stream, argument validation, error codes)."""
 import ctypes
+import json
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -44,6 +46,22 @@ def test_library_is_gfx950_code_object():
     # the offload bundle of the library built here names its target triple
     blob = open(A.LIB_PATH, "rb").read()
     assert b"amdgcn-amd-amdhsa--gfx950" in blob
+
+
+def test_one_hip_runtime_with_torch():
+    """Loading the library and then importing torch maps one HIP runtime and
+    one HSA runtime (torch's wheel carries its own copies, which it would
+    otherwise load beside ours).  Loading and importing make no GPU call."""
+    pytest.importorskip("torch")
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "import analogsat_amd as A; A.lib()\n"
+            "import torch\n"
+            "import json; print(json.dumps([sorted(A._mapped('libamdhip64.so')), sorted(A._mapped('libhsa-runtime64.so'))]))\n"
+            % os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    hip, hsa = json.loads(out.stdout.strip().splitlines()[-1])
+    assert len(hip) == 1 and len(hsa) == 1, out.stdout
 
 
 def test_version_and_device_count(L):
