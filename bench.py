@@ -125,8 +125,10 @@ def make_c4(count, seed=0):
     return out
 
 
-def run_batch(insts, max_steps, eps, chunk_steps=64, graph_chunks=4, log=None):
-    """Batched solver over insts; returns (batch, wall_s, accepted steps, attempts)."""
+def run_batch(insts, max_steps, eps, chunk_steps=64, graph_chunks=4, log=None, slice_chunks=0):
+    """Batched solver over insts; returns (batch, wall_s, accepted steps, attempts).
+    slice_chunks > 0 runs the batch in slices of that many chunks and logs
+    progress after each (long budgets); wall_s sums the slices."""
     import analogsat_amd as A
     b = A.HipBatch()
     for p, pub in insts:
@@ -138,9 +140,23 @@ def run_batch(insts, max_steps, eps, chunk_steps=64, graph_chunks=4, log=None):
     if log:
         log("batch prepared+tuned in %.2f s: %s" % (time.perf_counter() - t0,
             [(x["trajectories"], x["max_slots"], x["block"], x["lds"]) for x in b.buckets()]))
-    t0 = time.perf_counter()
-    b.run(graph_chunks=graph_chunks)
-    wall = time.perf_counter() - t0
+    if slice_chunks > 0:
+        wall = 0.0
+        while True:
+            t0 = time.perf_counter()
+            b.run(max_chunks=slice_chunks, graph_chunks=graph_chunks)
+            wall += time.perf_counter() - t0
+            res = b.results()
+            running = sum(r["status"] == 0 for r in res)
+            if log:
+                log("batch: %.1f s, %d accepted steps, %d trajectories running"
+                    % (wall, sum(r["steps"] for r in res), running))
+            if running == 0:
+                break
+    else:
+        t0 = time.perf_counter()
+        b.run(graph_chunks=graph_chunks)
+        wall = time.perf_counter() - t0
     res = b.results()
     steps = sum(r["steps"] for r in res)
     attempts = sum(r["attempts"] for r in res)
@@ -315,7 +331,10 @@ def workload_c4(args, log):
     per instance; exact minimum violation count per instance and the
     autotuned tile per size bucket."""
     insts = make_c4(args.batch_count)
-    b, wall, steps, attempts, res = run_batch(insts, args.batch_steps, args.eps, log=log)
+    # budgets beyond a few thousand steps run for minutes: slices of 16 chunks
+    # (1024 steps) with a progress line each
+    b, wall, steps, attempts, res = run_batch(insts, args.batch_steps, args.eps, log=log,
+                                              slice_chunks=16 if args.batch_steps > 2048 else 0)
     slots = sum(int(np.count_nonzero(p.literals)) for p, _ in insts)
     return {"workload": "c4", "trajectories": len(res), "step_budget": args.batch_steps, "slots": slots,
             "value": round(steps / wall, 1), "unit": "trajectory-steps/s", "wall_s": round(wall, 3),
@@ -323,6 +342,8 @@ def workload_c4(args, log):
             "slot_attempts_per_s": round(sum(int(np.count_nonzero(p.literals)) * r["attempts"]
                                              for (p, _), r in zip(insts, res)) / wall, 1),
             "solved": sum(r["status"] == 1 for r in res),
+            "min_violations_mean": round(float(np.mean([r["min_violations"] for r in res])), 2),
+            "min_violations_max": int(max(r["min_violations"] for r in res)),
             "min_violations_first64": [r["min_violations"] for r in res[:64]],
             "buckets": b.buckets()}
 
