@@ -84,9 +84,13 @@ def build():
 
 def _mapped(stem):
     """Real paths of the mapped shared objects whose file name starts with stem."""
+    paths = set()
     try:
         with open("/proc/self/maps") as f:
-            paths = {line.split()[-1] for line in f if "/" + stem in line}
+            for line in f:
+                fields = line.split(None, 5)   # address perms offset dev inode [path]
+                if len(fields) == 6 and "/" + stem in fields[5]:
+                    paths.add(fields[5].strip().removesuffix(" (deleted)"))
     except OSError:
         return set()
     return {os.path.realpath(p) for p in paths if os.path.basename(p).startswith(stem)}
