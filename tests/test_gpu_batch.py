@@ -186,6 +186,35 @@ def test_batch_reset_reproduces():
         assert np.array_equal(first[i], b.state(i))
 
 
+def test_batch_runs_in_slices_like_one_run():
+    """asat_batch_run with a chunk limit stops early and the next call goes
+    on from there: one-chunk slices give the results and states of a single
+    run (long budgets are run this way to report progress)."""
+    say("batch run in slices")
+    insts = make_instances(10, 8)
+    b = A.HipBatch()
+    for p, pub in insts:
+        b.add(A.SatProblem(p.literals, p.lengths), pub)
+    b.set_params(eps=1e-6, max_steps=70, chunk_steps=8)
+    b.set_tiling(block=256, lds_mode=-1, nbuckets=2)
+    b.prepare(autotune=False)
+    b.run(graph_chunks=2)
+    keys = ("status", "steps", "attempts", "rhscount", "time", "hdid", "min_violations", "sat_step")
+    whole = [{k: b.result(i)[k] for k in keys} for i in range(len(insts))]
+    states = [b.state(i).copy() for i in range(len(insts))]
+    assert any(r["steps"] > 8 for r in whole)
+    b.reset()
+    slices = 0
+    while any(r["status"] == 0 for r in b.results()):
+        b.run(max_chunks=1, graph_chunks=1)
+        slices += 1
+        assert slices <= 20
+    assert slices > 1
+    for i in range(len(insts)):
+        assert {k: b.result(i)[k] for k in keys} == whole[i], i
+        assert np.array_equal(b.state(i), states[i]), i
+
+
 def test_register_kernel_matches_oracle():
     """The register-resident kernel (lds_mode 2: state and stage derivatives
     in registers) on instance sizes that select its different variants,
