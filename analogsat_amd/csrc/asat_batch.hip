@@ -1234,13 +1234,32 @@ static void run(Batch& B, int max_chunks, int graph_chunks)
     require(B.prepared, ASAT_ERR_NOT_CONFIGURED, "batch has not been prepared");
     require(max_chunks > 0 && graph_chunks > 0, ASAT_ERR_INVALID_ARGUMENT, "chunk counts must be positive");
     const int T = (int)B.inst.size();
-    hipEvent_t e0, e1;
-    HIP_CHECK(hipEventCreate(&e0));
-    HIP_CHECK(hipEventCreate(&e1));
+    // events, graph and executable are released on every exit, and a capture
+    // left open by an error is ended, so the stream stays usable
+    struct Owned {
+        hipStream_t st;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        bool capturing = false;
+        ~Owned()
+        {
+            if (capturing) {
+                hipGraph_t g = nullptr;
+                if (hipStreamEndCapture(st, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+            }
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        }
+    } o{B.stream};
+    HIP_CHECK(hipEventCreate(&o.e0));
+    HIP_CHECK(hipEventCreate(&o.e1));
+    hipEvent_t e0 = o.e0, e1 = o.e1;
     // one graph = graph_chunks x (per bucket: reset its queue, persistent launch)
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
     HIP_CHECK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+    o.capturing = true;
     for (int c = 0; c < graph_chunks; c++) {
         for (auto& b : B.buckets) {
             Params P = params_of(B, b);
@@ -1248,8 +1267,10 @@ static void run(Batch& B, int max_chunks, int graph_chunks)
             dispatch(B.stream, P, b);
         }
     }
-    HIP_CHECK(hipStreamEndCapture(B.stream, &graph));
-    HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    o.capturing = false;
+    HIP_CHECK(hipStreamEndCapture(B.stream, &o.graph));
+    HIP_CHECK(hipGraphInstantiate(&o.exec, o.graph, nullptr, nullptr, 0));
+    hipGraphExec_t exec = o.exec;
     auto t0 = std::chrono::steady_clock::now();
     HIP_CHECK(hipEventRecord(e0, B.stream));
     int launched = 0;
@@ -1273,10 +1294,6 @@ static void run(Batch& B, int max_chunks, int graph_chunks)
     HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
     B.run_ms += ms;
     B.chunks_run += launched;
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
 }
 
 }  // namespace batch
