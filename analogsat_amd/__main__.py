@@ -24,9 +24,17 @@ solver, result, wall s, cpu s, RHS count); ``bench`` skips samples already
 listed there unless -rerun; ``speedtest`` cycles through the existing
 samples with the -timeout wall limit until 5 s of total solve time (at most
 200 runs) per N.  -config FILE reads further options from a file, one
-"-option value" per line, recursively.  Options of the reference this path
-does not cover (-nogpu, -minisat, -ramsey, -ramseycircular, make_ramsey,
-run_ramsey) are rejected.
+"-option value" per line, recursively.
+
+``make_ramsey`` and ``run_ramsey`` follow Frontend/src/make_ramsey_problems.cpp
+and Frontend/src/ramsey_runner.cpp (``analogsat_amd.ramsey``): for node counts
+N = nstart .. nend, -ramsey 3,4,5 (the digits of R(3,4,5); the reference's
+help text calls it -R, its parser -ramsey) and -ramseycircular (block-circulant
+adjacency), write <problemfolder>/ramsey_<regular|circular>_<digits>_N<N>.cnf,
+or solve those files samplestart .. sampleend - 1 times each, appending result
+lines to <resultfolder>/perf_<digits>_N<N>.dat and writing a solved coloring to
+<resultfolder>/coloring_<digits>_N<N>.dat.  Options of the reference this
+path does not cover (-nogpu, -minisat) are rejected.
 """
 import math
 import os
@@ -40,16 +48,16 @@ DEFAULTS = {"eps": 1e-6, "tmax": 1e8, "bias": 0.0, "batch": 50, "stepmax": 50000
             "problem": None, "trajectory": False, "float": False, "tanh": False, "seed": None,
             "k": 3, "alpha": 4.25, "samplestart": 0, "sampleend": 100, "nstart": 1.0, "nend": 5.0, "nstep": 0.5,
             "problemfolder": ".", "timeout": 3600.0, "rerun": False, "force_save_cnf": False, "version": 1,
-            "usegpu": 0}
+            "usegpu": 0, "ramsey": "", "ramseycircular": False}
 RANGES = {"eps": (1e-8, 1e-1), "tmax": (0, 1e80), "bias": (0.0, 1e6), "batch": (1, 2**31 - 1),
           "stepmax": (0, 2**31 - 1), "seed": (0, 2**63), "k": (2, 256), "alpha": (0.0, 1e10),
           "samplestart": (0, 9999), "sampleend": (0, 10000), "nstart": (1.0, 100.0), "nend": (1.0, 100.0),
           "nstep": (0.0, 100.0), "timeout": (0, 1e80), "version": (1, 3), "usegpu": (0, 99)}
 INTS = ("batch", "stepmax", "seed", "k", "samplestart", "sampleend", "version", "usegpu")
-FLAGS = ("trajectory", "float", "tanh", "rerun", "force_save_cnf")
-STRINGS = ("problem", "resultfolder", "problemfolder")
-COMMANDS = ("run", "bench", "speedtest")
-UNSUPPORTED = ("-nogpu", "-minisat", "-ramsey", "-ramseycircular")
+FLAGS = ("trajectory", "float", "tanh", "rerun", "force_save_cnf", "ramseycircular")
+STRINGS = ("problem", "resultfolder", "problemfolder", "ramsey")
+COMMANDS = ("run", "bench", "speedtest", "make_ramsey", "run_ramsey")
+UNSUPPORTED = ("-nogpu", "-minisat")
 
 
 def parse_options(args, conf, depth=0):
@@ -95,7 +103,8 @@ def parse_options(args, conf, depth=0):
 
 def parse(argv):
     if not argv or argv[0] not in COMMANDS:
-        raise SystemExit("usage: python -m analogsat_amd run|bench|speedtest [-config FILE] [-problem FILE.cnf] "
+        raise SystemExit("usage: python -m analogsat_amd run|bench|speedtest|make_ramsey|run_ramsey "
+                         "[-config FILE] [-problem FILE.cnf] [-ramsey 3,4,5] [-ramseycircular] "
                          "[-eps E] [-batch B] [-stepmax S] [-tmax T] [-timeout S] [-resultfolder DIR] "
                          "[-problemfolder DIR] [-k K] [-alpha A] [-nstart P] [-nend P] [-nstep P] "
                          "[-samplestart I] [-sampleend I] [-bias B] [-tanh] [-trajectory] [-float] [-seed S] "
@@ -105,6 +114,16 @@ def parse(argv):
     parse_options(argv[1:], conf)
     if conf["command"] == "run" and not conf["problem"]:
         raise SystemExit("-problem is required")
+    if conf["command"] in ("make_ramsey", "run_ramsey"):
+        from .ramsey import parse_ramsey
+        try:
+            R = parse_ramsey(conf["ramsey"])
+        except ValueError as e:
+            raise SystemExit(str(e))
+        if not 2 <= len(R) <= 8 or min(R) < 2:
+            raise SystemExit("-ramsey needs 2 to 8 comma-separated digits, each at least 2 (e.g. -ramsey 3,4,5)")
+        if int(conf["nstart"]) < 2:
+            raise SystemExit("-nstart: a Ramsey graph needs at least 2 nodes")
     if conf["usegpu"] != 0:
         raise SystemExit("-usegpu: this build runs on device 0 only")
     return conf
@@ -241,43 +260,52 @@ class BenchRunner:
         return solved
 
     def _run_gpu(self):
-        A, conf = self.A, self.conf
-        prec = "f32" if conf["float"] else "f64"
-        sat = A.HipSatTanh(prec) if conf["tanh"] else A.HipSat(prec)
-        rk = A.HipRungeKutta()
-        rk.SetEpsilon(conf["eps"])
-        solver = A.SatSolver(sat, rk)
-        sat.Set_B(conf["bias"])
-        solver.SetProblem(self.problem)
-        solver.SetMaxTime(conf["tmax"])
-        solver.SetMaxSteps(max(1, conf["stepmax"]))
-        solver.SetBatchSize(conf["batch"])
         if self.rng is None:
-            self.rng = A.HipRandom(conf["seed"])
-        solver.SetRandomInitialState(self.rng)
-        min_v = [self.problem.Get_M()]
-
-        def callback(s):
-            v = s.GetLastClauseViolationCount()
-            min_v[0] = min(min_v[0], v)
-            if s.GetStepCount() % 100 == 0:
-                self.out.write("  %d\t%5.4f\t%e\t%d\t%d\n" % (s.GetStepCount(), s.GetElapsedTime(),
-                                                               s.GetLastStepSize(), v, min_v[0]))
-            if s.GetElapsedWallTime() > conf["timeout"]:
-                self.out.write("  walltime limit reached\n")
-                return False
-            return True
-
-        solver.SetCallback(callback)
-        res = solver.Solve()
-        msg = A.GetSatResultMessage(res)
-        self.out.write(msg + "\n")
-        with open(self.outfile, "a") as f:
-            f.write("%s\t%s\tanalogsat%d%s\t%s\t%f\t%f\t%d\n" % (
-                self.problem_name, A.GetGpuName(), conf["version"], "tanh" if conf["tanh"] else "orig", msg,
-                solver.GetElapsedWallTime(), solver.GetElapsedCpuTime(), solver.GetRhsCount()))
+            self.rng = self.A.HipRandom(self.conf["seed"])
+        solved, solver = solve_on_gpu(self.conf, self.problem, self.rng, self.out, self.problem_name, self.outfile)
         self.last_solve_time = solver.GetElapsedWallTime()
-        return res == A.SatResult.SAT_SOLUTION_FOUND
+        return solved
+
+
+def solve_on_gpu(conf, problem, rng, out, problem_name, outfile):
+    """One solve from a fresh random start, as the reference's runners do it
+    (Frontend/src/bench_runner.cpp:425-499, Frontend/src/ramsey_runner.cpp:
+    183-257): progress every 100 steps, the -timeout wall limit, then one
+    result line appended to outfile.  Returns (solved, solver)."""
+    import analogsat_amd as A
+    prec = "f32" if conf["float"] else "f64"
+    sat = A.HipSatTanh(prec) if conf["tanh"] else A.HipSat(prec)
+    rk = A.HipRungeKutta()
+    rk.SetEpsilon(conf["eps"])
+    solver = A.SatSolver(sat, rk)
+    sat.Set_B(conf["bias"])
+    solver.SetProblem(problem)
+    solver.SetMaxTime(conf["tmax"])
+    solver.SetMaxSteps(max(1, conf["stepmax"]))
+    solver.SetBatchSize(conf["batch"])
+    solver.SetRandomInitialState(rng)
+    min_v = [problem.Get_M()]
+
+    def callback(s):
+        v = s.GetLastClauseViolationCount()
+        min_v[0] = min(min_v[0], v)
+        if s.GetStepCount() % 100 == 0:
+            out.write("  %d\t%5.4f\t%e\t%d\t%d\n" % (s.GetStepCount(), s.GetElapsedTime(), s.GetLastStepSize(), v,
+                                                    min_v[0]))
+        if s.GetElapsedWallTime() > conf["timeout"]:
+            out.write("  walltime limit reached\n")
+            return False
+        return True
+
+    solver.SetCallback(callback)
+    res = solver.Solve()
+    msg = A.GetSatResultMessage(res)
+    out.write(msg + "\n")
+    with open(outfile, "a") as f:
+        f.write("%s\t%s\tanalogsat%d%s\t%s\t%f\t%f\t%d\n" % (
+            problem_name, A.GetGpuName(), conf["version"], "tanh" if conf["tanh"] else "orig", msg,
+            solver.GetElapsedWallTime(), solver.GetElapsedCpuTime(), solver.GetRhsCount()))
+    return res == A.SatResult.SAT_SOLUTION_FOUND, solver
 
 
 def run_bench_series(conf, out=sys.stdout):
@@ -316,6 +344,12 @@ def main(argv=None):
         run(conf)
     elif conf["command"] == "bench":
         run_bench_series(conf)
+    elif conf["command"] == "make_ramsey":
+        from .ramsey import make_ramsey_problems
+        make_ramsey_problems(conf)
+    elif conf["command"] == "run_ramsey":
+        from .ramsey import run_ramsey_series
+        run_ramsey_series(conf)
     else:
         run_speedtest(conf)
     print("All done.")

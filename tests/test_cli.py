@@ -55,23 +55,27 @@ REF_CONFIGS = "/root/reference/Frontend/Config"
 
 @pytest.mark.skipif(not os.path.isdir(REF_CONFIGS), reason="reference config files not present")
 def test_reference_gpu_configs_parse():
-    """Every GPU config file the reference ships parses with the same values;
-    the CPU-solver, MiniSat and Ramsey ones are refused."""
-    n_gpu = 0
+    """Every GPU config file the reference ships (Ramsey ones included)
+    parses with the same values; the CPU-solver and MiniSat ones are refused."""
+    n_gpu = n_ramsey = 0
     for path in sorted(glob.glob(os.path.join(REF_CONFIGS, "*.cfg"))):
         text = open(path).read()
-        if any(o in text for o in ("-nogpu", "-minisat", "-ramsey")):
+        if any(o in text for o in ("-nogpu", "-minisat")):
             with pytest.raises(SystemExit):
                 cli.parse(["bench", "-config", path])
             continue
-        c = cli.parse(["speedtest", "-config", path])
+        ramsey = "-ramsey" in text
+        c = cli.parse(["run_ramsey" if ramsey else "speedtest", "-config", path])
         for line in text.splitlines():
             opt, _, val = line.partition(" ")
             if val:
-                want = val if opt in ("-problemfolder", "-resultfolder", "-problem") else float(val)
+                want = val if opt in ("-problemfolder", "-resultfolder", "-problem", "-ramsey") else float(val)
                 assert c[opt[1:]] == want, (path, opt)
+            elif opt:
+                assert c[opt[1:]] is True, (path, opt)
         n_gpu += 1
-    assert n_gpu >= 10
+        n_ramsey += ramsey
+    assert n_gpu >= 10 and n_ramsey >= 6
 
 
 def test_series_sizes():
