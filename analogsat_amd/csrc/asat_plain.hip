@@ -896,6 +896,60 @@ __global__ __launch_bounds__(BLK) void k_var(StageArgs<T> a)
     }
 }
 
+// Wide var pass, for the gather layout when variables have long rows (at
+// least WIDE_MIN_ROW contributions on average): one workgroup per variable.
+// The workgroup stages the variable's contributions in LDS, WIDE_STAGE at a
+// time with one load per thread and round, and one thread adds them in slot
+// order, so the sum is the gather kernel's (and the reference's) exactly.
+constexpr int WIDE_BLK = 256;
+constexpr int WIDE_STAGE = 4096;
+constexpr long long WIDE_MIN_ROW = 64;
+
+template <typename T, int J>
+__global__ __launch_bounds__(WIDE_BLK) void k_var_wide(StageArgs<T> a)
+{
+    __shared__ double sh[WIDE_BLK / 64];
+    __shared__ T vals[WIDE_STAGE];
+    const Ctrl* c = a.ctrl;
+    int parity = 0;
+    double h = 0.0, hnext = 0.0, floor = TabD::TINY;
+    bool fresh = true;
+    if (c) {
+        if (!c->active) return;
+        parity = c->parity; h = c->h; hnext = c->hnext; floor = c->floor;
+        fresh = (J != 1) || c->step_start;
+    }
+    const T hs = (T)h;
+    const T* x = parity ? a.xb : a.xa;
+    T* xnew = parity ? a.xa : a.xb;
+    const int i = (int)blockIdx.x + 1;          // grid: one workgroup per variable 1..n
+    double err = 0.0;
+    const T pref = (a.bias && fresh) ? bias_prefactor<T, WIDE_BLK>(a, sh) : (T)0;
+    T sum = (T)0;
+    if (fresh) {
+        const int e0 = a.csr_ptr[i], e1 = a.csr_ptr[i + 1];
+        for (int r0 = e0; r0 < e1; r0 += WIDE_STAGE) {
+            const int r1 = std::min(e1, r0 + WIDE_STAGE);
+            for (int e = r0 + (int)threadIdx.x; e < r1; e += WIDE_BLK) vals[e - r0] = a.contrib[a.csr_pos[e]];
+            __syncthreads();
+            if (threadIdx.x == 0) {
+#pragma unroll 8
+                for (int e = 0; e < r1 - r0; e++) sum += vals[e];
+            }
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        T kj = sum;
+        if (a.bias && fresh) kj += pref * bias_sine<T>(J == 1 ? x[i] : a.tmp[i]);
+        var_stage<T, J>(a, i, fresh, kj, var_load<T, J>(a, i, x, fresh), xnew, hs, h, hnext, floor, err);
+    }
+    if constexpr (J == 6) {
+        double bm = block_max<WIDE_BLK>(err, sh);
+        if (threadIdx.x == 0) a.partial[a.nblk_clause + blockIdx.x] = bm;
+    }
+}
+
 // ---- control: max-norm + Cash-Karp step control (CpuRungeKutta.cpp:217-267)
 __global__ __launch_bounds__(CTRL_BLK) void k_control(Ctrl* c, const double* partial, int np)
 {
@@ -1134,6 +1188,7 @@ struct Sys {
     double q = 1.1;              // tanh family parameter (CpuSatTanh.cpp:39)
     bool tiled = false;          // clause pass: tiled kernel (K = 3, 4) or generic
     bool lds_var = false;        // var pass: LDS region kernel or gather kernel
+    bool wide_var = false;       // gather path with long rows: one workgroup per variable
     int var_lds = 0;             // LDS region kernel: dynamic LDS bytes (region + offsets)
     int var_region_cap = 0;      // LDS region kernel: elements reserved for a region
     void* d_contrib = nullptr;
@@ -1158,7 +1213,12 @@ struct Sys {
 
 // clause-pass and var-pass grid sizes of a configured system
 static int clause_grid(const Sys* s) { return 8 * ceil_div(s->tiled ? ceil_div(s->m, TC) : ceil_div(s->m, BLK), 8); }
-static int var_grid(const Sys* s) { return s->lds_var ? s->ngroups : 8 * s->vg_blocks * ceil_div(s->ngroups, 8); }
+static int var_grid(const Sys* s)
+{
+    if (s->lds_var) return s->ngroups;
+    if (s->wide_var) return s->n;
+    return 8 * s->vg_blocks * ceil_div(s->ngroups, 8);
+}
 
 struct Rng {
     unsigned long long seed = 0, offset = 0;
@@ -1221,6 +1281,10 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
     s->vg_blocks = ceil_div(P.vg, BLK);
     s->tiled = tiled;
     s->lds_var = lds_var;
+    // few variables with long rows (dense problems such as Ramsey encodings):
+    // the gather kernel would add a block's rows one after another, so each
+    // variable gets a workgroup of its own
+    s->wide_var = !lds_var && s->nslots >= WIDE_MIN_ROW * (long long)P.n;
     s->var_region_cap = lds_var ? (int)P.max_region : 0;
     s->var_lds = lds_var ? var_lds_bytes(P.max_region, s->esize) : 0;
     // clause arrays padded to whole tiles (padding literal 0, position -1)
@@ -1380,6 +1444,8 @@ static void launch_var(const StageArgs<T>& a, const Sys* s, hipStream_t st)
     if (s->lds_var) {
         var_lds_attr<T, J>();
         hipLaunchKernelGGL((k_var_region<T, J>), dim3(var_grid(s)), dim3(VBLK), (size_t)s->var_lds, st, a);
+    } else if (s->wide_var) {
+        hipLaunchKernelGGL((k_var_wide<T, J>), dim3(var_grid(s)), dim3(WIDE_BLK), 0, st, a);
     } else {
         hipLaunchKernelGGL((k_var<T, J>), dim3(var_grid(s)), dim3(BLK), 0, st, a);
     }
@@ -2253,7 +2319,15 @@ int asat_util_gpu_name(char* buf, int len)
     HIP_CHECK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, dev));
-    std::snprintf(buf, (size_t)len, "%s", prop.name);
+    if (prop.name[0] != '\0') {
+        std::snprintf(buf, (size_t)len, "%s", prop.name);
+    } else {
+        // some drivers leave the marketing name empty: the architecture then
+        // names the device (the part of gcnArchName before its feature flags)
+        std::string arch(prop.gcnArchName);
+        arch = arch.substr(0, arch.find(':'));
+        std::snprintf(buf, (size_t)len, "AMD GPU %s", arch.empty() ? "(unknown)" : arch.c_str());
+    }
     API_END
 }
 

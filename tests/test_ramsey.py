@@ -215,3 +215,42 @@ def test_run_ramsey_series_on_gpu(tmp_path):
     assert M.shape == (5, 5) and (np.diag(M) == 0).all() and (M == M.T).all()
     colors = M - 1
     assert valid_coloring(colors, (3, 3))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("N", [16, 20])
+def test_dense_ramsey_problem_bitexact_vs_oracle(precision, N):
+    """R(5,5) encodings have few variables with long rows (728 and 1632
+    contributions on average at N = 16, 20), too long for the LDS var pass:
+    the one-workgroup-per-variable var pass.  RHS and 20 steps bit-identical
+    to the oracle."""
+    import analogsat_amd as A
+    import oracle as O
+    print("[ramsey] dense R(5,5) N=%d %s" % (N, precision), flush=True)
+    lits, lens = RamseyClauseMaker(N, [5, 5]).clause_arrays()
+    p = O.Problem(lits, lens)
+    sat = A.HipSat(precision)
+    sat.SetProblem(A.SatProblem(lits, lens))
+    n, m = sat.Get_N(), sat.Get_M()
+    mode = "f64" if precision == "f64" else "mix"
+    dt = np.float64 if precision == "f64" else np.float32
+    pub = O.random_state_f64(N, n, m).astype(dt)
+    x, d = sat.MakeState(), sat.MakeState()
+    x.CopyFrom(pub)
+    sat.GetDerivatives(d, x, 0.0)
+    want = O.rhs(p, np.concatenate([[-1.0], pub]).astype(dt), mode=mode)[1:]
+    assert np.array_equal(d.CopyTo(), want)
+    rk = A.HipRungeKutta()
+    rk.SetEpsilon(1e-6)
+    rk.Configure(sat)
+    old, new = sat.MakeState(), sat.MakeState()
+    old.CopyFrom(pub)
+    with O.pow_mode("cr"):
+        orc = O.RK(p, eps=1e-6, mode=mode)
+        orc.set_state(pub)
+        rk.Step(old, new, 20)
+        orc.step(20)
+        assert np.array_equal(old.CopyTo(), orc.get_state())
+        st = rk.stats()
+        assert (st.hdid, st.attempts, st.rhscount) == (orc.hdid, orc.attempts, orc.rhscount)
