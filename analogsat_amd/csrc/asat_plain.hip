@@ -26,8 +26,9 @@
 //                     recomputed inside the clause pass and never stored)
 // Problems outside the fast paths use a generic clause pass (K other than 3
 // and 4: pos[m*K + q] per slot) and a gather var pass (a group whose region
-// exceeds the LDS: csr_pos global positions, ~400 KB groups); both gather the
-// stage spins by variable from x / tmp.
+// exceeds the LDS: csr_pos global positions, ~400 KB groups, or one
+// workgroup per variable when rows are long); both gather the stage spins by
+// variable from x / tmp.
 //
 // One RK45 attempt = 6 x (clause pass + var pass) + 1 control kernel:
 //   clause pass J: per tile, read the slot spins through the tile's sorted
