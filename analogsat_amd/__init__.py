@@ -132,8 +132,7 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise RuntimeError("libanalogsat_hip.so is not built: run python -m analogsat_amd.builder "
-                               "(or __graft_entry__.build())")
+            raise RuntimeError("libanalogsat_hip.so is not built: run python -m analogsat_amd.builder")
         L = ctypes.CDLL(LIB_PATH)
         _share_hip_runtime()
         vp, ip, i = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int

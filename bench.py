@@ -451,7 +451,7 @@ def main():
 
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); "
-              "multi-GPU sharding is described in DESIGN.md but not implemented" % args.gpus, file=sys.stderr)
+              "multi-GPU sharding is not implemented" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import analogsat_amd as A
