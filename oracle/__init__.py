@@ -1,7 +1,7 @@
 """CPU oracle for the AnalogSat integrator -- test infrastructure only.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
-``bench.py`` import this package, and only as the checker.  It wraps two C
+``bench.py`` import this package, and only to check results.  It wraps two C
 libraries built by ``oracle/Makefile``:
 
 * ``libasat_oracle.so`` -- the C restatement (``asat_oracle.c``) of the

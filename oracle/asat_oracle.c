@@ -1,7 +1,7 @@
 /*
  * CPU oracle for the AnalogSat CTDS integrator -- TEST INFRASTRUCTURE ONLY.
  *
- * This library is the checker for the HIP implementation.  Only the test
+ * This library checks the results of the HIP implementation.  Only the test
  * suite, the smoke check and the CPU-baseline leg of the benchmark may load
  * it; the product library never links or calls it.
  *
