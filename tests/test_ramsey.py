@@ -254,3 +254,70 @@ def test_dense_ramsey_problem_bitexact_vs_oracle(precision, N):
         assert np.array_equal(old.CopyTo(), orc.get_state())
         st = rk.stats()
         assert (st.hdid, st.attempts, st.rhscount) == (orc.hdid, orc.attempts, orc.rhscount)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_dense_ramsey_tanh_with_bias(precision):
+    """The reference's Ramsey runs use the tanh family with a bias term
+    (Frontend/Config/run_ramsey_55_tanh.cfg: -tanh -bias 500).  On the dense
+    R(5,5) N = 16 encoding (one-workgroup-per-variable var pass), against the
+    oracle with the GPU's definition of the bias mean: RHS within 16 ulp of
+    the largest per-variable sum of |slot term| (the device tanh is not
+    glibc's bit for bit), and 20 steps with the same attempts and states
+    within 1e-9 (FP64) / 1e-4 (FP32) relative."""
+    import analogsat_amd as A
+    import oracle as O
+    print("[ramsey] dense R(5,5) N=16 tanh bias 500 %s" % precision, flush=True)
+    q, b = 1.1, 500.0
+    lits, lens = RamseyClauseMaker(16, [5, 5]).clause_arrays()
+    p = O.Problem(lits, lens)
+    sat = A.HipSatTanh(precision, q=q)
+    sat.SetProblem(A.SatProblem(lits, lens))
+    sat.Set_B(b)
+    n, m = sat.Get_N(), sat.Get_M()
+    mode = "f64" if precision == "f64" else "mix"
+    dt = np.float64 if precision == "f64" else np.float32
+    pub = O.random_state_f64(16, n, m).astype(dt)
+    x, d = sat.MakeState(), sat.MakeState()
+    x.CopyFrom(pub)
+    sat.GetDerivatives(d, x, 0.0)
+    xi = np.concatenate([[-1.0], pub]).astype(dt)
+    with O.mean_mode("double"):
+        want = O.rhs(p, xi, mode=mode, b=b, family="tanh", q=q)[1:].astype(np.float64)
+    got = d.CopyTo().astype(np.float64)
+    # largest per-variable sum of |slot term| (bounds the rounding of the sums)
+    k = int(lens.max())
+    C = np.zeros((m, k), np.int64)
+    off = np.concatenate([[0], np.cumsum(lens)])
+    for j in range(m):
+        C[j, :lens[j]] = lits[off[j]:off[j + 1]]
+    xs = np.asarray(xi, np.float64)
+    s = np.where(C != 0, xs[np.abs(C)], -1.0)
+    f = np.where(C < 0, 1.0 + s, 1.0 - s)
+    a = xs[n + 1:n + 1 + m]
+    tot = np.zeros(n + 1)
+    for qi in range(k):
+        prod = np.prod(np.delete(f, qi, axis=1), axis=1)
+        term = 2.0 ** -k * a * np.abs(prod) * np.abs(np.tanh(q * f[:, qi]))
+        np.add.at(tot, np.abs(C[:, qi]), np.where(C[:, qi] != 0, term, 0.0))
+    scale = max(tot[1:].max(), np.abs(want[:n]).max())
+    ulp = 2.0 ** -52 if precision == "f64" else 2.0 ** -23
+    err = np.abs(got - want).max()
+    print("[ramsey]   max |diff| %.3e = %.2f ulp of scale" % (err, err / (ulp * scale)), flush=True)
+    assert err <= 16 * ulp * scale
+    rk = A.HipRungeKutta()
+    rk.SetEpsilon(1e-6)
+    rk.Configure(sat)
+    old, new = sat.MakeState(), sat.MakeState()
+    old.CopyFrom(pub)
+    with O.pow_mode("cr"), O.mean_mode("double"):
+        orc = O.RK(p, eps=1e-6, mode=mode, b=b, family="tanh", q=q)
+        orc.set_state(pub)
+        rk.Step(old, new, 20)
+        orc.step(20)
+    got_x, ref = old.CopyTo().astype(np.float64), orc.get_state().astype(np.float64)
+    rel = np.abs(got_x - ref).max() / np.abs(ref).max()
+    print("[ramsey]   rel diff %.3e, attempts %d vs %d" % (rel, rk.stats().attempts, orc.attempts), flush=True)
+    assert rk.stats().steps == 20 and rk.stats().attempts == orc.attempts
+    assert rel <= (1e-9 if precision == "f64" else 1e-4)
