@@ -10,6 +10,7 @@
 * the checkpoint interval does not change the gradient
 * torch.autograd.Function returns the C ABI gradient
 """
+import os
 import sys
 
 import numpy as np
@@ -128,3 +129,29 @@ def test_torch_autograd_function():
     g = adj2.backward() * np.array([1.0, 2.0, -0.5])[:, None]
     np.testing.assert_allclose(s0.grad.numpy(), g[:, :n].astype(np.float32), rtol=1e-6)
     np.testing.assert_allclose(a0.grad.numpy(), g[:, n:].astype(np.float32), rtol=1e-6)
+
+
+def test_fp32_with_fp64_steps_within_1e5_of_fp64_oracle(golden_dir):
+    """FP32 state error alone: the C1 instance and start, the FP64 oracle's
+    accepted step sizes replayed with FP32 state (asat_adj_replay).  Without
+    the divergence of the step sequences that chaotic trajectories bring, the
+    FP32 state stays within 1e-5 relative of the FP64 oracle after 10, 100
+    and 1000 steps."""
+    say("FP32 frozen-step replay vs FP64 oracle, 1000 steps")
+    g = np.load(os.path.join(golden_dir, "c1_traj.npz"), allow_pickle=False)
+    p = O.fastsat(50, 213, 3, 0)
+    pub = g["pub0"].astype(np.float64)
+    for T in (10, 100, 1000):
+        with O.pow_mode("cr"):
+            orc = O.RK(p, eps=1e-6, mode="f64")
+            orc.set_state(pub)
+            hs = []
+            for _ in range(T):
+                orc.step(1)
+                hs.append(orc.hdid)
+            x64 = orc.get_state().astype(np.float64)
+        adj = A.HipAdjoint(A.SatProblem(p.literals, p.lengths), eps=1e-6, steps=T, checkpoint_every=32)
+        _, xT = adj.replay(pub.astype(np.float32)[None], np.array(hs)[None])
+        rel = np.abs(xT[0].astype(np.float64) - x64).max() / np.abs(x64).max()
+        say("  T = %d: max rel %.3e" % (T, rel))
+        assert rel <= 1e-5, (T, rel)
