@@ -899,18 +899,19 @@ __global__ __launch_bounds__(BLK) void k_var(StageArgs<T> a)
 
 // Wide var pass, for the gather layout when variables have long rows (at
 // least WIDE_MIN_ROW contributions on average): one workgroup per variable.
-// The workgroup stages the variable's contributions in LDS, WIDE_STAGE at a
-// time with one load per thread and round, and one thread adds them in slot
-// order, so the sum is the gather kernel's (and the reference's) exactly.
+// The variable's contributions pass through two LDS buffers of WIDE_STAGE
+// values: while thread 0 adds one buffer in slot order (so the sum is the
+// gather kernel's, and the reference's, exactly), waves 1..3 stage the next
+// chunk into the other.
 constexpr int WIDE_BLK = 256;
-constexpr int WIDE_STAGE = 4096;
+constexpr int WIDE_STAGE = 2048;
 constexpr long long WIDE_MIN_ROW = 64;
 
 template <typename T, int J>
 __global__ __launch_bounds__(WIDE_BLK) void k_var_wide(StageArgs<T> a)
 {
     __shared__ double sh[WIDE_BLK / 64];
-    __shared__ T vals[WIDE_STAGE];
+    __shared__ T vals[2][WIDE_STAGE];
     const Ctrl* c = a.ctrl;
     int parity = 0;
     double h = 0.0, hnext = 0.0, floor = TabD::TINY;
@@ -929,13 +930,23 @@ __global__ __launch_bounds__(WIDE_BLK) void k_var_wide(StageArgs<T> a)
     T sum = (T)0;
     if (fresh) {
         const int e0 = a.csr_ptr[i], e1 = a.csr_ptr[i + 1];
-        for (int r0 = e0; r0 < e1; r0 += WIDE_STAGE) {
+        const int t = (int)threadIdx.x;
+        {   // chunk 0, every thread
+            const int r1 = std::min(e1, e0 + WIDE_STAGE);
+            for (int e = e0 + t; e < r1; e += WIDE_BLK) vals[0][e - e0] = a.contrib[a.csr_pos[e]];
+        }
+        __syncthreads();
+        for (int r0 = e0, ch = 0; r0 < e1; r0 += WIDE_STAGE, ch++) {
             const int r1 = std::min(e1, r0 + WIDE_STAGE);
-            for (int e = r0 + (int)threadIdx.x; e < r1; e += WIDE_BLK) vals[e - r0] = a.contrib[a.csr_pos[e]];
-            __syncthreads();
-            if (threadIdx.x == 0) {
+            if (t >= 64) {
+                // stage chunk ch + 1 into the buffer added in round ch - 1
+                const int n1 = std::min(e1, r1 + WIDE_STAGE);
+                T* dst = vals[(ch + 1) & 1];
+                for (int e = r1 + (t - 64); e < n1; e += WIDE_BLK - 64) dst[e - r1] = a.contrib[a.csr_pos[e]];
+            } else if (t == 0) {
+                const T* v = vals[ch & 1];
 #pragma unroll 8
-                for (int e = 0; e < r1 - r0; e++) sum += vals[e];
+                for (int e = 0; e < r1 - r0; e++) sum += v[e];
             }
             __syncthreads();
         }
