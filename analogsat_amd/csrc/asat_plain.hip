@@ -911,7 +911,7 @@ template <typename T, int J>
 __global__ __launch_bounds__(WIDE_BLK) void k_var_wide(StageArgs<T> a)
 {
     __shared__ double sh[WIDE_BLK / 64];
-    __shared__ T vals[2][WIDE_STAGE];
+    __shared__ __attribute__((aligned(16))) T vals[2][WIDE_STAGE];
     const Ctrl* c = a.ctrl;
     int parity = 0;
     double h = 0.0, hnext = 0.0, floor = TabD::TINY;
@@ -944,9 +944,22 @@ __global__ __launch_bounds__(WIDE_BLK) void k_var_wide(StageArgs<T> a)
                 T* dst = vals[(ch + 1) & 1];
                 for (int e = r1 + (t - 64); e < n1; e += WIDE_BLK - 64) dst[e - r1] = a.contrib[a.csr_pos[e]];
             } else if (t == 0) {
+                // 16-byte LDS reads, eight ahead; the additions stay in slot order
+                constexpr int V = 16 / (int)sizeof(T);
+                typedef T vT __attribute__((ext_vector_type(V)));
                 const T* v = vals[ch & 1];
-#pragma unroll 8
-                for (int e = 0; e < r1 - r0; e++) sum += v[e];
+                const int len = r1 - r0;
+                int e = 0;
+                for (; e + 8 * V <= len; e += 8 * V) {
+                    vT q[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) q[u] = *(const vT*)(v + e + u * V);
+#pragma unroll
+                    for (int u = 0; u < 8; u++)
+#pragma unroll
+                        for (int w = 0; w < V; w++) sum += q[u][w];
+                }
+                for (; e < len; e++) sum += v[e];
             }
             __syncthreads();
         }
