@@ -77,6 +77,15 @@ class RkStats(ctypes.Structure):
                 ("rhscount", ctypes.c_longlong)]
 
 
+class LayoutInfo(ctypes.Structure):
+    """asat_layout_t: the passes and launch shapes SetProblem chose."""
+    _fields_ = [(nm, ctypes.c_int) for nm in ("clause_pass", "var_pass", "groups", "group_vars", "tiles",
+                                              "clause_grid", "var_grid", "var_lds_bytes", "slot_spins")]
+
+    def as_dict(self):
+        return {nm: getattr(self, nm) for nm, _ in self._fields_}
+
+
 def build():
     from . import builder
     return builder.build()
@@ -144,6 +153,8 @@ def lib():
             "asat_problem_fastsat": ([i, i, i, ctypes.c_ulong, vp], i),
             "asat_problem_seed_engine": ([ctypes.c_ulong, ctypes.POINTER(ctypes.c_ulonglong)], i),
             "asat_problem_fastsat_stream": ([i, i, i, ctypes.POINTER(ctypes.c_ulonglong), vp], i),
+            "asat_problem_randomsat": ([i, i, i, ctypes.c_ulong, vp], i),
+            "asat_problem_randomsat_stream": ([i, i, i, ctypes.POINTER(ctypes.c_ulonglong), vp], i),
             "asat_sys_create": ([i, pvp], i),
             "asat_sys_destroy": ([vp], i),
             "asat_sys_set_problem": ([vp, vp, vp, i], i),
@@ -154,6 +165,7 @@ def lib():
             "asat_sys_violations": ([vp, vp, ip], i),
             "asat_sys_set_random_state": ([vp, vp, vp], i),
             "asat_sys_clause_order": ([vp, vp], i),
+            "asat_sys_layout": ([vp, ctypes.POINTER(LayoutInfo)], i),
             "asat_sys_set_family": ([vp, ctypes.c_int], i),
             "asat_sys_get_family": ([vp, ip], i),
             "asat_sys_set_q": ([vp, ctypes.c_double], i),
@@ -384,6 +396,27 @@ class FastSatMaker:
         return p
 
 
+class RandomSatMaker(FastSatMaker):
+    """AnalogSAT/problem/RandomSatMaker.h:29-63 -- FastSatMaker's draw with
+    repeated clauses drawn again; same seed, same instance stream."""
+
+    def MakeSatProblem(self, problem=None):
+        out = np.zeros(self.m * self.k, np.int32)
+        _check(lib().asat_problem_randomsat_stream(self.n, self.m, self.k, ctypes.byref(self._engine),
+                                                   out.ctypes.data))
+        p = SatProblem(out, np.full(self.m, self.k, np.int32))
+        if problem is not None:
+            problem.literals, problem.lengths = p.literals, p.lengths
+        return p
+
+
+def randomsat(n, m, k, seed):
+    """One RandomSatMaker(seed) instance."""
+    out = np.zeros(m * k, np.int32)
+    _check(lib().asat_problem_randomsat(n, m, k, seed, out.ctypes.data))
+    return SatProblem(out, np.full(m, k, np.int32))
+
+
 def fastsat(n, m, k, seed):
     """One FastSatMaker(seed) instance."""
     out = np.zeros(m * k, np.int32)
@@ -535,6 +568,14 @@ class HipSat:
         out = np.zeros(self.Get_M(), np.int32)
         _check(lib().asat_sys_clause_order(self._h, out.ctypes.data))
         return out
+
+    def Layout(self):
+        """The passes and launch shapes chosen for the current problem (dict):
+        clause_pass 0 tiled / 1 generic, var_pass 0 LDS regions / 1 gather /
+        2 one workgroup per variable, groups, group_vars, tiles, grids."""
+        out = LayoutInfo()
+        _check(lib().asat_sys_layout(self._h, ctypes.byref(out)))
+        return out.as_dict()
 
 
 class HipSatTanh(HipSat):

@@ -1,11 +1,14 @@
 """Build libanalogsat_hip.so in-tree with hipcc for gfx950.
 
+Every source is compiled to an object of its own (in parallel, under
+analogsat_amd/_build/), then the objects are linked into the shared library.
 All HIP sources are compiled with
   -ffp-contract=off   no fused multiply-add, so the FP64 path rounds like the
                       reference's plain -O3 host build
   -Xclang -target-feature -Xclang -packed-fp32-ops
                       keep the compiler from emitting packed FP32 instructions
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -14,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libanalogsat_hip.so")
+OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = ["asat_plain.hip", "asat_batch.hip", "asat_adjoint.hip", "asat_problem.cpp"]
 
@@ -25,7 +29,6 @@ FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
@@ -34,16 +37,25 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-o", tmp] + srcs
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building %s" % LIB)
+        raise RuntimeError("hipcc failed: %s" % " ".join(cmd[-3:]))
+
+
+def build(verbose=False):
+    os.makedirs(OBJDIR, exist_ok=True)
+    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    objs = [os.path.join(OBJDIR, os.path.splitext(s)[0] + ".o") for s in srcs]
+    jobs = [[HIPCC] + FLAGS + ["-c", "-o", o, os.path.join(CSRC, s)] for s, o in zip(srcs, objs)]
+    with concurrent.futures.ThreadPoolExecutor(max(1, min(len(jobs), os.cpu_count() or 1))) as ex:
+        for f in [ex.submit(_run, j, verbose) for j in jobs]:
+            f.result()
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs, verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -1189,6 +1189,7 @@ struct State {
     Sys* sys = nullptr;
     void* d = nullptr;     // device buffer of the system's precision
     int alloc = 0;
+    unsigned long long gen = 0;   // the system's problem generation when the state was made
 };
 
 struct Sys {
@@ -1220,6 +1221,10 @@ struct Sys {
     double* d_apart = nullptr;   // bias term: per clause-block aux sums
     int* d_count = nullptr;
     bool ready = false;
+    // bumped by every SetProblem: states and integrators record the value
+    // they were made for, and a later problem (even one of the same size)
+    // invalidates them
+    unsigned long long gen = 0;
 
     void free_problem()
     {
@@ -1297,6 +1302,7 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
     HIP_CHECK(hipStreamSynchronize(s->stream));
     s->free_problem();
     s->ready = false;
+    s->gen++;
     s->n = P.n; s->m = P.m; s->k = P.k;
     s->aoff = (int)round_up((long long)P.n + 1, 64);
     s->alloc = (int)(s->aoff + round_up(P.m, 64));
@@ -1345,6 +1351,7 @@ static State* state_new(Sys* s)
     std::unique_ptr<State> st(new State());
     st->sys = s;
     st->alloc = s->alloc;
+    st->gen = s->gen;
     HIP_CHECK(hipMalloc(&st->d, (size_t)s->esize * s->alloc));
     HIP_CHECK(hipMemsetAsync(st->d, 0, (size_t)s->esize * s->alloc, s->stream));
     if (s->prec == ASAT_F64) hipLaunchKernelGGL(k_fill_const<double>, dim3(1), dim3(64), 0, s->stream, (double*)st->d);
@@ -1357,7 +1364,7 @@ static State* state_new(Sys* s)
 static void state_check(const State* st)
 {
     require(st && st->sys, ASAT_ERR_INVALID_ARGUMENT, "null state");
-    require(st->alloc == st->sys->alloc && st->sys->ready, ASAT_ERR_INVALID_ARGUMENT,
+    require(st->alloc == st->sys->alloc && st->gen == st->sys->gen && st->sys->ready, ASAT_ERR_INVALID_ARGUMENT,
             "state does not belong to the current problem");
 }
 
@@ -1543,6 +1550,7 @@ struct RK {
     double* hlog = nullptr; // optional device log of accepted step sizes (set before Reset)
     long long hlog_len = 0;
     int alloc = 0, prec = -1;
+    unsigned long long gen = 0;   // the system's problem generation at Configure
     void* k[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     void* tmp = nullptr;
     double* partial = nullptr;
@@ -1587,6 +1595,10 @@ struct RK {
     {
         require(configured && sys && sys->ready, ASAT_ERR_NOT_CONFIGURED,
                 "the RK4 solver has not been configured with an ODE yet");
+        // scratch (k1..k5, tmp, partials) is sized for the problem present at
+        // Configure; a new problem needs a new Configure
+        require(gen == sys->gen, ASAT_ERR_NOT_CONFIGURED,
+                "the SAT problem changed since the RK4 solver was configured");
     }
 };
 
@@ -1609,6 +1621,7 @@ static void rk_configure(RK* r, Sys* s)
     r->configured = false;
     r->sys = s;
     r->alloc = s->alloc;
+    r->gen = s->gen;
     r->prec = s->prec;
     const size_t bytes = (size_t)s->esize * s->alloc;
     for (auto& p : r->k) {
@@ -2076,6 +2089,24 @@ int asat_sys_clause_order(const asat_sys_t* sys, int* order)
     const Sys* s = &sys->s;
     s->require_ready();
     for (int j = 0; j < s->m; j++) order[j] = s->perm.empty() ? j : s->perm[j];
+    API_END
+}
+
+int asat_sys_layout(const asat_sys_t* sys, asat_layout_t* out)
+{
+    API_BEGIN
+    CHECK_PTR(sys); CHECK_PTR(out);
+    const Sys* s = &sys->s;
+    s->require_ready();
+    out->clause_pass = s->tiled ? ASAT_CLAUSE_TILED : ASAT_CLAUSE_GENERIC;
+    out->var_pass = s->lds_var ? ASAT_VAR_REGION : (s->wide_var ? ASAT_VAR_WIDE : ASAT_VAR_GATHER);
+    out->groups = s->ngroups;
+    out->group_vars = s->vg;
+    out->tiles = s->tiled ? (int)ceil_div(s->m, TC) : 0;
+    out->clause_grid = clause_grid(s);
+    out->var_grid = var_grid(s);
+    out->var_lds_bytes = s->var_lds;
+    out->slot_spins = (s->tiled && s->lds_var) ? 1 : 0;
     API_END
 }
 

@@ -8,11 +8,18 @@
 // which combines two engine outputs as ((u1-1) + (u2-1) r) / r^2, r = 2^31-2.
 // The clause rule: k variables floor(u*n)+1, sorted ascending, redrawn while
 // any repeats, then each literal kept positive when u < 0.5, else negated.
+//
+// asat_problem_randomsat reproduces RandomSatMaker
+// (AnalogSAT/problem/RandomSatMaker.cpp:59-125): the same draw, and a clause
+// equal to one already accepted (same signed literals in sorted order) is
+// drawn again.  The reference looks clauses up in a hash set; only equality
+// decides, so the set's hash function does not change the instance.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <unordered_set>
 
 #include "analogsat_hip.h"
 
@@ -86,6 +93,73 @@ static int fastsat_from(asat::MinStd& g, int n, int m, int k, int* literals_out)
         std::memcpy(literals_out + (size_t)j * k, clause, sizeof(int) * (size_t)k);
     }
     return ASAT_OK;
+}
+
+// Accepted clauses of one RandomSatMaker call, keyed by clause index into the
+// output array (the literals stay where they were written).
+struct ClauseKey {
+    const int* base;
+    int k;
+    size_t operator()(int j) const
+    {
+        uint64_t h = 1469598103934665603ULL;
+        for (int i = 0; i < k; i++) { h ^= (uint32_t)base[(size_t)j * k + i]; h *= 1099511628211ULL; }
+        return (size_t)h;
+    }
+    bool operator()(int a, int b) const
+    {
+        return std::memcmp(base + (size_t)a * k, base + (size_t)b * k, sizeof(int) * (size_t)k) == 0;
+    }
+};
+
+static int randomsat_from(asat::MinStd& g, int n, int m, int k, int* literals_out)
+{
+    if (n <= 0 || m < 0 || k <= 0 || k > 64 || k > n || !literals_out) {
+        asat::g_last_error = "randomsat: need n > 0, m >= 0, 0 < k <= min(n, 64) and an output array";
+        return ASAT_ERR_INVALID_ARGUMENT;
+    }
+    // C(n,k) 2^k distinct clauses exist; asking for more would draw forever
+    double distinct = 1.0;
+    for (int i = 0; i < k; i++) distinct = distinct * (double)(n - i) / (double)(i + 1) * 2.0;
+    if ((double)m > distinct) {
+        asat::g_last_error = "randomsat: more clauses requested than distinct clauses exist";
+        return ASAT_ERR_INVALID_ARGUMENT;
+    }
+    ClauseKey key{literals_out, k};
+    std::unordered_set<int, ClauseKey, ClauseKey> seen((size_t)m + 16, key, key);
+    for (int j = 0; j < m; j++) {
+        int* clause = literals_out + (size_t)j * k;
+        for (;;) {
+            bool repeated;
+            do {
+                for (int i = 0; i < k; i++) clause[i] = (int)std::floor(g.canonical() * n) + 1;
+                asat::sort_small(clause, k);
+                repeated = false;
+                for (int i = 0; i + 1 < k; i++) repeated |= clause[i] == clause[i + 1];
+            } while (repeated);
+            for (int i = 0; i < k; i++) clause[i] *= g.canonical() < 0.5 ? 1 : -1;
+            if (seen.insert(j).second) break;
+        }
+    }
+    return ASAT_OK;
+}
+
+extern "C" int asat_problem_randomsat(int n, int m, int k, unsigned long seed, int* literals_out)
+{
+    asat::MinStd g(seed);
+    return randomsat_from(g, n, m, k, literals_out);
+}
+
+extern "C" int asat_problem_randomsat_stream(int n, int m, int k, unsigned long long* engine, int* literals_out)
+{
+    if (!engine || *engine == 0 || *engine >= 2147483647ULL) {
+        asat::g_last_error = "engine state must be in [1, 2^31 - 2]";
+        return ASAT_ERR_INVALID_ARGUMENT;
+    }
+    asat::MinStd g = asat::MinStd::from_state(*engine);
+    int rc = randomsat_from(g, n, m, k, literals_out);
+    if (rc == ASAT_OK) *engine = g.state();
+    return rc;
 }
 
 extern "C" int asat_problem_fastsat(int n, int m, int k, unsigned long seed, int* literals_out)

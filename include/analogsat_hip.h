@@ -18,7 +18,8 @@
  *                  implemented by CudaRungeKutta (AnalogSAT/cuda_sat/CudaRungeKutta.h:30-136)
  *   asat_rng_*     ISatRandom<TFloat> (AnalogSAT/solver/ISatRandom.h:25-33),
  *                  implemented by CudaRandom (AnalogSAT/cuda_base/CudaRandom.h:29-46)
- *   asat_problem_* FastSatMaker (AnalogSAT/problem/FastSatMaker.h:29-61) and
+ *   asat_problem_* FastSatMaker (AnalogSAT/problem/FastSatMaker.h:29-61),
+ *                  RandomSatMaker (AnalogSAT/problem/RandomSatMaker.h:29-63) and
  *                  SatProblem::AddClauses (AnalogSAT/problem/SatProblem.h:49)
  *   asat_batch_*   no reference counterpart: batched ragged solver (many
  *                  trajectories / instances in one persistent launch)
@@ -71,6 +72,13 @@ int asat_problem_fastsat(int n, int m, int k, unsigned long seed, int* literals_
    several problems): seed the engine once, then each call advances *engine. */
 int asat_problem_seed_engine(unsigned long seed, unsigned long long* engine);
 int asat_problem_fastsat_stream(int n, int m, int k, unsigned long long* engine, int* literals_out);
+/* RandomSatMaker(seed) (AnalogSAT/problem/RandomSatMaker.cpp:59-125): the
+   FastSatMaker draw, with a clause equal to one already made (same signed
+   literals, sorted) drawn again, so no clause repeats.  Same stream as
+   FastSatMaker.  Asking for more than C(n,k) 2^k clauses, where the
+   reference would draw forever, is an ASAT_ERR_INVALID_ARGUMENT. */
+int asat_problem_randomsat(int n, int m, int k, unsigned long seed, int* literals_out);
+int asat_problem_randomsat_stream(int n, int m, int k, unsigned long long* engine, int* literals_out);
 
 /* ---------------- CTDS system (ISat) --------------------------------------- */
 
@@ -110,6 +118,26 @@ int asat_sys_get_q(const asat_sys_t* sys, double* q);
    of a state's device aux block.  Copies to and from host memory
    (asat_state_set_public / get_public) already use the public clause order. */
 int asat_sys_clause_order(const asat_sys_t* sys, int* order);
+
+/* No reference counterpart (the CUDA classes pick their kernels by class):
+   which passes and launch shapes SetProblem chose for this problem. */
+#define ASAT_CLAUSE_TILED 0    /* tiles of 2048 clauses, K = 3 or 4 */
+#define ASAT_CLAUSE_GENERIC 1  /* thread per clause, any K <= 32 */
+#define ASAT_VAR_REGION 0      /* one workgroup per variable group, region in LDS */
+#define ASAT_VAR_GATHER 1      /* gather from global memory, many variables per workgroup */
+#define ASAT_VAR_WIDE 2        /* gather, one workgroup per variable (long rows) */
+typedef struct {
+    int clause_pass;     /* ASAT_CLAUSE_* */
+    int var_pass;        /* ASAT_VAR_* */
+    int groups;          /* variable groups (contribution regions) */
+    int group_vars;      /* variables per group */
+    int tiles;           /* tiles of the tiled clause pass (0 otherwise) */
+    int clause_grid;     /* workgroups of one clause pass */
+    int var_grid;        /* workgroups of one var pass */
+    int var_lds_bytes;   /* dynamic LDS of the region var pass */
+    int slot_spins;      /* 1: the integrator carries the stage spins in contribution layout */
+} asat_layout_t;
+int asat_sys_layout(const asat_sys_t* sys, asat_layout_t* out);
 
 /* ---------------- states (ISatState) -------------------------------------- */
 

@@ -104,6 +104,11 @@ def lib():
         L.orc_uniform_f64.argtypes = [ctypes.c_uint, _f64p, ctypes.c_int]
         L.orc_random_state_f64.argtypes = [ctypes.c_uint, ctypes.c_int, ctypes.c_int, _f64p]
         L.orc_fastsat.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulong, _i32p]
+        L.orc_randomsat.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulong, _i32p]
+        L.orc_randomsat.restype = ctypes.c_int
+        _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+        L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+        L.orc_philox_uniform.argtypes = [ctypes.c_ulonglong, ctypes.c_ulonglong, ctypes.c_int, ctypes.c_int, _vp]
         L.orc_set_pow_mode.argtypes = [ctypes.c_int]
         L.orc_set_mean_mode.argtypes = [ctypes.c_int]
         L.orc_get_mean_mode.restype = ctypes.c_int
@@ -125,6 +130,7 @@ def ref():
         R = ctypes.CDLL(REF_PATH)
         R.ref_problem_sizes.argtypes = [_i32p, _i32p, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
         R.ref_rhs_f64.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_double, _f64p, _f64p]
+        R.ref_rhs_f32.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_double, _f32p, _f32p]
         R.ref_rhs_tanh_f64.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_double, ctypes.c_double, _f64p, _f64p]
         R.ref_rhs_tanh_f32.argtypes = [_i32p, _i32p, ctypes.c_int, ctypes.c_double, ctypes.c_double, _f32p, _f32p]
         R.ref_violations_f64.argtypes = [_i32p, _i32p, ctypes.c_int, _f64p]
@@ -154,6 +160,8 @@ def ref():
         R.ref_solve.restype = ctypes.c_int
         R.ref_fastsat.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulong, _i32p]
         R.ref_fastsat.restype = ctypes.c_int
+        R.ref_randomsat.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulong, _i32p]
+        R.ref_randomsat.restype = ctypes.c_int
         R.ref_uniform_f64.argtypes = [ctypes.c_uint, _f64p, ctypes.c_int]
         _ref = R
     return _ref
@@ -196,6 +204,29 @@ def fastsat(n, m, k, seed):
     out = np.zeros(m * k, dtype=np.int32)
     lib().orc_fastsat(n, m, k, seed, out)
     return Problem(out, np.full(m, k, dtype=np.int32))
+
+
+def randomsat(n, m, k, seed):
+    """RandomSatMaker(seed) instance as a Problem (restated generator: the
+    FastSatMaker draw with exact-duplicate clauses drawn again)."""
+    out = np.zeros(m * k, dtype=np.int32)
+    if lib().orc_randomsat(n, m, k, seed, out) != 0:
+        raise ValueError("more clauses than distinct %d-clauses over %d variables" % (k, n))
+    return Problem(out, np.full(m, k, dtype=np.int32))
+
+
+def philox4x32_10(ctr, key):
+    """One Philox4x32-10 block: four 32-bit counter words, two key words."""
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(np.ascontiguousarray(ctr, np.uint32), np.ascontiguousarray(key, np.uint32), out)
+    return out
+
+
+def philox_uniform(seed, offset, length, dtype=np.float64):
+    """The library's ISatRandom stream: `length` uniforms at stream offset `offset`."""
+    out = np.zeros(length, dtype=dtype)
+    lib().orc_philox_uniform(seed, offset, length, 1 if dtype == np.float32 else 0, out.ctypes.data)
+    return out
 
 
 def uniform_f64(seed, length):

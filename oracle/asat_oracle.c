@@ -267,3 +267,95 @@ void orc_fastsat(int n, int m, int k, unsigned long seed, int *out)
         memcpy(out + (size_t)j * k, clause, sizeof(int) * (size_t)k);
     }
 }
+
+/* RandomSatMaker(seed) with Set_N/Set_M/Set_K, MakeSatProblem
+   (AnalogSAT/problem/RandomSatMaker.cpp:59-125): each clause is drawn as in
+   FastSatMaker (same stream, same rule), and a clause equal to one already
+   accepted (same signed literals in the same sorted order) is drawn again.
+   The reference keeps the accepted clauses in an unordered_set keyed by a
+   hash of the literals; only equality decides, so any hash gives the same
+   instance.  Returns 0, or -1 if m exceeds the number of distinct clauses
+   C(n,k) 2^k (where the reference would draw forever). */
+static uint64_t clause_hash(const int *c, int k)
+{
+    uint64_t h = 1469598103934665603ULL;
+    for (int i = 0; i < k; i++) { h ^= (uint32_t)c[i]; h *= 1099511628211ULL; }
+    return h;
+}
+
+int orc_randomsat(int n, int m, int k, unsigned long seed, int *out)
+{
+    double distinct = 1.0;                   /* C(n,k) 2^k, as a double */
+    for (int i = 0; i < k; i++) distinct = distinct * (double)(n - i) / (double)(i + 1) * 2.0;
+    if ((double)m > distinct) return -1;
+    size_t cap = 16;
+    while (cap < 2 * (size_t)m + 16) cap <<= 1;
+    int *slot = (int *)malloc(sizeof(int) * cap);   /* index of an accepted clause, -1 empty */
+    for (size_t i = 0; i < cap; i++) slot[i] = -1;
+    orc_lcg_t g;
+    lcg_seed(&g, (uint64_t)seed);
+    for (int j = 0; j < m; j++) {
+        int *clause = out + (size_t)j * k;
+        for (;;) {
+            int rep;
+            do {
+                for (int i = 0; i < k; i++) clause[i] = (int)floor(canonical_double(&g) * n) + 1;
+                local_sort(clause, k);
+                rep = 0;
+                for (int i = 0; i < k - 1; i++) if (clause[i] == clause[i + 1]) rep = 1;
+            } while (rep);
+            for (int i = 0; i < k; i++) clause[i] *= canonical_double(&g) < 0.5 ? 1 : -1;
+            size_t at = (size_t)clause_hash(clause, k) & (cap - 1);
+            int seen = 0;
+            while (slot[at] >= 0) {
+                if (memcmp(out + (size_t)slot[at] * k, clause, sizeof(int) * (size_t)k) == 0) { seen = 1; break; }
+                at = (at + 1) & (cap - 1);
+            }
+            if (!seen) { slot[at] = j; break; }
+        }
+    }
+    free(slot);
+    return 0;
+}
+
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+   easy as 1, 2, 3", SC'11): ten rounds of two 32x32->64 multiplies with the
+   published multipliers and Weyl key increments.  Not a reference function:
+   the reference draws its device initial conditions from cuRAND MTGP32
+   (AnalogSAT/cuda_base/CudaRandom.cu:24-60), a closed library replaced by
+   this counter-based generator. */
+void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4])
+{
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; r++) {
+        const uint64_t p0 = (uint64_t)M0 * c0, p1 = (uint64_t)M1 * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += W0;
+        k1 += W1;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* The uniform stream of the library's ISatRandom: element i of a draw at
+   stream offset `offset` is Philox of counter {idx lo, idx hi, 0x5EED, 0}
+   (idx = offset + i) under key {seed lo, seed hi}; a double takes the top 53
+   bits of (word0 << 32 | word1), a float the top 24 bits of word0. */
+void orc_philox_uniform(unsigned long long seed, unsigned long long offset, int len, int f32, void *out)
+{
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (int i = 0; i < len; i++) {
+        const unsigned long long idx = offset + (unsigned long long)i;
+        const uint32_t ctr[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), 0x5EEDu, 0u};
+        uint32_t w[4];
+        orc_philox4x32_10(ctr, key, w);
+        if (f32) ((float *)out)[i] = (float)(w[0] >> 8) * 0x1.0p-24f;
+        else ((double *)out)[i] = (double)((((uint64_t)w[0] << 32) | w[1]) >> 11) * 0x1.0p-53;
+    }
+}

@@ -4,9 +4,10 @@
 // (read in place under /root/reference, never copied) into
 // oracle/_ref/libasat_ref.so.  The tests use it to write golden fixtures and
 // to pin the C restatement in asat_oracle.c bit for bit.  The classes used are
-// the reference's: CpuSat<double> (AnalogSAT/cpu_sat/CpuSat.h),
+// the reference's: CpuSat<double> and CpuSat<float> (AnalogSAT/cpu_sat/CpuSat.h),
 // CpuRungeKutta<double> (AnalogSAT/cpu_sat/CpuRungeKutta.h), SatSolver
-// (AnalogSAT/solver/SatSolver.h), SatProblem, FastSatMaker and CpuRandom.
+// (AnalogSAT/solver/SatSolver.h), SatProblem, FastSatMaker, RandomSatMaker and
+// CpuRandom.
 #include <memory>
 #include <stdexcept>
 #include <vector>
@@ -18,6 +19,7 @@
 #include "solver/SatSolver.h"
 #include "problem/SatProblem.h"
 #include "problem/FastSatMaker.h"
+#include "problem/RandomSatMaker.h"
 
 using namespace analogsat;
 
@@ -99,19 +101,36 @@ int ref_problem_sizes(const int* lits, const int* lens, int nclauses, int* n, in
     return 0;
 }
 
+}  // extern "C"
+
+// CpuSat<T>::GetDerivatives on the internal layout [-1, s, a]
+template <typename T>
+static int rhs_ctds(const int* lits, const int* lens, int nclauses, double b, const T* x, T* dxdt)
+{
+    SatProblem p = make_problem(lits, lens, nclauses);
+    CpuSat<T> sat;
+    sat.Set_B((T)b);
+    sat.SetProblem(p);
+    std::unique_ptr<CpuSatStateImpl<T>> xs(sat.MakeState()), ds(sat.MakeState());
+    int D = p.Get_N() + p.Get_M() + 1;
+    xs->GetState().assign(x, x + D);
+    sat.GetDerivatives(*ds, *xs, (T)0);
+    for (int i = 0; i < D; i++) dxdt[i] = ds->GetState()[i];
+    return 0;
+}
+
+extern "C" {
+
 // one RHS evaluation on the internal layout [-1, s, a]
 int ref_rhs_f64(const int* lits, const int* lens, int nclauses, double b, const double* x, double* dxdt)
 {
-    SatProblem p = make_problem(lits, lens, nclauses);
-    CpuSat<double> sat;
-    sat.Set_B(b);
-    sat.SetProblem(p);
-    std::unique_ptr<CpuSatStateImpl<double>> xs(sat.MakeState()), ds(sat.MakeState());
-    int D = p.Get_N() + p.Get_M() + 1;
-    xs->GetState().assign(x, x + D);
-    sat.GetDerivatives(*ds, *xs, 0.0);
-    for (int i = 0; i < D; i++) dxdt[i] = ds->GetState()[i];
-    return 0;
+    return rhs_ctds<double>(lits, lens, nclauses, b, x, dxdt);
+}
+
+// the same with the reference's float instantiation (AnalogSAT/cpu_sat/CpuSat.cpp:391)
+int ref_rhs_f32(const int* lits, const int* lens, int nclauses, double b, const float* x, float* dxdt)
+{
+    return rhs_ctds<float>(lits, lens, nclauses, b, x, dxdt);
 }
 
 int ref_rhs_tanh_f64(const int* lits, const int* lens, int nclauses, double b, double q, const double* x, double* dxdt)
@@ -219,6 +238,22 @@ int ref_solve(const int* lits, const int* lens, int nclauses, const double* pub0
 int ref_fastsat(int n, int m, int k, unsigned long seed, int* out)
 {
     FastSatMaker maker(seed);
+    maker.Set_K(k);
+    maker.Set_N(n);
+    maker.Set_M(m);
+    SatProblem p;
+    maker.MakeSatProblem(p);
+    std::vector<int> lits, lens;
+    p.GetAllClauses(lits, lens);
+    for (size_t i = 0; i < lits.size(); i++) out[i] = lits[i];
+    return (int)lits.size();
+}
+
+// RandomSatMaker(seed): as FastSatMaker, but a clause equal to one already made
+// (same signed literals, in sorted order) is drawn again
+int ref_randomsat(int n, int m, int k, unsigned long seed, int* out)
+{
+    RandomSatMaker maker(seed);
     maker.Set_K(k);
     maker.Set_N(n);
     maker.Set_M(m);

@@ -21,6 +21,10 @@ Fixtures (numpy .npz, no pickles; DIMACS .cnf for instances):
   c1_traj_ld.npz       CpuSat<long double> + CpuRungeKutta<long double> from the
                        c1_traj start: states after 10, 100 and 1000 steps and the
                        1000 step sizes (x87 extended precision)
+  rhs_cases_f32.npz    CpuSat<float> RHS evaluations on the cases of
+                       rhs_cases.npz, the state rounded to float
+  randomsat_cases.npz  RandomSatMaker literal streams, including instances
+                       where most draws repeat an earlier clause
 """
 import ctypes
 import os
@@ -180,6 +184,27 @@ def main():
     for sd in (0, 1, 2, 4294967295):
         gen["uniform_%d" % sd] = ref_uniform(sd, 2048)
     np.savez(os.path.join(HERE, "gen_cases.npz"), **gen)
+
+    # ---- CpuSat<float> on the RHS cases ------------------------------------
+    f32 = {}
+    for name in ("k3", "k4", "mixed", "k12", "bias", "n2000"):
+        lt, ln = cases[name + "_lits"], cases[name + "_lens"]
+        x32 = cases[name + "_x"].astype(np.float32)
+        out32 = np.zeros_like(x32)
+        R.ref_rhs_f32(lt, ln, len(ln), float(cases[name + "_b"]), x32, out32)
+        f32[name + "_rhs32"] = out32
+    np.savez(os.path.join(HERE, "rhs_cases_f32.npz"), **f32)
+
+    # ---- RandomSatMaker streams ------------------------------------------
+    # (n, m, k, seed): C(6,3) 2^3 = 160 and C(5,4) 2^4 = 80 distinct clauses,
+    # so most late draws repeat; k = 1 with m = 2n takes every clause
+    rs = {}
+    for (nn, mm, kk, sd) in [(50, 213, 3, 0), (6, 150, 3, 1), (8, 400, 3, 2), (5, 70, 4, 3), (4, 8, 1, 0),
+                             (300, 2979, 4, 2)]:
+        out = np.zeros(mm * kk, dtype=np.int32)
+        R.ref_randomsat(nn, mm, kk, sd, out)
+        rs["randomsat_%d_%d_%d_%d" % (nn, mm, kk, sd)] = out
+    np.savez(os.path.join(HERE, "randomsat_cases.npz"), **rs)
     print("golden fixtures written to", HERE)
 
 

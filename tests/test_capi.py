@@ -135,3 +135,22 @@ def test_solver_setters_validate():
         s.SetMaxSteps(0)
     assert (s.GetMaxTime(), s.GetMaxSteps(), s.GetBatchSize()) == (100.0, 100000, 20)
     assert A.GetSatResultMessage(A.SatResult.SAT_SOLUTION_FOUND) == "solution found"
+
+
+def test_randomsat_matches_golden_stream(golden_dir):
+    d = np.load(os.path.join(golden_dir, "randomsat_cases.npz"), allow_pickle=False)
+    for key in d.files:
+        n, m, k, sd = map(int, key.split("_")[1:])
+        p = A.randomsat(n, m, k, sd)
+        assert np.array_equal(p.literals, d[key]), key
+
+
+def test_randomsat_maker_stream_and_errors():
+    # one maker making two problems continues one stream, as the reference object does
+    mk = A.RandomSatMaker(7)
+    mk.Set_N(30); mk.Set_M(120); mk.Set_K(3)
+    a, b = mk.MakeSatProblem(), mk.MakeSatProblem()
+    assert not np.array_equal(a.literals, b.literals)
+    assert np.array_equal(a.literals, A.randomsat(30, 120, 3, 7).literals)
+    with pytest.raises(ValueError):
+        A.randomsat(4, 9, 1, 0)      # more clauses than the 8 distinct ones

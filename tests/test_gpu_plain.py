@@ -465,3 +465,30 @@ def test_bias_term_rhs_and_trajectory(rhs_cases, precision):
             % (mm, rel, rk.stats().attempts, orc.attempts))
         assert rk.stats().attempts == orc.attempts
         assert rel <= (1e-9 if precision == "f64" else 1e-3)
+
+
+def test_new_problem_invalidates_states_and_integrator():
+    """After a second SetProblem, states and an integrator made for the first
+    problem are rejected, even when the second problem has the same buffer
+    sizes (N = 100, M = 200 and N = 127, M = 250 both take 384 elements)."""
+    say("problem generation")
+    p1, p2 = O.fastsat(100, 200, 3, 1), O.fastsat(127, 250, 3, 2)
+    sat = _sys_with(p1.literals, p1.lengths, "f64")
+    rk = A.HipRungeKutta()
+    rk.SetEpsilon(1e-6)
+    rk.Configure(sat)
+    old, new = sat.MakeState(), sat.MakeState()
+    old.CopyFrom(O.random_state_f64(1, p1.n, p1.m))
+    assert rk.Step(old, new, 3)
+    assert old.GetAllocSize() == 384
+    sat.SetProblem(A.SatProblem(p2.literals, p2.lengths))
+    fresh = sat.MakeState()
+    assert fresh.GetAllocSize() == 384
+    with pytest.raises(RuntimeError, match="changed since"):
+        rk.Step(old, new, 1)
+    with pytest.raises(ValueError, match="current problem"):
+        sat.GetClauseViolationCount(old)
+    rk.Configure(sat)
+    a, b = sat.MakeState(), sat.MakeState()
+    a.CopyFrom(O.random_state_f64(2, p2.n, p2.m))
+    assert rk.Step(a, b, 3)

@@ -16,6 +16,7 @@ pinned instead by what the mathematics fixes:
 """
 import itertools
 import math
+from math import comb
 import os
 
 import numpy as np
@@ -219,12 +220,16 @@ def test_run_ramsey_series_on_gpu(tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", ["f64", "f32"])
-@pytest.mark.parametrize("N", [16, 20])
+@pytest.mark.parametrize("N", [16, 20, 23, 24])
 def test_dense_ramsey_problem_bitexact_vs_oracle(precision, N):
-    """R(5,5) encodings have few variables with long rows (728 and 1632
-    contributions on average at N = 16, 20), too long for the LDS var pass:
-    the one-workgroup-per-variable var pass.  RHS and 20 steps bit-identical
-    to the oracle."""
+    """R(5,5) encodings have few variables with long rows (every variable has
+    20 C(N,5) / C(N,2) contributions: 728, 1632, 2660 and 3080 at N = 16, 20,
+    23, 24), too long for the LDS var pass: the one-workgroup-per-variable var
+    pass.  Its rows pass through LDS in chunks of 2048 values; at N = 23 and
+    24 a row spans two chunks (the second staged while the first is summed),
+    and the last chunk (612, 1032 values) is not a multiple of the 8-vector
+    read group.  RHS, 20 steps, attempts and RHS counts bit-identical to the
+    oracle."""
     import analogsat_amd as A
     import oracle as O
     print("[ramsey] dense R(5,5) N=%d %s" % (N, precision), flush=True)
@@ -233,6 +238,9 @@ def test_dense_ramsey_problem_bitexact_vs_oracle(precision, N):
     sat = A.HipSat(precision)
     sat.SetProblem(A.SatProblem(lits, lens))
     n, m = sat.Get_N(), sat.Get_M()
+    assert sat.Layout()["var_pass"] == 2          # one workgroup per variable
+    rows = np.bincount(np.abs(lits), minlength=n + 1)[1:]
+    assert rows.min() == rows.max() == 20 * comb(N, 5) // comb(N, 2)
     mode = "f64" if precision == "f64" else "mix"
     dt = np.float64 if precision == "f64" else np.float32
     pub = O.random_state_f64(N, n, m).astype(dt)
