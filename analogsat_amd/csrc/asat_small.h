@@ -1,0 +1,375 @@
+// One-workgroup persistent integrator for problems that fit in LDS.
+//
+// The plain solver's attempt is 13 kernels; for a small instance (C1-sized:
+// N = 50, M = 213) every one of them is a few microseconds of dispatch for
+// well under a microsecond of work.  Here one workgroup holds the whole
+// integrator in LDS -- state x, x_new, k1..k5, the stage spin input, the slot
+// contributions, the literals and the per-variable slot lists -- and runs
+// attempt after attempt with workgroup barriers between the clause and var
+// phases of each stage.  Every thread evaluates the FP64 step controller
+// itself from the block-wide error maximum, so no thread waits for a
+// broadcast.  Optionally the SatSolver loop runs on the device too: after
+// every `batch` accepted steps the violated-clause count of the thresholded
+// state is taken and the run ends on SAT, on the step or time limit, or on
+// step-size underflow (AnalogSAT/solver/SatSolver.h:313-331, the branch
+// without a callback).
+//
+// Arithmetic follows the multi-kernel path operation for operation (the
+// generic clause pass k_clause_gen and the var passes' var_stage), so the
+// two paths give bit-identical states, step sizes and counters:
+//   clause j:  f_q = 1 -+ s (padding literal: s = -1), K_m = knorm prod f_q
+//              left to right, aux stage input from x and k_1..k_{J-1},
+//              k_J(a) = a K^2, slot terms (KmiProd order, stop at the first
+//              padding literal for K <= 10; CalculateAll order above)
+//   var i:     contributions added in the reference's accumulation order
+//              (clauses in input order, literals in clause order), then the
+//              stage combination (AnalogSAT/cpu_sat/CpuRungeKutta.cpp:180-215)
+//   control:   AnalogSAT/cpu_sat/CpuRungeKutta.cpp:217-267 in FP64
+// The bias term (b > 0) is not covered: such systems take the multi-kernel
+// path.
+#pragma once
+
+#include "asat_common.h"
+
+namespace asat {
+
+constexpr int SMALL_LDS_MAX = 150 * 1024;   // dynamic LDS of the small path
+constexpr long long SMALL_ATTEMPT_CAP = 8192;   // attempts per launch before the kernel pauses
+
+// SatSolver loop state on the device, kept across launches
+struct SmallSolve {
+    long long solver_steps;   // SatSolver::steps: accepted steps of completed batches
+    int bleft;                // accepted steps left in the current batch (0: at the top of the loop)
+    int result;               // -1 while running, else a SatResult code
+    int last_viol;            // SatSolver::lastViolationCount
+    int pad_;
+};
+
+// SatResult codes (AnalogSAT/solver/SatResult.h:24-32)
+constexpr int SMALL_SAT = 0, SMALL_UNDERFLOW = 1, SMALL_MAXTIME = 3, SMALL_MAXITER = 4;
+
+struct SmallArgs {
+    const int* lit;       // m*K literals in processing order (zero padded)
+    const int* vptr;      // n+2 slot-list starts per variable
+    const int* vslot;     // slot indices (processing order) per variable, accumulation order
+    void* x;              // state buffer: read at entry, newest state written at exit
+    Ctrl* ctrl;
+    SmallSolve* solve;    // null: Step mode (run until ctrl->steps reaches target)
+    long long target;     // Step mode: accepted-step count to reach
+    long long max_steps;  // solve mode: SatSolver::maxSteps
+    double max_time;      // solve mode: SatSolver::maxTime
+    int time_float;       // solve mode: compare the time as a float (TFloat = float)
+    int batch;            // solve mode: SatSolver::batchSize
+    int n, m, k, aoff, alloc, nlist;
+    int family;
+    double q;
+    double knorm;
+};
+
+// LDS bytes of the small path for a problem (0 if it does not qualify)
+inline long long small_lds_bytes(int n, int m, int k, long long alloc, long long nlist, int esize)
+{
+    const long long mk = (long long)m * k;
+    return (long long)esize * (7 * alloc + (n + 1) + mk) + 4 * (mk + (n + 2) + nlist);
+}
+
+template <typename T, int KT, int BS>
+struct SmallLds {
+    T *x, *xn, *k1, *k2, *k3, *k4, *k5, *sp, *cb;
+    int *lit, *vptr, *vslot;
+};
+
+// clause phase of stage J: thread per clause; sp_in holds the stage's spins
+// (x itself for J = 1)
+template <typename T, int KT, int BS, int J>
+__device__ __forceinline__ void small_clause(const SmallArgs& a, const SmallLds<T, KT, BS>& L, const T* sp_in,
+                                             T hs, double h, double hnext, double floor, T knorm, double& err)
+{
+    constexpr int KM = KT ? KT : GEN_KMAX;
+    const int K = KT ? KT : a.k;
+    for (int j = (int)threadIdx.x; j < a.m; j += BS) {
+        const int* lm = L.lit + j * K;
+        T f[KM];
+        T km = knorm;
+#pragma unroll
+        for (int q = 0; q < KM; q++) {
+            if (KT == 0 && q >= K) break;
+            const T fq = spin_factor<T>(lm[q], sp_in);
+            km *= fq;
+            f[q] = fq;
+        }
+        const int ai = a.aoff + j;
+        const T xa = L.x[ai];
+        T am, k1 = 0, k3 = 0, k4 = 0, k5 = 0;
+        if constexpr (J == 1) {
+            am = xa;
+        } else {
+            k1 = L.k1[ai];
+            const T k2 = J > 2 ? L.k2[ai] : (T)0;
+            k3 = J > 3 ? L.k3[ai] : (T)0;
+            k4 = J > 4 ? L.k4[ai] : (T)0;
+            k5 = J > 5 ? L.k5[ai] : (T)0;
+            am = stage_input<T, J>(xa, k1, k2, k3, k4, k5, hs);
+        }
+        const T kj = am * km * km;
+        if constexpr (J == 1) L.k1[ai] = kj;
+        else if constexpr (J == 2) L.k2[ai] = kj;
+        else if constexpr (J == 3) L.k3[ai] = kj;
+        else if constexpr (J == 4) L.k4[ai] = kj;
+        else if constexpr (J == 5) L.k5[ai] = kj;
+        else {
+            T xn;
+            double e;
+            final_combination<T>(xa, k1, k3, k4, k5, kj, hs, h, hnext, floor, xn, e);
+            err = fmax(err, e);
+            L.xn[ai] = xn;
+        }
+        T* out = L.cb + j * K;
+        if (a.family == ASAT_FAMILY_TANH) {
+            const T r = knorm * am;
+            const bool ex = K >= 3 && K <= 10;
+#pragma unroll
+            for (int q = 0; q < KM; q++) {
+                if (KT == 0 && q >= K) break;
+                const int l = lm[q];
+                if (l == 0) break;
+                T p = (T)1.0;
+                bool first = true;
+#pragma unroll
+                for (int qq = 0; qq < KM; qq++) {
+                    if (KT == 0 && qq >= K) break;
+                    if (qq == q) continue;
+                    p = first ? f[qq] : p * f[qq];
+                    first = false;
+                }
+                out[q] = tanh_term<T>(r, p, f[q], (T)a.q, l < 0, ex);
+            }
+        } else if (K <= 10) {
+            const T r = km * knorm * am;
+#pragma unroll
+            for (int q = 0; q < KM; q++) {
+                if (KT == 0 && q >= K) break;
+                const int l = lm[q];
+                if (l == 0) break;
+                T p = (T)1.0;
+                bool first = true;
+#pragma unroll
+                for (int qq = 0; qq < KM; qq++) {
+                    if (KT == 0 && qq >= K) break;
+                    if (qq == q) continue;
+                    p = first ? f[qq] : p * f[qq];
+                    first = false;
+                }
+                T rr = r * p;
+                rr *= l < 0 ? (T)-2.0 : (T)2.0;
+                out[q] = rr;
+            }
+        } else {
+            for (int q = 0; q < K; q++) {
+                const int l = lm[q];
+                if (l == 0) continue;
+                T temp = km;
+                for (int qq = 0; qq < K; qq++) temp *= (qq == q) ? (T)1.0 : f[qq];
+                out[q] = (T)2.0 * temp * knorm * am * (l < 0 ? (T)-1.0 : (T)1.0);
+            }
+        }
+    }
+}
+
+// var phase of stage J: thread per variable; fresh = false only on a retry's
+// J = 1 (k1 kept from the step's first attempt)
+template <typename T, int KT, int BS, int J>
+__device__ __forceinline__ void small_var(const SmallArgs& a, const SmallLds<T, KT, BS>& L, bool fresh, T hs,
+                                          double h, double hnext, double floor, double& err)
+{
+    for (int i = (int)threadIdx.x + 1; i <= a.n; i += BS) {
+        T kj = (T)0;
+        if (fresh) {
+            T sum = (T)0;
+            const int e1 = L.vptr[i + 1];
+            for (int e = L.vptr[i]; e < e1; e++) sum += L.cb[L.vslot[e]];
+            kj = sum;
+        }
+        const T xi = L.x[i];
+        if constexpr (J == 1) {
+            if (fresh) L.k1[i] = kj;
+            else kj = L.k1[i];
+            L.sp[i] = stage_input<T, 2>(xi, kj, 0, 0, 0, 0, hs);
+        } else if constexpr (J == 2) {
+            L.k2[i] = kj;
+            L.sp[i] = stage_input<T, 3>(xi, L.k1[i], kj, 0, 0, 0, hs);
+        } else if constexpr (J == 3) {
+            L.k3[i] = kj;
+            L.sp[i] = stage_input<T, 4>(xi, L.k1[i], L.k2[i], kj, 0, 0, hs);
+        } else if constexpr (J == 4) {
+            L.k4[i] = kj;
+            L.sp[i] = stage_input<T, 5>(xi, L.k1[i], L.k2[i], L.k3[i], kj, 0, hs);
+        } else if constexpr (J == 5) {
+            L.k5[i] = kj;
+            L.sp[i] = stage_input<T, 6>(xi, L.k1[i], L.k2[i], L.k3[i], L.k4[i], kj, hs);
+        } else {
+            T xn;
+            double e;
+            final_combination<T>(xi, L.k1[i], L.k3[i], L.k4[i], L.k5[i], kj, hs, h, hnext, floor, xn, e);
+            err = fmax(err, e);
+            L.xn[i] = xn;
+        }
+    }
+}
+
+// violated clauses of the thresholded state (k_violations semantics), in
+// every thread
+template <typename T, int KT, int BS>
+__device__ __forceinline__ int small_violations(const SmallArgs& a, const SmallLds<T, KT, BS>& L, int* shi)
+{
+    const int K = KT ? KT : a.k;
+    int viol = 0;
+    for (int j = (int)threadIdx.x; j < a.m; j += BS) {
+        int sat = 0;
+        for (int q = 0; q < K && !sat; q++) {
+            const int l = L.lit[j * K + q];
+            if (l == 0) continue;   // padding literal is constant FALSE
+            const int t = L.x[l < 0 ? -l : l] > (T)0;
+            sat = l < 0 ? !t : t;
+        }
+        viol += !sat;
+    }
+    return block_sum_all<BS>(viol, shi);
+}
+
+template <typename T, int KT, int BS>
+__global__ __launch_bounds__(BS) void k_small(SmallArgs a)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char small_lds[];
+    __shared__ double shd[BS / 64];
+    __shared__ int shi[BS / 64];
+    const int alloc = a.alloc, n = a.n, mk = a.m * (KT ? KT : a.k);
+    SmallLds<T, KT, BS> L;
+    T* base = (T*)small_lds;
+    L.x = base;
+    L.xn = base + alloc;
+    L.k1 = base + 2 * alloc; L.k2 = base + 3 * alloc; L.k3 = base + 4 * alloc; L.k4 = base + 5 * alloc;
+    L.k5 = base + 6 * alloc;
+    L.sp = base + 7 * alloc;
+    L.cb = L.sp + (n + 1);
+    L.lit = (int*)(L.cb + mk);
+    L.vptr = L.lit + mk;
+    L.vslot = L.vptr + (n + 2);
+    {
+        const T* gx = (const T*)a.x;
+        for (int i = (int)threadIdx.x; i < alloc; i += BS) {
+            const T v = gx[i];
+            L.x[i] = v;
+            L.xn[i] = v;
+            L.k1[i] = L.k2[i] = L.k3[i] = L.k4[i] = L.k5[i] = (T)0;
+        }
+        for (int i = (int)threadIdx.x; i <= n; i += BS) L.sp[i] = (T)0;
+        for (int i = (int)threadIdx.x; i < mk; i += BS) { L.lit[i] = a.lit[i]; L.cb[i] = (T)0; }
+        for (int i = (int)threadIdx.x; i < n + 2; i += BS) L.vptr[i] = a.vptr[i];
+        for (int i = (int)threadIdx.x; i < a.nlist; i += BS) L.vslot[i] = a.vslot[i];
+    }
+    // the control block, identical in every thread
+    const Ctrl* c = a.ctrl;
+    double h = c->h, hnext = c->hnext, time = c->time, hdid = c->hdid, errmax = c->errmax;
+    const double eps = c->eps, floor = c->floor;
+    long long steps = c->steps, attempts = c->attempts, rhscount = c->rhscount;
+    int step_start = c->step_start, underflow = c->underflow;
+    double* hlog = c->hlog;
+    const long long hlog_len = c->hlog_len;
+    SmallSolve ss = {0, 0, -1, 0, 0};
+    if (a.solve) ss = *a.solve;
+    __syncthreads();
+    const T knorm = (T)a.knorm;
+    long long launched = 0;
+    bool done = underflow || (a.solve ? ss.result >= 0 : steps >= a.target);
+    while (!done) {
+        if (a.solve && ss.bleft == 0) {
+            // top of the SatSolver loop: while (time < maxTime && steps < maxSteps)
+            const double tc = a.time_float ? (double)(float)time : time;
+            if (!(tc < a.max_time && ss.solver_steps < a.max_steps)) {
+                ss.result = tc >= a.max_time ? SMALL_MAXTIME : SMALL_MAXITER;
+                break;
+            }
+            ss.bleft = a.batch;
+        }
+        // pause between steps (k1 of the current x is not carried over)
+        if (step_start && launched >= SMALL_ATTEMPT_CAP) break;
+        const T hs = (T)h;
+        double err = 0.0;
+        if (step_start) small_clause<T, KT, BS, 1>(a, L, L.x, hs, h, hnext, floor, knorm, err);
+        __syncthreads();
+        small_var<T, KT, BS, 1>(a, L, step_start != 0, hs, h, hnext, floor, err);
+        __syncthreads();
+        small_clause<T, KT, BS, 2>(a, L, L.sp, hs, h, hnext, floor, knorm, err);
+        __syncthreads();
+        small_var<T, KT, BS, 2>(a, L, true, hs, h, hnext, floor, err);
+        __syncthreads();
+        small_clause<T, KT, BS, 3>(a, L, L.sp, hs, h, hnext, floor, knorm, err);
+        __syncthreads();
+        small_var<T, KT, BS, 3>(a, L, true, hs, h, hnext, floor, err);
+        __syncthreads();
+        small_clause<T, KT, BS, 4>(a, L, L.sp, hs, h, hnext, floor, knorm, err);
+        __syncthreads();
+        small_var<T, KT, BS, 4>(a, L, true, hs, h, hnext, floor, err);
+        __syncthreads();
+        small_clause<T, KT, BS, 5>(a, L, L.sp, hs, h, hnext, floor, knorm, err);
+        __syncthreads();
+        small_var<T, KT, BS, 5>(a, L, true, hs, h, hnext, floor, err);
+        __syncthreads();
+        small_clause<T, KT, BS, 6>(a, L, L.sp, hs, h, hnext, floor, knorm, err);
+        __syncthreads();
+        small_var<T, KT, BS, 6>(a, L, true, hs, h, hnext, floor, err);
+        // block maximum (its barriers also order the x_new writes before any
+        // read of the next attempt)
+        const double emax = block_max_all<BS>(err, shd) / eps;
+        launched++;
+        errmax = emax;
+        attempts += 1;
+        rhscount += step_start ? 6 : 5;
+        if (emax <= 1.0) {
+            const double hn = emax > TabD::ERRCON ? TabD::SAFETY * h * pow_ctrl(emax, 5) : 5.0 * h;
+            hdid = h;
+            time += h;
+            if (hlog && steps < hlog_len && threadIdx.x == 0) hlog[steps] = h;
+            steps += 1;
+            T* t = L.x; L.x = L.xn; L.xn = t;
+            step_start = 1;
+            hnext = hn;
+            h = hn;
+            if (a.solve) {
+                if (--ss.bleft == 0) {
+                    ss.solver_steps += a.batch;
+                    const int v = small_violations<T, KT, BS>(a, L, shi);
+                    ss.last_viol = v;
+                    if (v == 0) { ss.result = SMALL_SAT; done = true; }
+                }
+            } else if (steps >= a.target) {
+                done = true;
+            }
+        } else {
+            const double htemp = TabD::SAFETY * h * pow_ctrl(emax, 4);
+            const double hn = h >= 0.0 ? fmax(htemp, TabD::HFACTOR * h) : fmin(htemp, TabD::HFACTOR * h);
+            h = hn;
+            step_start = 0;
+            if (time + hn == time) {
+                underflow = 1;
+                if (a.solve) ss.result = SMALL_UNDERFLOW;
+                done = true;
+            }
+        }
+    }
+    __syncthreads();
+    {
+        T* gx = (T*)a.x;
+        for (int i = (int)threadIdx.x; i < alloc; i += BS) gx[i] = L.x[i];
+    }
+    if (threadIdx.x == 0) {
+        Ctrl* cw = a.ctrl;
+        cw->h = h; cw->hnext = hnext; cw->time = time; cw->hdid = hdid; cw->errmax = errmax;
+        cw->steps = steps; cw->attempts = attempts; cw->rhscount = rhscount;
+        cw->step_start = step_start; cw->underflow = underflow; cw->active = 0;
+        if (a.solve) *a.solve = ss;
+    }
+}
+
+}  // namespace asat

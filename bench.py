@@ -13,14 +13,24 @@ Roofline: before the timed region each kernel kind of one attempt is timed
 on its own with HIP events on the integrator's stream (asat_rk_profile,
 back-to-back launches of one kind), which names the dominant kernel.  Inside
 the timed region the integrator records a HIP event pair around that kernel
-in every attempt (asat_rk_set_kernel_timer); its algorithmic bytes per launch
-/ that in-sequence average duration gives `achieved`.  Algorithmic bytes
-(kernel_bytes; L = K*M literal slots, esize = 4 for FP32 state):
-  clause pass J : (6 + 2 esize) L + esize M A_J,           A = (2, 3, 4, 5, 6, 7)
-  var pass J    : (2 + 2 esize) L + 4 (N+1) + esize N B_J, B = (3, 4, 5, 6, 7, 6)
-(per slot: sorted position, rank, slot spin read and contribution written,
-resp. contribution and region offset read and slot spin written; A_J/B_J =
-state-length arrays read or written).
+in every attempt (asat_rk_set_kernel_timer).  `achieved` = that kernel's
+algorithmic bytes per launch / its in-sequence average duration, with the
+algorithmic bytes of the RK45 attempt B_alg = 96 L + 136 D (L = K M literal
+slots, D = N + M, FP32 state, int32 indices) split over the kernels:
+  clause pass J : 8 L + 4 M (J + 1)    (literal read 4 + contribution write 4
+                                        per slot; x and k_1..k_{J-1} read,
+                                        k_J written per clause)
+  var pass J    : 8 L + 4 N (J + 1)    (contribution read 4 + CSR read 4)
+The six stages of both passes add up to 96 L + 108 D; the remaining 28 D of
+B_alg is the final combine (x, k1, k3..k6 read, x_new written), which the
+J = 6 passes fuse into stage 6.  The attempt-level fraction
+B_alg x attempts/s / 8 TB/s and the floor B_floor = 24 L + 136 D are
+reported beside it, and the bytes the implementation moves by design
+(bytes_impl: sorted positions, ranks, region offsets, slot spins) next to
+them.  `traffic` is the HBM bytes per launch of the dominant kernel read
+from two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over a short run
+of the same instance, each scaled by what the counter reads for a plain copy
+of known size in the same run; null where rocprofv3 is unavailable.
 
 CPU baseline: the reference's own CPU solver (oracle/_ref, CpuSat<double> +
 CpuRungeKutta<double>, single thread) on the same instance and start for a
@@ -42,10 +52,131 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def alg_kernel_bytes(n, m, nslots, esize=4):
+    """Algorithmic bytes per launch of each kernel of one attempt, the RK45
+    attempt's B_alg split by pass (int32 indices, state of esize bytes; for
+    FP32 B_alg = 96 L + 136 D): clause pass J (4 + esize) L + esize M (J + 1),
+    var pass J (esize + 4) L + esize N (J + 1).  The final combine's 7 vectors
+    are fused into the J = 6 passes and not attributed."""
+    L = nslots
+    out = {}
+    for j in range(1, 7):
+        out["clause%d" % j] = (4 + esize) * L + esize * m * (j + 1)
+        out["var%d" % j] = (esize + 4) * L + esize * n * (j + 1)
+    out["control"] = 0
+    return out
+
+
+def alg_attempt_bytes(n, m, nslots, esize=4):
+    """(B_alg, B_floor) of one RK45 attempt; FP32: 96 L + 136 D and 24 L + 136 D."""
+    d = n + m
+    return 12 * (4 + esize) * nslots + 34 * esize * d, 24 * nslots + 34 * esize * d
+
+
+def kernel_regex(kind):
+    """Pattern of the rocprofv3 kernel names that implement a kernel kind of
+    asat_rk_profile's order (clause1..6, var1..6, control)."""
+    import re
+    if kind.startswith("clause"):
+        j = int(kind[6:])
+        return re.compile(r"k_clause_tile<\s*\w+\s*,\s*\d+\s*,\s*%d\s*,|k_clause_gen<\s*\w+\s*,\s*%d\s*>" % (j, j))
+    if kind.startswith("var"):
+        j = int(kind[3:])
+        return re.compile(r"k_var(_region|_wide)?<\s*\w+\s*,\s*%d\s*>" % j)
+    return re.compile(r"k_control\b")
+
+
+def traffic_probe(args):
+    """Child of measure_traffic, run under rocprofv3 --pmc: a few attempts of
+    the benchmark instance, then one plain copy of 256 MiB (counter scale)."""
+    import analogsat_amd as A
+    n, k = args.n, args.k
+    m = int(round(args.alpha * n))
+    prob = A.fastsat(n, m, k, 0)
+    pub0 = np.concatenate([np.random.default_rng(0).uniform(-1.0, 1.0, n), np.ones(m)])
+    sat = A.HipSat(args.precision)
+    sat.SetProblem(prob)
+    rk = A.HipRungeKutta()
+    rk.SetEpsilon(args.eps)
+    rk.Configure(sat)
+    old, new = sat.MakeState(), sat.MakeState()
+    old.CopyFrom(pub0.astype(sat.dtype))
+    rk.Step(old, new, 3)
+    A.copy_bench(256 << 20, reps=1)
+    print("probe done", flush=True)
+
+
+def measure_traffic(args, kind, log):
+    """HBM bytes per launch of kernel kind `kind`: two rocprofv3 --pmc passes
+    (FETCH_SIZE, then WRITE_SIZE) over traffic_probe, each counter scaled by
+    its reading for the 256 MiB copy (k_copy16 reads and writes 256 MiB).
+    Returns (read_bytes, write_bytes, detail) or None with the reason."""
+    import csv
+    import glob
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3") or ("/opt/rocm/bin/rocprofv3" if os.path.exists("/opt/rocm/bin/rocprofv3")
+                                         else None)
+    if prof is None:
+        return None, "rocprofv3 not found"
+    pat = kernel_regex(kind)
+    copy_bytes = 256 << 20
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="asat_pmc_", dir="/tmp")
+        cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "t", "--",
+               sys.executable, os.path.join(ROOT, "bench.py"), "--traffic-probe", "--n", str(args.n),
+               "--alpha", str(args.alpha), "--k", str(args.k), "--eps", str(args.eps),
+               "--precision", args.precision]
+        env = dict(os.environ, TMPDIR="/tmp")
+        log("traffic: rocprofv3 --pmc %s over a 3-step run" % counter)
+        p = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                             start_new_session=True)
+        try:
+            out = p.communicate(timeout=150)[0].decode(errors="replace")
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, signal.SIGKILL)
+            p.communicate()
+            shutil.rmtree(d, ignore_errors=True)
+            return None, "rocprofv3 --pmc %s timed out" % counter
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            shutil.rmtree(d, ignore_errors=True)
+            return None, "rocprofv3 --pmc %s wrote no counter file (rc %d): %s" % (counter, p.returncode,
+                                                                                  out[-300:])
+        vals, copy = [], []
+        for fn in files:
+            with open(fn) as f:
+                for row in csv.DictReader(f):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    name = row.get("Kernel_Name", "")
+                    v = float(row["Counter_Value"])
+                    if "k_copy16" in name:
+                        copy.append(v)
+                    elif pat.search(name):
+                        vals.append(v)
+        shutil.rmtree(d, ignore_errors=True)
+        if not vals or not copy:
+            return None, "counter file lacks the %s kernel or the copy kernel" % kind
+        res[counter] = (float(np.mean(vals)), float(np.mean(copy)), len(vals))
+    f_fetch = copy_bytes / res["FETCH_SIZE"][1]
+    f_write = copy_bytes / res["WRITE_SIZE"][1]
+    rd = res["FETCH_SIZE"][0] * f_fetch
+    wr = res["WRITE_SIZE"][0] * f_write
+    detail = {"read": int(rd), "write": int(wr), "launches": res["FETCH_SIZE"][2],
+              "fetch_KiB": round(res["FETCH_SIZE"][0], 1), "write_KiB": round(res["WRITE_SIZE"][0], 1),
+              "copy_fetch_KiB": round(res["FETCH_SIZE"][1], 1), "copy_write_KiB": round(res["WRITE_SIZE"][1], 1),
+              "bytes_per_fetch_unit": round(f_fetch, 2), "bytes_per_write_unit": round(f_write, 2)}
+    return (rd, wr, detail), None
+
+
 def kernel_bytes(n, m, k, nslots, esize=4):
-    """Algorithmic bytes per launch of each plain-solver kernel (every array
-    the kernel must touch, once; index arrays int32/uint16, state arrays of
-    esize bytes).
+    """Bytes each plain-solver kernel moves by design (every array the kernel
+    touches, once; index arrays int32/uint16, state arrays of esize bytes),
+    reported as bytes_impl beside the algorithmic figure.
 
     clause pass J (tiled, K = 3, 4, slot spins): per slot a sorted position
     (4) and a rank with the literal's sign (2) read, the slot's stage spin
@@ -68,8 +199,18 @@ def kernel_bytes(n, m, k, nslots, esize=4):
 
 
 def attempt_bytes(n, m, k, nslots, esize=4):
-    """Algorithmic bytes of one attempt (sum over the 13 kernels)."""
+    """Bytes one attempt moves by design (sum over the 13 kernels)."""
     return sum(kernel_bytes(n, m, k, nslots, esize).values())
+
+
+def host_cores():
+    """(cores used for the all-core CPU leg, nproc): the CPUs this process
+    may run on, at most 16 (one GPU's share of the host on the MI355X box)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return min(16, avail), os.cpu_count() or 1
 
 
 def cpu_baseline(lits, lens, pub0, eps, steps):
@@ -233,7 +374,7 @@ def batched_section(workload, count, max_steps, eps, plain_sample, log):
                                "p50_time_to_solution_s": (sorted(plain_tts)[len(plain_tts) // 2]
                                                           if plain_tts else None)},
         "batched_over_plain": round((steps / wall) / plain_rate, 1),
-        "cpu_reference_all_cores": cpu_cores_rate(insts, eps, 400, min(16, os.cpu_count() or 1)),
+        "cpu_reference_all_cores": cpu_cores_rate(insts, eps, 400, host_cores()[0]),
         "buckets": [{k: v for k, v in x.items()} for x in b.buckets()],
     }
     if workload == "c4":
@@ -244,6 +385,14 @@ def batched_section(workload, count, max_steps, eps, plain_sample, log):
 def p50(xs):
     xs = sorted(xs)
     return xs[len(xs) // 2] if xs else None
+
+
+def p50_all(solved_times, total):
+    """Median time to solution over all `total` runs, an unsolved run counting
+    as infinite: None unless at least half of the runs solved."""
+    xs = sorted(solved_times) + [float("inf")] * (total - len(solved_times))
+    v = xs[(total - 1) // 2] if total else float("inf")
+    return None if v == float("inf") else v
 
 
 def workload_c1(args, log):
@@ -268,11 +417,12 @@ def workload_c1(args, log):
     out = {"workload": "c1", "trajectories": len(res), "step_cap": 10000,
            "batched": {"dtype": "fp32-state/fp64-control", "wall_s": round(wall, 3),
                        "trajectory_steps_per_s": round(steps / wall, 1), "solved": len(solved),
-                       "p50_time_to_solution_s": p50([r["solved_wall_ms"] / 1e3 for r in solved]),
-                       "p50_steps_to_solution": p50([r["sat_step"] for r in solved])}}
+                       "p50_time_to_solution_s": p50_all([r["solved_wall_ms"] / 1e3 for r in solved], len(res)),
+                       "p50_steps_to_solution": p50_all([r["sat_step"] for r in solved], len(res)),
+                       "p50_solved_only_s": p50([r["solved_wall_ms"] / 1e3 for r in solved])}}
     sample = insts[:args.c1_plain_sample]
     log("C1: plain FP64 solver, %d trajectories one at a time" % len(sample))
-    tts, steps_p, wall_p = [], 0, 0.0
+    tts, tts_steps, steps_p, wall_p = [], [], 0, 0.0
     for p, x in sample:
         sat = A.HipSat("f64")
         rk = A.HipRungeKutta()
@@ -290,12 +440,14 @@ def workload_c1(args, log):
         steps_p += solver.GetStepCount()
         if r == A.SatResult.SAT_SOLUTION_FOUND:
             tts.append(w)
+            tts_steps.append(solver.GetStepCount())
     out["plain"] = {"dtype": "f64", "sample": len(sample), "trajectory_steps_per_s": round(steps_p / wall_p, 1),
-                    "solved": len(tts), "p50_time_to_solution_s": p50(tts)}
+                    "solved": len(tts), "p50_time_to_solution_s": p50_all(tts, len(sample)),
+                    "p50_steps_to_solution": p50_all(tts_steps, len(sample))}
     if not args.no_cpu:
         kind = "reference" if O.ref_available() else "port"
         log("C1: CPU solver (%s), %d trajectories one at a time, 1 thread" % (kind, len(sample)))
-        tts_c, steps_c, wall_c = [], 0, 0.0
+        tts_c, tts_cs, steps_c, wall_c = [], [], 0, 0.0
         for p, x in sample:
             lits = np.ascontiguousarray(p.literals, np.int32)
             lens = np.ascontiguousarray(p.lengths, np.int32)
@@ -320,9 +472,11 @@ def workload_c1(args, log):
             steps_c += st
             if code == 0:
                 tts_c.append(w)
+                tts_cs.append(st)
         out["cpu"] = {"kind": kind, "cores": 1, "sample": len(sample),
                       "trajectory_steps_per_s": round(steps_c / wall_c, 1), "solved": len(tts_c),
-                      "p50_time_to_solution_s": p50(tts_c)}
+                      "p50_time_to_solution_s": p50_all(tts_c, len(sample)),
+                      "p50_steps_to_solution": p50_all(tts_cs, len(sample))}
     return out
 
 
@@ -445,6 +599,10 @@ def main():
                     help="c2: the headline plain-solver line; c1/c4/c5: the other measured workloads")
     ap.add_argument("--c1-seeds", type=int, default=100)
     ap.add_argument("--c1-plain-sample", type=int, default=20)
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure the dominant kernel's HBM traffic")
+    ap.add_argument("--no-c1", action="store_true", help="skip the C1 time-to-solution leg of the default line")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run the plain copy kernel on this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -455,6 +613,10 @@ def main():
         sys.exit(2)
 
     import analogsat_amd as A
+
+    if args.traffic_probe:
+        traffic_probe(args)
+        return
 
     if args.copy_calibration:
         nbytes = args.copy_calibration << 20
@@ -515,14 +677,23 @@ def main():
                                                       for nm, v in kernels.items()))
     dom_us = t_ms * 1e3 / max(t_n, 1)
     log("%s in sequence: %.2f us average over %d launches" % (dom, dom_us, t_n))
-    achieved = float(kb[dom] / (dom_us * 1e-6) / 1e9)
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "plain_c2_traffic.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            traffic = json.load(f).get(dom)
+    ab = alg_kernel_bytes(n, m, nslots, esz)
+    achieved = float(ab[dom] / (dom_us * 1e-6) / 1e9)
     attempt_us = float(us[:13].sum())
     att_bytes = attempt_bytes(n, m, k, nslots, esz)
+    b_alg, b_floor = alg_attempt_bytes(n, m, nslots, esz)
+    att_rate = attempts / wall
+    traffic, traffic_detail = None, None
+    if not args.no_traffic:
+        got, why = measure_traffic(args, dom, log)
+        if got is None:
+            traffic_detail = {"unavailable": why}
+            log("traffic: " + why)
+        else:
+            traffic = int(got[0] + got[1])
+            traffic_detail = got[2]
+            log("traffic: %s reads %.1f MB and writes %.1f MB per launch (algorithmic %.1f MB)"
+                % (dom, got[0] / 1e6, got[1] / 1e6, ab[dom] / 1e6))
 
     hbm = A.stream_bench(1 << 30, 5)
     log("practical HBM rates: copy %.0f GB/s, read %.0f GB/s" % (hbm["copy_GBps"], hbm["read_GBps"]))
@@ -540,9 +711,16 @@ def main():
         log("batched %.0f traj-steps/s vs plain per instance %.0f (x%.1f)" % (
             batched["value"], batched["plain_per_instance"]["value"], batched["batched_over_plain"]))
 
+    c1 = None
+    if not args.no_c1:
+        c1 = workload_c1(args, log)
+        log("C1 p50 time to solution: batched %s s, plain FP64 %s s, CPU %s s" % (
+            c1["batched"]["p50_time_to_solution_s"], c1["plain"]["p50_time_to_solution_s"],
+            c1.get("cpu", {}).get("p50_time_to_solution_s")))
+
     value = steps / wall
     line = {
-        "metric": "RK45 accepted trajectory-steps/s (single large instance)",
+        "metric": "RK45 accepted trajectory-steps/s + p50 time-to-solution, batched vs plain",
         "value": round(value, 3),
         "unit": "trajectory-steps/s",
         "n_gpus": 1,
@@ -559,16 +737,23 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                      "kernel_us": round(dom_us, 2), "kernel_launches": int(t_n),
-                     "kernel_us_alone": round(float(us[names.index(dom)]), 2), "bytes_per_launch": int(kb[dom]),
+                     "kernel_us_alone": round(float(us[names.index(dom)]), 2),
+                     "alg_bytes_per_launch": int(ab[dom]), "bytes_impl": int(kb[dom]),
+                     "impl_GBps": round(float(kb[dom] / (dom_us * 1e-6) / 1e9), 1),
+                     "traffic_detail": traffic_detail,
                      "practical_GBps": {"copy": round(hbm["copy_GBps"], 1), "read": round(hbm["read_GBps"], 1)}},
-        "attempt": {"attempts_per_s": round(attempts / wall, 2), "reject_ratio": round(1 - steps / max(attempts, 1), 4),
-                    "kernel_us_sum": round(attempt_us, 1), "B_alg": int(att_bytes),
-                    "B_alg_GBps_at_kernel_sum": round(att_bytes / (attempt_us * 1e-6) / 1e9, 1),
-                    "B_alg_GBps_wall": round(att_bytes * attempts / wall / 1e9, 1)},
+        "attempt": {"attempts_per_s": round(att_rate, 2), "reject_ratio": round(1 - steps / max(attempts, 1), 4),
+                    "B_alg": int(b_alg), "B_floor": int(b_floor),
+                    "B_alg_GBps": round(b_alg * att_rate / 1e9, 1),
+                    "frac_B_alg": round(b_alg * att_rate / 1e9 / HBM_PEAK_GBS, 4),
+                    "frac_B_floor": round(b_floor * att_rate / 1e9 / HBM_PEAK_GBS, 4),
+                    "kernel_us_sum": round(attempt_us, 1), "bytes_impl": int(att_bytes),
+                    "impl_GBps_at_kernel_sum": round(att_bytes / (attempt_us * 1e-6) / 1e9, 1)},
         "kernels_alone": kernels,
         "device_ms": round(dev_ms, 3),
         "cpu_baseline": cpu,
         "batched": batched,
+        "time_to_solution": c1,
     }
     print(json.dumps(line), flush=True)
 
