@@ -77,6 +77,12 @@ class RkStats(ctypes.Structure):
                 ("rhscount", ctypes.c_longlong)]
 
 
+class SolveResult(ctypes.Structure):
+    """asat_solve_result_t: outcome of SatSolver's loop run by asat_rk_solve."""
+    _fields_ = [("result", ctypes.c_int), ("last_violations", ctypes.c_int), ("steps", ctypes.c_longlong),
+                ("on_device", ctypes.c_int), ("pad_", ctypes.c_int)]
+
+
 class LayoutInfo(ctypes.Structure):
     """asat_layout_t: the passes and launch shapes SetProblem chose."""
     _fields_ = [(nm, ctypes.c_int) for nm in ("clause_pass", "var_pass", "groups", "group_vars", "tiles",
@@ -187,6 +193,9 @@ def lib():
             "asat_batch_set_xscal_floor": ([vp, ctypes.c_double], i),
             "asat_rk_configure": ([vp, vp], i),
             "asat_rk_step": ([vp, vp, vp, i], i),
+            "asat_rk_set_one_workgroup": ([vp, i], i),
+            "asat_rk_uses_one_workgroup": ([vp, ctypes.POINTER(i)], i),
+            "asat_rk_solve": ([vp, vp, vp, i, ctypes.c_longlong, ctypes.c_double, i, ctypes.POINTER(SolveResult)], i),
             "asat_rk_reset": ([vp], i),
             "asat_rk_stats": ([vp, ctypes.POINTER(RkStats)], i),
             "asat_rk_last_step_ms": ([vp, ctypes.POINTER(ctypes.c_float)], i),
@@ -622,6 +631,25 @@ class HipRungeKutta:
     def Reset(self):
         _check(lib().asat_rk_reset(self._h))
 
+    def SetOneWorkgroup(self, enable):
+        """Allow (True, the default) or forbid the one-workgroup persistent
+        path for problems that fit in one workgroup's LDS."""
+        _check(lib().asat_rk_set_one_workgroup(self._h, 1 if enable else 0))
+
+    def UsesOneWorkgroup(self):
+        v = ctypes.c_int()
+        _check(lib().asat_rk_uses_one_workgroup(self._h, ctypes.byref(v)))
+        return bool(v.value)
+
+    def SolveBatches(self, old_state, new_state, batch, max_steps, max_time, time_float=False):
+        """SatSolver's loop without a callback (asat_rk_solve): batches of
+        `batch` accepted steps, SAT checked after each; on the one-workgroup
+        path the whole loop runs on the device.  Returns the SolveResult."""
+        out = SolveResult()
+        _check(lib().asat_rk_solve(self._h, old_state._h, new_state._h, int(batch), int(max_steps), float(max_time),
+                                   1 if time_float else 0, ctypes.byref(out)))
+        return out
+
     def stats(self):
         s = RkStats()
         _check(lib().asat_rk_stats(self._h, ctypes.byref(s)))
@@ -751,6 +779,20 @@ class SatSolver:
         self._wallstart = _time.perf_counter()
         self._cpustart = _time.process_time()
         result = SatResult.SAT_UNKNOWN
+        if self.callback is None and hasattr(self.odeint, "SolveBatches"):
+            # no callback: the integrator runs the same loop itself
+            r = self.odeint.SolveBatches(self.oldState, self.newState, self.batchSize, self.maxSteps, self.maxTime)
+            self.steps = int(r.steps)
+            self.lastViolationCount = int(r.last_violations)
+            result = SatResult(r.result)
+            self.walltime = _time.perf_counter() - self._wallstart
+            self.cputime = _time.process_time() - self._cpustart
+            st = self.odeint.stats()
+            self.time = st.time
+            self.rhsCount = int(st.rhscount)
+            self.lastStepSize = st.hdid
+            self.running = False
+            return result
         while self.time < self.maxTime and self.steps < self.maxSteps:
             before = self.odeint.GetStepCount()
             if not self.odeint.Step(self.oldState, self.newState, self.batchSize):

@@ -13,6 +13,8 @@
 
 namespace asat {
 
+constexpr int GEN_KMAX = 32;      // longest clause supported by the generic kernels
+
 // ---------------------------------------------------------------------------
 // errors: every failing HIP call becomes an exception carrying a status code;
 // the extern "C" layer turns it into a return code + thread-local message.

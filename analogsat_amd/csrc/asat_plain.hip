@@ -46,6 +46,7 @@
 // Step(count) fills the slot spins of x, enqueues attempts without host
 // round trips and reads the control block back once per round of attempts.
 #include "asat_common.h"
+#include "asat_small.h"
 
 #include <algorithm>
 #include <cmath>
@@ -61,7 +62,6 @@ thread_local std::string g_last_error;
 
 constexpr int BLK = 256;          // stage-kernel block size (4 waves)
 constexpr int CTRL_BLK = 1024;    // control kernel block size
-constexpr int GEN_KMAX = 32;      // longest clause supported by the generic kernels
 
 static inline long long round_up(long long a, long long b) { return (a + b - 1) / b * b; }
 
@@ -1209,6 +1209,13 @@ struct Sys {
     long long* d_gstart = nullptr;
     int* d_perm = nullptr;       // processing position -> clause index (null: identity)
     std::vector<int> perm;       // host copy, for the public <-> internal a-block order
+    // one-workgroup path (asat_small.h) for problems whose integrator fits in
+    // LDS: per variable, its slots (processing-order slot index) in the
+    // reference's accumulation order; small_lds = 0 when the problem is too big
+    int* d_vptr = nullptr;
+    int* d_vslot = nullptr;
+    int nlist = 0;
+    long long small_lds = 0;
     int vg = 0, ngroups = 0, vg_blocks = 0;
     int family = ASAT_FAMILY_CTDS;
     double q = 1.1;              // tanh family parameter (CpuSatTanh.cpp:39)
@@ -1230,7 +1237,10 @@ struct Sys {
     {
         dfree(d_lit); dfree(d_pos); dfree(d_spos); dfree(d_srank); dfree(d_csr_ptr); dfree(d_csr_pos);
         dfree(d_csr_loc); dfree(d_gstart); dfree(d_perm); dfree(d_contrib); dfree(d_apart);
+        dfree(d_vptr); dfree(d_vslot);
         perm.clear();
+        nlist = 0;
+        small_lds = 0;
     }
     ~Sys()
     {
@@ -1291,6 +1301,38 @@ static bool choose_layout(HostProblem& P, int esize)
     return false;
 }
 
+// Slot lists of the one-workgroup path, when its LDS footprint fits: for
+// every variable, the processing-order slot indices j*K + q of its
+// literals, clauses taken in input order and literals in clause order (the
+// order CpuSat::CalculateAllEx adds them, AnalogSAT/cpu_sat/CpuSat.cpp:206-229).
+static void build_small_lists(Sys* s, const HostProblem& P)
+{
+    const int n = P.n, m = P.m, K = P.k;
+    std::vector<int> vptr((size_t)n + 2, 0);
+    long long nl = 0;
+    for (long long i = 0; i < (long long)m * K; i++) {
+        const int l = P.lit[(size_t)i];
+        if (l) { vptr[(size_t)std::abs(l) + 1]++; nl++; }
+    }
+    const long long bytes = small_lds_bytes(n, m, K, s->alloc, nl, s->esize);
+    if (bytes > SMALL_LDS_MAX) return;
+    for (int v = 1; v <= n; v++) vptr[(size_t)v + 1] += vptr[v];
+    std::vector<int> fill(vptr.begin(), vptr.end() - 1), vslot((size_t)std::max<long long>(nl, 1), 0);
+    std::vector<int> proc((size_t)m);
+    for (int p = 0; p < m; p++) proc[(size_t)(P.perm.empty() ? p : P.perm[p])] = p;
+    for (int c = 0; c < m; c++) {
+        const int p = proc[(size_t)c];
+        for (int q = 0; q < K; q++) {
+            const int l = P.lit[(size_t)p * K + q];
+            if (l) vslot[(size_t)fill[(size_t)std::abs(l)]++] = p * K + q;
+        }
+    }
+    upload(s->d_vptr, vptr);
+    upload(s->d_vslot, vslot);
+    s->nlist = (int)nl;
+    s->small_lds = bytes;
+}
+
 static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclauses)
 {
     HostProblem P;
@@ -1335,6 +1377,7 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
     }
     else upload(s->d_csr_pos, P.csr_pos);
     upload(s->d_gstart, P.gstart);
+    build_small_lists(s, P);
     if (!P.perm.empty()) upload(s->d_perm, P.perm);
     s->perm = std::move(P.perm);
     const size_t ncontrib = (size_t)P.gstart[P.ngroups] + REGION_ALIGN;
@@ -1557,6 +1600,9 @@ struct RK {
     int npartial = 0;
     Ctrl* d_ctrl = nullptr;
     Ctrl* h_ctrl = nullptr;   // pinned mirror
+    int small_mode = 1;              // 1: one-workgroup path where the problem fits, 0: never
+    SmallSolve* d_solve = nullptr;   // device SatSolver loop state (one-workgroup path)
+    SmallSolve* h_solve = nullptr;   // pinned mirror
     void* pair[2] = {nullptr, nullptr};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = -1.f;
@@ -1581,8 +1627,9 @@ struct RK {
     {
         drop_graph();
         for (auto& p : k) dfree(p);
-        dfree(tmp); dfree(partial); dfree(d_ctrl);
+        dfree(tmp); dfree(partial); dfree(d_ctrl); dfree(d_solve);
         if (h_ctrl) { (void)hipHostFree(h_ctrl); h_ctrl = nullptr; }
+        if (h_solve) { (void)hipHostFree(h_solve); h_solve = nullptr; }
     }
     ~RK()
     {
@@ -1636,6 +1683,9 @@ static void rk_configure(RK* r, Sys* s)
     HIP_CHECK(hipMalloc((void**)&r->d_ctrl, sizeof(Ctrl)));
     HIP_CHECK(hipHostMalloc((void**)&r->h_ctrl, sizeof(Ctrl), hipHostMallocDefault));
     memset(r->h_ctrl, 0, sizeof(Ctrl));
+    HIP_CHECK(hipMalloc((void**)&r->d_solve, sizeof(SmallSolve)));
+    HIP_CHECK(hipHostMalloc((void**)&r->h_solve, sizeof(SmallSolve), hipHostMallocDefault));
+    memset(r->h_solve, 0, sizeof(SmallSolve));
     if (!r->ev0) HIP_CHECK(hipEventCreate(&r->ev0));
     if (!r->ev1) HIP_CHECK(hipEventCreate(&r->ev1));
     r->configured = true;
@@ -1774,18 +1824,178 @@ static int rk_step_impl(RK* r, State* olds, State* news, int count)
     return r->h_ctrl->underflow ? ASAT_UNDERFLOW : ASAT_OK;
 }
 
-static int rk_step(RK* r, State* olds, State* news, int count)
+// ---------------------------------------------------------------------------
+// one-workgroup path (asat_small.h): problems whose whole integrator fits in
+// one workgroup's LDS and that use no bias term.  The kernel timer (a
+// per-kernel measurement hook) needs the multi-kernel path.
+// ---------------------------------------------------------------------------
+static bool small_path(const RK* r)
 {
-    r->require_configured();
+    const Sys* s = r->sys;
+    return r->small_mode && s->small_lds > 0 && !(s->b > 0.0) && r->timer_kind < 0;
+}
+
+template <typename T, int KT, int BS>
+static void small_kernel_launch(const SmallArgs& a, size_t lds, hipStream_t st)
+{
+    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_small<T, KT, BS>),
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, SMALL_LDS_MAX);
+    HIP_CHECK(attr);
+    hipLaunchKernelGGL((k_small<T, KT, BS>), dim3(1), dim3(BS), lds, st, a);
+    HIP_CHECK(hipGetLastError());
+}
+
+// 256 threads (a clause and a variable each) up to 512 clauses and
+// variables, 1024 threads above
+template <typename T>
+static void small_launch(const Sys* s, const SmallArgs& a, hipStream_t st)
+{
+    const size_t lds = (size_t)s->small_lds;
+    const bool wide = std::max(s->m, s->n) > 512;
+    if (s->k == 3) {
+        if (wide) small_kernel_launch<T, 3, 1024>(a, lds, st);
+        else small_kernel_launch<T, 3, 256>(a, lds, st);
+    } else if (s->k == 4) {
+        if (wide) small_kernel_launch<T, 4, 1024>(a, lds, st);
+        else small_kernel_launch<T, 4, 256>(a, lds, st);
+    } else {
+        if (wide) small_kernel_launch<T, 0, 1024>(a, lds, st);
+        else small_kernel_launch<T, 0, 256>(a, lds, st);
+    }
+}
+
+static SmallArgs small_args(const RK* r, void* x)
+{
+    const Sys* s = r->sys;
+    SmallArgs a;
+    memset(&a, 0, sizeof(a));
+    a.lit = s->d_lit; a.vptr = s->d_vptr; a.vslot = s->d_vslot;
+    a.x = x;
+    a.ctrl = r->d_ctrl;
+    a.n = s->n; a.m = s->m; a.k = s->k; a.aoff = s->aoff; a.alloc = s->alloc; a.nlist = s->nlist;
+    a.family = s->family;
+    a.q = s->q;
+    a.knorm = std::pow(2.0, -s->k);
+    a.batch = 1;
+    return a;
+}
+
+// Launches the persistent kernel until it reports done: each launch runs at
+// most SMALL_ATTEMPT_CAP attempts and leaves its state in the control block
+// (and the solver block), so the next launch resumes where it stopped.
+template <typename T>
+static void small_run(RK* r, const SmallArgs& a)
+{
+    Sys* s = r->sys;
+    hipStream_t st = s->stream;
+    HIP_CHECK(hipEventRecord(r->ev0, st));
+    long long last = r->h_ctrl->attempts;
+    for (;;) {
+        small_launch<T>(s, a, st);
+        HIP_CHECK(hipMemcpyAsync(r->h_ctrl, r->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+        if (a.solve) HIP_CHECK(hipMemcpyAsync(r->h_solve, r->d_solve, sizeof(SmallSolve), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        if (r->h_ctrl->underflow) break;
+        if (a.solve ? r->h_solve->result >= 0 : r->h_ctrl->steps >= a.target) break;
+        require(r->h_ctrl->attempts > last, ASAT_ERR_HIP, "integrator made no progress");
+        last = r->h_ctrl->attempts;
+    }
+    HIP_CHECK(hipEventRecord(r->ev1, st));
+    HIP_CHECK(hipEventSynchronize(r->ev1));
+    HIP_CHECK(hipEventElapsedTime(&r->last_ms, r->ev0, r->ev1));
+}
+
+static void rk_check_states(const RK* r, const State* olds, const State* news)
+{
     state_check(olds); state_check(news);
     require(olds != news && olds->d != news->d, ASAT_ERR_INVALID_ARGUMENT, "old and new state must differ");
     require(olds->sys == r->sys && news->sys == r->sys, ASAT_ERR_INVALID_ARGUMENT,
             "states belong to another system");
+}
+
+static int rk_step(RK* r, State* olds, State* news, int count)
+{
+    r->require_configured();
+    rk_check_states(r, olds, news);
     require(count >= 0, ASAT_ERR_INVALID_ARGUMENT, "negative step count");
     if (r->h_ctrl->underflow) return ASAT_UNDERFLOW;
     if (count == 0) return ASAT_OK;
+    if (small_path(r)) {
+        // the state stays in olds; news is not touched
+        SmallArgs a = small_args(r, olds->d);
+        a.target = r->h_ctrl->steps + count;
+        if (r->prec == ASAT_F64) small_run<double>(r, a);
+        else small_run<float>(r, a);
+        return r->h_ctrl->underflow ? ASAT_UNDERFLOW : ASAT_OK;
+    }
     return r->prec == ASAT_F64 ? rk_step_impl<double>(r, olds, news, count)
                                : rk_step_impl<float>(r, olds, news, count);
+}
+
+// SatSolver::DoSolve without a callback (AnalogSAT/solver/SatSolver.h:313-331):
+// batches of `batch` accepted steps, the violated-clause count of `olds`
+// after each, ending on SAT, underflow, or when the time (compared as a
+// float if time_float) or the solver's step count reaches its limit before a
+// batch.  The solver's counters start at zero; the integrator's continue
+// from where they are (SatSolver::Solve configures it first).  On the
+// one-workgroup path the whole loop runs in the persistent kernel; otherwise
+// it runs here on the host over rk_step and the violation kernel.
+static void rk_solve(RK* r, State* olds, State* news, int batch, long long max_steps, double max_time,
+                     int time_float, asat_solve_result_t* out)
+{
+    r->require_configured();
+    rk_check_states(r, olds, news);
+    require(batch >= 1, ASAT_ERR_INVALID_ARGUMENT, "Batch size must be positive");
+    require(max_steps >= 1, ASAT_ERR_INVALID_ARGUMENT, "Max iterations must be positive");
+    require(max_time > 0.0, ASAT_ERR_INVALID_ARGUMENT, "Max time must be positive");
+    memset(out, 0, sizeof(*out));
+    if (small_path(r) && !r->h_ctrl->underflow) {
+        SmallSolve init;
+        memset(&init, 0, sizeof(init));
+        init.result = -1;
+        *r->h_solve = init;
+        HIP_CHECK(hipMemcpyAsync(r->d_solve, r->h_solve, sizeof(SmallSolve), hipMemcpyHostToDevice, r->sys->stream));
+        SmallArgs a = small_args(r, olds->d);
+        a.solve = r->d_solve;
+        a.batch = batch;
+        a.max_steps = max_steps;
+        a.max_time = max_time;
+        a.time_float = time_float;
+        if (r->prec == ASAT_F64) small_run<double>(r, a);
+        else small_run<float>(r, a);
+        out->result = r->h_solve->result;
+        out->steps = r->h_solve->solver_steps;
+        out->last_violations = r->h_solve->last_viol;
+        out->on_device = 1;
+        return;
+    }
+    auto tcmp = [&](double t) { return time_float ? (double)(float)t : t; };
+    long long steps = 0;
+    double time = 0.0;
+    int lastv = 0, result = -1;
+    while (tcmp(time) < max_time && steps < max_steps) {
+        const long long before = r->h_ctrl->steps;
+        if (rk_step(r, olds, news, batch) == ASAT_UNDERFLOW) {
+            result = SMALL_UNDERFLOW;
+            break;
+        }
+        steps += r->h_ctrl->steps - before;
+        time = r->h_ctrl->time;
+        lastv = r->prec == ASAT_F64 ? sys_violations<double>(r->sys, olds) : sys_violations<float>(r->sys, olds);
+        if (lastv == 0) {
+            result = SMALL_SAT;
+            break;
+        }
+    }
+    if (result < 0) {
+        if (tcmp(time) >= max_time) result = SMALL_MAXTIME;
+        else if (steps >= max_steps) result = SMALL_MAXITER;
+        else result = 5;   // SAT_UNKNOWN
+    }
+    out->result = result;
+    out->steps = steps;
+    out->last_violations = lastv;
+    out->on_device = 0;
 }
 
 // Times each kernel kind of one attempt over `reps` launches with HIP events
@@ -2232,6 +2442,32 @@ int asat_rk_step(asat_rk_t* rk, asat_state_t* old_state, asat_state_t* new_state
         asat::g_last_error = "step size underflow";
         return rc;
     }
+    API_END
+}
+
+int asat_rk_set_one_workgroup(asat_rk_t* rk, int enable)
+{
+    API_BEGIN
+    CHECK_PTR(rk);
+    rk->r.small_mode = enable ? 1 : 0;
+    API_END
+}
+
+int asat_rk_uses_one_workgroup(const asat_rk_t* rk, int* uses)
+{
+    API_BEGIN
+    CHECK_PTR(rk); CHECK_PTR(uses);
+    rk->r.require_configured();
+    *uses = small_path(&rk->r) ? 1 : 0;
+    API_END
+}
+
+int asat_rk_solve(asat_rk_t* rk, asat_state_t* old_state, asat_state_t* new_state, int batch, long long max_steps,
+                  double max_time, int time_float, asat_solve_result_t* out)
+{
+    API_BEGIN
+    CHECK_PTR(rk); CHECK_PTR(old_state); CHECK_PTR(new_state); CHECK_PTR(out);
+    rk_solve(&rk->r, &old_state->st, &new_state->st, batch, max_steps, max_time, time_float, out);
     API_END
 }
 

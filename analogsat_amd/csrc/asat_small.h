@@ -91,9 +91,8 @@ __device__ __forceinline__ void small_clause(const SmallArgs& a, const SmallLds<
         const int* lm = L.lit + j * K;
         T f[KM];
         T km = knorm;
-#pragma unroll
-        for (int q = 0; q < KM; q++) {
-            if (KT == 0 && q >= K) break;
+#pragma unroll 4
+        for (int q = 0; q < K; q++) {
             const T fq = spin_factor<T>(lm[q], sp_in);
             km *= fq;
             f[q] = fq;
@@ -128,16 +127,14 @@ __device__ __forceinline__ void small_clause(const SmallArgs& a, const SmallLds<
         if (a.family == ASAT_FAMILY_TANH) {
             const T r = knorm * am;
             const bool ex = K >= 3 && K <= 10;
-#pragma unroll
-            for (int q = 0; q < KM; q++) {
-                if (KT == 0 && q >= K) break;
+#pragma unroll 4
+            for (int q = 0; q < K; q++) {
                 const int l = lm[q];
                 if (l == 0) break;
                 T p = (T)1.0;
                 bool first = true;
-#pragma unroll
-                for (int qq = 0; qq < KM; qq++) {
-                    if (KT == 0 && qq >= K) break;
+#pragma unroll 4
+                for (int qq = 0; qq < K; qq++) {
                     if (qq == q) continue;
                     p = first ? f[qq] : p * f[qq];
                     first = false;
@@ -146,16 +143,14 @@ __device__ __forceinline__ void small_clause(const SmallArgs& a, const SmallLds<
             }
         } else if (K <= 10) {
             const T r = km * knorm * am;
-#pragma unroll
-            for (int q = 0; q < KM; q++) {
-                if (KT == 0 && q >= K) break;
+#pragma unroll 4
+            for (int q = 0; q < K; q++) {
                 const int l = lm[q];
                 if (l == 0) break;
                 T p = (T)1.0;
                 bool first = true;
-#pragma unroll
-                for (int qq = 0; qq < KM; qq++) {
-                    if (KT == 0 && qq >= K) break;
+#pragma unroll 4
+                for (int qq = 0; qq < K; qq++) {
                     if (qq == q) continue;
                     p = first ? f[qq] : p * f[qq];
                     first = false;

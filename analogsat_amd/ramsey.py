@@ -279,7 +279,10 @@ class RamseyRunner:
         self.out.write("ANALOGSAT GPUv%d:\n" % self.conf["version"])
         self.out.write("Running on GPU\n  %d-SAT\n  clauses: %d\n  variables: %d\n\n"
                        % (p.Get_K(), p.Get_M(), p.Get_N()))
-        solved, solver = solve_on_gpu(self.conf, p, self.rng, self.out, self.problem_name, self.outfile)
+        # RamseyRunner::RunAnalogsatGpu always instantiates the <double>
+        # solver (Frontend/src/ramsey_runner.cpp:219-305), so -float is ignored
+        conf = dict(self.conf, float=False)
+        solved, solver = solve_on_gpu(conf, p, self.rng, self.out, self.problem_name, self.outfile)
         self.solution = solver.GetSatState().CopyTo()
         self.last_solve_time = solver.GetElapsedWallTime()
         return solved

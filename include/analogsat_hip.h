@@ -196,6 +196,35 @@ int asat_rk_configure(asat_rk_t* rk, asat_sys_t* sys);
    holds the newest state (the two device buffers are swapped as needed).
    Returns ASAT_UNDERFLOW if the step size underflowed. */
 int asat_rk_step(asat_rk_t* rk, asat_state_t* old_state, asat_state_t* new_state, int count);
+/* No reference counterpart.  Problems whose whole integrator fits in one
+   workgroup's LDS (about 150 KB: 7 state vectors, the spin stage input, the
+   slot contributions, literals and slot lists) and that use no bias term run
+   Step and asat_rk_solve in one persistent single-workgroup kernel, with the
+   same arithmetic as the multi-kernel path (bit-identical results).
+   enable = 0 forces the multi-kernel path (13 launches per attempt);
+   asat_rk_uses_one_workgroup reports which path the configured problem takes. */
+int asat_rk_set_one_workgroup(asat_rk_t* rk, int enable);
+int asat_rk_uses_one_workgroup(const asat_rk_t* rk, int* uses);
+/* SatSolver::DoSolve's loop without a callback (AnalogSAT/solver/SatSolver.h:313-331):
+   while (time < max_time && steps < max_steps) { Step(batch); count the
+   violated clauses of old_state; stop on 0 }, ending with SAT_UNDERFLOW on
+   step-size underflow.  `time` is the integrator's time, compared as a float
+   when time_float != 0 (TFloat = float); the solver's step count starts at 0,
+   the integrator's counters continue (SatSolver::Solve configures the
+   integrator first).  Problems whose whole integrator fits in one
+   workgroup's LDS (and use no bias term) run the loop inside one persistent
+   kernel with no host round trip per batch; others run it on the host over
+   asat_rk_step and asat_sys_violations.  Same results either way.  No
+   reference counterpart as one call (the reference loops on the host). */
+typedef struct {
+    int result;              /* SatResult code (AnalogSAT/solver/SatResult.h:24-32) */
+    int last_violations;     /* SatSolver::lastViolationCount: after the last completed batch, 0 before */
+    long long steps;         /* SatSolver::steps: accepted steps of the completed batches */
+    int on_device;           /* 1: the loop ran inside the persistent kernel */
+    int pad_;
+} asat_solve_result_t;
+int asat_rk_solve(asat_rk_t* rk, asat_state_t* old_state, asat_state_t* new_state, int batch, long long max_steps,
+                  double max_time, int time_float, asat_solve_result_t* out);
 /* IODEInt::Reset */
 int asat_rk_reset(asat_rk_t* rk);
 int asat_rk_stats(const asat_rk_t* rk, asat_rk_stats_t* out);

@@ -228,6 +228,19 @@ public:
     int GetStepCount() const override { return (int)stats().steps; }
     int GetAttemptCount() const override { return (int)stats().attempts; }
     void Reset() override { HipSafe(asat_rk_reset(h)); }
+    bool SolveBatches(IODEState<TFloat, TState>& oldState, IODEState<TFloat, TState>& newState, int batch,
+                      int maxSteps, TFloat maxTime, SatResult& result, int& steps, int& lastViolations) override
+    {
+        auto& o = dynamic_cast<HipSatState<TFloat>&>(oldState);
+        auto& n = dynamic_cast<HipSatState<TFloat>&>(newState);
+        asat_solve_result_t r;
+        HipSafe(asat_rk_solve(h, o.handle(), n.handle(), batch, maxSteps, (double)maxTime,
+                              sizeof(TFloat) == sizeof(float) ? 1 : 0, &r));
+        result = (SatResult)r.result;
+        steps = (int)r.steps;
+        lastViolations = r.last_violations;
+        return true;
+    }
     TFloat GetEpsilon() const
     {
         double e = 0;
@@ -237,6 +250,9 @@ public:
     void SetEpsilon(TFloat eps) { HipSafe(asat_rk_set_eps(h, (double)eps)); }
     // constant term of the error scale (reference: TINY = 1e-20, the default)
     void SetErrorFloor(double floor) { HipSafe(asat_rk_set_xscal_floor(h, floor)); }
+    // allow (default) or forbid the one-workgroup persistent path for
+    // problems whose integrator fits in one workgroup's LDS
+    void SetOneWorkgroup(bool enable) { HipSafe(asat_rk_set_one_workgroup(h, enable ? 1 : 0)); }
 
 private:
     asat_rk_t* h = nullptr;

@@ -111,6 +111,19 @@ public:
     virtual int GetStepCount() const = 0;
     virtual int GetAttemptCount() const = 0;
     virtual void Reset() = 0;
+    // No reference counterpart: SatSolver's loop without a callback
+    // (AnalogSAT/solver/SatSolver.h:313-331) run by the integrator itself.
+    // Batches of `batch` accepted steps, the violated-clause count of
+    // oldState after each; fills result, the solver's step count and the
+    // last count, and returns true.  The default returns false, and
+    // SatSolver then runs the loop on the host.
+    virtual bool SolveBatches(IODEState<TFloat, TState>& oldState, IODEState<TFloat, TState>& newState, int batch,
+                              int maxSteps, TFloat maxTime, SatResult& result, int& steps, int& lastViolations)
+    {
+        (void)oldState; (void)newState; (void)batch; (void)maxSteps; (void)maxTime; (void)result; (void)steps;
+        (void)lastViolations;
+        return false;
+    }
 };
 
 template <class TFloat>

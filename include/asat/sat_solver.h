@@ -83,6 +83,17 @@ public:
         wallstart = std::chrono::steady_clock::now();
         cpustart = std::clock();
         SatResult result = SAT_UNKNOWN;
+        // without a callback the integrator may run the whole loop itself
+        if (!(useCallback && callback) &&
+            odeint.SolveBatches(*oldState, *newState, batchSize, maxSteps, maxTime, result, steps, lastViolationCount)) {
+            walltime = std::chrono::duration<double>(std::chrono::steady_clock::now() - wallstart).count();
+            cputime = double(std::clock() - cpustart) / CLOCKS_PER_SEC;
+            time = odeint.GetTime();
+            rhsCount = odeint.GetRHSCount();
+            lastStepSize = odeint.GetLastStepSize();
+            running = false;
+            return result;
+        }
         while (time < maxTime && steps < maxSteps) {
             int before = odeint.GetStepCount();
             if (!odeint.Step(*oldState, *newState, batchSize)) {

@@ -2,11 +2,14 @@
 // (Frontend/src/cnf_runner.cpp:28-127, SolveOnGpu) written against the MI355X
 // headers; only the class names differ (CudaSat1 -> HipSat, CudaRungeKutta ->
 // HipRungeKutta, CudaSatSolver -> HipSatSolver, CudaRandom -> HipRandom).
-// Usage: dropin_run <file.cnf> <eps> <batch> <stepmax> [public-start-file]
+// Usage: dropin_run <file.cnf> <eps> <batch> <stepmax> [public-start-file] [nocallback]
 // Prints: result, steps, rhs count, violations, then the final spins' signs.
+// With "nocallback" no callback is set (the reference's other branch of
+// SatSolver::DoSolve); the minimum violation and batch counts print as -1.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <string>
 #include <vector>
 
 #include "asat/hip_sat.h"
@@ -38,6 +41,7 @@ int main(int argc, char** argv)
         solver.SetMaxSteps(std::atoi(argv[4]));
         solver.SetBatchSize(std::atoi(argv[3]));
         int n = problem.Get_N(), m = problem.Get_M();
+        const bool callback = !(argc > 6 && std::string(argv[6]) == "nocallback");
         if (argc > 5) {
             std::vector<double> start(n + m);
             std::ifstream in(argv[5]);
@@ -48,12 +52,16 @@ int main(int argc, char** argv)
             solver.SetRandomInitialState(rand);
         }
         int minViolation = m, batches = 0;
-        solver.SetCallback([&](HipSatSolver<double>::Interface& s) -> bool {
-            int v = s.GetLastClauseViolationCount();
-            if (v < minViolation) minViolation = v;
-            batches++;
-            return true;
-        });
+        if (callback) {
+            solver.SetCallback([&](HipSatSolver<double>::Interface& s) -> bool {
+                int v = s.GetLastClauseViolationCount();
+                if (v < minViolation) minViolation = v;
+                batches++;
+                return true;
+            });
+        } else {
+            minViolation = batches = -1;
+        }
         SatResult result = solver.Solve();
         std::vector<double> last;
         solver.GetSatState().CopyTo(last);

@@ -9,8 +9,8 @@ Bar (all through the C ABI):
     rounded (the GPU definition); on the committed fixture that oracle equals
     the reference's own CPU trajectory at every step, so the GPU does too
   * FP32-state trajectories: bit-identical to the oracle's mixed mode; the
-    distance to the FP64 oracle is printed (chaotic dynamics amplify the
-    float rounding, so no bound is asserted on it)
+    distance to the FP64 oracle (adaptive, its own step sizes) is printed at
+    steps 10 and 100 and bounded by 1e-5 at step 1000
 """
 import os
 import sys
@@ -157,6 +157,14 @@ def test_trajectory_f32_vs_oracle_mixed_and_f64(golden_dir):
                 ref = o64.get_state()
                 rel = np.abs(x - ref).max() / np.abs(ref).max()
                 say("step %d: max rel err vs FP64 oracle = %.3e (h %s)" % (s, rel, rk.stats().hdid == o64.hdid))
+                # The two runs choose their own step sizes, so before they
+                # settle step s of one is at another analog time than step s
+                # of the other (5.7e-5 apart at step 100 on this start); at
+                # step 1000 both sit on the same fixed point.  The matched-
+                # step bound is pinned with frozen step sizes in
+                # test_fp32_with_fp64_steps_within_1e5_of_fp64_oracle.
+                if s == 1000:
+                    assert rel <= 1e-5, rel
 
 
 def test_satsolver_loop_matches_reference(golden_dir):
