@@ -335,31 +335,6 @@ struct TCtl {
     int steps, attempts, active, underflow, accept, ckpt;   // ckpt: checkpoint slot to write on accept, or -1
 };
 
-// |xerr / xscal| per element, block maxima per trajectory; xerr in FP64 from
-// the FP32 stage derivatives, xscal = |x| + |k1 hnext| + TINY (as the plain
-// solver's stage-6 passes)
-__global__ __launch_bounds__(BLK) void k_err_parts(Dev d, const float* X, const float* k1, const float* k3,
-                                                   const float* k4, const float* k5, const float* k6,
-                                                   const TCtl* ctl, double* parts, int nblk)
-{
-    const TB tb = traj_block();
-    __shared__ double sh[BLK / 64];
-    const long long i = (long long)tb.x * BLK + threadIdx.x;
-    const long long tr = tb.tr;
-    if (tr >= d.ntraj) return;
-    double e = 0.0;
-    if (i <= (long long)d.n + d.m) {
-        const long long o = tr * d.D + i;
-        const double h = ctl[tr].h, hn = ctl[tr].hnext;
-        const double xerr = h * (TabD::dc1 * (double)k1[o] + TabD::dc3 * (double)k3[o] + TabD::dc4 * (double)k4[o] +
-                                 TabD::dc5 * (double)k5[o] + TabD::dc6 * (double)k6[o]);
-        const double xscal = fabs((double)X[o]) + fabs((double)k1[o] * hn) + TabD::TINY;
-        e = fabs(xerr / xscal);
-    }
-    const double bm = block_max<BLK>(e, sh);
-    if (threadIdx.x == 0) parts[tr * nblk + tb.x] = bm;
-}
-
 // per-trajectory controller (the plain solver's k_control); one block each
 __global__ __launch_bounds__(BLK) void k_ctl_batch(TCtl* ctl, const double* parts, int nblk, double eps, int T,
                                                    int C, double* hlog, double* hcur)
@@ -431,6 +406,153 @@ __global__ void k_ctl_init(TCtl* ctl, double* hcur, int ntraj)
     hcur[tr] = 1e-9;
 }
 
+// ---- fused stages (FP32 state): the plain solver's stage fusion ------------
+// Stage J of every trajectory in two passes, with the stage inputs formed
+// inside them (the plain solver's k_clause_gen / var_stage order), so there
+// are no stage-input, final-combination or error passes of their own:
+//   clause pass J: K_m from the stage spins, the aux stage input from x and
+//     k_1..k_{J-1}, k_J(a) = a K^2 (J = 6: x_new of the aux element, and in
+//     the adaptive run |xerr/xscal|), slot terms at their var-major positions
+//   var pass J:    each variable's terms added in slot order, k_J(s), then
+//     stage J+1's spin input (J = 6: x_new, and |xerr/xscal|)
+// FZ = false: the adaptive lockstep run (h from the control block, inactive
+// trajectories skipped, spin inputs in one buffer S, the J = 6 passes leave
+// one error maximum per block in parts[tr]: clause blocks, then var blocks).
+// FZ = true: a frozen step with the logged h of every trajectory; the six
+// stage inputs Y_1..Y_6 of every element are kept (Y_1 = x) for the
+// adjoint, and the spins of Y_J are stage J's spin input.
+struct FArgs {
+    Dev d;
+    const float* X;
+    float* Xn;            // null: no x_new (frozen step whose successor is not needed)
+    float* k[6];          // k1..k6 (k6 is not stored)
+    float* S;             // adaptive: spin input of the next stage
+    float* Y;             // frozen: [6][ntraj][D] stage inputs
+    float* c;             // [ntraj][L] slot terms, var-major
+    long long V;          // ntraj * D
+    const double* hcol;   // frozen: step size of every trajectory
+    const TCtl* ctl;      // adaptive: control blocks
+    double* parts;
+    int nblk_c, nblk_v;
+};
+
+template <int K, int J, bool FZ>
+__global__ __launch_bounds__(BLK) void k_fclause(FArgs a)
+{
+    __shared__ double sh[BLK / 64];
+    const TB tb = traj_block();
+    const long long tr = tb.tr;
+    if (tr >= a.d.ntraj) return;
+    if (!FZ && !a.ctl[tr].active) return;
+    const int m = tb.x * BLK + threadIdx.x;
+    const double h = FZ ? a.hcol[tr] : a.ctl[tr].h;
+    const double hnext = FZ ? h : a.ctl[tr].hnext;
+    const float hs = (float)h;
+    const long long row = tr * a.d.D;
+    const float* x = a.X + row;
+    const float* sp = J == 1 ? x : (FZ ? a.Y + (J - 1) * a.V : a.S) + row;
+    double err = 0.0;
+    if (m < a.d.m) {
+        int l[K];
+        float f[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) l[q] = a.d.lit[(size_t)m * K + q];
+        float km = (float)(1.0 / (1 << K));
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            f[q] = spin_factor<float>(l[q], sp);
+            km *= f[q];
+        }
+        const long long ai = a.d.n + 1 + m;
+        float am, k1 = 0.f, k3 = 0.f, k4 = 0.f, k5 = 0.f;
+        if constexpr (J == 1) {
+            am = x[ai];
+        } else {
+            k1 = a.k[0][row + ai];
+            const float k2 = J > 2 ? a.k[1][row + ai] : 0.f;
+            k3 = J > 3 ? a.k[2][row + ai] : 0.f;
+            k4 = J > 4 ? a.k[3][row + ai] : 0.f;
+            k5 = J > 5 ? a.k[4][row + ai] : 0.f;
+            am = stage_input<float, J>(x[ai], k1, k2, k3, k4, k5, hs);
+        }
+        if constexpr (FZ) a.Y[(J - 1) * a.V + row + ai] = am;
+        const float kj = am * km * km;
+        if constexpr (J < 6) {
+            a.k[J - 1][row + ai] = kj;
+        } else if (a.Xn) {
+            float xn;
+            final_combination<float>(x[ai], k1, k3, k4, k5, kj, hs, h, hnext, TabD::TINY, xn, err);
+            a.Xn[row + ai] = xn;
+            if (m == 0) a.Xn[row] = -1.0f;
+        }
+        const float r = km * (float)(1.0 / (1 << K)) * am;
+        float o[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) o[q] = 0.f;
+        write_contribs<float, K, 0>(l, f, r, o);
+        float* cr = a.c + tr * a.d.L;
+#pragma unroll
+        for (int q = 0; q < K; q++)
+            if (l[q] != 0) cr[a.d.slot_pos[(size_t)m * K + q]] = o[q];
+    }
+    if constexpr (J == 6 && !FZ) {
+        const double bm = block_max<BLK>(err, sh);
+        if (threadIdx.x == 0) a.parts[tr * (a.nblk_c + a.nblk_v) + tb.x] = bm;
+    }
+}
+
+template <int J, bool FZ>
+__global__ __launch_bounds__(BLK) void k_fvar(FArgs a)
+{
+    __shared__ double sh[BLK / 64];
+    const TB tb = traj_block();
+    const long long tr = tb.tr;
+    if (tr >= a.d.ntraj) return;
+    if (!FZ && !a.ctl[tr].active) return;
+    const int i = tb.x * BLK + threadIdx.x + 1;
+    const double h = FZ ? a.hcol[tr] : a.ctl[tr].h;
+    const double hnext = FZ ? h : a.ctl[tr].hnext;
+    const float hs = (float)h;
+    const long long row = tr * a.d.D;
+    double err = 0.0;
+    if (i <= a.d.n) {
+        const float* cr = a.c + tr * a.d.L;
+        float sum = 0.f;
+        for (int e = a.d.csr_ptr[i]; e < a.d.csr_ptr[i + 1]; e++) sum += cr[e];
+        const float kj = sum;
+        const long long o = row + i;
+        const float xi = a.X[o];
+        // stage J+1's spin input goes to S, or to Y_{J+1} in a frozen step
+        float* nxt = FZ ? a.Y + J * a.V : a.S;
+        if constexpr (FZ && J == 1) a.Y[o] = xi;
+        if constexpr (J == 1) {
+            a.k[0][o] = kj;
+            nxt[o] = stage_input<float, 2>(xi, kj, 0.f, 0.f, 0.f, 0.f, hs);
+        } else if constexpr (J == 2) {
+            a.k[1][o] = kj;
+            nxt[o] = stage_input<float, 3>(xi, a.k[0][o], kj, 0.f, 0.f, 0.f, hs);
+        } else if constexpr (J == 3) {
+            a.k[2][o] = kj;
+            nxt[o] = stage_input<float, 4>(xi, a.k[0][o], a.k[1][o], kj, 0.f, 0.f, hs);
+        } else if constexpr (J == 4) {
+            a.k[3][o] = kj;
+            nxt[o] = stage_input<float, 5>(xi, a.k[0][o], a.k[1][o], a.k[2][o], kj, 0.f, hs);
+        } else if constexpr (J == 5) {
+            a.k[4][o] = kj;
+            nxt[o] = stage_input<float, 6>(xi, a.k[0][o], a.k[1][o], a.k[2][o], a.k[3][o], kj, hs);
+        } else if (a.Xn) {
+            float xn;
+            final_combination<float>(xi, a.k[0][o], a.k[2][o], a.k[3][o], a.k[4][o], kj, hs, h, hnext, TabD::TINY,
+                                     xn, err);
+            a.Xn[o] = xn;
+        }
+    }
+    if constexpr (J == 6 && !FZ) {
+        const double bm = block_max<BLK>(err, sh);
+        if (threadIdx.x == 0) a.parts[tr * (a.nblk_c + a.nblk_v) + a.nblk_c + tb.x] = bm;
+    }
+}
+
 // ---------------------------------------------------------------------------
 struct Adj {
     int n = 0, m = 0, k = 0;
@@ -446,6 +568,7 @@ struct Adj {
     float* ckpt = nullptr;       // [ncheck][ntraj][D]
     float* seg = nullptr;        // [C+1][ntraj][D]
     float* Y = nullptr;          // [6][ntraj][D]
+    float* Yseg = nullptr;       // [C][6][ntraj][D] stage inputs of a segment's steps (backward)
     float* kk = nullptr;         // [6][ntraj][D]
     float* c32 = nullptr;        // [ntraj][L]
     double* c64 = nullptr;       // [ntraj][L]
@@ -467,11 +590,11 @@ struct Adj {
 
     void free_batch()
     {
-        for (void* p : {(void*)ckpt, (void*)seg, (void*)Y, (void*)kk, (void*)c32, (void*)c64, (void*)lam,
+        for (void* p : {(void*)ckpt, (void*)seg, (void*)Y, (void*)Yseg, (void*)kk, (void*)c32, (void*)c64, (void*)lam,
                         (void*)mu, (void*)nu, (void*)hlog, (void*)hcol, (void*)hcolr, (void*)vpart, (void*)dV,
                         (void*)ctl, (void*)hstep, (void*)eparts})
             (void)hipFree(p);
-        ckpt = seg = Y = kk = c32 = nullptr;
+        ckpt = seg = Y = Yseg = kk = c32 = nullptr;
         c64 = lam = mu = nu = hlog = hcol = hcolr = vpart = dV = hstep = eparts = nullptr;
         ctl = nullptr;
     }
@@ -552,6 +675,7 @@ static void alloc_batch(Adj& A, int ntraj)
     HIP_CHECK(hipMalloc((void**)&A.ckpt, sizeof(float) * V * A.ncheck));
     HIP_CHECK(hipMalloc((void**)&A.seg, sizeof(float) * V * (A.C + 1)));
     HIP_CHECK(hipMalloc((void**)&A.Y, sizeof(float) * V * 6));
+    HIP_CHECK(hipMalloc((void**)&A.Yseg, sizeof(float) * V * 6 * A.C));
     HIP_CHECK(hipMalloc((void**)&A.kk, sizeof(float) * V * 6));
     HIP_CHECK(hipMalloc((void**)&A.c32, sizeof(float) * (size_t)ntraj * A.L));
     HIP_CHECK(hipMalloc((void**)&A.c64, sizeof(double) * (size_t)ntraj * A.L));
@@ -566,7 +690,7 @@ static void alloc_batch(Adj& A, int ntraj)
     HIP_CHECK(hipMalloc((void**)&A.dV, sizeof(double) * ntraj));
     HIP_CHECK(hipMalloc((void**)&A.ctl, sizeof(TCtl) * ntraj));
     HIP_CHECK(hipMalloc((void**)&A.hstep, sizeof(double) * ntraj));
-    HIP_CHECK(hipMalloc((void**)&A.eparts, sizeof(double) * (size_t)ntraj * ceil_div((long long)A.n + A.m + 1, BLK)));
+    HIP_CHECK(hipMalloc((void**)&A.eparts, sizeof(double) * (size_t)ntraj * (ceil_div(A.m, BLK) + ceil_div(A.n, BLK))));
     HIP_CHECK(hipMemset(A.seg, 0, sizeof(float) * V * (A.C + 1)));
     HIP_CHECK(hipMemset(A.Y, 0, sizeof(float) * V * 6));
     HIP_CHECK(hipMemset(A.kk, 0, sizeof(float) * V * 6));
@@ -613,11 +737,6 @@ static void frozen_step(Adj& A, const T* X, T* Xn, T* Y, T* kk, T* contrib)
     }
 }
 
-// the FP32 recompute of the forward, into the batch's own buffers
-static void frozen_step(Adj& A, const float* X, float* Xn)
-{
-    frozen_step<float>(A, X, Xn, A.Y, A.kk, A.c32);
-}
 
 // V (and grad V into lam if want_grad) at X for every trajectory
 template <typename TX>
@@ -636,6 +755,53 @@ static void energy(Adj& A, const TX* X, bool want_grad)
     hipLaunchKernelGGL(k_sum_parts, dim3(A.ntraj), dim3(64), 0, A.stream, (const double*)A.vpart, nblk, A.dV);
 }
 
+// one attempt (adaptive) or one frozen step of every trajectory: six fused stages
+template <int K, bool FZ>
+static void fused_stages(Adj& A, const FArgs& fa)
+{
+    const dim3 gc(fa.nblk_c, ntraj8(A.ntraj)), gv(fa.nblk_v, ntraj8(A.ntraj));
+    hipLaunchKernelGGL((k_fclause<K, 1, FZ>), gc, dim3(BLK), 0, A.stream, fa);
+    hipLaunchKernelGGL((k_fvar<1, FZ>), gv, dim3(BLK), 0, A.stream, fa);
+    hipLaunchKernelGGL((k_fclause<K, 2, FZ>), gc, dim3(BLK), 0, A.stream, fa);
+    hipLaunchKernelGGL((k_fvar<2, FZ>), gv, dim3(BLK), 0, A.stream, fa);
+    hipLaunchKernelGGL((k_fclause<K, 3, FZ>), gc, dim3(BLK), 0, A.stream, fa);
+    hipLaunchKernelGGL((k_fvar<3, FZ>), gv, dim3(BLK), 0, A.stream, fa);
+    hipLaunchKernelGGL((k_fclause<K, 4, FZ>), gc, dim3(BLK), 0, A.stream, fa);
+    hipLaunchKernelGGL((k_fvar<4, FZ>), gv, dim3(BLK), 0, A.stream, fa);
+    hipLaunchKernelGGL((k_fclause<K, 5, FZ>), gc, dim3(BLK), 0, A.stream, fa);
+    hipLaunchKernelGGL((k_fvar<5, FZ>), gv, dim3(BLK), 0, A.stream, fa);
+    hipLaunchKernelGGL((k_fclause<K, 6, FZ>), gc, dim3(BLK), 0, A.stream, fa);
+    hipLaunchKernelGGL((k_fvar<6, FZ>), gv, dim3(BLK), 0, A.stream, fa);
+}
+
+static FArgs fused_args(Adj& A, const float* X, float* Xn)
+{
+    FArgs fa;
+    memset(&fa, 0, sizeof(fa));
+    fa.d = A.dev();
+    fa.X = X;
+    fa.Xn = Xn;
+    const size_t V = (size_t)A.ntraj * A.D;
+    for (int i = 0; i < 6; i++) fa.k[i] = A.kk + i * V;
+    fa.V = (long long)V;
+    fa.c = A.c32;
+    fa.nblk_c = ceil_div(A.m, BLK);
+    fa.nblk_v = ceil_div(A.n, BLK);
+    return fa;
+}
+
+// the FP32 frozen recompute of one step with the current h column: the six
+// stage inputs into Y (the adjoint's linearisation points), x_next into Xn
+// unless null
+static void frozen_step_fused(Adj& A, const float* X, float* Xn, float* Y)
+{
+    FArgs fa = fused_args(A, X, Xn);
+    fa.Y = Y;
+    fa.hcol = A.hcur;
+    if (A.k == 3) fused_stages<3, true>(A, fa);
+    else fused_stages<4, true>(A, fa);
+}
+
 // Adaptive forward of every trajectory in lockstep: T accepted steps each,
 // step sizes logged in hlog, checkpoints c = 0..ceil(T/C) (x at step c*C,
 // the last one x_T).  X0 holds the start states in the internal layout.
@@ -649,18 +815,19 @@ static void forward_lockstep(Adj& A, const std::vector<float>& X0)
     HIP_CHECK(hipMemcpyAsync(A.ckpt, X, sizeof(float) * V, hipMemcpyDeviceToDevice, A.stream));
     hipLaunchKernelGGL(k_ctl_init, dim3(ceil_div(A.ntraj, 64)), dim3(64), 0, A.stream, A.ctl, A.hstep, A.ntraj);
     A.hcur = A.hstep;
-    const int nblk = ceil_div((long long)A.n + A.m + 1, BLK);
-    const dim3 ge(nblk, ntraj8(A.ntraj));
-    const float* kl[6];
-    for (int i = 0; i < 6; i++) kl[i] = A.kk + i * V;
+    const dim3 ge(ceil_div((long long)A.n + A.m + 1, BLK), ntraj8(A.ntraj));
+    FArgs fa = fused_args(A, X, Xn);
+    fa.S = A.Y;
+    fa.ctl = A.ctl;
+    fa.parts = A.eparts;
+    const int nblk = fa.nblk_c + fa.nblk_v;
     std::vector<TCtl> hc((size_t)A.ntraj);
     const int per_round = 16;
     // every trajectory needs at least T attempts; the round loop ends when none is active
     for (long long round = 0;; round++) {
         for (int r = 0; r < per_round; r++) {
-            frozen_step(A, X, Xn);
-            hipLaunchKernelGGL(k_err_parts, ge, dim3(BLK), 0, A.stream, d, (const float*)X, kl[0], kl[2], kl[3],
-                               kl[4], kl[5], (const TCtl*)A.ctl, A.eparts, nblk);
+            if (A.k == 3) fused_stages<3, false>(A, fa);
+            else fused_stages<4, false>(A, fa);
             hipLaunchKernelGGL(k_ctl_batch, dim3(A.ntraj), dim3(BLK), 0, A.stream, A.ctl, (const double*)A.eparts,
                                nblk, A.eps, A.T, A.C, A.hlog, A.hstep);
             hipLaunchKernelGGL(k_accept, ge, dim3(BLK), 0, A.stream, d, (const TCtl*)A.ctl, (const float*)Xn, X,
@@ -828,7 +995,7 @@ int asat_adj_replay(asat_adj_t* a, int ntraj, const float* pub0, const double* h
     float* nxt = A.seg + V;
     for (int s = 0; s < A.T; s++) {
         load_h(A, s);
-        frozen_step(A, cur, nxt);
+        frozen_step_fused(A, cur, nxt, A.Y);
         std::swap(cur, nxt);
     }
     energy(A, cur, false);
@@ -900,18 +1067,19 @@ int asat_adj_backward(asat_adj_t* a, double* grad_out)
         const int s0 = sg * A.C, s1 = std::min(A.T, s0 + A.C);
         // recompute the segment's states from its checkpoint
         HIP_CHECK(hipMemcpyAsync(A.seg, A.ckpt + (size_t)sg * V, sizeof(float) * V, hipMemcpyDeviceToDevice, A.stream));
-        for (int s = s0; s < s1 - 1; s++) {
+        // ... keeping every step's six stage inputs (6 V floats per step)
+        for (int s = s0; s < s1; s++) {
             load_h(A, s);
-            frozen_step(A, A.seg + (size_t)(s - s0) * V, A.seg + (size_t)(s - s0 + 1) * V);
+            frozen_step_fused(A, A.seg + (size_t)(s - s0) * V, s < s1 - 1 ? A.seg + (size_t)(s - s0 + 1) * V : nullptr,
+                              A.Yseg + (size_t)(s - s0) * 6 * V);
         }
         for (int s = s1 - 1; s >= s0; s--) {
             load_h(A, s);
-            frozen_step(A, A.seg + (size_t)(s - s0) * V, nullptr);
             for (int l = 6; l >= 1; l--) {
                 hipLaunchKernelGGL(k_mu, ge, dim3(BLK), 0, A.stream, d, l, (const double*)A.lam,
                                    (const double*)A.nu, A.hcurr, A.mu);
                 double* nul = A.nu + (size_t)(l - 1) * V;
-                const float* Yl = A.Y + (size_t)(l - 1) * V;
+                const float* Yl = A.Yseg + ((size_t)(s - s0) * 6 + (l - 1)) * V;
                 if (A.k == 3) hipLaunchKernelGGL(k_vjp_clause<3>, gc, dim3(BLK), 0, A.stream, d, Yl, (const double*)A.mu, nul, A.c64);
                 else hipLaunchKernelGGL(k_vjp_clause<4>, gc, dim3(BLK), 0, A.stream, d, Yl, (const double*)A.mu, nul, A.c64);
                 hipLaunchKernelGGL(k_rhs_var<double>, gv, dim3(BLK), 0, A.stream, d, (const double*)A.c64, nul);

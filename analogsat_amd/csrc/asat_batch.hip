@@ -130,10 +130,13 @@ __global__ __launch_bounds__(BS) void k_batch(Params P)
     for (;;) {
         if (tid == 0) s_traj = atomicAdd(P.queue, 1);
         __syncthreads();
-        const int slot = s_traj;
+        // the trajectory id is the same in every lane: reading it as a
+        // scalar lets the descriptor and the instance's base pointers live
+        // in scalar registers, leaving the vector registers to the state
+        const int slot = __builtin_amdgcn_readfirstlane(s_traj);
         __syncthreads();
         if (slot >= P.ntraj) break;
-        const int tr = P.order[slot];
+        const int tr = __builtin_amdgcn_readfirstlane(P.order[slot]);
         const Desc d = P.desc[tr];
         TrajCtrl* cp = P.ctrl + tr;
         if (cp->status != RUNNING) continue;    // uniform: every thread reads the same word
@@ -426,11 +429,16 @@ struct RegState {
     int e0[VPT], e1[VPT];
 };
 
-template <int BS, int CPT, int VPT, int J>
+// KK = 3 or 4: every instance of the launch has clauses of KK literal slots
+// (the code is compiled for that length); KK = 0: the length k is read at
+// run time (at most KMAX)
+template <int BS, int CPT, int VPT, int J, int KK>
 __device__ __forceinline__ void reg_stage(RegState<CPT, VPT>& R, const uint32_t* code, int n, int m, int k,
                                           float knorm, float* sp, float* contrib, float hs, double h, double hnext,
                                           double floor, double& err)
 {
+    constexpr int KQ = KK ? KK : KMAX;
+    if constexpr (KK != 0) k = KK;
     const int tid = threadIdx.x;
     // ---------------- clause phase
 #pragma unroll
@@ -438,14 +446,14 @@ __device__ __forceinline__ void reg_stage(RegState<CPT, VPT>& R, const uint32_t*
         const int j = tid + u * BS;
         if (j < m) {
             const uint32_t* cj = code + (long long)j * k;
-            float f[KMAX];
-            uint32_t cc[KMAX];
+            float f[KQ];
+            uint32_t cc[KQ];
             float km = knorm;
 #pragma unroll
-            for (int q = 0; q < KMAX; q++) {
+            for (int q = 0; q < KQ; q++) {
                 f[q] = 1.0f;
                 cc[q] = 0;
-                if (q < k) {
+                if (KK != 0 || q < k) {
                     uint32_t c = cj[q];
                     cc[q] = c;
                     int v = (int)(c >> 19);
@@ -472,13 +480,13 @@ __device__ __forceinline__ void reg_stage(RegState<CPT, VPT>& R, const uint32_t*
             }
             const float r = km * knorm * am;
 #pragma unroll
-            for (int q = 0; q < KMAX; q++) {
-                if (q < k && (cc[q] >> 19) != 0) {
+            for (int q = 0; q < KQ; q++) {
+                if ((KK != 0 || q < k) && (cc[q] >> 19) != 0) {
                     float p = 1.0f;
                     bool first = true;
 #pragma unroll
-                    for (int qq = 0; qq < KMAX; qq++) {
-                        if (qq == q || qq >= k) continue;
+                    for (int qq = 0; qq < KQ; qq++) {
+                        if (qq == q || (KK == 0 && qq >= k)) continue;
                         p = first ? f[qq] : p * f[qq];
                         first = false;
                     }
@@ -529,7 +537,7 @@ __device__ __forceinline__ void reg_stage(RegState<CPT, VPT>& R, const uint32_t*
     __syncthreads();
 }
 
-template <int BS, int CPT, int VPT>
+template <int BS, int CPT, int VPT, int KK>
 __global__ __launch_bounds__(BS) void k_batch_reg(Params P)
 {
     extern __shared__ float lds[];          // [stage spins: sp_stride] then contributions
@@ -543,10 +551,13 @@ __global__ __launch_bounds__(BS) void k_batch_reg(Params P)
     for (;;) {
         if (tid == 0) s_traj = atomicAdd(P.queue, 1);
         __syncthreads();
-        const int slot = s_traj;
+        // the trajectory id is the same in every lane: reading it as a
+        // scalar lets the descriptor and the instance's base pointers live
+        // in scalar registers, leaving the vector registers to the state
+        const int slot = __builtin_amdgcn_readfirstlane(s_traj);
         __syncthreads();
         if (slot >= P.ntraj) break;
-        const int tr = P.order[slot];
+        const int tr = __builtin_amdgcn_readfirstlane(P.order[slot]);
         const Desc d = P.desc[tr];
         TrajCtrl* cp = P.ctrl + tr;
         if (cp->status != RUNNING) continue;    // uniform: every thread reads the same word
@@ -597,13 +608,13 @@ __global__ __launch_bounds__(BS) void k_batch_reg(Params P)
                 const bool had_stage1 = first_attempt;
                 double err_local = 0.0;
                 if (first_attempt)
-                    reg_stage<BS, CPT, VPT, 1>(R, code, n, m, k, knorm, sp, contrib, hs, h, hnext, P.floor, err_local);
+                    reg_stage<BS, CPT, VPT, 1, KK>(R, code, n, m, k, knorm, sp, contrib, hs, h, hnext, P.floor, err_local);
                 first_attempt = false;
-                reg_stage<BS, CPT, VPT, 2>(R, code, n, m, k, knorm, sp, contrib, hs, h, hnext, P.floor, err_local);
-                reg_stage<BS, CPT, VPT, 3>(R, code, n, m, k, knorm, sp, contrib, hs, h, hnext, P.floor, err_local);
-                reg_stage<BS, CPT, VPT, 4>(R, code, n, m, k, knorm, sp, contrib, hs, h, hnext, P.floor, err_local);
-                reg_stage<BS, CPT, VPT, 5>(R, code, n, m, k, knorm, sp, contrib, hs, h, hnext, P.floor, err_local);
-                reg_stage<BS, CPT, VPT, 6>(R, code, n, m, k, knorm, sp, contrib, hs, h, hnext, P.floor, err_local);
+                reg_stage<BS, CPT, VPT, 2, KK>(R, code, n, m, k, knorm, sp, contrib, hs, h, hnext, P.floor, err_local);
+                reg_stage<BS, CPT, VPT, 3, KK>(R, code, n, m, k, knorm, sp, contrib, hs, h, hnext, P.floor, err_local);
+                reg_stage<BS, CPT, VPT, 4, KK>(R, code, n, m, k, knorm, sp, contrib, hs, h, hnext, P.floor, err_local);
+                reg_stage<BS, CPT, VPT, 5, KK>(R, code, n, m, k, knorm, sp, contrib, hs, h, hnext, P.floor, err_local);
+                reg_stage<BS, CPT, VPT, 6, KK>(R, code, n, m, k, knorm, sp, contrib, hs, h, hnext, P.floor, err_local);
                 // ---------------- error norm + control (FP64), uniform in every thread
                 const double errmax = block_max_all<BS>(err_local, red) / P.eps;
                 errmax_last = errmax;
@@ -753,6 +764,7 @@ struct Bucket {
     int chunk = 0;                 // clause-chunked mode: clauses per chunk
     bool vcode = false;            // clause-chunked records built for this bucket's instances
     int cpt = 0, vpt = 0;          // register kernel: clauses / variables per thread
+    int kk = 0;                    // register kernel: 3 when every instance has 3-literal clauses, else 0
     int grid = 0;
     int maxn = 0, maxm = 0, maxl = 0;
     int sp_stride = 0;
@@ -906,20 +918,22 @@ static const RegVariant* reg_variant(const Bucket& b, int bs = 0)
 
 #define ASAT_REG_VARIANTS(X) X(256, 1, 1) X(256, 4, 2) X(512, 8, 4) X(768, 12, 3) X(1024, 9, 2) X(512, 17, 4)
 
-static const void* reg_kernel(int bs, int cpt, int vpt)
+static const void* reg_kernel(int bs, int cpt, int vpt, int kk)
 {
-#define ASAT_REG_PTR(B, C, V) \
-    if (bs == B && cpt == C && vpt == V) return (const void*)k_batch_reg<B, C, V>;
+#define ASAT_REG_PTR(B, C, V)                                                          \
+    if (bs == B && cpt == C && vpt == V)                                               \
+        return kk == 3 ? (const void*)k_batch_reg<B, C, V, 3> : (const void*)k_batch_reg<B, C, V, 0>;
     ASAT_REG_VARIANTS(ASAT_REG_PTR)
 #undef ASAT_REG_PTR
     return nullptr;
 }
 
-static void launch_reg(hipStream_t st, const Params& P, int grid, int bs, int cpt, int vpt, size_t shmem)
+static void launch_reg(hipStream_t st, const Params& P, int grid, int bs, int cpt, int vpt, int kk, size_t shmem)
 {
 #define ASAT_REG_LAUNCH(B, C, V)                                                               \
     if (bs == B && cpt == C && vpt == V) {                                                     \
-        hipLaunchKernelGGL((k_batch_reg<B, C, V>), dim3(grid), dim3(B), shmem, st, P);         \
+        if (kk == 3) hipLaunchKernelGGL((k_batch_reg<B, C, V, 3>), dim3(grid), dim3(B), shmem, st, P); \
+        else hipLaunchKernelGGL((k_batch_reg<B, C, V, 0>), dim3(grid), dim3(B), shmem, st, P); \
         return;                                                                                \
     }
     ASAT_REG_VARIANTS(ASAT_REG_LAUNCH)
@@ -952,7 +966,7 @@ static void dispatch(hipStream_t st, const Params& P, const Bucket& b)
 {
     const int grid = b.grid, bs = b.block;
     const size_t shmem = b.lds_bytes;
-    if (b.lds == 2) launch_reg(st, P, grid, bs, b.cpt, b.vpt, shmem);
+    if (b.lds == 2) launch_reg(st, P, grid, bs, b.cpt, b.vpt, b.kk, shmem);
     else if (b.lds == 1) launch_mode<1>(st, P, grid, bs, shmem);
     else if (b.lds == 3) launch_mode<3>(st, P, grid, bs, shmem);
     else launch_mode<0>(st, P, grid, bs, shmem);
@@ -1024,7 +1038,7 @@ static void configure_bucket(Batch& B, Bucket& b, int bs, int mode, int grid_cap
     HIP_CHECK(hipMalloc((void**)&b.d_scratch, sbytes));
     HIP_CHECK(hipMemset(b.d_scratch, 0, sbytes));
     if (mode == 2)
-        HIP_CHECK(hipFuncSetAttribute(reg_kernel(bs, b.cpt, b.vpt), hipFuncAttributeMaxDynamicSharedMemorySize,
+        HIP_CHECK(hipFuncSetAttribute(reg_kernel(bs, b.cpt, b.vpt, b.kk), hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)b.lds_bytes));
     else
         set_lds_attr(bs, mode, b.lds_bytes);
@@ -1166,11 +1180,14 @@ static void prepare(Batch& B, bool tune)
         Bucket& b = B.buckets[q];
         int lo = (int)((long long)T * q / nb), hi = (int)((long long)T * (q + 1) / nb);
         b.ids.assign(order.begin() + lo, order.begin() + hi);
+        bool allk3 = true;
         for (int id : b.ids) {
             b.maxn = std::max(b.maxn, B.inst[id].n);
             b.maxm = std::max(b.maxm, B.inst[id].m);
             b.maxl = std::max(b.maxl, B.inst[id].nslots);
+            allk3 = allk3 && B.inst[id].k == 3;
         }
+        b.kk = allk3 ? 3 : 0;
         b.sp_stride = (int)round_up_ll((long long)b.maxn + 1, 4);
         b.lds_fits = bucket_lds(b, true) <= B.lds_limit;
         HIP_CHECK(hipMalloc((void**)&b.d_order, sizeof(int) * b.ids.size()));

@@ -1210,10 +1210,11 @@ struct Sys {
     int* d_perm = nullptr;       // processing position -> clause index (null: identity)
     std::vector<int> perm;       // host copy, for the public <-> internal a-block order
     // one-workgroup path (asat_small.h) for problems whose integrator fits in
-    // LDS: per variable, its slots (processing-order slot index) in the
-    // reference's accumulation order; small_lds = 0 when the problem is too big
+    // LDS: each slot's position in var-major order (a variable's terms in the
+    // reference's accumulation order) and each variable's range of positions;
+    // small_lds = 0 when the problem is too big
     int* d_vptr = nullptr;
-    int* d_vslot = nullptr;
+    int* d_spos_small = nullptr;
     int nlist = 0;
     long long small_lds = 0;
     int vg = 0, ngroups = 0, vg_blocks = 0;
@@ -1237,7 +1238,7 @@ struct Sys {
     {
         dfree(d_lit); dfree(d_pos); dfree(d_spos); dfree(d_srank); dfree(d_csr_ptr); dfree(d_csr_pos);
         dfree(d_csr_loc); dfree(d_gstart); dfree(d_perm); dfree(d_contrib); dfree(d_apart);
-        dfree(d_vptr); dfree(d_vslot);
+        dfree(d_vptr); dfree(d_spos_small);
         perm.clear();
         nlist = 0;
         small_lds = 0;
@@ -1301,10 +1302,11 @@ static bool choose_layout(HostProblem& P, int esize)
     return false;
 }
 
-// Slot lists of the one-workgroup path, when its LDS footprint fits: for
-// every variable, the processing-order slot indices j*K + q of its
-// literals, clauses taken in input order and literals in clause order (the
-// order CpuSat::CalculateAllEx adds them, AnalogSAT/cpu_sat/CpuSat.cpp:206-229).
+// Var-major term positions of the one-workgroup path, when its LDS
+// footprint fits: variable v's terms occupy positions vptr[v] .. vptr[v+1]-1,
+// its slots taken with clauses in input order and literals in clause order
+// (the order CpuSat::CalculateAllEx adds them, AnalogSAT/cpu_sat/CpuSat.cpp:
+// 206-229); spos[j*K + q] is the position of processing-order slot (j, q).
 static void build_small_lists(Sys* s, const HostProblem& P)
 {
     const int n = P.n, m = P.m, K = P.k;
@@ -1317,18 +1319,18 @@ static void build_small_lists(Sys* s, const HostProblem& P)
     const long long bytes = small_lds_bytes(n, m, K, s->alloc, nl, s->esize);
     if (bytes > SMALL_LDS_MAX) return;
     for (int v = 1; v <= n; v++) vptr[(size_t)v + 1] += vptr[v];
-    std::vector<int> fill(vptr.begin(), vptr.end() - 1), vslot((size_t)std::max<long long>(nl, 1), 0);
+    std::vector<int> fill(vptr.begin(), vptr.end() - 1), spos((size_t)m * K, -1);
     std::vector<int> proc((size_t)m);
     for (int p = 0; p < m; p++) proc[(size_t)(P.perm.empty() ? p : P.perm[p])] = p;
     for (int c = 0; c < m; c++) {
         const int p = proc[(size_t)c];
         for (int q = 0; q < K; q++) {
             const int l = P.lit[(size_t)p * K + q];
-            if (l) vslot[(size_t)fill[(size_t)std::abs(l)]++] = p * K + q;
+            if (l) spos[(size_t)p * K + q] = fill[(size_t)std::abs(l)]++;
         }
     }
     upload(s->d_vptr, vptr);
-    upload(s->d_vslot, vslot);
+    upload(s->d_spos_small, spos);
     s->nlist = (int)nl;
     s->small_lds = bytes;
 }
@@ -1596,6 +1598,7 @@ struct RK {
     unsigned long long gen = 0;   // the system's problem generation at Configure
     void* k[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     void* tmp = nullptr;
+    int tmp_n = 0;            // tmp holds n+1 values of the problem at Configure
     double* partial = nullptr;
     int npartial = 0;
     Ctrl* d_ctrl = nullptr;
@@ -1664,6 +1667,21 @@ static void rk_configure(RK* r, Sys* s)
 {
     s->require_ready();
     HIP_CHECK(hipStreamSynchronize(s->stream));
+    const int npartial = clause_grid(s) + var_grid(s);
+    if (r->configured && r->sys == s && r->alloc == s->alloc && r->prec == s->prec && r->npartial == npartial &&
+        r->tmp_n == s->n) {
+        // same sizes (SatSolver configures before every Solve): keep the
+        // buffers, clear them and reset the control block
+        r->drop_graph();
+        r->sys = s;
+        r->gen = s->gen;
+        const size_t bytes = (size_t)s->esize * s->alloc;
+        for (auto& p : r->k) HIP_CHECK(hipMemsetAsync(p, 0, bytes, s->stream));
+        HIP_CHECK(hipMemsetAsync(r->tmp, 0, (size_t)s->esize * (s->n + 1), s->stream));
+        HIP_CHECK(hipMemsetAsync(r->partial, 0, sizeof(double) * r->npartial, s->stream));
+        rk_reset(r);
+        return;
+    }
     r->free_all();
     r->configured = false;
     r->sys = s;
@@ -1676,8 +1694,9 @@ static void rk_configure(RK* r, Sys* s)
         HIP_CHECK(hipMemset(p, 0, bytes));
     }
     HIP_CHECK(hipMalloc(&r->tmp, (size_t)s->esize * (s->n + 1)));
+    r->tmp_n = s->n;
     HIP_CHECK(hipMemset(r->tmp, 0, (size_t)s->esize * (s->n + 1)));
-    r->npartial = clause_grid(s) + var_grid(s);
+    r->npartial = npartial;
     HIP_CHECK(hipMalloc((void**)&r->partial, sizeof(double) * r->npartial));
     HIP_CHECK(hipMemset(r->partial, 0, sizeof(double) * r->npartial));
     HIP_CHECK(hipMalloc((void**)&r->d_ctrl, sizeof(Ctrl)));
@@ -1869,7 +1888,7 @@ static SmallArgs small_args(const RK* r, void* x)
     const Sys* s = r->sys;
     SmallArgs a;
     memset(&a, 0, sizeof(a));
-    a.lit = s->d_lit; a.vptr = s->d_vptr; a.vslot = s->d_vslot;
+    a.lit = s->d_lit; a.vptr = s->d_vptr; a.spos = s->d_spos_small;
     a.x = x;
     a.ctrl = r->d_ctrl;
     a.n = s->n; a.m = s->m; a.k = s->k; a.aoff = s->aoff; a.alloc = s->alloc; a.nlist = s->nlist;

@@ -4,7 +4,9 @@
 // N = 50, M = 213) every one of them is a few microseconds of dispatch for
 // well under a microsecond of work.  Here one workgroup holds the whole
 // integrator in LDS -- state x, x_new, k1..k5, the stage spin input, the slot
-// contributions, the literals and the per-variable slot lists -- and runs
+// contributions (in var-major order: each variable's terms are one run, in
+// the reference's accumulation order), the literals and each slot's
+// var-major position -- and runs
 // attempt after attempt with workgroup barriers between the clause and var
 // phases of each stage.  Every thread evaluates the FP64 step controller
 // itself from the block-wide error maximum, so no thread waits for a
@@ -50,8 +52,8 @@ constexpr int SMALL_SAT = 0, SMALL_UNDERFLOW = 1, SMALL_MAXTIME = 3, SMALL_MAXIT
 
 struct SmallArgs {
     const int* lit;       // m*K literals in processing order (zero padded)
-    const int* vptr;      // n+2 slot-list starts per variable
-    const int* vslot;     // slot indices (processing order) per variable, accumulation order
+    const int* vptr;      // n+2: variable i's terms are var-major positions vptr[i] .. vptr[i+1]-1
+    const int* spos;      // m*K: each slot's var-major position (-1: padding)
     void* x;              // state buffer: read at entry, newest state written at exit
     Ctrl* ctrl;
     SmallSolve* solve;    // null: Step mode (run until ctrl->steps reaches target)
@@ -66,29 +68,42 @@ struct SmallArgs {
     double knorm;
 };
 
-// LDS bytes of the small path for a problem (0 if it does not qualify)
+// LDS bytes of the small path for a problem
 inline long long small_lds_bytes(int n, int m, int k, long long alloc, long long nlist, int esize)
 {
     const long long mk = (long long)m * k;
-    return (long long)esize * (7 * alloc + (n + 1) + mk) + 4 * (mk + (n + 2) + nlist);
+    return (long long)esize * (7 * alloc + (n + 1) + nlist) + 4 * (2 * mk + (n + 2));
 }
 
-template <typename T, int KT, int BS>
+template <typename T>
 struct SmallLds {
     T *x, *xn, *k1, *k2, *k3, *k4, *k5, *sp, *cb;
-    int *lit, *vptr, *vslot;
+    int *lit, *vptr, *spos;
 };
 
-// clause phase of stage J: thread per clause; sp_in holds the stage's spins
-// (x itself for J = 1)
-template <typename T, int KT, int BS, int J>
-__device__ __forceinline__ void small_clause(const SmallArgs& a, const SmallLds<T, KT, BS>& L, const T* sp_in,
-                                             T hs, double h, double hnext, double floor, T knorm, double& err)
+// Register mode (REG): 256 threads, clauses of KT = 3 or 4 slots, at most
+// 512 clauses and variables: each thread keeps its (at most two) clauses'
+// literals and var-major positions and its variables' term ranges in
+// registers for the whole launch.
+constexpr int SMALL_RPT = 2;
+template <int KT, int BS>
+struct SmallReg {
+    static constexpr bool on = KT > 0 && BS == 256;
+    int lit[SMALL_RPT][KT > 0 ? KT : 1];
+    int pos[SMALL_RPT][KT > 0 ? KT : 1];
+    int e0[SMALL_RPT], e1[SMALL_RPT];
+};
+
+// one clause of the clause phase of stage J; lm / pm: its literals and its
+// slots' var-major positions (LDS pointers or register arrays)
+template <typename T, int KT, int J, typename LitT, typename PosT>
+__device__ __forceinline__ void small_clause_one(const SmallArgs& a, const SmallLds<T>& L, const T* sp_in,
+                                                 const LitT& lm, const PosT& pm, int j, T hs, double h,
+                                                 double hnext, double floor, T knorm, double& err)
 {
     constexpr int KM = KT ? KT : GEN_KMAX;
     const int K = KT ? KT : a.k;
-    for (int j = (int)threadIdx.x; j < a.m; j += BS) {
-        const int* lm = L.lit + j * K;
+    {
         T f[KM];
         T km = knorm;
 #pragma unroll 4
@@ -123,7 +138,7 @@ __device__ __forceinline__ void small_clause(const SmallArgs& a, const SmallLds<
             err = fmax(err, e);
             L.xn[ai] = xn;
         }
-        T* out = L.cb + j * K;
+        T* out = L.cb;
         if (a.family == ASAT_FAMILY_TANH) {
             const T r = knorm * am;
             const bool ex = K >= 3 && K <= 10;
@@ -139,7 +154,7 @@ __device__ __forceinline__ void small_clause(const SmallArgs& a, const SmallLds<
                     p = first ? f[qq] : p * f[qq];
                     first = false;
                 }
-                out[q] = tanh_term<T>(r, p, f[q], (T)a.q, l < 0, ex);
+                out[pm[q]] = tanh_term<T>(r, p, f[q], (T)a.q, l < 0, ex);
             }
         } else if (K <= 10) {
             const T r = km * knorm * am;
@@ -157,7 +172,7 @@ __device__ __forceinline__ void small_clause(const SmallArgs& a, const SmallLds<
                 }
                 T rr = r * p;
                 rr *= l < 0 ? (T)-2.0 : (T)2.0;
-                out[q] = rr;
+                out[pm[q]] = rr;
             }
         } else {
             for (int q = 0; q < K; q++) {
@@ -165,24 +180,45 @@ __device__ __forceinline__ void small_clause(const SmallArgs& a, const SmallLds<
                 if (l == 0) continue;
                 T temp = km;
                 for (int qq = 0; qq < K; qq++) temp *= (qq == q) ? (T)1.0 : f[qq];
-                out[q] = (T)2.0 * temp * knorm * am * (l < 0 ? (T)-1.0 : (T)1.0);
+                out[pm[q]] = (T)2.0 * temp * knorm * am * (l < 0 ? (T)-1.0 : (T)1.0);
             }
         }
     }
 }
 
+// clause phase of stage J: thread per clause; sp_in holds the stage's spins
+// (x itself for J = 1)
+template <typename T, int KT, int BS, int J>
+__device__ __forceinline__ void small_clause(const SmallArgs& a, const SmallLds<T>& L, const SmallReg<KT, BS>& R,
+                                             const T* sp_in, T hs, double h, double hnext, double floor, T knorm,
+                                             double& err)
+{
+    if constexpr (SmallReg<KT, BS>::on) {
+#pragma unroll
+        for (int u = 0; u < SMALL_RPT; u++) {
+            const int j = (int)threadIdx.x + u * BS;
+            if (j < a.m)
+                small_clause_one<T, KT, J>(a, L, sp_in, R.lit[u], R.pos[u], j, hs, h, hnext, floor, knorm, err);
+        }
+    } else {
+        const int K = KT ? KT : a.k;
+        for (int j = (int)threadIdx.x; j < a.m; j += BS)
+            small_clause_one<T, KT, J>(a, L, sp_in, L.lit + j * K, L.spos + j * K, j, hs, h, hnext, floor, knorm,
+                                       err);
+    }
+}
+
 // var phase of stage J: thread per variable; fresh = false only on a retry's
 // J = 1 (k1 kept from the step's first attempt)
-template <typename T, int KT, int BS, int J>
-__device__ __forceinline__ void small_var(const SmallArgs& a, const SmallLds<T, KT, BS>& L, bool fresh, T hs,
-                                          double h, double hnext, double floor, double& err)
+template <typename T, int J>
+__device__ __forceinline__ void small_var_one(const SmallLds<T>& L, int i, int e0, int e1, bool fresh, T hs,
+                                              double h, double hnext, double floor, double& err)
 {
-    for (int i = (int)threadIdx.x + 1; i <= a.n; i += BS) {
+    {
         T kj = (T)0;
         if (fresh) {
             T sum = (T)0;
-            const int e1 = L.vptr[i + 1];
-            for (int e = L.vptr[i]; e < e1; e++) sum += L.cb[L.vslot[e]];
+            for (int e = e0; e < e1; e++) sum += L.cb[e];
             kj = sum;
         }
         const T xi = L.x[i];
@@ -212,10 +248,26 @@ __device__ __forceinline__ void small_var(const SmallArgs& a, const SmallLds<T, 
     }
 }
 
+template <typename T, int KT, int BS, int J>
+__device__ __forceinline__ void small_var(const SmallArgs& a, const SmallLds<T>& L, const SmallReg<KT, BS>& R,
+                                          bool fresh, T hs, double h, double hnext, double floor, double& err)
+{
+    if constexpr (SmallReg<KT, BS>::on) {
+#pragma unroll
+        for (int u = 0; u < SMALL_RPT; u++) {
+            const int i = (int)threadIdx.x + 1 + u * BS;
+            if (i <= a.n) small_var_one<T, J>(L, i, R.e0[u], R.e1[u], fresh, hs, h, hnext, floor, err);
+        }
+    } else {
+        for (int i = (int)threadIdx.x + 1; i <= a.n; i += BS)
+            small_var_one<T, J>(L, i, L.vptr[i], L.vptr[i + 1], fresh, hs, h, hnext, floor, err);
+    }
+}
+
 // violated clauses of the thresholded state (k_violations semantics), in
 // every thread
 template <typename T, int KT, int BS>
-__device__ __forceinline__ int small_violations(const SmallArgs& a, const SmallLds<T, KT, BS>& L, int* shi)
+__device__ __forceinline__ int small_violations(const SmallArgs& a, const SmallLds<T>& L, int* shi)
 {
     const int K = KT ? KT : a.k;
     int viol = 0;
@@ -239,7 +291,7 @@ __global__ __launch_bounds__(BS) void k_small(SmallArgs a)
     __shared__ double shd[BS / 64];
     __shared__ int shi[BS / 64];
     const int alloc = a.alloc, n = a.n, mk = a.m * (KT ? KT : a.k);
-    SmallLds<T, KT, BS> L;
+    SmallLds<T> L;
     T* base = (T*)small_lds;
     L.x = base;
     L.xn = base + alloc;
@@ -247,9 +299,9 @@ __global__ __launch_bounds__(BS) void k_small(SmallArgs a)
     L.k5 = base + 6 * alloc;
     L.sp = base + 7 * alloc;
     L.cb = L.sp + (n + 1);
-    L.lit = (int*)(L.cb + mk);
-    L.vptr = L.lit + mk;
-    L.vslot = L.vptr + (n + 2);
+    L.lit = (int*)(L.cb + a.nlist);
+    L.spos = L.lit + mk;
+    L.vptr = L.spos + mk;
     {
         const T* gx = (const T*)a.x;
         for (int i = (int)threadIdx.x; i < alloc; i += BS) {
@@ -259,9 +311,24 @@ __global__ __launch_bounds__(BS) void k_small(SmallArgs a)
             L.k1[i] = L.k2[i] = L.k3[i] = L.k4[i] = L.k5[i] = (T)0;
         }
         for (int i = (int)threadIdx.x; i <= n; i += BS) L.sp[i] = (T)0;
-        for (int i = (int)threadIdx.x; i < mk; i += BS) { L.lit[i] = a.lit[i]; L.cb[i] = (T)0; }
+        for (int i = (int)threadIdx.x; i < mk; i += BS) { L.lit[i] = a.lit[i]; L.spos[i] = a.spos[i]; }
+        for (int i = (int)threadIdx.x; i < a.nlist; i += BS) L.cb[i] = (T)0;
         for (int i = (int)threadIdx.x; i < n + 2; i += BS) L.vptr[i] = a.vptr[i];
-        for (int i = (int)threadIdx.x; i < a.nlist; i += BS) L.vslot[i] = a.vslot[i];
+    }
+    SmallReg<KT, BS> R;
+    if constexpr (SmallReg<KT, BS>::on) {
+#pragma unroll
+        for (int u = 0; u < SMALL_RPT; u++) {
+            const int j = (int)threadIdx.x + u * BS;
+#pragma unroll
+            for (int q = 0; q < (KT > 0 ? KT : 1); q++) {
+                R.lit[u][q] = j < a.m ? a.lit[j * KT + q] : 0;
+                R.pos[u][q] = j < a.m ? a.spos[j * KT + q] : -1;
+            }
+            const int i = (int)threadIdx.x + 1 + u * BS;
+            R.e0[u] = i <= n ? a.vptr[i] : 0;
+            R.e1[u] = i <= n ? a.vptr[i + 1] : 0;
+        }
     }
     // the control block, identical in every thread
     const Ctrl* c = a.ctrl;
@@ -291,29 +358,29 @@ __global__ __launch_bounds__(BS) void k_small(SmallArgs a)
         if (step_start && launched >= SMALL_ATTEMPT_CAP) break;
         const T hs = (T)h;
         double err = 0.0;
-        if (step_start) small_clause<T, KT, BS, 1>(a, L, L.x, hs, h, hnext, floor, knorm, err);
+        if (step_start) small_clause<T, KT, BS, 1>(a, L, R, L.x, hs, h, hnext, floor, knorm, err);
         __syncthreads();
-        small_var<T, KT, BS, 1>(a, L, step_start != 0, hs, h, hnext, floor, err);
+        small_var<T, KT, BS, 1>(a, L, R, step_start != 0, hs, h, hnext, floor, err);
         __syncthreads();
-        small_clause<T, KT, BS, 2>(a, L, L.sp, hs, h, hnext, floor, knorm, err);
+        small_clause<T, KT, BS, 2>(a, L, R, L.sp, hs, h, hnext, floor, knorm, err);
         __syncthreads();
-        small_var<T, KT, BS, 2>(a, L, true, hs, h, hnext, floor, err);
+        small_var<T, KT, BS, 2>(a, L, R, true, hs, h, hnext, floor, err);
         __syncthreads();
-        small_clause<T, KT, BS, 3>(a, L, L.sp, hs, h, hnext, floor, knorm, err);
+        small_clause<T, KT, BS, 3>(a, L, R, L.sp, hs, h, hnext, floor, knorm, err);
         __syncthreads();
-        small_var<T, KT, BS, 3>(a, L, true, hs, h, hnext, floor, err);
+        small_var<T, KT, BS, 3>(a, L, R, true, hs, h, hnext, floor, err);
         __syncthreads();
-        small_clause<T, KT, BS, 4>(a, L, L.sp, hs, h, hnext, floor, knorm, err);
+        small_clause<T, KT, BS, 4>(a, L, R, L.sp, hs, h, hnext, floor, knorm, err);
         __syncthreads();
-        small_var<T, KT, BS, 4>(a, L, true, hs, h, hnext, floor, err);
+        small_var<T, KT, BS, 4>(a, L, R, true, hs, h, hnext, floor, err);
         __syncthreads();
-        small_clause<T, KT, BS, 5>(a, L, L.sp, hs, h, hnext, floor, knorm, err);
+        small_clause<T, KT, BS, 5>(a, L, R, L.sp, hs, h, hnext, floor, knorm, err);
         __syncthreads();
-        small_var<T, KT, BS, 5>(a, L, true, hs, h, hnext, floor, err);
+        small_var<T, KT, BS, 5>(a, L, R, true, hs, h, hnext, floor, err);
         __syncthreads();
-        small_clause<T, KT, BS, 6>(a, L, L.sp, hs, h, hnext, floor, knorm, err);
+        small_clause<T, KT, BS, 6>(a, L, R, L.sp, hs, h, hnext, floor, knorm, err);
         __syncthreads();
-        small_var<T, KT, BS, 6>(a, L, true, hs, h, hnext, floor, err);
+        small_var<T, KT, BS, 6>(a, L, R, true, hs, h, hnext, floor, err);
         // block maximum (its barriers also order the x_new writes before any
         // read of the next attempt)
         const double emax = block_max_all<BS>(err, shd) / eps;

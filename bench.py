@@ -420,18 +420,34 @@ def workload_c1(args, log):
                        "p50_time_to_solution_s": p50_all([r["solved_wall_ms"] / 1e3 for r in solved], len(res)),
                        "p50_steps_to_solution": p50_all([r["sat_step"] for r in solved], len(res)),
                        "p50_solved_only_s": p50([r["solved_wall_ms"] / 1e3 for r in solved])}}
-    sample = insts[:args.c1_plain_sample]
+    # the one-at-a-time legs take every start of instance seeds spread
+    # evenly over the S seeds (unsolvable instances included)
+    n_inst = max(1, min(S, args.c1_plain_sample // 10))
+    pick = [(S * i) // n_inst * 10 + ic for i in range(n_inst) for ic in range(10)]
+    sample = [insts[i] for i in pick]
+    sres = [res[i] for i in pick]
+    ssolved = [r for r in sres if r["status"] == 1]
+    out["batched"]["sample"] = {"trajectories": len(pick), "solved": len(ssolved),
+                                "p50_time_to_solution_s": p50_all([r["solved_wall_ms"] / 1e3 for r in ssolved],
+                                                                  len(pick)),
+                                "p50_steps_to_solution": p50_all([r["sat_step"] for r in ssolved], len(pick))}
     log("C1: plain FP64 solver, %d trajectories one at a time" % len(sample))
-    tts, tts_steps, steps_p, wall_p = [], [], 0, 0.0
+    tts, tts_steps, dev_ms, steps_p, wall_p = [], [], [], 0, 0.0
+    # one solver for the whole sample, as a caller solving instance after
+    # instance would keep it: SetProblem per instance, SetSatState per start;
+    # the timed region is Solve() (SatSolver::GetElapsedWallTime's span)
+    sat = A.HipSat("f64")
+    rk = A.HipRungeKutta()
+    rk.SetEpsilon(1e-6)
+    solver = A.SatSolver(sat, rk)
+    solver.SetMaxTime(1e300)
+    solver.SetMaxSteps(10000)
+    solver.SetBatchSize(1)
+    cur = None
     for p, x in sample:
-        sat = A.HipSat("f64")
-        rk = A.HipRungeKutta()
-        rk.SetEpsilon(1e-6)
-        solver = A.SatSolver(sat, rk)
-        solver.SetProblem(p)
-        solver.SetMaxTime(1e300)
-        solver.SetMaxSteps(10000)
-        solver.SetBatchSize(1)
+        if p is not cur:
+            solver.SetProblem(p)
+            cur = p
         solver.SetSatState(x)
         t0 = time.perf_counter()
         r = solver.Solve()
@@ -441,9 +457,11 @@ def workload_c1(args, log):
         if r == A.SatResult.SAT_SOLUTION_FOUND:
             tts.append(w)
             tts_steps.append(solver.GetStepCount())
+            dev_ms.append(rk.last_step_ms())
     out["plain"] = {"dtype": "f64", "sample": len(sample), "trajectory_steps_per_s": round(steps_p / wall_p, 1),
                     "solved": len(tts), "p50_time_to_solution_s": p50_all(tts, len(sample)),
-                    "p50_steps_to_solution": p50_all(tts_steps, len(sample))}
+                    "p50_steps_to_solution": p50_all(tts_steps, len(sample)),
+                    "p50_device_ms_solved": p50(dev_ms), "one_workgroup": rk.UsesOneWorkgroup()}
     if not args.no_cpu:
         kind = "reference" if O.ref_available() else "port"
         log("C1: CPU solver (%s), %d trajectories one at a time, 1 thread" % (kind, len(sample)))
@@ -598,7 +616,8 @@ def main():
     ap.add_argument("--workload", choices=["c2", "c1", "c4", "c5"], default="c2",
                     help="c2: the headline plain-solver line; c1/c4/c5: the other measured workloads")
     ap.add_argument("--c1-seeds", type=int, default=100)
-    ap.add_argument("--c1-plain-sample", type=int, default=20)
+    ap.add_argument("--c1-plain-sample", type=int, default=100,
+                    help="C1 trajectories run one at a time on the plain solver and the CPU (whole instances of 10 starts)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure the dominant kernel's HBM traffic")
     ap.add_argument("--no-c1", action="store_true", help="skip the C1 time-to-solution leg of the default line")

@@ -198,7 +198,7 @@ int asat_rk_configure(asat_rk_t* rk, asat_sys_t* sys);
 int asat_rk_step(asat_rk_t* rk, asat_state_t* old_state, asat_state_t* new_state, int count);
 /* No reference counterpart.  Problems whose whole integrator fits in one
    workgroup's LDS (about 150 KB: 7 state vectors, the spin stage input, the
-   slot contributions, literals and slot lists) and that use no bias term run
+   slot contributions, literals and term positions) and that use no bias term run
    Step and asat_rk_solve in one persistent single-workgroup kernel, with the
    same arithmetic as the multi-kernel path (bit-identical results).
    enable = 0 forces the multi-kernel path (13 launches per attempt);

@@ -145,7 +145,9 @@ def test_batch_autotune_and_buckets_do_not_change_results():
         if tiling[0] == 0:
             bk = b.buckets()
             say("buckets: %s" % bk)
-            assert all(x["block"] in (256, 512, 1024) for x in bk)
+            # workgroup sizes of the contribution kernels (256, 512, 1024)
+            # and of the register-resident variants (also 768)
+            assert all(x["block"] in (256, 512, 768, 1024) for x in bk)
     (r0, s0), (r1, s1) = res
     for i in range(len(insts)):
         for key in ("status", "steps", "attempts", "time", "min_violations", "sat_step"):

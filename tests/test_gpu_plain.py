@@ -270,7 +270,10 @@ def test_step_log_records_accepted_steps():
 
 
 def test_large_instance_rhs_property():
-    """N = 1e6-scale RHS: GPU FP32 equals the oracle's float mode bit for bit."""
+    """Standalone RHS (spin-gather clause pass) at N = 2e5, M = 8.52e5: GPU
+    FP32 equals the oracle's float mode bit for bit.  The benchmark's own
+    size (N = 1e6) and its slot-spin integrator path are checked in
+    test_gpu_full_size.py."""
     say("rhs f32 N=200000 M=852000")
     n, m = 200000, 852000
     p = O.fastsat(n, m, 3, 1)
