@@ -193,6 +193,8 @@ def lib():
             "asat_batch_set_xscal_floor": ([vp, ctypes.c_double], i),
             "asat_rk_configure": ([vp, vp], i),
             "asat_rk_step": ([vp, vp, vp, i], i),
+            "asat_solve_noalloc": ([i, vp, vp, i, vp, ctypes.c_double, i, ctypes.c_longlong, ctypes.c_double, i, vp,
+                                    ctypes.POINTER(SolveResult), ctypes.POINTER(RkStats)], i),
             "asat_rk_set_one_workgroup": ([vp, i], i),
             "asat_rk_uses_one_workgroup": ([vp, ctypes.POINTER(i)], i),
             "asat_rk_solve": ([vp, vp, vp, i, ctypes.c_longlong, ctypes.c_double, i, ctypes.POINTER(SolveResult)], i),

@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libanalogsat_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["asat_plain.hip", "asat_batch.hip", "asat_adjoint.hip", "asat_problem.cpp"]
+SOURCES = ["asat_plain.hip", "asat_batch.hip", "asat_adjoint.hip", "asat_noalloc.hip", "asat_problem.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ASAT_OFFLOAD_ARCH", "gfx950")

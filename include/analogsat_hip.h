@@ -225,6 +225,25 @@ typedef struct {
 } asat_solve_result_t;
 int asat_rk_solve(asat_rk_t* rk, asat_state_t* old_state, asat_state_t* new_state, int batch, long long max_steps,
                   double max_time, int time_float, asat_solve_result_t* out);
+/* No reference counterpart.  For callers that cannot allocate device memory
+   (code under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc): asat_rk_solve for one problem given as host arrays (clauses as
+   SatProblem::AddClauses, the public start [s, a] of the given precision),
+   with no handles and no allocation.  eps as CudaRungeKutta::SetEpsilon;
+   batch, max_steps, max_time and time_float as asat_rk_solve; the integrator
+   starts from CudaRungeKutta's reset state (h = 1e-9, error floor 1e-20).
+   Writes the newest state to pub_out, the loop's outcome to *out and, if
+   stats is not null, the integrator's counters.  Every device buffer (control
+   and solver blocks, state, literals, term positions) lives in one
+   static array of ASAT_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+   ends; the problem must fit it and the one-workgroup path's LDS (about
+   150 KB), or the call fails with ASAT_ERR_INVALID_ARGUMENT.  Calls on one
+   GPU share that array: calls that overlap on one GPU race.  Results equal
+   asat_rk_solve's on the same problem and start, bit for bit. */
+#define ASAT_NOALLOC_SCRATCH_BYTES (512 * 1024)
+int asat_solve_noalloc(int precision, const int* literals, const int* lengths, int nclauses, const void* pub0,
+                       double eps, int batch, long long max_steps, double max_time, int time_float, void* pub_out,
+                       asat_solve_result_t* out, asat_rk_stats_t* stats);
 /* IODEInt::Reset */
 int asat_rk_reset(asat_rk_t* rk);
 int asat_rk_stats(const asat_rk_t* rk, asat_rk_stats_t* out);
