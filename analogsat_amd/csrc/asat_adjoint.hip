@@ -120,9 +120,9 @@ __global__ __launch_bounds__(BLK) void k_rhs_var(Dev d, const T* contrib, T* out
     const T* c = contrib + tr * d.L;
     T sum = (T)0;
     if constexpr (sizeof(T) == 4)
-        for (int e = d.csr_ptr[i]; e < d.csr_ptr[i + 1]; e++) sum += c[e];
+        sum = sum_run(sum, c, d.csr_ptr[i], d.csr_ptr[i + 1]);
     else
-        for (int e = d.csr_ptr[i]; e < d.csr_ptr[i + 1]; e++) sum += c[d.csr_slot[e]];
+        sum = sum_run_idx(sum, c, d.csr_slot, d.csr_ptr[i], d.csr_ptr[i + 1]);
     out[tr * d.D + i] = sum;
 }
 
@@ -518,7 +518,7 @@ __global__ __launch_bounds__(BLK) void k_fvar(FArgs a)
     if (i <= a.d.n) {
         const float* cr = a.c + tr * a.d.L;
         float sum = 0.f;
-        for (int e = a.d.csr_ptr[i]; e < a.d.csr_ptr[i + 1]; e++) sum += cr[e];
+        sum = sum_run(sum, cr, a.d.csr_ptr[i], a.d.csr_ptr[i + 1]);
         const float kj = sum;
         const long long o = row + i;
         const float xi = a.X[o];

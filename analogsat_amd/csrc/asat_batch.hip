@@ -302,7 +302,7 @@ __global__ __launch_bounds__(BS) void k_batch(Params P)
                             sum = vsum[i];
                         } else {
                             const int e0 = rowptr[i], e1 = rowptr[i + 1];
-                            for (int e = e0; e < e1; e++) sum += contrib[e];
+                            sum = sum_run(sum, contrib, e0, e1);
                         }
                         const float kj = sum;
                         const float xi = x[i];
@@ -441,11 +441,22 @@ __device__ __forceinline__ void reg_stage(RegState<CPT, VPT>& R, const uint32_t*
     if constexpr (KK != 0) k = KK;
     const int tid = threadIdx.x;
     // ---------------- clause phase
+    // the next clause's codes are loaded while this one is processed
+    uint32_t nx[KQ];
+#pragma unroll
+    for (int q = 0; q < KQ; q++) nx[q] = (tid < m && (KK != 0 || q < k)) ? code[(long long)tid * k + q] : 0u;
 #pragma unroll
     for (int u = 0; u < CPT; u++) {
         const int j = tid + u * BS;
+        uint32_t cur[KQ];
+#pragma unroll
+        for (int q = 0; q < KQ; q++) cur[q] = nx[q];
+        if (u + 1 < CPT) {
+            const int jn = j + BS;
+#pragma unroll
+            for (int q = 0; q < KQ; q++) nx[q] = (jn < m && (KK != 0 || q < k)) ? code[(long long)jn * k + q] : 0u;
+        }
         if (j < m) {
-            const uint32_t* cj = code + (long long)j * k;
             float f[KQ];
             uint32_t cc[KQ];
             float km = knorm;
@@ -454,7 +465,7 @@ __device__ __forceinline__ void reg_stage(RegState<CPT, VPT>& R, const uint32_t*
                 f[q] = 1.0f;
                 cc[q] = 0;
                 if (KK != 0 || q < k) {
-                    uint32_t c = cj[q];
+                    uint32_t c = cur[q];
                     cc[q] = c;
                     int v = (int)(c >> 19);
                     float s = v ? sp[v - 1] : -1.0f;
@@ -507,7 +518,7 @@ __device__ __forceinline__ void reg_stage(RegState<CPT, VPT>& R, const uint32_t*
         const int i = tid + v * BS;
         if (i < n) {
             float sum = 0.0f;
-            for (int e = R.e0[v]; e < R.e1[v]; e++) sum += contrib[e];
+            sum = sum_run(sum, contrib, R.e0[v], R.e1[v]);
             const float kj = sum;
             const float xi = R.xv[v];
             if constexpr (J == 1) {

@@ -312,6 +312,41 @@ __device__ __forceinline__ int block_sum_all(int v, int* sh)
     return r;
 }
 
+// Left-to-right sum s + v[e0] + ... + v[e1 - 1] (the reference's
+// accumulation order), with the loads issued four at a time so that one
+// memory latency covers four terms.
+template <typename T>
+__device__ __forceinline__ T sum_run(T s, const T* v, int e0, int e1)
+{
+    int e = e0;
+    for (; e + 4 <= e1; e += 4) {
+        const T a = v[e], b = v[e + 1], c = v[e + 2], d = v[e + 3];
+        s += a;
+        s += b;
+        s += c;
+        s += d;
+    }
+    for (; e < e1; e++) s += v[e];
+    return s;
+}
+
+// the same through an index array: s + v[idx[e0]] + ... + v[idx[e1 - 1]]
+template <typename T, typename I>
+__device__ __forceinline__ T sum_run_idx(T s, const T* v, const I* idx, int e0, int e1)
+{
+    int e = e0;
+    for (; e + 4 <= e1; e += 4) {
+        const I i0 = idx[e], i1 = idx[e + 1], i2 = idx[e + 2], i3 = idx[e + 3];
+        const T a = v[i0], b = v[i1], c = v[i2], d = v[i3];
+        s += a;
+        s += b;
+        s += c;
+        s += d;
+    }
+    for (; e < e1; e++) s += v[idx[e]];
+    return s;
+}
+
 inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 }  // namespace asat

@@ -897,7 +897,7 @@ __global__ __launch_bounds__(BLK) void k_var(StageArgs<T> a)
             for (int e = r0 + threadIdx.x; e < r1; e += BLK) vals[e - r0] = a.contrib[a.csr_pos[e]];
             __syncthreads();
             const int lo = std::max(e0, r0), hi = std::min(e1, r1);
-            for (int e = lo; e < hi; e++) sum += vals[e - r0];
+            sum = sum_run(sum, vals, lo - r0, hi - r0);
             __syncthreads();
         }
         kj = sum;

@@ -217,9 +217,7 @@ __device__ __forceinline__ void small_var_one(const SmallLds<T>& L, int i, int e
     {
         T kj = (T)0;
         if (fresh) {
-            T sum = (T)0;
-            for (int e = e0; e < e1; e++) sum += L.cb[e];
-            kj = sum;
+            kj = sum_run((T)0, L.cb, e0, e1);
         }
         const T xi = L.x[i];
         if constexpr (J == 1) {
