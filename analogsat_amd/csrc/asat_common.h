@@ -11,6 +11,9 @@
 
 #include "analogsat_hip.h"
 
+#define ASAT_HD __host__ __device__
+#include "asat_powctrl.h"
+
 namespace asat {
 
 constexpr int GEN_KMAX = 32;      // longest clause supported by the generic kernels
@@ -90,38 +93,6 @@ struct TabD {
     static constexpr double TINY = 1e-20, SAFETY = 0.9, PGROW = -0.2, PSHRNK = -0.25, ERRCON = 1.89e-4,
                             HFACTOR = 0.1;
 };
-
-// ---------------------------------------------------------------------------
-// Correctly rounded pow for the step controller's exponents.
-// y = -1/q for q = 4 (PSHRNK = -0.25, exact) and q = 5 (PGROW = -0.2, which is
-// -(1/5 + d) with d = double(0.2) - 1/5 = 1.1102230246251565e-17).
-// Start from r = pow(x, y) (a few ulp), apply one Newton step for x r^q = 1
-// with x r^q formed in double-double (FMA two-product), and the first-order
-// factor x^-d = 1 - d ln x; the result is x^y to ~1e-30 relative, rounded once.
-// ---------------------------------------------------------------------------
-__host__ __device__ inline void dd_mul_d(double& hi, double& lo, double b)
-{
-    double p = hi * b;
-    double e = fma(hi, b, -p);
-    e = fma(lo, b, e);
-    double s = p + e;
-    lo = e - (s - p);
-    hi = s;
-}
-
-__host__ __device__ inline double pow_ctrl(double x, int q)
-{
-    const double y = q == 4 ? -0.25 : -0.2;
-    if (!(x > 0.0) || !(x < 1.0e300) || !(x > 1.0e-300)) return pow(x, y);
-    double r = pow(x, y);
-    double hi = r, lo = 0.0;
-    for (int i = 1; i < q; i++) dd_mul_d(hi, lo, r);
-    dd_mul_d(hi, lo, x);
-    double e = (1.0 - hi) - lo;              // 1 - x r^q, exact leading part (Sterbenz)
-    double corr = r * e / (double)q;
-    if (q == 5) corr -= r * (1.1102230246251565404e-17 * log(x));
-    return r + corr;
-}
 
 // stage input of stage J (2..6) from x and k1..k(J-1), in the reference's
 // operation order (AnalogSAT/cpu_sat/CpuRungeKutta.cpp:190-210)
@@ -219,6 +190,39 @@ __device__ __forceinline__ double wave_max(double v)
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
     return v;
+}
+
+// A lane permutation within rows of 16 lanes (DPP), for a double.  Every
+// lane of the wave must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// wave_max without LDS round trips: within each row of 16 lanes by DPP
+// (swap neighbours, swap pairs, half-row mirror, row mirror), then the four
+// row maxima read from lanes 0, 16, 32 and 48.  Same value as wave_max (the
+// maximum does not depend on the order), wave-uniform.  Every lane must be
+// active.
+__device__ __forceinline__ double wave_max_dpp(double v)
+{
+    v = fmax(v, dpp_f64<0xB1>(v));    // quad_perm [1,0,3,2]
+    v = fmax(v, dpp_f64<0x4E>(v));    // quad_perm [2,3,0,1]
+    v = fmax(v, dpp_f64<0x141>(v));   // row_half_mirror
+    v = fmax(v, dpp_f64<0x140>(v));   // row_mirror
+    return fmax(fmax(readlane_f64(v, 0), readlane_f64(v, 16)), fmax(readlane_f64(v, 32), readlane_f64(v, 48)));
 }
 
 __device__ __forceinline__ int wave_sum(int v)

@@ -76,10 +76,10 @@ Sections sections(int n, int m, int k, long long alloc, int esize)
 template <typename T, int KT, int BS>
 void launch(const SmallArgs& a, size_t lds)
 {
-    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_small<T, KT, BS>),
+    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_small<T, KT, BS, ASAT_FAMILY_CTDS>),
                                                        hipFuncAttributeMaxDynamicSharedMemorySize, SMALL_LDS_MAX);
     HIP_CHECK(attr);
-    hipLaunchKernelGGL((k_small<T, KT, BS>), dim3(1), dim3(BS), lds, 0, a);
+    hipLaunchKernelGGL((k_small<T, KT, BS, ASAT_FAMILY_CTDS>), dim3(1), dim3(BS), lds, 0, a);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -169,19 +169,14 @@ int solve_noalloc(int precision, const int* literals, const int* lengths, int nc
             const int l = literals[off + i];
             if (l == 0) continue;
             lit[(size_t)row * k + q++] = l;
-            vptr[std::abs(l) + 1]++;
         }
         if (q > 0) row++;
         off += lengths[j];
     }
-    for (int v = 1; v <= n; v++) vptr[v + 1] += vptr[v];
-    // fill cursors: vptr[v] is advanced while filling and restored after
-    for (long long s = 0; s < (long long)m * k; s++) {
-        const int l = lit[s];
-        spos[s] = l ? vptr[std::abs(l)]++ : -1;
-    }
-    for (int v = n; v >= 1; v--) vptr[v + 1] = vptr[v];
-    vptr[1] = 0;
+    long long nlist = 0;
+    const int runlen = small_build_lists(lit, nullptr, n, m, k, alloc, esize, vptr, spos, &nlist);
+    // small_build_lists takes uniform runs only where they fit in SMALL_LDS_MAX
+    const long long lds_run = small_lds_bytes(n, m, k, alloc, nlist, esize);
     HIP_CHECK(hipMemcpy(dev, st, sec.end, hipMemcpyHostToDevice));
 
     SmallArgs a;
@@ -196,14 +191,15 @@ int solve_noalloc(int precision, const int* literals, const int* lengths, int nc
     a.max_time = max_time;
     a.time_float = time_float;
     a.batch = batch;
-    a.n = n; a.m = m; a.k = k; a.aoff = (int)aoff; a.alloc = (int)alloc; a.nlist = (int)nl;
+    a.n = n; a.m = m; a.k = k; a.aoff = (int)aoff; a.alloc = (int)alloc; a.nlist = (int)nlist;
+    a.runlen = runlen;
     a.family = ASAT_FAMILY_CTDS;
     a.knorm = std::pow(2.0, -k);
     // relaunch until the loop ends (each launch runs a bounded number of attempts)
     long long last = 0;
     for (;;) {
-        if (esize == 8) launch_small<double>(a, (size_t)lds);
-        else launch_small<float>(a, (size_t)lds);
+        if (esize == 8) launch_small<double>(a, (size_t)lds_run);
+        else launch_small<float>(a, (size_t)lds_run);
         HIP_CHECK(hipMemcpy(st, dev, sec.x, hipMemcpyDeviceToHost));
         if (ss->result >= 0) break;
         require(c->attempts > last, ASAT_ERR_HIP, "integrator made no progress");

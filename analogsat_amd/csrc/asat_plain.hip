@@ -1229,6 +1229,7 @@ struct Sys {
     int* d_vptr = nullptr;
     int* d_spos_small = nullptr;
     int nlist = 0;
+    int runlen = 0;                // uniform run length of the lists (0: ragged)
     long long small_lds = 0;
     int vg = 0, ngroups = 0, vg_blocks = 0;
     int family = ASAT_FAMILY_CTDS;
@@ -1254,6 +1255,7 @@ struct Sys {
         dfree(d_vptr); dfree(d_spos_small);
         perm.clear();
         nlist = 0;
+        runlen = 0;
         small_lds = 0;
     }
     ~Sys()
@@ -1323,28 +1325,20 @@ static bool choose_layout(HostProblem& P, int esize)
 static void build_small_lists(Sys* s, const HostProblem& P)
 {
     const int n = P.n, m = P.m, K = P.k;
-    std::vector<int> vptr((size_t)n + 2, 0);
-    long long nl = 0;
-    for (long long i = 0; i < (long long)m * K; i++) {
-        const int l = P.lit[(size_t)i];
-        if (l) { vptr[(size_t)std::abs(l) + 1]++; nl++; }
+    std::vector<int> vptr((size_t)n + 2), spos((size_t)m * K), visit;
+    if (!P.perm.empty()) {
+        visit.resize((size_t)m);
+        for (int p = 0; p < m; p++) visit[(size_t)P.perm[p]] = p;
     }
+    long long nl = 0;
+    const int runlen = small_build_lists(P.lit.data(), visit.empty() ? nullptr : visit.data(), n, m, K, s->alloc,
+                                         s->esize, vptr.data(), spos.data(), &nl);
     const long long bytes = small_lds_bytes(n, m, K, s->alloc, nl, s->esize);
     if (bytes > SMALL_LDS_MAX) return;
-    for (int v = 1; v <= n; v++) vptr[(size_t)v + 1] += vptr[v];
-    std::vector<int> fill(vptr.begin(), vptr.end() - 1), spos((size_t)m * K, -1);
-    std::vector<int> proc((size_t)m);
-    for (int p = 0; p < m; p++) proc[(size_t)(P.perm.empty() ? p : P.perm[p])] = p;
-    for (int c = 0; c < m; c++) {
-        const int p = proc[(size_t)c];
-        for (int q = 0; q < K; q++) {
-            const int l = P.lit[(size_t)p * K + q];
-            if (l) spos[(size_t)p * K + q] = fill[(size_t)std::abs(l)]++;
-        }
-    }
     upload(s->d_vptr, vptr);
     upload(s->d_spos_small, spos);
     s->nlist = (int)nl;
+    s->runlen = runlen;
     s->small_lds = bytes;
 }
 
@@ -1867,24 +1861,27 @@ static bool small_path(const RK* r)
     return r->small_mode && s->small_lds > 0 && !(s->b > 0.0) && r->timer_kind < 0;
 }
 
-template <typename T, int KT, int BS>
+template <typename T, int KT, int BS, int FAM = ASAT_FAMILY_CTDS>
 static void small_kernel_launch(const SmallArgs& a, size_t lds, hipStream_t st)
 {
-    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_small<T, KT, BS>),
+    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_small<T, KT, BS, FAM>),
                                                        hipFuncAttributeMaxDynamicSharedMemorySize, SMALL_LDS_MAX);
     HIP_CHECK(attr);
-    hipLaunchKernelGGL((k_small<T, KT, BS>), dim3(1), dim3(BS), lds, st, a);
+    hipLaunchKernelGGL((k_small<T, KT, BS, FAM>), dim3(1), dim3(BS), lds, st, a);
     HIP_CHECK(hipGetLastError());
 }
 
 // 256 threads (a clause and a variable each) up to 512 clauses and
-// variables, 1024 threads above
+// variables, 1024 threads above; the tanh family runs the generic kernel
 template <typename T>
 static void small_launch(const Sys* s, const SmallArgs& a, hipStream_t st)
 {
     const size_t lds = (size_t)s->small_lds;
     const bool wide = std::max(s->m, s->n) > 512;
-    if (s->k == 3) {
+    if (s->family == ASAT_FAMILY_TANH) {
+        if (wide) small_kernel_launch<T, 0, 1024, ASAT_FAMILY_TANH>(a, lds, st);
+        else small_kernel_launch<T, 0, 256, ASAT_FAMILY_TANH>(a, lds, st);
+    } else if (s->k == 3) {
         if (wide) small_kernel_launch<T, 3, 1024>(a, lds, st);
         else small_kernel_launch<T, 3, 256>(a, lds, st);
     } else if (s->k == 4) {
@@ -1905,6 +1902,7 @@ static SmallArgs small_args(const RK* r, void* x)
     a.x = x;
     a.ctrl = r->d_ctrl;
     a.n = s->n; a.m = s->m; a.k = s->k; a.aoff = s->aoff; a.alloc = s->alloc; a.nlist = s->nlist;
+    a.runlen = s->runlen;
     a.family = s->family;
     a.q = s->q;
     a.knorm = std::pow(2.0, -s->k);

@@ -29,6 +29,13 @@
 //   control:   AnalogSAT/cpu_sat/CpuRungeKutta.cpp:217-267 in FP64
 // The bias term (b > 0) is not covered: such systems take the multi-kernel
 // path.
+//
+// The family is a template parameter (FAM): the whole attempt loop is inlined,
+// and a tanh branch compiled into every stage of the CTDS kernel would more
+// than double its code (about 10 700 instructions for the FP64 K = 3 kernel
+// against 3 800 without), past what the instruction cache holds, with one
+// wave per SIMD to hide the misses.  The tanh family is built in the generic
+// (KT = 0) form only.
 #pragma once
 
 #include "asat_common.h"
@@ -63,16 +70,98 @@ struct SmallArgs {
     int time_float;       // solve mode: compare the time as a float (TFloat = float)
     int batch;            // solve mode: SatSolver::batchSize
     int n, m, k, aoff, alloc, nlist;
-    int family;
+    int runlen;           // > 0: every variable's run is runlen long (uniform runs), 0: ragged runs
+    int family;           // selects the kernel instantiation (FAM)
     double q;
     double knorm;
 };
+
+// offset of the term lists after the stage spins (elements): 16-byte
+// aligned for the var phase's vector loads
+__host__ __device__ inline int small_cb_offset(int n) { return (n + 1 + 3) / 4 * 4; }
 
 // LDS bytes of the small path for a problem
 inline long long small_lds_bytes(int n, int m, int k, long long alloc, long long nlist, int esize)
 {
     const long long mk = (long long)m * k;
-    return (long long)esize * (7 * alloc + (n + 1) + nlist) + 4 * (2 * mk + (n + 2));
+    return (long long)esize * (7 * alloc + small_cb_offset(n) + nlist) + 4 * (2 * mk + (n + 2));
+}
+
+// Uniform runs: every variable's run of terms has the same length D, the
+// largest degree rounded up to SMALL_RUN_CHUNK, with its terms first and -0.0
+// after them.  x + (-0.0) == x for every x under round to nearest, so the
+// padding leaves each sum bit for bit as it was; the var phase then adds
+// whole chunks read with 16-byte loads, with no per-lane bounds.  Chosen when
+// the padded lists are at most SMALL_RUN_WASTE times the terms (plus a chunk
+// per variable) and fit in LDS.
+constexpr int SMALL_RUN_CHUNK = 8;
+constexpr int SMALL_RUN_WASTE = 2;
+
+// Var-major term lists of the one-workgroup kernel.  lit: m rows of k
+// literals in processing order (zero padded); visit: the rows in the order
+// the reference accumulates them (visit[c] = row of its clause c; null: row
+// order).  Writes vptr (n + 2: variable i's run is vptr[i] .. vptr[i+1]-1)
+// and spos (m * k: each slot's list position, -1 for padding).  Returns the
+// run length (0: ragged runs, each exactly the variable's degree) and the
+// list length in nlist.  Allocates nothing.
+inline int small_build_lists(const int* lit, const int* visit, int n, int m, int k, long long alloc, int esize,
+                             int* vptr, int* spos, long long* nlist)
+{
+    for (int v = 0; v < n + 2; v++) vptr[v] = 0;
+    long long nl = 0;
+    for (long long s = 0; s < (long long)m * k; s++)
+        if (lit[s]) { vptr[(lit[s] < 0 ? -lit[s] : lit[s]) + 1]++; nl++; }
+    int dmax = 0;
+    for (int v = 1; v <= n; v++) dmax = vptr[v + 1] > dmax ? vptr[v + 1] : dmax;
+    const long long d = (long long)(dmax + SMALL_RUN_CHUNK - 1) / SMALL_RUN_CHUNK * SMALL_RUN_CHUNK;
+    const bool uniform = d > 0 && (long long)n * d <= SMALL_RUN_WASTE * nl + (long long)SMALL_RUN_CHUNK * n &&
+                         (long long)n * d < (1LL << 30) &&
+                         small_lds_bytes(n, m, k, alloc, (long long)n * d, esize) <= SMALL_LDS_MAX;
+    // run starts
+    if (uniform) {
+        for (int v = 1; v <= n + 1; v++) vptr[v] = (int)((v - 1) * d);
+    } else {
+        for (int v = 1; v <= n; v++) vptr[v + 1] += vptr[v];
+    }
+    // fill: vptr[v] is the cursor of variable v while filling, restored after
+    for (int c = 0; c < m; c++) {
+        const long long r = visit ? visit[c] : c;
+        for (int q = 0; q < k; q++) {
+            const int l = lit[r * k + q];
+            spos[r * k + q] = l ? vptr[l < 0 ? -l : l]++ : -1;
+        }
+    }
+    if (uniform) {
+        for (int v = 1; v <= n; v++) vptr[v] = (int)((v - 1) * d);
+    } else {
+        for (int v = n; v >= 1; v--) vptr[v + 1] = vptr[v];
+        vptr[1] = 0;
+    }
+    *nlist = uniform ? (long long)n * d : nl;
+    return uniform ? (int)d : 0;
+}
+
+// sum of one uniform run (length a multiple of SMALL_RUN_CHUNK, 16-byte
+// aligned), left to right from 0
+template <typename T>
+__device__ __forceinline__ T sum_uniform(const T* run, int len)
+{
+    constexpr int V = 16 / sizeof(T);
+    typedef T vec __attribute__((ext_vector_type(V)));
+    T s = (T)0;
+#pragma unroll 2
+    for (int c = 0; c < len; c += SMALL_RUN_CHUNK) {
+        T t[SMALL_RUN_CHUNK];
+#pragma unroll
+        for (int q = 0; q < SMALL_RUN_CHUNK; q += V) {
+            const vec w = *(const vec*)(run + c + q);
+#pragma unroll
+            for (int u = 0; u < V; u++) t[q + u] = w[u];
+        }
+#pragma unroll
+        for (int q = 0; q < SMALL_RUN_CHUNK; q++) s += t[q];
+    }
+    return s;
 }
 
 template <typename T>
@@ -96,7 +185,7 @@ struct SmallReg {
 
 // one clause of the clause phase of stage J; lm / pm: its literals and its
 // slots' var-major positions (LDS pointers or register arrays)
-template <typename T, int KT, int J, typename LitT, typename PosT>
+template <typename T, int KT, int J, int FAM, typename LitT, typename PosT>
 __device__ __forceinline__ void small_clause_one(const SmallArgs& a, const SmallLds<T>& L, const T* sp_in,
                                                  const LitT& lm, const PosT& pm, int j, T hs, double h,
                                                  double hnext, double floor, T knorm, double& err)
@@ -139,7 +228,7 @@ __device__ __forceinline__ void small_clause_one(const SmallArgs& a, const Small
             L.xn[ai] = xn;
         }
         T* out = L.cb;
-        if (a.family == ASAT_FAMILY_TANH) {
+        if constexpr (FAM == ASAT_FAMILY_TANH) {
             const T r = knorm * am;
             const bool ex = K >= 3 && K <= 10;
 #pragma unroll 4
@@ -188,7 +277,7 @@ __device__ __forceinline__ void small_clause_one(const SmallArgs& a, const Small
 
 // clause phase of stage J: thread per clause; sp_in holds the stage's spins
 // (x itself for J = 1)
-template <typename T, int KT, int BS, int J>
+template <typename T, int KT, int BS, int J, int FAM>
 __device__ __forceinline__ void small_clause(const SmallArgs& a, const SmallLds<T>& L, const SmallReg<KT, BS>& R,
                                              const T* sp_in, T hs, double h, double hnext, double floor, T knorm,
                                              double& err)
@@ -198,12 +287,12 @@ __device__ __forceinline__ void small_clause(const SmallArgs& a, const SmallLds<
         for (int u = 0; u < SMALL_RPT; u++) {
             const int j = (int)threadIdx.x + u * BS;
             if (j < a.m)
-                small_clause_one<T, KT, J>(a, L, sp_in, R.lit[u], R.pos[u], j, hs, h, hnext, floor, knorm, err);
+                small_clause_one<T, KT, J, FAM>(a, L, sp_in, R.lit[u], R.pos[u], j, hs, h, hnext, floor, knorm, err);
         }
     } else {
         const int K = KT ? KT : a.k;
         for (int j = (int)threadIdx.x; j < a.m; j += BS)
-            small_clause_one<T, KT, J>(a, L, sp_in, L.lit + j * K, L.spos + j * K, j, hs, h, hnext, floor, knorm,
+            small_clause_one<T, KT, J, FAM>(a, L, sp_in, L.lit + j * K, L.spos + j * K, j, hs, h, hnext, floor, knorm,
                                        err);
     }
 }
@@ -211,13 +300,13 @@ __device__ __forceinline__ void small_clause(const SmallArgs& a, const SmallLds<
 // var phase of stage J: thread per variable; fresh = false only on a retry's
 // J = 1 (k1 kept from the step's first attempt)
 template <typename T, int J>
-__device__ __forceinline__ void small_var_one(const SmallLds<T>& L, int i, int e0, int e1, bool fresh, T hs,
+__device__ __forceinline__ void small_var_one(const SmallLds<T>& L, int i, int e0, int e1, int runlen, bool fresh, T hs,
                                               double h, double hnext, double floor, double& err)
 {
     {
         T kj = (T)0;
         if (fresh) {
-            kj = sum_run((T)0, L.cb, e0, e1);
+            kj = runlen ? sum_uniform(L.cb + e0, runlen) : sum_run((T)0, L.cb, e0, e1);
         }
         const T xi = L.x[i];
         if constexpr (J == 1) {
@@ -254,19 +343,44 @@ __device__ __forceinline__ void small_var(const SmallArgs& a, const SmallLds<T>&
 #pragma unroll
         for (int u = 0; u < SMALL_RPT; u++) {
             const int i = (int)threadIdx.x + 1 + u * BS;
-            if (i <= a.n) small_var_one<T, J>(L, i, R.e0[u], R.e1[u], fresh, hs, h, hnext, floor, err);
+            if (i <= a.n) small_var_one<T, J>(L, i, R.e0[u], R.e1[u], a.runlen, fresh, hs, h, hnext, floor, err);
         }
     } else {
         for (int i = (int)threadIdx.x + 1; i <= a.n; i += BS)
-            small_var_one<T, J>(L, i, L.vptr[i], L.vptr[i + 1], fresh, hs, h, hnext, floor, err);
+            small_var_one<T, J>(L, i, L.vptr[i], L.vptr[i + 1], a.runlen, fresh, hs, h, hnext, floor, err);
     }
 }
 
 // violated clauses of the thresholded state (k_violations semantics), in
-// every thread
+// every thread.  REG mode: the literals come from registers, every spin load
+// of a clause is issued at once, and each wave counts its violated clauses
+// with one ballot per clause slot.
 template <typename T, int KT, int BS>
-__device__ __forceinline__ int small_violations(const SmallArgs& a, const SmallLds<T>& L, int* shi)
+__device__ __forceinline__ int small_violations(const SmallArgs& a, const SmallLds<T>& L, const SmallReg<KT, BS>& R,
+                                                int* shi)
 {
+    if constexpr (SmallReg<KT, BS>::on) {
+        int cnt = 0;
+#pragma unroll
+        for (int u = 0; u < SMALL_RPT; u++) {
+            const int j = (int)threadIdx.x + u * BS;
+            bool sat = false;
+#pragma unroll
+            for (int q = 0; q < KT; q++) {
+                const int l = R.lit[u][q];   // 0: padding literal, constant FALSE (reads the spin slot 0)
+                const bool t = L.x[l < 0 ? -l : l] > (T)0;
+                sat = sat || (l > 0 && t) || (l < 0 && !t);
+            }
+            cnt += (int)__popcll(__ballot(j < a.m && !sat));
+        }
+        if ((threadIdx.x & 63) == 0) shi[threadIdx.x >> 6] = cnt;
+        __syncthreads();
+        int r = 0;
+#pragma unroll
+        for (int w = 0; w < BS / 64; w++) r += shi[w];
+        __syncthreads();
+        return r;
+    }
     const int K = KT ? KT : a.k;
     int viol = 0;
     for (int j = (int)threadIdx.x; j < a.m; j += BS) {
@@ -282,7 +396,21 @@ __device__ __forceinline__ int small_violations(const SmallArgs& a, const SmallL
     return block_sum_all<BS>(viol, shi);
 }
 
-template <typename T, int KT, int BS>
+// block maximum in every thread, the wave maxima taken by DPP
+template <int BS>
+__device__ __forceinline__ double small_block_max(double v, double* sh)
+{
+    v = wave_max_dpp(v);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = sh[0];
+#pragma unroll
+    for (int w = 1; w < BS / 64; w++) r = fmax(r, sh[w]);
+    __syncthreads();
+    return r;
+}
+
+template <typename T, int KT, int BS, int FAM>
 __global__ __launch_bounds__(BS) void k_small(SmallArgs a)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char small_lds[];
@@ -296,7 +424,7 @@ __global__ __launch_bounds__(BS) void k_small(SmallArgs a)
     L.k1 = base + 2 * alloc; L.k2 = base + 3 * alloc; L.k3 = base + 4 * alloc; L.k4 = base + 5 * alloc;
     L.k5 = base + 6 * alloc;
     L.sp = base + 7 * alloc;
-    L.cb = L.sp + (n + 1);
+    L.cb = L.sp + small_cb_offset(n);
     L.lit = (int*)(L.cb + a.nlist);
     L.spos = L.lit + mk;
     L.vptr = L.spos + mk;
@@ -310,7 +438,7 @@ __global__ __launch_bounds__(BS) void k_small(SmallArgs a)
         }
         for (int i = (int)threadIdx.x; i <= n; i += BS) L.sp[i] = (T)0;
         for (int i = (int)threadIdx.x; i < mk; i += BS) { L.lit[i] = a.lit[i]; L.spos[i] = a.spos[i]; }
-        for (int i = (int)threadIdx.x; i < a.nlist; i += BS) L.cb[i] = (T)0;
+        for (int i = (int)threadIdx.x; i < a.nlist; i += BS) L.cb[i] = (T)-0.0;   // uniform runs' padding
         for (int i = (int)threadIdx.x; i < n + 2; i += BS) L.vptr[i] = a.vptr[i];
     }
     SmallReg<KT, BS> R;
@@ -356,32 +484,32 @@ __global__ __launch_bounds__(BS) void k_small(SmallArgs a)
         if (step_start && launched >= SMALL_ATTEMPT_CAP) break;
         const T hs = (T)h;
         double err = 0.0;
-        if (step_start) small_clause<T, KT, BS, 1>(a, L, R, L.x, hs, h, hnext, floor, knorm, err);
+        if (step_start) small_clause<T, KT, BS, 1, FAM>(a, L, R, L.x, hs, h, hnext, floor, knorm, err);
         __syncthreads();
         small_var<T, KT, BS, 1>(a, L, R, step_start != 0, hs, h, hnext, floor, err);
         __syncthreads();
-        small_clause<T, KT, BS, 2>(a, L, R, L.sp, hs, h, hnext, floor, knorm, err);
+        small_clause<T, KT, BS, 2, FAM>(a, L, R, L.sp, hs, h, hnext, floor, knorm, err);
         __syncthreads();
         small_var<T, KT, BS, 2>(a, L, R, true, hs, h, hnext, floor, err);
         __syncthreads();
-        small_clause<T, KT, BS, 3>(a, L, R, L.sp, hs, h, hnext, floor, knorm, err);
+        small_clause<T, KT, BS, 3, FAM>(a, L, R, L.sp, hs, h, hnext, floor, knorm, err);
         __syncthreads();
         small_var<T, KT, BS, 3>(a, L, R, true, hs, h, hnext, floor, err);
         __syncthreads();
-        small_clause<T, KT, BS, 4>(a, L, R, L.sp, hs, h, hnext, floor, knorm, err);
+        small_clause<T, KT, BS, 4, FAM>(a, L, R, L.sp, hs, h, hnext, floor, knorm, err);
         __syncthreads();
         small_var<T, KT, BS, 4>(a, L, R, true, hs, h, hnext, floor, err);
         __syncthreads();
-        small_clause<T, KT, BS, 5>(a, L, R, L.sp, hs, h, hnext, floor, knorm, err);
+        small_clause<T, KT, BS, 5, FAM>(a, L, R, L.sp, hs, h, hnext, floor, knorm, err);
         __syncthreads();
         small_var<T, KT, BS, 5>(a, L, R, true, hs, h, hnext, floor, err);
         __syncthreads();
-        small_clause<T, KT, BS, 6>(a, L, R, L.sp, hs, h, hnext, floor, knorm, err);
+        small_clause<T, KT, BS, 6, FAM>(a, L, R, L.sp, hs, h, hnext, floor, knorm, err);
         __syncthreads();
         small_var<T, KT, BS, 6>(a, L, R, true, hs, h, hnext, floor, err);
         // block maximum (its barriers also order the x_new writes before any
         // read of the next attempt)
-        const double emax = block_max_all<BS>(err, shd) / eps;
+        const double emax = small_block_max<BS>(err, shd) / eps;
         launched++;
         errmax = emax;
         attempts += 1;
@@ -399,7 +527,7 @@ __global__ __launch_bounds__(BS) void k_small(SmallArgs a)
             if (a.solve) {
                 if (--ss.bleft == 0) {
                     ss.solver_steps += a.batch;
-                    const int v = small_violations<T, KT, BS>(a, L, shi);
+                    const int v = small_violations<T, KT, BS>(a, L, R, shi);
                     ss.last_viol = v;
                     if (v == 0) { ss.result = SMALL_SAT; done = true; }
                 }

@@ -5,7 +5,10 @@ SatSolver's loop (no callback) in one persistent kernel.  Bar:
   * states, h, time, errmax and every counter bit-identical to the
     multi-kernel path (forced with SetOneWorkgroup(False)) on the same
     problem and start, FP64 and FP32, across several Step calls, for K = 3,
-    K = 4, mixed clause lengths with padding, K = 12 and the tanh family
+    K = 4, mixed clause lengths with padding, K = 12, the tanh family and a
+    problem with one variable in every clause (its term lists stay ragged:
+    padding every variable's run to the hub's length would cost more than
+    twice the terms; the others take uniform runs padded with -0.0)
   * the device-side SatSolver loop gives the reference's own solve of the C1
     fixture (result, steps, RHS count, final state: AnalogSAT/solver/SatSolver.h:313-355)
     and, on problems that end by SAT, by the step limit and by the time
@@ -35,6 +38,13 @@ def _problem(case):
         return O.fastsat(50, 213, 3, 0)
     if case == "k4":
         return O.fastsat(40, 400, 4, 3)
+    if case == "hub":
+        rng = np.random.default_rng(13)
+        lits = []
+        for _ in range(200):
+            v = np.concatenate([[1], rng.choice(np.arange(2, 41), size=2, replace=False)])
+            lits.extend((v * rng.choice([-1, 1], size=3)).tolist())
+        return O.Problem(np.array(lits, np.int32), np.full(200, 3, np.int32))
     rng = np.random.default_rng(11 if case == "mixed" else 12)
     n, m = (60, 250) if case == "mixed" else (40, 90)
     lens = rng.integers(1, 6, m) if case == "mixed" else np.full(m, 12)
@@ -67,7 +77,7 @@ def _pair(p, prec, family):
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("case,family", [("k3", "ctds"), ("k4", "ctds"), ("mixed", "ctds"), ("k12", "ctds"),
-                                         ("k3", "tanh"), ("mixed", "tanh")])
+                                         ("hub", "ctds"), ("k3", "tanh"), ("mixed", "tanh")])
 def test_one_workgroup_equals_multi_kernel(case, family, prec):
     say("one workgroup vs multi-kernel: %s %s %s" % (case, family, prec))
     p = _problem(case)
