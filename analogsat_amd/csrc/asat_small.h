@@ -12,9 +12,10 @@
 // itself from the block-wide error maximum, so no thread waits for a
 // broadcast.  Optionally the SatSolver loop runs on the device too: after
 // every `batch` accepted steps the violated-clause count of the thresholded
-// state is taken and the run ends on SAT, on the step or time limit, or on
-// step-size underflow (AnalogSAT/solver/SatSolver.h:313-331, the branch
-// without a callback).
+// state is taken (by the next step's first clause phase, which reads those
+// spins anyway, with one ballot per wave) and the run ends on SAT, on the
+// step or time limit, or on step-size underflow
+// (AnalogSAT/solver/SatSolver.h:313-331, the branch without a callback).
 //
 // Arithmetic follows the multi-kernel path operation for operation (the
 // generic clause pass k_clause_gen and the var passes' var_stage), so the
@@ -51,7 +52,7 @@ struct SmallSolve {
     int bleft;                // accepted steps left in the current batch (0: at the top of the loop)
     int result;               // -1 while running, else a SatResult code
     int last_viol;            // SatSolver::lastViolationCount
-    int pad_;
+    int check;                // a batch just ended: count the violated clauses of x before going on
 };
 
 // SatResult codes (AnalogSAT/solver/SatResult.h:24-32)
@@ -84,7 +85,7 @@ __host__ __device__ inline int small_cb_offset(int n) { return (n + 1 + 3) / 4 *
 inline long long small_lds_bytes(int n, int m, int k, long long alloc, long long nlist, int esize)
 {
     const long long mk = (long long)m * k;
-    return (long long)esize * (7 * alloc + small_cb_offset(n) + nlist) + 4 * (2 * mk + (n + 2));
+    return (long long)esize * (7 * alloc + small_cb_offset(n) + nlist + 1) + 4 * (2 * mk + (n + 2));
 }
 
 // Uniform runs: every variable's run of terms has the same length D, the
@@ -101,9 +102,10 @@ constexpr int SMALL_RUN_WASTE = 2;
 // literals in processing order (zero padded); visit: the rows in the order
 // the reference accumulates them (visit[c] = row of its clause c; null: row
 // order).  Writes vptr (n + 2: variable i's run is vptr[i] .. vptr[i+1]-1)
-// and spos (m * k: each slot's list position, -1 for padding).  Returns the
-// run length (0: ragged runs, each exactly the variable's degree) and the
-// list length in nlist.  Allocates nothing.
+// and spos (m * k: each slot's list position; a padding slot gets nlist, a
+// dummy entry after the lists that the clause phase may write and nothing
+// reads).  Returns the run length (0: ragged runs, each exactly the
+// variable's degree) and the list length in nlist.  Allocates nothing.
 inline int small_build_lists(const int* lit, const int* visit, int n, int m, int k, long long alloc, int esize,
                              int* vptr, int* spos, long long* nlist)
 {
@@ -123,12 +125,14 @@ inline int small_build_lists(const int* lit, const int* visit, int n, int m, int
     } else {
         for (int v = 1; v <= n; v++) vptr[v + 1] += vptr[v];
     }
-    // fill: vptr[v] is the cursor of variable v while filling, restored after
+    // fill: vptr[v] is the cursor of variable v while filling, restored
+    // after; padding slots get the dummy entry after the lists
+    const int dummy = (int)(uniform ? (long long)n * d : nl);
     for (int c = 0; c < m; c++) {
         const long long r = visit ? visit[c] : c;
         for (int q = 0; q < k; q++) {
             const int l = lit[r * k + q];
-            spos[r * k + q] = l ? vptr[l < 0 ? -l : l]++ : -1;
+            spos[r * k + q] = l ? vptr[l < 0 ? -l : l]++ : dummy;
         }
     }
     if (uniform) {
@@ -142,24 +146,31 @@ inline int small_build_lists(const int* lit, const int* visit, int n, int m, int
 }
 
 // sum of one uniform run (length a multiple of SMALL_RUN_CHUNK, 16-byte
-// aligned), left to right from 0
+// aligned), left to right from 0; the loads of up to four chunks are issued
+// before the first add
 template <typename T>
 __device__ __forceinline__ T sum_uniform(const T* run, int len)
 {
     constexpr int V = 16 / sizeof(T);
     typedef T vec __attribute__((ext_vector_type(V)));
+    constexpr int C = SMALL_RUN_CHUNK, B = 4;   // up to B chunks in flight (len is wave-uniform)
     T s = (T)0;
-#pragma unroll 2
-    for (int c = 0; c < len; c += SMALL_RUN_CHUNK) {
-        T t[SMALL_RUN_CHUNK];
+    for (int c = 0; c < len; c += B * C) {
+        T t[B * C];
 #pragma unroll
-        for (int q = 0; q < SMALL_RUN_CHUNK; q += V) {
-            const vec w = *(const vec*)(run + c + q);
+        for (int b = 0; b < B; b++)
+            if (b == 0 || c + b * C < len)
 #pragma unroll
-            for (int u = 0; u < V; u++) t[q + u] = w[u];
-        }
+                for (int q = 0; q < C; q += V) {
+                    const vec w = *(const vec*)(run + c + b * C + q);
 #pragma unroll
-        for (int q = 0; q < SMALL_RUN_CHUNK; q++) s += t[q];
+                    for (int u = 0; u < V; u++) t[b * C + q + u] = w[u];
+                }
+#pragma unroll
+        for (int b = 0; b < B; b++)
+            if (b == 0 || c + b * C < len)
+#pragma unroll
+                for (int q = 0; q < C; q++) s += t[b * C + q];
     }
     return s;
 }
@@ -186,18 +197,37 @@ struct SmallReg {
 // one clause of the clause phase of stage J; lm / pm: its literals and its
 // slots' var-major positions (LDS pointers or register arrays)
 template <typename T, int KT, int J, int FAM, typename LitT, typename PosT>
-__device__ __forceinline__ void small_clause_one(const SmallArgs& a, const SmallLds<T>& L, const T* sp_in,
+__device__ __forceinline__ bool small_clause_one(const SmallArgs& a, const SmallLds<T>& L, const T* sp_in,
                                                  const LitT& lm, const PosT& pm, int j, T hs, double h,
-                                                 double hnext, double floor, T knorm, double& err)
+                                                 double hnext, double floor, T knorm, double& err, bool count)
 {
     constexpr int KM = KT ? KT : GEN_KMAX;
     const int K = KT ? KT : a.k;
+    // J = 1 with count: is the clause violated by the thresholded spins
+    // (k_violations semantics: literal true when its spin is > 0, resp. not
+    // > 0 when negated; a padding literal is FALSE; it reads spin slot 0)
+    bool violated = false;
+    if constexpr (J == 1) {
+        if (count) {
+            bool sat = false;
+            for (int q = 0; q < K; q++) {
+                const int l = lm[q];
+                const bool t = sp_in[l < 0 ? -l : l] > (T)0;
+                sat = sat || (l > 0 && t) || (l < 0 && !t);
+            }
+            violated = !sat;
+        }
+    }
     {
         T f[KM];
         T km = knorm;
 #pragma unroll 4
         for (int q = 0; q < K; q++) {
-            const T fq = spin_factor<T>(lm[q], sp_in);
+            // spin_factor without its branch: slot 0 of x and of the stage
+            // spins holds -1, the padding literal's spin
+            const int l = lm[q];
+            const T sv = sp_in[l < 0 ? -l : l];
+            const T fq = l < 0 ? (T)1.0 + sv : (T)1.0 - sv;
             km *= fq;
             f[q] = fq;
         }
@@ -246,11 +276,12 @@ __device__ __forceinline__ void small_clause_one(const SmallArgs& a, const Small
                 out[pm[q]] = tanh_term<T>(r, p, f[q], (T)a.q, l < 0, ex);
             }
         } else if (K <= 10) {
+            // padding literals (zero padded at the end of the row) store
+            // into the dummy list entry, which nothing reads
             const T r = km * knorm * am;
 #pragma unroll 4
             for (int q = 0; q < K; q++) {
                 const int l = lm[q];
-                if (l == 0) break;
                 T p = (T)1.0;
                 bool first = true;
 #pragma unroll 4
@@ -273,28 +304,38 @@ __device__ __forceinline__ void small_clause_one(const SmallArgs& a, const Small
             }
         }
     }
+    return violated;
 }
 
 // clause phase of stage J: thread per clause; sp_in holds the stage's spins
-// (x itself for J = 1)
+// (x itself for J = 1).  count (J = 1): also count the violated clauses of x;
+// the wave's count, in lane 0 (lane 0 takes part in every pass of its wave
+// over the clauses), is returned.
 template <typename T, int KT, int BS, int J, int FAM>
-__device__ __forceinline__ void small_clause(const SmallArgs& a, const SmallLds<T>& L, const SmallReg<KT, BS>& R,
-                                             const T* sp_in, T hs, double h, double hnext, double floor, T knorm,
-                                             double& err)
+__device__ __forceinline__ int small_clause(const SmallArgs& a, const SmallLds<T>& L, const SmallReg<KT, BS>& R,
+                                            const T* sp_in, T hs, double h, double hnext, double floor, T knorm,
+                                            double& err, bool count = false)
 {
+    int cnt = 0;
     if constexpr (SmallReg<KT, BS>::on) {
 #pragma unroll
         for (int u = 0; u < SMALL_RPT; u++) {
             const int j = (int)threadIdx.x + u * BS;
-            if (j < a.m)
-                small_clause_one<T, KT, J, FAM>(a, L, sp_in, R.lit[u], R.pos[u], j, hs, h, hnext, floor, knorm, err);
+            if (j < a.m) {
+                const bool v = small_clause_one<T, KT, J, FAM>(a, L, sp_in, R.lit[u], R.pos[u], j, hs, h, hnext,
+                                                               floor, knorm, err, count);
+                if (count) cnt += (int)__popcll(__ballot(v));
+            }
         }
     } else {
         const int K = KT ? KT : a.k;
-        for (int j = (int)threadIdx.x; j < a.m; j += BS)
-            small_clause_one<T, KT, J, FAM>(a, L, sp_in, L.lit + j * K, L.spos + j * K, j, hs, h, hnext, floor, knorm,
-                                       err);
+        for (int j = (int)threadIdx.x; j < a.m; j += BS) {
+            const bool v = small_clause_one<T, KT, J, FAM>(a, L, sp_in, L.lit + j * K, L.spos + j * K, j, hs, h,
+                                                           hnext, floor, knorm, err, count);
+            if (count) cnt += (int)__popcll(__ballot(v));
+        }
     }
+    return cnt;
 }
 
 // var phase of stage J: thread per variable; fresh = false only on a retry's
@@ -303,35 +344,38 @@ template <typename T, int J>
 __device__ __forceinline__ void small_var_one(const SmallLds<T>& L, int i, int e0, int e1, int runlen, bool fresh, T hs,
                                               double h, double hnext, double floor, double& err)
 {
-    {
-        T kj = (T)0;
-        if (fresh) {
-            kj = runlen ? sum_uniform(L.cb + e0, runlen) : sum_run((T)0, L.cb, e0, e1);
-        }
-        const T xi = L.x[i];
-        if constexpr (J == 1) {
-            if (fresh) L.k1[i] = kj;
-            else kj = L.k1[i];
-            L.sp[i] = stage_input<T, 2>(xi, kj, 0, 0, 0, 0, hs);
-        } else if constexpr (J == 2) {
-            L.k2[i] = kj;
-            L.sp[i] = stage_input<T, 3>(xi, L.k1[i], kj, 0, 0, 0, hs);
-        } else if constexpr (J == 3) {
-            L.k3[i] = kj;
-            L.sp[i] = stage_input<T, 4>(xi, L.k1[i], L.k2[i], kj, 0, 0, hs);
-        } else if constexpr (J == 4) {
-            L.k4[i] = kj;
-            L.sp[i] = stage_input<T, 5>(xi, L.k1[i], L.k2[i], L.k3[i], kj, 0, hs);
-        } else if constexpr (J == 5) {
-            L.k5[i] = kj;
-            L.sp[i] = stage_input<T, 6>(xi, L.k1[i], L.k2[i], L.k3[i], L.k4[i], kj, hs);
-        } else {
-            T xn;
-            double e;
-            final_combination<T>(xi, L.k1[i], L.k3[i], L.k4[i], L.k5[i], kj, hs, h, hnext, floor, xn, e);
-            err = fmax(err, e);
-            L.xn[i] = xn;
-        }
+    // the stage combination's inputs, loaded ahead of the sum so that one
+    // LDS latency covers both
+    const T xi = L.x[i];
+    const T c1 = L.k1[i];
+    const T c2 = J > 2 ? L.k2[i] : (T)0;
+    const T c3 = J > 3 ? L.k3[i] : (T)0;
+    const T c4 = J > 4 ? L.k4[i] : (T)0;
+    const T c5 = J > 5 ? L.k5[i] : (T)0;
+    T kj = (T)0;
+    if (fresh) kj = runlen ? sum_uniform(L.cb + e0, runlen) : sum_run((T)0, L.cb, e0, e1);
+    if constexpr (J == 1) {
+        if (fresh) L.k1[i] = kj;
+        else kj = c1;
+        L.sp[i] = stage_input<T, 2>(xi, kj, 0, 0, 0, 0, hs);
+    } else if constexpr (J == 2) {
+        L.k2[i] = kj;
+        L.sp[i] = stage_input<T, 3>(xi, c1, kj, 0, 0, 0, hs);
+    } else if constexpr (J == 3) {
+        L.k3[i] = kj;
+        L.sp[i] = stage_input<T, 4>(xi, c1, c2, kj, 0, 0, hs);
+    } else if constexpr (J == 4) {
+        L.k4[i] = kj;
+        L.sp[i] = stage_input<T, 5>(xi, c1, c2, c3, kj, 0, hs);
+    } else if constexpr (J == 5) {
+        L.k5[i] = kj;
+        L.sp[i] = stage_input<T, 6>(xi, c1, c2, c3, c4, kj, hs);
+    } else {
+        T xn;
+        double e;
+        final_combination<T>(xi, c1, c3, c4, c5, kj, hs, h, hnext, floor, xn, e);
+        err = fmax(err, e);
+        L.xn[i] = xn;
     }
 }
 
@@ -349,51 +393,6 @@ __device__ __forceinline__ void small_var(const SmallArgs& a, const SmallLds<T>&
         for (int i = (int)threadIdx.x + 1; i <= a.n; i += BS)
             small_var_one<T, J>(L, i, L.vptr[i], L.vptr[i + 1], a.runlen, fresh, hs, h, hnext, floor, err);
     }
-}
-
-// violated clauses of the thresholded state (k_violations semantics), in
-// every thread.  REG mode: the literals come from registers, every spin load
-// of a clause is issued at once, and each wave counts its violated clauses
-// with one ballot per clause slot.
-template <typename T, int KT, int BS>
-__device__ __forceinline__ int small_violations(const SmallArgs& a, const SmallLds<T>& L, const SmallReg<KT, BS>& R,
-                                                int* shi)
-{
-    if constexpr (SmallReg<KT, BS>::on) {
-        int cnt = 0;
-#pragma unroll
-        for (int u = 0; u < SMALL_RPT; u++) {
-            const int j = (int)threadIdx.x + u * BS;
-            bool sat = false;
-#pragma unroll
-            for (int q = 0; q < KT; q++) {
-                const int l = R.lit[u][q];   // 0: padding literal, constant FALSE (reads the spin slot 0)
-                const bool t = L.x[l < 0 ? -l : l] > (T)0;
-                sat = sat || (l > 0 && t) || (l < 0 && !t);
-            }
-            cnt += (int)__popcll(__ballot(j < a.m && !sat));
-        }
-        if ((threadIdx.x & 63) == 0) shi[threadIdx.x >> 6] = cnt;
-        __syncthreads();
-        int r = 0;
-#pragma unroll
-        for (int w = 0; w < BS / 64; w++) r += shi[w];
-        __syncthreads();
-        return r;
-    }
-    const int K = KT ? KT : a.k;
-    int viol = 0;
-    for (int j = (int)threadIdx.x; j < a.m; j += BS) {
-        int sat = 0;
-        for (int q = 0; q < K && !sat; q++) {
-            const int l = L.lit[j * K + q];
-            if (l == 0) continue;   // padding literal is constant FALSE
-            const int t = L.x[l < 0 ? -l : l] > (T)0;
-            sat = l < 0 ? !t : t;
-        }
-        viol += !sat;
-    }
-    return block_sum_all<BS>(viol, shi);
 }
 
 // block maximum in every thread, the wave maxima taken by DPP
@@ -425,20 +424,20 @@ __global__ __launch_bounds__(BS) void k_small(SmallArgs a)
     L.k5 = base + 6 * alloc;
     L.sp = base + 7 * alloc;
     L.cb = L.sp + small_cb_offset(n);
-    L.lit = (int*)(L.cb + a.nlist);
+    L.lit = (int*)(L.cb + a.nlist + 1);   // after the lists and their dummy entry
     L.spos = L.lit + mk;
     L.vptr = L.spos + mk;
     {
         const T* gx = (const T*)a.x;
         for (int i = (int)threadIdx.x; i < alloc; i += BS) {
-            const T v = gx[i];
+            const T v = i == 0 ? (T)-1.0 : gx[i];   // slot 0: the padding literal's spin
             L.x[i] = v;
             L.xn[i] = v;
             L.k1[i] = L.k2[i] = L.k3[i] = L.k4[i] = L.k5[i] = (T)0;
         }
-        for (int i = (int)threadIdx.x; i <= n; i += BS) L.sp[i] = (T)0;
+        for (int i = (int)threadIdx.x; i <= n; i += BS) L.sp[i] = i == 0 ? (T)-1.0 : (T)0;   // slot 0: padding spin
         for (int i = (int)threadIdx.x; i < mk; i += BS) { L.lit[i] = a.lit[i]; L.spos[i] = a.spos[i]; }
-        for (int i = (int)threadIdx.x; i < a.nlist; i += BS) L.cb[i] = (T)-0.0;   // uniform runs' padding
+        for (int i = (int)threadIdx.x; i <= a.nlist; i += BS) L.cb[i] = (T)-0.0;   // uniform runs' padding
         for (int i = (int)threadIdx.x; i < n + 2; i += BS) L.vptr[i] = a.vptr[i];
     }
     SmallReg<KT, BS> R;
@@ -471,6 +470,30 @@ __global__ __launch_bounds__(BS) void k_small(SmallArgs a)
     long long launched = 0;
     bool done = underflow || (a.solve ? ss.result >= 0 : steps >= a.target);
     while (!done) {
+        const T hs = (T)h;
+        double err = 0.0;
+        // When the previous accepted step ended a batch, this step's first
+        // clause phase (which reads every clause's spins of x) also counts
+        // the violated clauses of x; SAT is decided on that count before the
+        // loop's limits are checked, as in SatSolver (SatSolver.h:313-331).
+        // An attempt that ends the loop here leaves x, the clock and the
+        // counters as they were.
+        const bool check = a.solve && ss.check;
+        int cnt = 0;
+        if (step_start) cnt = small_clause<T, KT, BS, 1, FAM>(a, L, R, L.x, hs, h, hnext, floor, knorm, err, check);
+        if (check && (threadIdx.x & 63) == 0) shi[threadIdx.x >> 6] = cnt;
+        __syncthreads();
+        if (check) {
+            int v = 0;
+#pragma unroll
+            for (int w = 0; w < BS / 64; w++) v += shi[w];
+            ss.check = 0;
+            ss.last_viol = v;
+            if (v == 0) {
+                ss.result = SMALL_SAT;
+                break;
+            }
+        }
         if (a.solve && ss.bleft == 0) {
             // top of the SatSolver loop: while (time < maxTime && steps < maxSteps)
             const double tc = a.time_float ? (double)(float)time : time;
@@ -482,10 +505,6 @@ __global__ __launch_bounds__(BS) void k_small(SmallArgs a)
         }
         // pause between steps (k1 of the current x is not carried over)
         if (step_start && launched >= SMALL_ATTEMPT_CAP) break;
-        const T hs = (T)h;
-        double err = 0.0;
-        if (step_start) small_clause<T, KT, BS, 1, FAM>(a, L, R, L.x, hs, h, hnext, floor, knorm, err);
-        __syncthreads();
         small_var<T, KT, BS, 1>(a, L, R, step_start != 0, hs, h, hnext, floor, err);
         __syncthreads();
         small_clause<T, KT, BS, 2, FAM>(a, L, R, L.sp, hs, h, hnext, floor, knorm, err);
@@ -527,9 +546,7 @@ __global__ __launch_bounds__(BS) void k_small(SmallArgs a)
             if (a.solve) {
                 if (--ss.bleft == 0) {
                     ss.solver_steps += a.batch;
-                    const int v = small_violations<T, KT, BS>(a, L, R, shi);
-                    ss.last_viol = v;
-                    if (v == 0) { ss.result = SMALL_SAT; done = true; }
+                    ss.check = 1;
                 }
             } else if (steps >= a.target) {
                 done = true;
