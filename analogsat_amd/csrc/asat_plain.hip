@@ -94,6 +94,17 @@ struct HostProblem {
     std::vector<uint16_t> srank;   // per slot: its rank in the tile's sorted list, the literal's sign
                                    // in bit 15, 0xFFFF for padding; per tile, literal q of every
                                    // clause first (index tile*TC*K + q*TC + clause in the tile)
+    // primary variables (tiled pass): pvar[j] is the primary variable of
+    // processing-order clause j and pq[j] the first of its literals on it
+    std::vector<int> pvar;
+    std::vector<unsigned char> pq;
+    // direct primary slots (tiled pass with LDS regions): a clause's primary
+    // slot is not in any region; its spin is read from the stage vector by
+    // variable and its term stored at the clause's processing position of
+    // pcontrib.  Regions hold the other slots.  pc0[g] is the first clause
+    // whose primary variable is in group g (ngroups+1 entries).
+    bool split = false;
+    std::vector<int> pc0;
 };
 
 // SatProblem::AddClauses semantics (AnalogSAT/problem/SatProblem.cpp:64-95) +
@@ -157,6 +168,8 @@ static void reorder_clauses(HostProblem& P)
         prim[j] = best;
         load[best]++;
     }
+    P.pvar.assign((size_t)m, 0);
+    P.pq.assign((size_t)m, 0);
     std::vector<int> start((size_t)n + 2, 0);
     for (int j = 0; j < m; j++) start[(size_t)prim[j] + 1]++;
     for (int v = 0; v <= n; v++) start[v + 1] += start[v];
@@ -166,34 +179,59 @@ static void reorder_clauses(HostProblem& P)
     for (int j = 0; j < m; j++)
         std::copy_n(P.lit.begin() + (size_t)P.perm[j] * K, K, lit2.begin() + (size_t)j * K);
     P.lit.swap(lit2);
+    for (int j = 0; j < m; j++) {
+        const int v = prim[(size_t)P.perm[j]];
+        P.pvar[(size_t)j] = v;
+        int q = 0;
+        while (std::abs(P.lit[(size_t)j * K + q]) != v) q++;
+        P.pq[(size_t)j] = (unsigned char)q;
+    }
 }
 
 // Contribution layout for groups of vg variables, and the transposed
 // var-major slot list (role of CudaSat3::InitProblem,
 // AnalogSAT/cuda_sat/CudaSat3.cu:266-307, without the pad-to-32).
-static void build_layout(HostProblem& P, int vg)
+// pv > 0 (tiled pass with primary variables): direct primary slots.  The
+// regions then hold only the other slots, and a variable's list entry for a
+// primary slot points past its group's region: the LDS var pass loads the
+// group's primary terms (pcontrib from pc0[g] rounded down to a multiple of pv
+// elements, its 16-byte load unit) right behind the region.
+static void build_layout(HostProblem& P, int vg, int pv)
 {
-    const int n = P.n;
+    const int n = P.n, K = P.k;
+    const bool split = pv > 0 && !P.pvar.empty();
+    P.split = split;
     P.vg = vg;
     P.ngroups = (n + vg - 1) / vg;
+    auto is_prim = [&](size_t s) { return split && P.pq[s / K] == (unsigned char)(s % K); };
     std::vector<long long> gcount((size_t)P.ngroups, 0);
     for (size_t s = 0; s < P.lit.size(); s++) {
         int l = P.lit[s];
-        if (l) gcount[(size_t)(std::abs(l) - 1) / vg]++;
+        if (l && !is_prim(s)) gcount[(size_t)(std::abs(l) - 1) / vg]++;
+    }
+    P.pc0.clear();
+    if (split) {
+        // clauses are sorted by primary variable: group g's primaries are the
+        // processing range [pc0[g], pc0[g+1])
+        P.pc0.assign((size_t)P.ngroups + 1, P.m);
+        for (int j = P.m - 1; j >= 0; j--) P.pc0[(size_t)(P.pvar[(size_t)j] - 1) / vg] = j;
+        for (int g = P.ngroups - 1; g >= 0; g--) P.pc0[g] = std::min(P.pc0[g], P.pc0[(size_t)g + 1]);
     }
     P.gstart.assign((size_t)P.ngroups + 1, 0);
     P.max_region = 0;
     for (int g = 0; g < P.ngroups; g++) {
         const long long sz = round_up(std::max<long long>(gcount[g], 1), REGION_ALIGN);
         P.gstart[g + 1] = P.gstart[g] + sz;
-        P.max_region = std::max(P.max_region, sz);
+        long long lds = sz;
+        if (split) lds += round_up(P.pc0[(size_t)g + 1] - (P.pc0[g] / pv) * pv, pv);
+        P.max_region = std::max(P.max_region, lds);
     }
     require(P.gstart[P.ngroups] < (1LL << 31), ASAT_ERR_UNSUPPORTED, "contribution array exceeds 2^31 entries");
     P.pos.assign(P.lit.size(), -1);
     std::vector<long long> gfill(P.gstart.begin(), P.gstart.end() - 1);
     for (size_t s = 0; s < P.lit.size(); s++) {
         int l = P.lit[s];
-        if (l) P.pos[s] = (int)gfill[(size_t)(std::abs(l) - 1) / vg]++;
+        if (l && !is_prim(s)) P.pos[s] = (int)gfill[(size_t)(std::abs(l) - 1) / vg]++;
     }
     // per-variable lists of contribution positions; slots visited in the
     // original clause order, which is the reference's accumulation order
@@ -204,22 +242,31 @@ static void build_layout(HostProblem& P, int vg)
     }
     for (int i = 0; i <= n; i++) P.csr_ptr[i + 1] += P.csr_ptr[i];
     P.csr_pos.assign((size_t)P.csr_ptr[n + 1], 0);
+    std::vector<long long> cloc(split ? P.csr_pos.size() : 0);
     std::vector<int> fill(P.csr_ptr.begin(), P.csr_ptr.end() - 1);
     std::vector<int> row((size_t)P.m);
     for (int j = 0; j < P.m; j++) row[P.perm.empty() ? j : P.perm[j]] = j;
-    const int K = P.k;
     for (int mo = 0; mo < P.m; mo++)
         for (int q = 0; q < K; q++) {
             const size_t s = (size_t)row[mo] * K + q;
             int l = P.lit[s];
-            if (l) P.csr_pos[(size_t)fill[std::abs(l)]++] = P.pos[s];
+            if (!l) continue;
+            const int v = std::abs(l);
+            const int e = fill[v]++;
+            P.csr_pos[(size_t)e] = P.pos[s];
+            if (split) {
+                const size_t g = (size_t)(v - 1) / vg;
+                cloc[(size_t)e] = is_prim(s) ? (P.gstart[g + 1] - P.gstart[g]) + row[mo] - (P.pc0[g] / pv) * pv
+                                             : P.pos[s] - P.gstart[g];
+            }
         }
     P.csr_loc.clear();
     if (P.max_region <= 65536) {
         P.csr_loc.resize(P.csr_pos.size());
         for (int i = 1; i <= n; i++) {
             const long long base = P.gstart[(size_t)(i - 1) / vg];
-            for (int e = P.csr_ptr[i]; e < P.csr_ptr[i + 1]; e++) P.csr_loc[e] = (uint16_t)(P.csr_pos[e] - base);
+            for (int e = P.csr_ptr[i]; e < P.csr_ptr[i + 1]; e++)
+                P.csr_loc[e] = (uint16_t)(split ? cloc[(size_t)e] : P.csr_pos[e] - base);
         }
     }
 }
@@ -227,23 +274,27 @@ static void build_layout(HostProblem& P, int vg)
 // Per tile of TC clauses: the tile's slots in ascending position order.
 // Positions inside a region follow slot order, so this is a stable counting
 // sort of the tile's slots by variable group; the clause pass then writes
-// each group's run of contributions with neighbouring lanes.
+// each group's run of contributions with neighbouring lanes.  With direct
+// primary slots only the other slots are sorted (K - 1 per clause), and a
+// primary slot's rank entry is 0x4000 with the sign bit.
 static void build_tile_order(HostProblem& P)
 {
     const int K = P.k;
+    const int NR = P.split ? K - 1 : K;
     const long long ntiles = ceil_div((long long)P.m, TC);
-    const size_t per = (size_t)TC * K;
-    P.spos.assign((size_t)ntiles * per, -1);
+    const size_t per = (size_t)TC * K, nper = (size_t)TC * NR;
+    P.spos.assign((size_t)ntiles * nper, -1);
     P.srank.assign((size_t)ntiles * per, 0xFFFF);
     std::vector<int> cnt((size_t)P.ngroups + 1, 0);
     std::vector<int> touched;
     touched.reserve(per);
     for (long long t = 0; t < ntiles; t++) {
         const size_t s0 = (size_t)t * per, s1 = std::min(P.lit.size(), s0 + per);
+        const size_t r0 = (size_t)t * nper;
         touched.clear();
         for (size_t s = s0; s < s1; s++) {
             int l = P.lit[s];
-            if (!l) continue;
+            if (!l || P.pos[s] < 0) continue;
             int g = (std::abs(l) - 1) / P.vg;
             if (cnt[g]++ == 0) touched.push_back(g);
         }
@@ -253,11 +304,17 @@ static void build_tile_order(HostProblem& P)
         for (size_t s = s0; s < s1; s++) {
             int l = P.lit[s];
             if (!l) continue;
-            int g = (std::abs(l) - 1) / P.vg;
-            const size_t j = s0 + (size_t)cnt[g]++;
-            P.spos[j] = P.pos[s];
             const size_t ls = s - s0;   // tile-local slot: clause ls / K, literal ls % K
-            P.srank[s0 + (ls % K) * TC + ls / K] = (uint16_t)((j - s0) | (l < 0 ? 0x8000u : 0u));
+            const unsigned sign = l < 0 ? 0x8000u : 0u;
+            uint16_t& rk = P.srank[s0 + (ls % K) * TC + ls / K];
+            if (P.pos[s] < 0) {   // a direct primary slot
+                rk = (uint16_t)(0x4000u | sign);
+                continue;
+            }
+            int g = (std::abs(l) - 1) / P.vg;
+            const size_t j = (size_t)cnt[g]++;
+            P.spos[r0 + j] = P.pos[s];
+            rk = (uint16_t)(j | sign);
         }
         for (int g : touched) cnt[g] = 0;
     }
@@ -276,6 +333,9 @@ struct StageArgs {
     const int* csr_pos;    // per-variable contribution positions (gather var pass)
     const uint16_t* csr_loc;   // the same relative to the region start (LDS var pass)
     const long long* gstart;   // region starts, ngroups+1
+    const int* pvar;       // direct primary slots: primary variable of each clause (processing order)
+    const int* pc0;        // direct primary slots: first clause of each group's primaries, ngroups+1
+    T* pcontrib;           // direct primary slots: each clause's primary term (processing order)
     T* contrib;
     T* xa;       // state buffer 0
     T* xb;       // state buffer 1 (x = parity ? xb : xa; x_new is the other)
@@ -296,6 +356,7 @@ struct StageArgs {
     int ngroups;
     int clause_per_xcd;   // clause-pass chunks (BLK clauses, or TC-clause tiles) per XCD
     int ntiles;           // tiled pass: tiles of TC clauses
+    int split;            // direct primary slots (tiled pass + LDS var pass)
     int family;           // ASAT_FAMILY_CTDS or ASAT_FAMILY_TANH
     int bias;             // sine bias term on (b > 0)
     T knorm;
@@ -363,13 +424,19 @@ __device__ __forceinline__ void st_stream(V* p, V v)
 }
 
 // ---- clause pass, compile-time K (Ex order), tiled -----------------------
+// Slot ranks: bits 0-12 the rank in the tile's sorted list, bit 14 a direct
+// primary slot (no rank), bit 15 the literal's sign, 0xFFFF padding.
+constexpr unsigned RANK_MASK = 0x1FFFu;
+constexpr unsigned RANK_PRIMARY = 0x4000u;
+
 // Slot terms of one clause, written to LDS at the slots' ranks in the tile's
-// position order.  CTDS: r = K_m 2^-k a_m times the product of the other
-// factors in KmiProd order, times +-2; tanh: tanh_term with r = 2^-k a_m.
-// Stops at the first padding literal (KmiFunc / KmiFuncTanh).
-template <typename T, int K, int Q>
+// position order (a direct primary slot's term goes to pt instead).  CTDS:
+// r = K_m 2^-k a_m times the product of the other factors in KmiProd order,
+// times +-2; tanh: tanh_term with r = 2^-k a_m.  Stops at the first padding
+// literal (KmiFunc / KmiFuncTanh).
+template <typename T, int K, int Q, bool PS>
 __device__ __forceinline__ void slot_terms(const int (&l)[K], const T (&f)[K], T r, bool tanh_family, T q,
-                                           const uint16_t* rank, T* s_out)
+                                           const uint16_t* rank, T* s_out, T& pt)
 {
     if constexpr (Q < K) {
         if (l[Q] != 0) {
@@ -381,31 +448,37 @@ __device__ __forceinline__ void slot_terms(const int (&l)[K], const T (&f)[K], T
                 rr = r * p;
                 rr *= l[Q] < 0 ? (T)-2.0 : (T)2.0;
             }
-            s_out[rank[Q] & 0x7FFF] = rr;
-            slot_terms<T, K, Q + 1>(l, f, r, tanh_family, q, rank, s_out);
+            if (PS && (rank[Q] & RANK_PRIMARY)) pt = rr;
+            else s_out[rank[Q] & RANK_MASK] = rr;
+            slot_terms<T, K, Q + 1, PS>(l, f, r, tanh_family, q, rank, s_out, pt);
         }
     }
 }
 
 // Phase 1 of one tile (see k_clause_tile): slot terms into s_out at their
-// ranks, this thread's share of the tile's sorted positions into ps.
+// ranks (direct primary terms into pt), this thread's share of the tile's
+// sorted positions into ps.
 // SS (slot spins): sp is contrib holding the stage spins in contribution
 // layout; the tile reads them through its sorted positions (the same runs
 // its phase 2 then overwrites with the slot terms) into s_out at their
 // ranks, and each clause thread takes its slots' spins from there (the
-// literal signs come from the ranks' bit 15).  Otherwise sp is the
-// N-element stage spin vector, gathered by variable.
-template <typename T, int K, int J, int BT, bool SS>
+// literal signs come from the ranks' bit 15).  A direct primary slot's spin
+// is read from the stage spin vector (x at J = 1, tmp after) at the clause's
+// primary variable: clauses are sorted by it, so a wave reads a few lines.
+// Otherwise sp is the N-element stage spin vector, gathered by variable.
+template <typename T, int K, int J, int BT, bool SS, bool PS>
 __device__ __forceinline__ void tile_phase1(const StageArgs<T>& a, long long c0, const T* x, T* xnew,
                                             const T* sp, T hs, double h, double hnext, double floor,
-                                            bool tanh_family, T* s_out, int (&ps)[TC / BT * K], double& err,
-                                            double& asum)
+                                            bool tanh_family, T* s_out, int (&ps)[TC / BT * (PS ? K - 1 : K)],
+                                            T (&pt)[TC / BT], double& err, double& asum)
 {
     constexpr int CPT = TC / BT;
+    constexpr int NR = PS ? K - 1 : K;
     int l[CPT][K];
     T sv[CPT][K];
     T ax[CPT], a1[CPT], a2[CPT], a3[CPT], a4[CPT], a5[CPT];
     uint16_t rk[CPT][K];
+    int pv[CPT];
     // every load that does not depend on another is issued first: the
     // literal rows and slot ranks (padded to whole tiles, read unguarded),
     // this thread's share of the tile's sorted positions and the aux values
@@ -418,10 +491,11 @@ __device__ __forceinline__ void tile_phase1(const StageArgs<T>& a, long long c0,
         }
 #pragma unroll
         for (int q = 0; q < K; q++) rk[u][q] = ld_g(a.srank + c0 * K + q * TC + (m - c0));
+        if constexpr (SS && PS) pv[u] = ld_g(a.pvar + m);
     }
-    const int* sp_t = a.spos + c0 * K;
+    const int* sp_t = a.spos + c0 * NR;
 #pragma unroll
-    for (int i = 0; i < CPT * K; i++) ps[i] = ld_g(sp_t + i * BT + threadIdx.x);
+    for (int i = 0; i < CPT * NR; i++) ps[i] = ld_g(sp_t + i * BT + threadIdx.x);
 #pragma unroll
     for (int u = 0; u < CPT; u++) {
         const long long m = c0 + u * BT + threadIdx.x;
@@ -437,11 +511,18 @@ __device__ __forceinline__ void tile_phase1(const StageArgs<T>& a, long long c0,
         }
     }
     if constexpr (SS) {
-        T so[CPT * K];
+        T so[CPT * NR];
 #pragma unroll
-        for (int i = 0; i < CPT * K; i++) so[i] = ps[i] >= 0 ? sp[ps[i]] : (T)0;
+        for (int i = 0; i < CPT * NR; i++) so[i] = ps[i] >= 0 ? sp[ps[i]] : (T)0;
+        T psv[CPT];
+        if constexpr (PS) {
+            // padding clauses have primary variable 0: x[0] / tmp[0], unused
+            const T* svec = (J == 1) ? x : a.tmp;
 #pragma unroll
-        for (int i = 0; i < CPT * K; i++) s_out[i * BT + threadIdx.x] = so[i];
+            for (int u = 0; u < CPT; u++) psv[u] = ld_g(svec + pv[u]);
+        }
+#pragma unroll
+        for (int i = 0; i < CPT * NR; i++) s_out[i * BT + threadIdx.x] = so[i];
         __syncthreads();
         // each rank is read here and overwritten with its slot term below by
         // the same thread (the owner of the slot), so one buffer serves both
@@ -451,7 +532,9 @@ __device__ __forceinline__ void tile_phase1(const StageArgs<T>& a, long long c0,
             for (int q = 0; q < K; q++) {
                 const unsigned r = rk[u][q];
                 l[u][q] = r == 0xFFFFu ? 0 : ((r & 0x8000u) ? -1 : 1);
-                sv[u][q] = r == 0xFFFFu ? (T)-1.0 : s_out[r & 0x7FFFu];
+                if (r == 0xFFFFu) sv[u][q] = (T)-1.0;
+                else if (PS && (r & RANK_PRIMARY)) sv[u][q] = psv[u];
+                else sv[u][q] = s_out[r & RANK_MASK];
             }
         }
     } else {
@@ -468,6 +551,7 @@ __device__ __forceinline__ void tile_phase1(const StageArgs<T>& a, long long c0,
     for (int u = 0; u < CPT; u++) {
         const int cl = u * BT + threadIdx.x;
         const long long m = c0 + cl;
+        pt[u] = (T)0;
         if (m < a.m) {
             T f[K];
             T km = a.knorm;
@@ -495,7 +579,7 @@ __device__ __forceinline__ void tile_phase1(const StageArgs<T>& a, long long c0,
                 st_g(xnew + ai, xn);
             }
             const T r = tanh_family ? a.knorm * am : km * a.knorm * am;
-            slot_terms<T, K, 0>(l[u], f, r, tanh_family, a.q, rk[u], s_out);
+            slot_terms<T, K, 0, PS>(l[u], f, r, tanh_family, a.q, rk[u], s_out, pt[u]);
         }
     }
 }
@@ -506,15 +590,17 @@ __device__ __forceinline__ void tile_phase1(const StageArgs<T>& a, long long c0,
 // loads are in flight together (with slot spins, the spins arrive through
 // the tile's sorted positions and LDS); then K_m, the aux stage input,
 // k_J(a_m) and the clause's slot terms, which go to LDS at the slots' ranks
-// in the tile's position order.  Phase 2, a thread per rank: store the
-// terms, so that neighbouring lanes write neighbouring positions of one
-// region.
-template <typename T, int K, int J, int BT, bool SS>
+// in the tile's position order.  Direct primary terms (PS) are stored at the
+// clauses' own positions of pcontrib: neighbouring lanes, neighbouring
+// entries.  Phase 2, a thread per rank: store the terms, so that
+// neighbouring lanes write neighbouring positions of one region.
+template <typename T, int K, int J, int BT, bool SS, bool PS>
 __global__ __launch_bounds__(BT) void k_clause_tile(StageArgs<T> a)
 {
-    constexpr int NP = TC / BT * K;
+    constexpr int CPT = TC / BT;
+    constexpr int NP = CPT * (PS ? K - 1 : K);
     __shared__ double sh[BT / 64];
-    __shared__ T s_out[TC * K];
+    __shared__ T s_out[TC * (PS ? K - 1 : K)];
     const Ctrl* c = a.ctrl;
     int parity = 0;
     double h = 0.0, hnext = 0.0, floor = TabD::TINY;
@@ -533,8 +619,15 @@ __global__ __launch_bounds__(BT) void k_clause_tile(StageArgs<T> a)
     const int tile = (b & 7) * a.clause_per_xcd + (b >> 3);
     if (tile < a.ntiles) {
         int ps[NP];
-        tile_phase1<T, K, J, BT, SS>(a, (long long)tile * TC, x, xnew, sp, hs, h, hnext, floor, tanh_family, s_out, ps,
-                                 err, asum);
+        T pt[CPT];
+        const long long c0 = (long long)tile * TC;
+        tile_phase1<T, K, J, BT, SS, PS>(a, c0, x, xnew, sp, hs, h, hnext, floor, tanh_family, s_out, ps, pt, err,
+                                         asum);
+        if constexpr (PS) {
+            // pcontrib is padded to whole tiles: padding clauses store 0
+#pragma unroll
+            for (int u = 0; u < CPT; u++) st_g(a.pcontrib + c0 + u * BT + threadIdx.x, pt[u]);
+        }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < NP; i++)
@@ -778,6 +871,7 @@ __global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
     T* s_next = a.slot_spins ? a.contrib : nullptr;
     const long long r0 = a.gstart[g];
     const int nreg = (int)(a.gstart[g + 1] - r0) / V;
+    const int rlen = nreg * V;
     // each thread's state values, loaded before the region (groups of at most
     // 2 * VBLK variables; larger groups load them in the loop)
     constexpr int VP = 2;
@@ -801,23 +895,33 @@ __global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
         ebase = e_first & ~7;
         const int nc = (e_end - ebase + 7) >> 3;
         const vT* src = (const vT*)(a.contrib + r0);
+        // direct primary slots: the group's primary terms follow the region
+        // in LDS, read from the 16-byte unit holding its first clause
+        const vT* psrc = src;
+        int npv = 0;
+        if (a.split && fresh) {
+            const int pb = a.pc0[g] / V * V;
+            psrc = (const vT*)(a.pcontrib + pb);
+            npv = (a.pc0[g + 1] - pb + V - 1) / V;
+        }
+        const int nt = nv + npv;
         const v4u* csrc = (const v4u*)(a.csr_loc + ebase);
         vT* dst = (vT*)reg;
         v4u* cdst = (v4u*)loc;
-        const int nmax = nv > nc ? nv : nc;
+        const int nmax = nt > nc ? nt : nc;
         for (int base = 0; base < nmax; base += R * VBLK) {
             vT rb[R];
             v4u cb[R];
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const int v = base + r * VBLK + (int)threadIdx.x;
-                if (v < nv) rb[r] = ld_g(src + v);
+                if (v < nt) rb[r] = ld_g(v < nv ? src + v : psrc + (v - nv));
                 if (v < nc) cb[r] = ld_g(csrc + v);
             }
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const int v = base + r * VBLK + (int)threadIdx.x;
-                if (v < nv) dst[v] = rb[r];
+                if (v < nt) dst[v] = rb[r];
                 if (v < nc) cdst[v] = cb[r];
             }
         }
@@ -839,8 +943,11 @@ __global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
         if (pre) in = u == 0 ? vin[0] : vin[1];
         else in = var_load<T, J>(a, i, x, fresh);
         const T out = var_stage<T, J>(a, i, fresh, kj, in, xnew, hs, h, hnext, floor, err);
-        if (s_next)
-            for (int e = e0; e < e1; e++) reg[loc[e]] = out;
+        if (s_next)   // region entries only (direct primary entries lie past the region)
+            for (int e = e0; e < e1; e++) {
+                const int o = loc[e];
+                if (o < rlen) reg[o] = out;
+            }
     }
     if (s_next) {
         // the region's entries not owned by a variable (alignment padding)
@@ -1117,8 +1224,13 @@ __global__ __launch_bounds__(BLK) void k_slot_fill(const T* x, const int* csr_pt
     const int i = blockIdx.x * BLK + (int)threadIdx.x + 1;
     if (i > n) return;
     const T v = x[i];
-    T* base = sl + gstart[(i - 1) / vg];
-    for (int e = csr_ptr[i]; e < csr_ptr[i + 1]; e++) base[csr_loc[e]] = v;
+    const int g = (i - 1) / vg;
+    T* base = sl + gstart[g];
+    const long long rlen = gstart[g + 1] - gstart[g];   // direct primary entries lie past the region
+    for (int e = csr_ptr[i]; e < csr_ptr[i + 1]; e++) {
+        const int o = csr_loc[e];
+        if (o < rlen) base[o] = v;
+    }
 }
 
 template <typename T>
@@ -1220,6 +1332,10 @@ struct Sys {
     int* d_csr_pos = nullptr;    // gather var pass only
     uint16_t* d_csr_loc = nullptr;   // LDS var pass only
     long long* d_gstart = nullptr;
+    int* d_pvar = nullptr;       // direct primary slots: primary variable per clause
+    int* d_pc0 = nullptr;        // direct primary slots: first clause of each group's primaries
+    void* d_pcontrib = nullptr;  // direct primary slots: primary term per clause
+    bool split = false;          // direct primary slots (tiled pass + LDS var pass)
     int* d_perm = nullptr;       // processing position -> clause index (null: identity)
     std::vector<int> perm;       // host copy, for the public <-> internal a-block order
     // one-workgroup path (asat_small.h) for problems whose integrator fits in
@@ -1252,6 +1368,8 @@ struct Sys {
     {
         dfree(d_lit); dfree(d_pos); dfree(d_spos); dfree(d_srank); dfree(d_csr_ptr); dfree(d_csr_pos);
         dfree(d_csr_loc); dfree(d_gstart); dfree(d_perm); dfree(d_contrib); dfree(d_apart);
+        dfree(d_pvar); dfree(d_pc0); dfree(d_pcontrib);
+        split = false;
         dfree(d_vptr); dfree(d_spos_small);
         perm.clear();
         nlist = 0;
@@ -1307,13 +1425,16 @@ static bool choose_layout(HostProblem& P, int esize)
     long long G = std::max(ceil_div((long long)P.n, target), std::min(256, ceil_div((long long)P.n, 256)));
     if (G > 256) G = round_up(G, 256);
     int vg = ceil_div((long long)P.n, G);
+    // direct primary slots wherever the tiled pass meets the LDS var pass
+    // (primary variables exist only for the tiled pass's clause order)
+    const int pv = 16 / esize;
     for (;;) {
-        build_layout(P, vg);
+        build_layout(P, vg, pv);
         if (P.max_region <= cap && !P.csr_loc.empty()) return true;
         if (vg <= 64) break;
         vg = (vg + 1) / 2;
     }
-    build_layout(P, esize == 4 ? 8192 : 4096);
+    build_layout(P, esize == 4 ? 8192 : 4096, 0);
     return false;
 }
 
@@ -1363,6 +1484,8 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
     s->vg_blocks = ceil_div(P.vg, BLK);
     s->tiled = tiled;
     s->lds_var = lds_var;
+    s->split = P.split;
+    require(!P.split || (tiled && lds_var), ASAT_ERR_INVALID_ARGUMENT, "direct primary slots without their passes");
     // few variables with long rows (dense problems such as Ramsey encodings):
     // the gather kernel would add a block's rows one after another, so each
     // variable gets a workgroup of its own
@@ -1375,6 +1498,15 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
     if (tiled) {
         upload(s->d_spos, P.spos);
         upload(s->d_srank, P.srank);
+        if (P.split) {
+            P.pvar.resize((size_t)round_up(P.m, TC), 0);   // padding clauses: variable 0
+            upload(s->d_pvar, P.pvar);
+            upload(s->d_pc0, P.pc0);
+            // one term per clause padded to whole tiles, plus a 16-byte unit
+            const size_t np = (size_t)round_up(P.m, TC) + 16;
+            HIP_CHECK(hipMalloc(&s->d_pcontrib, (size_t)s->esize * np));
+            HIP_CHECK(hipMemset(s->d_pcontrib, 0, (size_t)s->esize * np));
+        }
     } else {
         P.pos.resize((size_t)round_up(P.m, TC) * P.k, -1);
         upload(s->d_pos, P.pos);
@@ -1467,6 +1599,7 @@ static StageArgs<T> stage_args(Sys* s, T* xa, T* xb, T* const* kk, T* tmp, doubl
     memset(&a, 0, sizeof(a));   // padding too: the arguments are compared bytewise (captured attempts)
     a.lit = s->d_lit; a.pos = s->d_pos; a.spos = s->d_spos; a.srank = s->d_srank;
     a.csr_ptr = s->d_csr_ptr; a.csr_pos = s->d_csr_pos; a.csr_loc = s->d_csr_loc; a.gstart = s->d_gstart;
+    a.pvar = s->d_pvar; a.pc0 = s->d_pc0; a.pcontrib = (T*)s->d_pcontrib; a.split = s->split ? 1 : 0;
     a.contrib = (T*)s->d_contrib;
     a.xa = xa; a.xb = xb;
     a.k1 = kk[0]; a.k2 = kk[1]; a.k3 = kk[2]; a.k4 = kk[3]; a.k5 = kk[4];
@@ -1497,11 +1630,15 @@ static void launch_clause(const StageArgs<T>& a, const Sys* s, hipStream_t st)
     dim3 g(a.nblk_clause), b(BLK);
     const bool tiled = s->tiled;
     const bool ss = a.slot_spins != 0;
-    if (tiled && a.k == 3 && ss) hipLaunchKernelGGL((k_clause_tile<T, 3, J, TILE_BT, true>), g, dim3(TILE_BT), 0, st, a);
-    else if (tiled && a.k == 4 && ss)
-        hipLaunchKernelGGL((k_clause_tile<T, 4, J, TILE_BT, true>), g, dim3(TILE_BT), 0, st, a);
-    else if (tiled && a.k == 3) hipLaunchKernelGGL((k_clause_tile<T, 3, J, TILE_BT, false>), g, dim3(TILE_BT), 0, st, a);
-    else if (tiled && a.k == 4) hipLaunchKernelGGL((k_clause_tile<T, 4, J, TILE_BT, false>), g, dim3(TILE_BT), 0, st, a);
+    const bool ps = a.split != 0;   // slot spins come with direct primary slots (both need the LDS var pass)
+    require(!ss || ps, ASAT_ERR_INVALID_ARGUMENT, "slot spins without direct primary slots");
+    const dim3 tb(TILE_BT);
+    if (tiled && a.k == 3 && ss) hipLaunchKernelGGL((k_clause_tile<T, 3, J, TILE_BT, true, true>), g, tb, 0, st, a);
+    else if (tiled && a.k == 4 && ss) hipLaunchKernelGGL((k_clause_tile<T, 4, J, TILE_BT, true, true>), g, tb, 0, st, a);
+    else if (tiled && a.k == 3 && ps) hipLaunchKernelGGL((k_clause_tile<T, 3, J, TILE_BT, false, true>), g, tb, 0, st, a);
+    else if (tiled && a.k == 4 && ps) hipLaunchKernelGGL((k_clause_tile<T, 4, J, TILE_BT, false, true>), g, tb, 0, st, a);
+    else if (tiled && a.k == 3) hipLaunchKernelGGL((k_clause_tile<T, 3, J, TILE_BT, false, false>), g, tb, 0, st, a);
+    else if (tiled && a.k == 4) hipLaunchKernelGGL((k_clause_tile<T, 4, J, TILE_BT, false, false>), g, tb, 0, st, a);
     else hipLaunchKernelGGL((k_clause_gen<T, J>), g, b, 0, st, a);
 }
 
@@ -2347,6 +2484,7 @@ int asat_sys_layout(const asat_sys_t* sys, asat_layout_t* out)
     out->var_grid = var_grid(s);
     out->var_lds_bytes = s->var_lds;
     out->slot_spins = (s->tiled && s->lds_var) ? 1 : 0;
+    out->direct_primary = s->split ? 1 : 0;
     API_END
 }
 

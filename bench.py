@@ -178,22 +178,26 @@ def kernel_bytes(n, m, k, nslots, esize=4):
     touches, once; index arrays int32/uint16, state arrays of esize bytes),
     reported as bytes_impl beside the algorithmic figure.
 
-    clause pass J (tiled, K = 3, 4, slot spins): per slot a sorted position
-    (4) and a rank with the literal's sign (2) read, the slot's stage spin
-    (esize) read through the sorted position and one contribution (esize)
-    stored; per clause A_J aux values (x_a and k_1..k_{J-1} read, k_J or x_new
-    written).
+    clause pass J (tiled, K = 3, 4, slot spins, direct primary slots): per
+    slot a rank with the literal's sign (2) read; per slot other than its
+    clause's primary one a sorted position (4) and the slot's stage spin
+    (esize) read through it and one contribution (esize) stored; per clause
+    its primary variable (4) read and its primary term (esize) stored, plus
+    the N-element stage spin vector the primary spins come from (esize per
+    variable); per clause A_J aux values (x_a and k_1..k_{J-1} read, k_J or
+    x_new written).
     var pass J (LDS regions): per slot a contribution (esize) and a 16-bit
-    region offset (2) read and the next stage's slot spin (esize) written;
-    per variable a row pointer (4) and B_J state values.
+    offset (2) read, per non-primary slot the next stage's slot spin (esize)
+    written; per variable a row pointer (4) and B_J state values.
     """
     L = nslots
     A = (2, 3, 4, 5, 6, 7)
     B = (3, 4, 5, 6, 7, 6)
     out = {}
     for j in range(6):
-        out["clause%d" % (j + 1)] = (6 + 2 * esize) * L + esize * m * A[j]
-        out["var%d" % (j + 1)] = (2 + 2 * esize) * L + 4 * (n + 1) + esize * n * B[j]
+        out["clause%d" % (j + 1)] = (2 * L + (4 + 2 * esize) * (L - m) + (4 + esize) * m + esize * n
+                                     + esize * m * A[j])
+        out["var%d" % (j + 1)] = (2 + esize) * L + esize * (L - m) + 4 * (n + 1) + esize * n * B[j]
     out["control"] = 8 * (-(-m // 1024) + -(-n // 1024))
     return out
 

@@ -136,6 +136,7 @@ typedef struct {
     int var_grid;        /* workgroups of one var pass */
     int var_lds_bytes;   /* dynamic LDS of the region var pass */
     int slot_spins;      /* 1: the integrator carries the stage spins in contribution layout */
+    int direct_primary;  /* 1: each clause's primary slot bypasses the regions (own spin read and term store) */
 } asat_layout_t;
 int asat_sys_layout(const asat_sys_t* sys, asat_layout_t* out);
 
