@@ -934,8 +934,20 @@ __global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
         T kj = (T)0;
         const int e0 = a.csr_ptr[i] - ebase, e1 = a.csr_ptr[i + 1] - ebase;
         if (fresh) {
+            // four entries in flight per round (offsets, then values), added
+            // left to right; lanes past their row read element 0 and skip it
             T sum = (T)0;
-            for (int e = e0; e < e1; e++) sum += reg[loc[e]];
+            for (int e = e0; e < e1; e += 4) {
+                int o[4];
+                T v[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) o[q] = e + q < e1 ? loc[e + q] : 0;
+#pragma unroll
+                for (int q = 0; q < 4; q++) v[q] = reg[o[q]];
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    if (e + q < e1) sum += v[q];
+            }
             kj = sum;
             if (a.bias) kj += pref * bias_sine<T>(J == 1 ? x[i] : a.tmp[i]);
         }
@@ -944,9 +956,13 @@ __global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
         else in = var_load<T, J>(a, i, x, fresh);
         const T out = var_stage<T, J>(a, i, fresh, kj, in, xnew, hs, h, hnext, floor, err);
         if (s_next)   // region entries only (direct primary entries lie past the region)
-            for (int e = e0; e < e1; e++) {
-                const int o = loc[e];
-                if (o < rlen) reg[o] = out;
+            for (int e = e0; e < e1; e += 4) {
+                int o[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) o[q] = e + q < e1 ? loc[e + q] : rlen;
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    if (o[q] < rlen) reg[o[q]] = out;
             }
     }
     if (s_next) {
