@@ -72,6 +72,7 @@ constexpr int TC = 2048;                  // clauses per tile of the tiled claus
 constexpr int TILE_BT = 1024;             // threads per tile workgroup (a clause each)
 constexpr int VBLK = 1024;                // var-pass workgroup (one variable group)
 constexpr int VAR_LDS_BYTES = 147456;     // LDS holding one group's contribution region
+constexpr int VAR_LDS_CU = 163840 - 1024; // LDS two var workgroups may share on one CU
 constexpr int REGION_ALIGN = 64;          // region starts, in elements
 constexpr bool REORDER_CLAUSES = true;    // tiled pass: process clauses sorted by a primary variable
 
@@ -352,6 +353,7 @@ struct StageArgs {
     int nblk_clause;   // clause-pass grid (its J = 6 error partials come first)
     int vg;            // variables per group
     int var_region_cap;   // LDS var pass: elements reserved for a region
+    int var_stagger;      // LDS var pass: CUs whose second workgroup starts late (0: none)
     int vg_blocks;     // gather var pass: blocks per variable group
     int ngroups;
     int clause_per_xcd;   // clause-pass chunks (BLK clauses, or TC-clause tiles) per XCD
@@ -865,6 +867,14 @@ __global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
     const T* x = parity ? a.xb : a.xa;
     T* xnew = parity ? a.xa : a.xb;
     const int g = blockIdx.x;
+    // The dispatcher gives each CU one workgroup before any gets a second, so
+    // workgroups var_stagger .. 2 var_stagger - 1 are the second ones of the
+    // first round.  Started together, a CU's two workgroups would load, add
+    // and write back in lockstep, and HBM would idle while both add from LDS;
+    // the second starts about 8000 cycles later, so their phases alternate
+    // (J = 1..5 on N = 1e6: 31-34 -> 28-29 us per pass; the J = 6 pass
+    // measured slower with the delay and keeps none).
+    if (J < 6 && g >= a.var_stagger && g < 2 * a.var_stagger) __builtin_amdgcn_s_sleep(127);
     const int i0 = g * a.vg + 1, i1 = (int)std::min<long long>((long long)(g + 1) * a.vg, a.n);
     T* reg = (T*)var_lds;
     const uint16_t* loc = (const uint16_t*)(var_lds + (size_t)a.var_region_cap * sizeof(T));
@@ -1370,6 +1380,7 @@ struct Sys {
     bool lds_var = false;        // var pass: LDS region kernel or gather kernel
     bool wide_var = false;       // gather path with long rows: one workgroup per variable
     int var_lds = 0;             // LDS region kernel: dynamic LDS bytes (region + offsets)
+    int var_stagger = 0;         // LDS region kernel: CUs whose second workgroup starts late
     int var_region_cap = 0;      // LDS region kernel: elements reserved for a region
     void* d_contrib = nullptr;
     double* d_apart = nullptr;   // bias term: per clause-block aux sums
@@ -1508,6 +1519,15 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
     s->wide_var = !lds_var && s->nslots >= WIDE_MIN_ROW * (long long)P.n;
     s->var_region_cap = lds_var ? (int)P.max_region : 0;
     s->var_lds = lds_var ? var_lds_bytes(P.max_region, s->esize) : 0;
+    // two FP32 var workgroups per CU (LDS for both; the FP64 kernel's
+    // registers allow one) and more groups than CUs: stagger the pairs
+    s->var_stagger = 0;
+    if (lds_var && s->esize == 4 && 2LL * s->var_lds <= VAR_LDS_CU) {
+        int dev = 0, ncu = 0;
+        HIP_CHECK(hipGetDevice(&dev));
+        HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        if (ncu > 0 && P.ngroups > ncu) s->var_stagger = ncu;
+    }
     // clause arrays padded to whole tiles (padding literal 0, position -1)
     P.lit.resize((size_t)round_up(P.m, TC) * P.k, 0);
     upload(s->d_lit, P.lit);
@@ -1627,6 +1647,7 @@ static StageArgs<T> stage_args(Sys* s, T* xa, T* xb, T* const* kk, T* tmp, doubl
     a.nblk_clause = clause_grid(s);
     a.vg = s->vg;
     a.var_region_cap = s->var_region_cap;
+    a.var_stagger = s->var_stagger;
     a.vg_blocks = s->vg_blocks;
     a.ngroups = s->ngroups;
     a.knorm = (T)std::pow(2.0, -s->k);
