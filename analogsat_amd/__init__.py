@@ -220,6 +220,7 @@ def lib():
             "asat_adj_replay": ([vp, i, vp, vp, vp, vp], i),
             "asat_adj_replay_f64": ([vp, i, vp, vp, vp], i),
             "asat_adj_timing": ([vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], i),
+            "asat_adj_attempts": ([vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)], i),
             "asat_rng_create": ([ctypes.c_ulonglong, pvp], i),
             "asat_rng_destroy": ([vp], i),
             "asat_rng_uniform": ([vp, vp, i, i], i),
@@ -983,6 +984,12 @@ class HipAdjoint:
         f, b = ctypes.c_double(), ctypes.c_double()
         _check(lib().asat_adj_timing(self._h, ctypes.byref(f), ctypes.byref(b)))
         return f.value, b.value
+
+    def attempts(self):
+        """(attempts summed over the trajectories, lockstep attempts) of the last forward run."""
+        t, l = ctypes.c_longlong(), ctypes.c_longlong()
+        _check(lib().asat_adj_attempts(self._h, ctypes.byref(t), ctypes.byref(l)))
+        return t.value, l.value
 
 
 def energy_function():

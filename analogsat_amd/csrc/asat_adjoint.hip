@@ -587,6 +587,8 @@ struct Adj {
     double* eparts = nullptr;    // [ntraj][nblk] error-norm block maxima
     int ncheck = 0;
     double fwd_ms = 0, bwd_ms = 0;
+    long long fwd_attempts = 0;   // forward: attempts summed over the trajectories
+    long long fwd_lockstep = 0;   // forward: lockstep attempts launched
 
     void free_batch()
     {
@@ -837,10 +839,14 @@ static void forward_lockstep(Adj& A, const std::vector<float>& X0)
         HIP_CHECK(hipMemcpyAsync(hc.data(), A.ctl, sizeof(TCtl) * hc.size(), hipMemcpyDeviceToHost, A.stream));
         HIP_CHECK(hipStreamSynchronize(A.stream));
         int active = 0;
+        long long attempts = 0;
         for (const TCtl& c : hc) {
             if (c.underflow) throw asat::Error(ASAT_UNDERFLOW, "step size underflow in the forward run");
             active += c.active;
+            attempts += c.attempts;
         }
+        A.fwd_attempts = attempts;
+        A.fwd_lockstep = (round + 1) * per_round;
         if (!active) break;
         // each round accepts or rejects every active trajectory per attempt;
         // a trajectory that needs more than 64 attempts per step is stuck
@@ -1109,6 +1115,15 @@ int asat_adj_timing(const asat_adj_t* a, double* forward_ms, double* backward_ms
     require(a != nullptr, ASAT_ERR_INVALID_ARGUMENT, "null adjoint");
     if (forward_ms) *forward_ms = a->a.fwd_ms;
     if (backward_ms) *backward_ms = a->a.bwd_ms;
+    AAPI_END
+}
+
+int asat_adj_attempts(const asat_adj_t* a, long long* trajectory_attempts, long long* lockstep_attempts)
+{
+    AAPI_BEGIN
+    require(a != nullptr, ASAT_ERR_INVALID_ARGUMENT, "null adjoint");
+    if (trajectory_attempts) *trajectory_attempts = a->a.fwd_attempts;
+    if (lockstep_attempts) *lockstep_attempts = a->a.fwd_lockstep;
     AAPI_END
 }
 

@@ -202,6 +202,21 @@ def kernel_bytes(n, m, k, nslots, esize=4):
     return out
 
 
+def adjoint_bytes(L, D):
+    """Algorithmic HBM bytes of the C5 adjoint (FP32 forward, FP64 adjoint),
+    L literal slots and D = N + M elements per trajectory.
+
+    forward, per trajectory-attempt: SURVEY's 96 L + 136 D (one FP32 attempt).
+    backward, per trajectory-step: the step's FP32 recompute (96 L + 136 D),
+    then for each of the 6 stages the FP64 vector-Jacobian product of the
+    RHS at the stage input (per slot: literal 4, FP64 term written 8 and read
+    8, transposed-list entry 4; per element: stage input 4, adjoint input 8,
+    product written 8), and the adjoint recursion over the stages (per
+    element 20 FP64 vector reads and writes): 240 L + 416 D.
+    """
+    return 96 * L + 136 * D, 96 * L + 136 * D + 6 * (24 * L + 20 * D) + 160 * D
+
+
 def attempt_bytes(n, m, k, nslots, esize=4):
     """Bytes one attempt moves by design (sum over the 13 kernels)."""
     return sum(kernel_bytes(n, m, k, nslots, esize).values())
@@ -541,6 +556,7 @@ def workload_c5(args, log):
     g = adj.backward()
     t2 = time.perf_counter()
     fwd_ms, bwd_ms = adj.timing()
+    traj_attempts, lockstep = adj.attempts()
     # central differences of V(T) along 256 random coordinates of trajectory
     # 0, with its step sizes frozen (FP32 replays of the perturbed starts,
     # 64 at a time); the step is the FP32-representable difference
@@ -586,9 +602,24 @@ def workload_c5(args, log):
     conv = np.abs(fd64 - fd64b) / gmax
     log("C5: FD (FP64 replays, h = 1e-6): max %.2e, median %.2e; h = 1e-6 vs 1e-8: max %.2e"
         % (rel64.max(), np.median(rel64), conv.max()))
+    # HBM roofline of both halves (algorithmic bytes, see adjoint_bytes)
+    L, D = 3 * m, n + m
+    fb, bb = adjoint_bytes(L, D)
+    fwd_bytes, bwd_bytes = fb * traj_attempts, bb * ntraj * T
+    roof = {"bound": "hbm", "peak": 8000.0, "unit": "GB/s",
+            "forward": {"trajectory_attempts": traj_attempts, "lockstep_attempts": lockstep,
+                        "bytes_per_trajectory_attempt": fb, "achieved": round(fwd_bytes / (fwd_ms * 1e-3) / 1e9, 1),
+                        "frac": round(fwd_bytes / (fwd_ms * 1e-3) / 8e12, 4)},
+            "backward": {"trajectory_steps": ntraj * T, "bytes_per_trajectory_step": bb,
+                         "achieved": round(bwd_bytes / (bwd_ms * 1e-3) / 1e9, 1),
+                         "frac": round(bwd_bytes / (bwd_ms * 1e-3) / 8e12, 4)}}
+    log("C5: forward %.1f GB/s (%.3f of 8 TB/s) over %d trajectory-attempts; backward %.1f GB/s (%.3f)"
+        % (roof["forward"]["achieved"], roof["forward"]["frac"], traj_attempts, roof["backward"]["achieved"],
+           roof["backward"]["frac"]))
     return {"workload": "c5", "trajectories": ntraj, "steps": T, "checkpoint_every": C,
             "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
             "forward_device_ms": round(fwd_ms, 2), "backward_device_ms": round(bwd_ms, 2),
+            "roofline": roof,
             "backward_trajectory_steps_per_s": round(ntraj * T / (t2 - t1), 1),
             "grad_norm_first": float(np.linalg.norm(g[0])),
             "fd_check": {"coordinates": 256, "trajectory": 0, "frozen_steps": True,

@@ -390,6 +390,10 @@ int asat_adj_replay(asat_adj_t* a, int ntraj, const float* pub0, const double* h
 int asat_adj_replay_f64(asat_adj_t* a, int ntraj, const double* pub0, const double* h, double* V_out);
 /* device milliseconds of the last forward / backward */
 int asat_adj_timing(const asat_adj_t* a, double* forward_ms, double* backward_ms);
+/* Measurement hook, no counterpart in the reference: attempts of the last
+   forward run, summed over the trajectories, and lockstep attempts launched
+   (every launch covers all trajectories; finished ones return at once). */
+int asat_adj_attempts(const asat_adj_t* a, long long* trajectory_attempts, long long* lockstep_attempts);
 
 #ifdef __cplusplus
 }
