@@ -844,8 +844,13 @@ __device__ __forceinline__ T bias_sine(T s)
 // streams the region back to contrib as the next clause pass's slot spins.
 // Each region entry belongs to one slot of one variable, so the entries a
 // thread reads and overwrites are its own.
+// FP32: two 1024-thread workgroups per CU (LDS holds two regions) need 8
+// waves per SIMD, which caps the scalar registers: at J = 6 the compiler
+// otherwise takes 90 of them, only one workgroup fits, and the pass took 44
+// instead of 36 us.  FP64 systems fit one var workgroup per CU anyway.
 template <typename T, int J>
-__global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
+__global__ __launch_bounds__(VBLK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 1)))
+void k_var_region(StageArgs<T> a)
 {
     constexpr int V = 16 / (int)sizeof(T);
     constexpr int R = 4;              // 16-byte loads in flight per thread and round
@@ -872,9 +877,8 @@ __global__ __launch_bounds__(VBLK) void k_var_region(StageArgs<T> a)
     // first round.  Started together, a CU's two workgroups would load, add
     // and write back in lockstep, and HBM would idle while both add from LDS;
     // the second starts about 8000 cycles later, so their phases alternate
-    // (J = 1..5 on N = 1e6: 31-34 -> 28-29 us per pass; the J = 6 pass
-    // measured slower with the delay and keeps none).
-    if (J < 6 && g >= a.var_stagger && g < 2 * a.var_stagger) __builtin_amdgcn_s_sleep(127);
+    // (N = 1e6: 31-34 -> 28-29 us per pass at J = 1..5, 36 -> 32 us at J = 6).
+    if (g >= a.var_stagger && g < 2 * a.var_stagger) __builtin_amdgcn_s_sleep(127);
     const int i0 = g * a.vg + 1, i1 = (int)std::min<long long>((long long)(g + 1) * a.vg, a.n);
     T* reg = (T*)var_lds;
     const uint16_t* loc = (const uint16_t*)(var_lds + (size_t)a.var_region_cap * sizeof(T));
