@@ -69,6 +69,8 @@ static inline long long round_up(long long a, long long b) { return (a + b - 1) 
 // problem (host)
 // ===========================================================================
 constexpr int TC = 2048;                  // clauses per tile of the tiled clause pass (one workgroup)
+constexpr int TC_BIG = 4096;              // the same for FP32 systems with direct primary slots
+constexpr int TC_MAX = TC_BIG;            // clause arrays are padded to whole tiles of this size
 constexpr int TILE_BT = 1024;             // threads per tile workgroup (a clause each)
 constexpr int VBLK = 1024;                // var-pass workgroup (one variable group)
 constexpr int VAR_LDS_BYTES = 147456;     // LDS holding one group's contribution region
@@ -106,6 +108,7 @@ struct HostProblem {
     // whose primary variable is in group g (ngroups+1 entries).
     bool split = false;
     std::vector<int> pc0;
+    int tc = TC;                   // tiled pass: clauses per tile
 };
 
 // SatProblem::AddClauses semantics (AnalogSAT/problem/SatProblem.cpp:64-95) +
@@ -131,7 +134,7 @@ static void parse_clauses(const int* literals, const int* lengths, int nclauses,
     }
     require(m > 0 && k > 0, ASAT_ERR_INVALID_ARGUMENT, "empty SAT problem");
     require(k <= GEN_KMAX, ASAT_ERR_UNSUPPORTED, "clauses longer than 32 literals are not supported");
-    require((long long)m * k < (1LL << 31) - (long long)TC * k, ASAT_ERR_UNSUPPORTED, "more than 2^31 literal slots");
+    require((long long)m * k < (1LL << 31) - (long long)TC_MAX * k, ASAT_ERR_UNSUPPORTED, "more than 2^31 literal slots");
     P.n = n; P.m = m; P.k = k;
     P.lit.assign((size_t)m * k, 0);
     off = 0;
@@ -280,10 +283,10 @@ static void build_layout(HostProblem& P, int vg, int pv)
 // primary slot's rank entry is 0x4000 with the sign bit.
 static void build_tile_order(HostProblem& P)
 {
-    const int K = P.k;
+    const int K = P.k, TCt = P.tc;
     const int NR = P.split ? K - 1 : K;
-    const long long ntiles = ceil_div((long long)P.m, TC);
-    const size_t per = (size_t)TC * K, nper = (size_t)TC * NR;
+    const long long ntiles = ceil_div((long long)P.m, TCt);
+    const size_t per = (size_t)TCt * K, nper = (size_t)TCt * NR;
     P.spos.assign((size_t)ntiles * nper, -1);
     P.srank.assign((size_t)ntiles * per, 0xFFFF);
     std::vector<int> cnt((size_t)P.ngroups + 1, 0);
@@ -307,7 +310,7 @@ static void build_tile_order(HostProblem& P)
             if (!l) continue;
             const size_t ls = s - s0;   // tile-local slot: clause ls / K, literal ls % K
             const unsigned sign = l < 0 ? 0x8000u : 0u;
-            uint16_t& rk = P.srank[s0 + (ls % K) * TC + ls / K];
+            uint16_t& rk = P.srank[s0 + (ls % K) * TCt + ls / K];
             if (P.pos[s] < 0) {   // a direct primary slot
                 rk = (uint16_t)(0x4000u | sign);
                 continue;
@@ -426,9 +429,9 @@ __device__ __forceinline__ void st_stream(V* p, V v)
 }
 
 // ---- clause pass, compile-time K (Ex order), tiled -----------------------
-// Slot ranks: bits 0-12 the rank in the tile's sorted list, bit 14 a direct
-// primary slot (no rank), bit 15 the literal's sign, 0xFFFF padding.
-constexpr unsigned RANK_MASK = 0x1FFFu;
+// Slot ranks: bits 0-13 the rank in the tile's sorted list, bit 14 a direct
+// primary slot (its rank bits 0), bit 15 the literal's sign, 0xFFFF padding.
+constexpr unsigned RANK_MASK = 0x3FFFu;
 constexpr unsigned RANK_PRIMARY = 0x4000u;
 
 // Slot terms of one clause, written to LDS at the slots' ranks in the tile's
@@ -457,115 +460,89 @@ __device__ __forceinline__ void slot_terms(const int (&l)[K], const T (&f)[K], T
     }
 }
 
-// Phase 1 of one tile (see k_clause_tile): slot terms into s_out at their
-// ranks (direct primary terms into pt), this thread's share of the tile's
-// sorted positions into ps.
-// SS (slot spins): sp is contrib holding the stage spins in contribution
-// layout; the tile reads them through its sorted positions (the same runs
-// its phase 2 then overwrites with the slot terms) into s_out at their
-// ranks, and each clause thread takes its slots' spins from there (the
-// literal signs come from the ranks' bit 15).  A direct primary slot's spin
-// is read from the stage spin vector (x at J = 1, tmp after) at the clause's
-// primary variable: clauses are sorted by it, so a wave reads a few lines.
-// Otherwise sp is the N-element stage spin vector, gathered by variable.
-template <typename T, int K, int J, int BT, bool SS, bool PS>
-__device__ __forceinline__ void tile_phase1(const StageArgs<T>& a, long long c0, const T* x, T* xnew,
-                                            const T* sp, T hs, double h, double hnext, double floor,
-                                            bool tanh_family, T* s_out, int (&ps)[TC / BT * (PS ? K - 1 : K)],
-                                            T (&pt)[TC / BT], double& err, double& asum)
+// Phase 1 of a tile for G of the thread's clauses (its clauses u0 .. u0+G-1,
+// see k_clause_tile): K_m, the aux stage input, k_J(a_m) and the slot terms,
+// which go to s_out at their ranks (direct primary terms into pt).  Every
+// load that does not depend on another is issued first: the slot ranks
+// (padded to whole tiles, read unguarded), the literal rows when the spins
+// are gathered by variable, and the aux values.
+// SS (slot spins): s_out already holds each rank's stage spin (phase 0), and
+// psv the clauses' direct primary spins; the literal signs come from the
+// ranks' bit 15.  Otherwise sp is the N-element stage spin vector, gathered
+// by variable.
+template <typename T, int K, int J, int BT, int TCt, bool SS, bool PS, int G, int CPT>
+__device__ __forceinline__ void tile_group(const StageArgs<T>& a, long long c0, int u0, const T* x, T* xnew,
+                                           const T* sp, const T (&psv)[CPT], T hs, double h, double hnext,
+                                           double floor, bool tanh_family, T* s_out, double& err, double& asum)
 {
-    constexpr int CPT = TC / BT;
-    constexpr int NR = PS ? K - 1 : K;
-    int l[CPT][K];
-    T sv[CPT][K];
-    T ax[CPT], a1[CPT], a2[CPT], a3[CPT], a4[CPT], a5[CPT];
-    uint16_t rk[CPT][K];
-    int pv[CPT];
-    // every load that does not depend on another is issued first: the
-    // literal rows and slot ranks (padded to whole tiles, read unguarded),
-    // this thread's share of the tile's sorted positions and the aux values
+    int l[G][K];
+    T sv[G][K];
+    T ax[G], a1[G], a2[G], a3[G], a4[G], a5[G];
+    uint16_t rk[G][K];
 #pragma unroll
-    for (int u = 0; u < CPT; u++) {
-        const long long m = c0 + u * BT + threadIdx.x;
+    for (int g = 0; g < G; g++) {
+        const long long m = c0 + (u0 + g) * BT + threadIdx.x;
         if constexpr (!SS) {
 #pragma unroll
-            for (int q = 0; q < K; q++) l[u][q] = ld_g(a.lit + m * K + q);
+            for (int q = 0; q < K; q++) l[g][q] = ld_g(a.lit + m * K + q);
         }
 #pragma unroll
-        for (int q = 0; q < K; q++) rk[u][q] = ld_g(a.srank + c0 * K + q * TC + (m - c0));
-        if constexpr (SS && PS) pv[u] = ld_g(a.pvar + m);
+        for (int q = 0; q < K; q++) rk[g][q] = ld_g(a.srank + c0 * K + q * TCt + (m - c0));
     }
-    const int* sp_t = a.spos + c0 * NR;
 #pragma unroll
-    for (int i = 0; i < CPT * NR; i++) ps[i] = ld_g(sp_t + i * BT + threadIdx.x);
-#pragma unroll
-    for (int u = 0; u < CPT; u++) {
-        const long long m = c0 + u * BT + threadIdx.x;
+    for (int g = 0; g < G; g++) {
+        const long long m = c0 + (u0 + g) * BT + threadIdx.x;
         const size_t ai = (size_t)a.aoff + m;
-        ax[u] = a1[u] = a2[u] = a3[u] = a4[u] = a5[u] = (T)0;
+        ax[g] = a1[g] = a2[g] = a3[g] = a4[g] = a5[g] = (T)0;
         if (m < a.m) {
-            ax[u] = ld_g(x + ai);
-            if constexpr (J > 1) a1[u] = ld_g(a.k1 + ai);
-            if constexpr (J > 2) a2[u] = ld_g(a.k2 + ai);
-            if constexpr (J > 3) a3[u] = ld_g(a.k3 + ai);
-            if constexpr (J > 4) a4[u] = ld_g(a.k4 + ai);
-            if constexpr (J > 5) a5[u] = ld_g(a.k5 + ai);
+            ax[g] = ld_g(x + ai);
+            if constexpr (J > 1) a1[g] = ld_g(a.k1 + ai);
+            if constexpr (J > 2) a2[g] = ld_g(a.k2 + ai);
+            if constexpr (J > 3) a3[g] = ld_g(a.k3 + ai);
+            if constexpr (J > 4) a4[g] = ld_g(a.k4 + ai);
+            if constexpr (J > 5) a5[g] = ld_g(a.k5 + ai);
         }
     }
     if constexpr (SS) {
-        T so[CPT * NR];
-#pragma unroll
-        for (int i = 0; i < CPT * NR; i++) so[i] = ps[i] >= 0 ? sp[ps[i]] : (T)0;
-        T psv[CPT];
-        if constexpr (PS) {
-            // padding clauses have primary variable 0: x[0] / tmp[0], unused
-            const T* svec = (J == 1) ? x : a.tmp;
-#pragma unroll
-            for (int u = 0; u < CPT; u++) psv[u] = ld_g(svec + pv[u]);
-        }
-#pragma unroll
-        for (int i = 0; i < CPT * NR; i++) s_out[i * BT + threadIdx.x] = so[i];
-        __syncthreads();
         // each rank is read here and overwritten with its slot term below by
         // the same thread (the owner of the slot), so one buffer serves both
 #pragma unroll
-        for (int u = 0; u < CPT; u++) {
+        for (int g = 0; g < G; g++) {
 #pragma unroll
             for (int q = 0; q < K; q++) {
-                const unsigned r = rk[u][q];
-                l[u][q] = r == 0xFFFFu ? 0 : ((r & 0x8000u) ? -1 : 1);
-                if (r == 0xFFFFu) sv[u][q] = (T)-1.0;
-                else if (PS && (r & RANK_PRIMARY)) sv[u][q] = psv[u];
-                else sv[u][q] = s_out[r & RANK_MASK];
+                const unsigned r = rk[g][q];
+                l[g][q] = r == 0xFFFFu ? 0 : ((r & 0x8000u) ? -1 : 1);
+                if (r == 0xFFFFu) sv[g][q] = (T)-1.0;
+                else if (PS && (r & RANK_PRIMARY)) sv[g][q] = psv[u0 + g];
+                else sv[g][q] = s_out[r & RANK_MASK];
             }
         }
     } else {
 #pragma unroll
-        for (int u = 0; u < CPT; u++) {
+        for (int g = 0; g < G; g++) {
 #pragma unroll
             for (int q = 0; q < K; q++) {
-                const int v = l[u][q] < 0 ? -l[u][q] : l[u][q];
-                sv[u][q] = v != 0 ? sp[v] : (T)-1.0;
+                const int v = l[g][q] < 0 ? -l[g][q] : l[g][q];
+                sv[g][q] = v != 0 ? sp[v] : (T)-1.0;
             }
         }
     }
 #pragma unroll
-    for (int u = 0; u < CPT; u++) {
-        const int cl = u * BT + threadIdx.x;
-        const long long m = c0 + cl;
-        pt[u] = (T)0;
+    for (int g = 0; g < G; g++) {
+        const long long m = c0 + (u0 + g) * BT + threadIdx.x;
+        T pt = (T)0;
         if (m < a.m) {
             T f[K];
             T km = a.knorm;
 #pragma unroll
             for (int q = 0; q < K; q++) {
-                f[q] = l[u][q] < 0 ? (T)1.0 + sv[u][q] : (T)1.0 - sv[u][q];
+                f[q] = l[g][q] < 0 ? (T)1.0 + sv[g][q] : (T)1.0 - sv[g][q];
                 km *= f[q];
             }
             const size_t ai = (size_t)a.aoff + m;
             T am;
-            if constexpr (J == 1) am = ax[u];
-            else am = stage_input<T, J>(ax[u], a1[u], a2[u], a3[u], a4[u], a5[u], hs);
+            if constexpr (J == 1) am = ax[g];
+            else am = stage_input<T, J>(ax[g], a1[g], a2[g], a3[g], a4[g], a5[g], hs);
             T kj = am * km * km;
             asum += (double)am;
             if constexpr (J == 1) st_g(a.k1 + ai, kj);
@@ -576,33 +553,44 @@ __device__ __forceinline__ void tile_phase1(const StageArgs<T>& a, long long c0,
             else {
                 T xn;
                 double e;
-                final_combination<T>(ax[u], a1[u], a3[u], a4[u], a5[u], kj, hs, h, hnext, floor, xn, e);
+                final_combination<T>(ax[g], a1[g], a3[g], a4[g], a5[g], kj, hs, h, hnext, floor, xn, e);
                 err = fmax(err, e);
                 st_g(xnew + ai, xn);
             }
             const T r = tanh_family ? a.knorm * am : km * a.knorm * am;
-            slot_terms<T, K, 0, PS>(l[u], f, r, tanh_family, a.q, rk[u], s_out, pt[u]);
+            slot_terms<T, K, 0, PS>(l[g], f, r, tanh_family, a.q, rk[g], s_out, pt);
         }
+        // pcontrib is padded to whole tiles: padding clauses store 0
+        if constexpr (PS) st_g(a.pcontrib + m, pt);
     }
 }
 
-// One workgroup of BT threads per tile of TC clauses (XCD-contiguous tile
-// ranges).  Phase 1, a thread per clause (TC / BT clauses each): load every
-// rank, stage spin and aux value of the thread's clauses first, so that the
-// loads are in flight together (with slot spins, the spins arrive through
-// the tile's sorted positions and LDS); then K_m, the aux stage input,
-// k_J(a_m) and the clause's slot terms, which go to LDS at the slots' ranks
-// in the tile's position order.  Direct primary terms (PS) are stored at the
-// clauses' own positions of pcontrib: neighbouring lanes, neighbouring
-// entries.  Phase 2, a thread per rank: store the terms, so that
-// neighbouring lanes write neighbouring positions of one region.
-template <typename T, int K, int J, int BT, bool SS, bool PS>
-__global__ __launch_bounds__(BT) void k_clause_tile(StageArgs<T> a)
+// One workgroup of BT threads per tile of TCt clauses (XCD-contiguous tile
+// ranges), TCt / BT clauses per thread.
+// Phase 0, a thread per rank (slot spins only): the tile's sorted positions,
+// and the stage spins read through them into s_out (neighbouring lanes,
+// neighbouring positions of one region); with direct primary slots also the
+// thread's clauses' primary spins, from the stage spin vector.
+// Phase 1, a thread per clause, in groups of G clauses (tile_group): the
+// clause arithmetic, slot terms into s_out at their ranks; direct primary
+// terms (PS) are stored at the clauses' own positions of pcontrib.
+// Phase 2, a thread per rank: store the terms through the sorted positions.
+// 4096-clause tiles give each region runs of about 8 entries per tile in
+// place of 4; taking their clauses two at a time in phase 1 keeps the
+// registers of 2048-clause tiles.
+template <typename T, int K, int J, int BT, int TCt, bool SS, bool PS>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(TCt > TC ? 8 : 1)))
+void k_clause_tile(StageArgs<T> a)
 {
-    constexpr int CPT = TC / BT;
-    constexpr int NP = CPT * (PS ? K - 1 : K);
+    constexpr int CPT = TCt / BT;
+    constexpr int G = CPT < 2 ? CPT : 2;
+    constexpr int NR = PS ? K - 1 : K;
+    constexpr int NP = CPT * NR;
+    // the sorted positions of 4096-clause tiles wait in LDS for phase 2
+    constexpr bool PL = TCt > TC;
     __shared__ double sh[BT / 64];
-    __shared__ T s_out[TC * (PS ? K - 1 : K)];
+    __shared__ T s_out[TCt * NR];
+    __shared__ int s_pos[PL ? TCt * NR : 1];
     const Ctrl* c = a.ctrl;
     int parity = 0;
     double h = 0.0, hnext = 0.0, floor = TabD::TINY;
@@ -620,20 +608,43 @@ __global__ __launch_bounds__(BT) void k_clause_tile(StageArgs<T> a)
     double err = 0.0, asum = 0.0;
     const int tile = (b & 7) * a.clause_per_xcd + (b >> 3);
     if (tile < a.ntiles) {
+        const long long c0 = (long long)tile * TCt;
         int ps[NP];
-        T pt[CPT];
-        const long long c0 = (long long)tile * TC;
-        tile_phase1<T, K, J, BT, SS, PS>(a, c0, x, xnew, sp, hs, h, hnext, floor, tanh_family, s_out, ps, pt, err,
-                                         asum);
-        if constexpr (PS) {
-            // pcontrib is padded to whole tiles: padding clauses store 0
+        T psv[CPT];
+        const int* sp_t = a.spos + c0 * NR;
 #pragma unroll
-            for (int u = 0; u < CPT; u++) st_g(a.pcontrib + c0 + u * BT + threadIdx.x, pt[u]);
+        for (int i = 0; i < NP; i++) ps[i] = ld_g(sp_t + i * BT + threadIdx.x);
+        if constexpr (SS) {
+            if constexpr (PS) {
+                // padding clauses have primary variable 0: x[0] / tmp[0], unused
+                const T* svec = (J == 1) ? x : a.tmp;
+                int pv[CPT];
+#pragma unroll
+                for (int u = 0; u < CPT; u++) pv[u] = ld_g(a.pvar + c0 + u * BT + threadIdx.x);
+#pragma unroll
+                for (int u = 0; u < CPT; u++) psv[u] = ld_g(svec + pv[u]);
+            }
+            T so[NP];
+#pragma unroll
+            for (int i = 0; i < NP; i++) so[i] = ps[i] >= 0 ? sp[ps[i]] : (T)0;
+#pragma unroll
+            for (int i = 0; i < NP; i++) s_out[i * BT + threadIdx.x] = so[i];
         }
+        if constexpr (PL) {
+#pragma unroll
+            for (int i = 0; i < NP; i++) s_pos[i * BT + threadIdx.x] = ps[i];
+        }
+        if constexpr (SS) __syncthreads();
+#pragma unroll
+        for (int u0 = 0; u0 < CPT; u0 += G)
+            tile_group<T, K, J, BT, TCt, SS, PS, G, CPT>(a, c0, u0, x, xnew, sp, psv, hs, h, hnext, floor,
+                                                          tanh_family, s_out, err, asum);
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < NP; i++)
-            if (ps[i] >= 0) a.contrib[ps[i]] = s_out[i * BT + threadIdx.x];
+        for (int i = 0; i < NP; i++) {
+            const int p = PL ? s_pos[i * BT + threadIdx.x] : ps[i];
+            if (p >= 0) a.contrib[p] = s_out[i * BT + threadIdx.x];
+        }
     }
     if (a.bias) {
         const double bs = block_sum_all<BT>(asum, sh);
@@ -1381,6 +1392,7 @@ struct Sys {
     int family = ASAT_FAMILY_CTDS;
     double q = 1.1;              // tanh family parameter (CpuSatTanh.cpp:39)
     bool tiled = false;          // clause pass: tiled kernel (K = 3, 4) or generic
+    int tc = TC;                 // tiled pass: clauses per tile (TC or TC_BIG)
     bool lds_var = false;        // var pass: LDS region kernel or gather kernel
     bool wide_var = false;       // gather path with long rows: one workgroup per variable
     int var_lds = 0;             // LDS region kernel: dynamic LDS bytes (region + offsets)
@@ -1417,7 +1429,7 @@ struct Sys {
 };
 
 // clause-pass and var-pass grid sizes of a configured system
-static int clause_grid(const Sys* s) { return 8 * ceil_div(s->tiled ? ceil_div(s->m, TC) : ceil_div(s->m, BLK), 8); }
+static int clause_grid(const Sys* s) { return 8 * ceil_div(s->tiled ? ceil_div(s->m, s->tc) : ceil_div(s->m, BLK), 8); }
 static int var_grid(const Sys* s)
 {
     if (s->lds_var) return s->ngroups;
@@ -1501,6 +1513,10 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
     const bool tiled = P.k == 3 || P.k == 4;
     if (tiled && REORDER_CLAUSES) reorder_clauses(P);
     const bool lds_var = choose_layout(P, s->esize);
+    // FP32 3-literal clauses with direct primary slots: 4096-clause tiles
+    // (twice as long runs per region; the kernel takes their clauses two at
+    // a time, and two workgroups still fit on a CU)
+    P.tc = (tiled && P.split && s->esize == 4 && P.k == 3) ? TC_BIG : TC;
     if (tiled) build_tile_order(P);
     HIP_CHECK(hipStreamSynchronize(s->stream));
     s->free_problem();
@@ -1514,6 +1530,7 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
     s->ngroups = P.ngroups;
     s->vg_blocks = ceil_div(P.vg, BLK);
     s->tiled = tiled;
+    s->tc = P.tc;
     s->lds_var = lds_var;
     s->split = P.split;
     require(!P.split || (tiled && lds_var), ASAT_ERR_INVALID_ARGUMENT, "direct primary slots without their passes");
@@ -1533,22 +1550,22 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
         if (ncu > 0 && P.ngroups > ncu) s->var_stagger = ncu;
     }
     // clause arrays padded to whole tiles (padding literal 0, position -1)
-    P.lit.resize((size_t)round_up(P.m, TC) * P.k, 0);
+    P.lit.resize((size_t)round_up(P.m, TC_MAX) * P.k, 0);
     upload(s->d_lit, P.lit);
     if (tiled) {
         upload(s->d_spos, P.spos);
         upload(s->d_srank, P.srank);
         if (P.split) {
-            P.pvar.resize((size_t)round_up(P.m, TC), 0);   // padding clauses: variable 0
+            P.pvar.resize((size_t)round_up(P.m, TC_MAX), 0);   // padding clauses: variable 0
             upload(s->d_pvar, P.pvar);
             upload(s->d_pc0, P.pc0);
             // one term per clause padded to whole tiles, plus a 16-byte unit
-            const size_t np = (size_t)round_up(P.m, TC) + 16;
+            const size_t np = (size_t)round_up(P.m, TC_MAX) + 16;
             HIP_CHECK(hipMalloc(&s->d_pcontrib, (size_t)s->esize * np));
             HIP_CHECK(hipMemset(s->d_pcontrib, 0, (size_t)s->esize * np));
         }
     } else {
-        P.pos.resize((size_t)round_up(P.m, TC) * P.k, -1);
+        P.pos.resize((size_t)round_up(P.m, TC_MAX) * P.k, -1);
         upload(s->d_pos, P.pos);
     }
     upload(s->d_csr_ptr, P.csr_ptr);
@@ -1646,7 +1663,7 @@ static StageArgs<T> stage_args(Sys* s, T* xa, T* xb, T* const* kk, T* tmp, doubl
     a.tmp = tmp; a.partial = partial; a.ctrl = c;
     a.slot_spins = 0;
     a.n = s->n; a.m = s->m; a.k = s->k; a.aoff = s->aoff;
-    a.ntiles = ceil_div(s->m, TC);
+    a.ntiles = ceil_div(s->m, s->tc);
     a.clause_per_xcd = ceil_div(s->tiled ? a.ntiles : ceil_div(s->m, BLK), 8);
     a.nblk_clause = clause_grid(s);
     a.vg = s->vg;
@@ -1674,12 +1691,23 @@ static void launch_clause(const StageArgs<T>& a, const Sys* s, hipStream_t st)
     const bool ps = a.split != 0;   // slot spins come with direct primary slots (both need the LDS var pass)
     require(!ss || ps, ASAT_ERR_INVALID_ARGUMENT, "slot spins without direct primary slots");
     const dim3 tb(TILE_BT);
-    if (tiled && a.k == 3 && ss) hipLaunchKernelGGL((k_clause_tile<T, 3, J, TILE_BT, true, true>), g, tb, 0, st, a);
-    else if (tiled && a.k == 4 && ss) hipLaunchKernelGGL((k_clause_tile<T, 4, J, TILE_BT, true, true>), g, tb, 0, st, a);
-    else if (tiled && a.k == 3 && ps) hipLaunchKernelGGL((k_clause_tile<T, 3, J, TILE_BT, false, true>), g, tb, 0, st, a);
-    else if (tiled && a.k == 4 && ps) hipLaunchKernelGGL((k_clause_tile<T, 4, J, TILE_BT, false, true>), g, tb, 0, st, a);
-    else if (tiled && a.k == 3) hipLaunchKernelGGL((k_clause_tile<T, 3, J, TILE_BT, false, false>), g, tb, 0, st, a);
-    else if (tiled && a.k == 4) hipLaunchKernelGGL((k_clause_tile<T, 4, J, TILE_BT, false, false>), g, tb, 0, st, a);
+    constexpr int TB = TC_BIG, TS = TC;
+    const bool big = s->tc == TC_BIG;
+    require(!big || (sizeof(T) == 4 && ps && a.k == 3), ASAT_ERR_INVALID_ARGUMENT,
+            "4096-clause tiles without their layout");
+    if constexpr (sizeof(T) == 4) {
+        if (tiled && big) {
+            if (ss) hipLaunchKernelGGL((k_clause_tile<T, 3, J, TILE_BT, TB, true, true>), g, tb, 0, st, a);
+            else hipLaunchKernelGGL((k_clause_tile<T, 3, J, TILE_BT, TB, false, true>), g, tb, 0, st, a);
+            return;
+        }
+    }
+    if (tiled && a.k == 3 && ss) hipLaunchKernelGGL((k_clause_tile<T, 3, J, TILE_BT, TS, true, true>), g, tb, 0, st, a);
+    else if (tiled && a.k == 4 && ss) hipLaunchKernelGGL((k_clause_tile<T, 4, J, TILE_BT, TS, true, true>), g, tb, 0, st, a);
+    else if (tiled && a.k == 3 && ps) hipLaunchKernelGGL((k_clause_tile<T, 3, J, TILE_BT, TS, false, true>), g, tb, 0, st, a);
+    else if (tiled && a.k == 4 && ps) hipLaunchKernelGGL((k_clause_tile<T, 4, J, TILE_BT, TS, false, true>), g, tb, 0, st, a);
+    else if (tiled && a.k == 3) hipLaunchKernelGGL((k_clause_tile<T, 3, J, TILE_BT, TS, false, false>), g, tb, 0, st, a);
+    else if (tiled && a.k == 4) hipLaunchKernelGGL((k_clause_tile<T, 4, J, TILE_BT, TS, false, false>), g, tb, 0, st, a);
     else hipLaunchKernelGGL((k_clause_gen<T, J>), g, b, 0, st, a);
 }
 
@@ -2520,7 +2548,8 @@ int asat_sys_layout(const asat_sys_t* sys, asat_layout_t* out)
     out->var_pass = s->lds_var ? ASAT_VAR_REGION : (s->wide_var ? ASAT_VAR_WIDE : ASAT_VAR_GATHER);
     out->groups = s->ngroups;
     out->group_vars = s->vg;
-    out->tiles = s->tiled ? (int)ceil_div(s->m, TC) : 0;
+    out->tiles = s->tiled ? (int)ceil_div(s->m, s->tc) : 0;
+    out->tile_clauses = s->tiled ? s->tc : 0;
     out->clause_grid = clause_grid(s);
     out->var_grid = var_grid(s);
     out->var_lds_bytes = s->var_lds;
