@@ -69,8 +69,9 @@ static inline long long round_up(long long a, long long b) { return (a + b - 1) 
 // problem (host)
 // ===========================================================================
 constexpr int TC = 2048;                  // clauses per tile of the tiled clause pass (one workgroup)
-constexpr int TC_BIG = 4096;              // the same for FP32 systems with direct primary slots
-constexpr int TC_MAX = TC_BIG;            // clause arrays are padded to whole tiles of this size
+constexpr int TC_BIG = 4608;              // FP32 3-literal systems with direct primary slots: the most
+                                          // clauses a tile may have (its size is chosen per problem)
+constexpr int TC_MAX = TC_BIG;
 constexpr int TILE_BT = 1024;             // threads per tile workgroup (a clause each)
 constexpr int VBLK = 1024;                // var-pass workgroup (one variable group)
 constexpr int VAR_LDS_BYTES = 147456;     // LDS holding one group's contribution region
@@ -360,7 +361,8 @@ struct StageArgs {
     int vg_blocks;     // gather var pass: blocks per variable group
     int ngroups;
     int clause_per_xcd;   // clause-pass chunks (BLK clauses, or TC-clause tiles) per XCD
-    int ntiles;           // tiled pass: tiles of TC clauses
+    int ntiles;           // tiled pass: tiles
+    int tile_clauses;     // tiled pass: clauses per tile
     int split;            // direct primary slots (tiled pass + LDS var pass)
     int family;           // ASAT_FAMILY_CTDS or ASAT_FAMILY_TANH
     int bias;             // sine bias term on (b > 0)
@@ -470,8 +472,8 @@ __device__ __forceinline__ void slot_terms(const int (&l)[K], const T (&f)[K], T
 // psv the clauses' direct primary spins; the literal signs come from the
 // ranks' bit 15.  Otherwise sp is the N-element stage spin vector, gathered
 // by variable.
-template <typename T, int K, int J, int BT, int TCt, bool SS, bool PS, int G, int CPT>
-__device__ __forceinline__ void tile_group(const StageArgs<T>& a, long long c0, int u0, const T* x, T* xnew,
+template <typename T, int K, int J, int BT, bool SS, bool PS, int G, int CPT>
+__device__ __forceinline__ void tile_group(const StageArgs<T>& a, long long c0, int TS, int u0, const T* x, T* xnew,
                                            const T* sp, const T (&psv)[CPT], T hs, double h, double hnext,
                                            double floor, bool tanh_family, T* s_out, double& err, double& asum)
 {
@@ -481,20 +483,22 @@ __device__ __forceinline__ void tile_group(const StageArgs<T>& a, long long c0, 
     uint16_t rk[G][K];
 #pragma unroll
     for (int g = 0; g < G; g++) {
-        const long long m = c0 + (u0 + g) * BT + threadIdx.x;
+        const int lc = (u0 + g) * BT + (int)threadIdx.x;   // the clause in the tile (< TS: a clause of it)
+        const long long m = c0 + lc;
         if constexpr (!SS) {
 #pragma unroll
-            for (int q = 0; q < K; q++) l[g][q] = ld_g(a.lit + m * K + q);
+            for (int q = 0; q < K; q++) l[g][q] = lc < TS ? ld_g(a.lit + m * K + q) : 0;
         }
 #pragma unroll
-        for (int q = 0; q < K; q++) rk[g][q] = ld_g(a.srank + c0 * K + q * TCt + (m - c0));
+        for (int q = 0; q < K; q++) rk[g][q] = lc < TS ? ld_g(a.srank + c0 * K + (long long)q * TS + lc) : 0xFFFF;
     }
 #pragma unroll
     for (int g = 0; g < G; g++) {
-        const long long m = c0 + (u0 + g) * BT + threadIdx.x;
+        const int lc = (u0 + g) * BT + (int)threadIdx.x;
+        const long long m = c0 + lc;
         const size_t ai = (size_t)a.aoff + m;
         ax[g] = a1[g] = a2[g] = a3[g] = a4[g] = a5[g] = (T)0;
-        if (m < a.m) {
+        if (lc < TS && m < a.m) {
             ax[g] = ld_g(x + ai);
             if constexpr (J > 1) a1[g] = ld_g(a.k1 + ai);
             if constexpr (J > 2) a2[g] = ld_g(a.k2 + ai);
@@ -529,9 +533,10 @@ __device__ __forceinline__ void tile_group(const StageArgs<T>& a, long long c0, 
     }
 #pragma unroll
     for (int g = 0; g < G; g++) {
-        const long long m = c0 + (u0 + g) * BT + threadIdx.x;
+        const int lc = (u0 + g) * BT + (int)threadIdx.x;
+        const long long m = c0 + lc;
         T pt = (T)0;
-        if (m < a.m) {
+        if (lc < TS && m < a.m) {
             T f[K];
             T km = a.knorm;
 #pragma unroll
@@ -561,7 +566,8 @@ __device__ __forceinline__ void tile_group(const StageArgs<T>& a, long long c0, 
             slot_terms<T, K, 0, PS>(l[g], f, r, tanh_family, a.q, rk[g], s_out, pt);
         }
         // pcontrib is padded to whole tiles: padding clauses store 0
-        if constexpr (PS) st_g(a.pcontrib + m, pt);
+        if constexpr (PS)
+            if (lc < TS) st_g(a.pcontrib + m, pt);
     }
 }
 
@@ -575,18 +581,22 @@ __device__ __forceinline__ void tile_group(const StageArgs<T>& a, long long c0, 
 // clause arithmetic, slot terms into s_out at their ranks; direct primary
 // terms (PS) are stored at the clauses' own positions of pcontrib.
 // Phase 2, a thread per rank: store the terms through the sorted positions.
-// 4096-clause tiles give each region runs of about 8 entries per tile in
-// place of 4; taking their clauses two at a time in phase 1 keeps the
-// registers of 2048-clause tiles.
+// Big tiles (TCt = TC_BIG) hold up to 4608 clauses, a size chosen per
+// problem (tile_clauses) so that the tiles fill whole rounds of resident
+// workgroups: about 4100-clause tiles give each region runs of about 8
+// entries per tile in place of 4, and no nearly empty last round (one
+// extra tile running alone cost about 5 us per pass).  Taking their clauses
+// two at a time in phase 1 keeps the registers of 2048-clause tiles.
 template <typename T, int K, int J, int BT, int TCt, bool SS, bool PS>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(TCt > TC ? 8 : 1)))
 void k_clause_tile(StageArgs<T> a)
 {
-    constexpr int CPT = TCt / BT;
+    constexpr int CPT = (TCt + BT - 1) / BT;
     constexpr int G = CPT < 2 ? CPT : 2;
+    constexpr int CPTG = (CPT + G - 1) / G * G;   // clause slots of the groups (past CPT: none of the tile)
     constexpr int NR = PS ? K - 1 : K;
     constexpr int NP = CPT * NR;
-    // the sorted positions of 4096-clause tiles wait in LDS for phase 2
+    // the sorted positions of big tiles wait in LDS for phase 2
     constexpr bool PL = TCt > TC;
     __shared__ double sh[BT / 64];
     __shared__ T s_out[TCt * NR];
@@ -608,42 +618,60 @@ void k_clause_tile(StageArgs<T> a)
     double err = 0.0, asum = 0.0;
     const int tile = (b & 7) * a.clause_per_xcd + (b >> 3);
     if (tile < a.ntiles) {
-        const long long c0 = (long long)tile * TCt;
+        const int TS = PL ? a.tile_clauses : TCt;
+        const long long c0 = (long long)tile * TS;
         int ps[NP];
-        T psv[CPT];
+        T psv[CPTG];
         const int* sp_t = a.spos + c0 * NR;
 #pragma unroll
-        for (int i = 0; i < NP; i++) ps[i] = ld_g(sp_t + i * BT + threadIdx.x);
+        for (int i = 0; i < NP; i++) {
+            const int lr = i * BT + (int)threadIdx.x;
+            ps[i] = lr < TS * NR ? ld_g(sp_t + lr) : -1;
+        }
         if constexpr (SS) {
             if constexpr (PS) {
                 // padding clauses have primary variable 0: x[0] / tmp[0], unused
                 const T* svec = (J == 1) ? x : a.tmp;
-                int pv[CPT];
+                int pv[CPTG];
 #pragma unroll
-                for (int u = 0; u < CPT; u++) pv[u] = ld_g(a.pvar + c0 + u * BT + threadIdx.x);
+                for (int u = 0; u < CPTG; u++) {
+                    const int lc = u * BT + (int)threadIdx.x;
+                    pv[u] = lc < TS ? ld_g(a.pvar + c0 + lc) : 0;
+                }
 #pragma unroll
-                for (int u = 0; u < CPT; u++) psv[u] = ld_g(svec + pv[u]);
+                for (int u = 0; u < CPTG; u++) psv[u] = ld_g(svec + pv[u]);
             }
+            // LDS holds TCt * NR ranks; NP * BT may be more (TCt not a
+            // multiple of BT), and a tile has TS * NR <= TCt * NR of them
             T so[NP];
 #pragma unroll
             for (int i = 0; i < NP; i++) so[i] = ps[i] >= 0 ? sp[ps[i]] : (T)0;
 #pragma unroll
-            for (int i = 0; i < NP; i++) s_out[i * BT + threadIdx.x] = so[i];
+            for (int i = 0; i < NP; i++) {
+                const int lr = i * BT + (int)threadIdx.x;
+                if (lr < TS * NR) s_out[lr] = so[i];
+            }
         }
         if constexpr (PL) {
 #pragma unroll
-            for (int i = 0; i < NP; i++) s_pos[i * BT + threadIdx.x] = ps[i];
+            for (int i = 0; i < NP; i++) {
+                const int lr = i * BT + (int)threadIdx.x;
+                if (lr < TS * NR) s_pos[lr] = ps[i];
+            }
         }
         if constexpr (SS) __syncthreads();
 #pragma unroll
         for (int u0 = 0; u0 < CPT; u0 += G)
-            tile_group<T, K, J, BT, TCt, SS, PS, G, CPT>(a, c0, u0, x, xnew, sp, psv, hs, h, hnext, floor,
-                                                          tanh_family, s_out, err, asum);
+            tile_group<T, K, J, BT, SS, PS, G, CPTG>(a, c0, TS, u0, x, xnew, sp, psv, hs, h, hnext, floor,
+                                                      tanh_family, s_out, err, asum);
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < NP; i++) {
-            const int p = PL ? s_pos[i * BT + threadIdx.x] : ps[i];
-            if (p >= 0) a.contrib[p] = s_out[i * BT + threadIdx.x];
+            const int lr = i * BT + (int)threadIdx.x;
+            if (lr < TS * NR) {
+                const int p = PL ? s_pos[lr] : ps[i];
+                if (p >= 0) a.contrib[p] = s_out[lr];
+            }
         }
     }
     if (a.bias) {
@@ -1513,10 +1541,23 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
     const bool tiled = P.k == 3 || P.k == 4;
     if (tiled && REORDER_CLAUSES) reorder_clauses(P);
     const bool lds_var = choose_layout(P, s->esize);
-    // FP32 3-literal clauses with direct primary slots: 4096-clause tiles
-    // (twice as long runs per region; the kernel takes their clauses two at
-    // a time, and two workgroups still fit on a CU)
-    P.tc = (tiled && P.split && s->esize == 4 && P.k == 3) ? TC_BIG : TC;
+    int ncu = 0;
+    {
+        int dev = 0;
+        HIP_CHECK(hipGetDevice(&dev));
+        HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    // FP32 3-literal clauses with direct primary slots: big tiles (the
+    // kernel takes their clauses two at a time, and two workgroups still fit
+    // on a CU), sized so that they fill whole rounds of the 2 x CUs resident
+    // workgroups: C2 (M = 4.2e6) gets 1010 tiles of 4160 clauses, two rounds
+    P.tc = TC;
+    if (tiled && P.split && s->esize == 4 && P.k == 3) {
+        const long long slots = 2LL * (ncu > 0 ? ncu : 1);
+        const long long rounds = std::max<long long>(1, ceil_div((long long)P.m, slots * TC_BIG));
+        const long long ts = round_up(ceil_div((long long)P.m, rounds * slots), 64);
+        P.tc = (int)std::min<long long>(std::max<long long>(ts, 1024), TC_BIG);
+    }
     if (tiled) build_tile_order(P);
     HIP_CHECK(hipStreamSynchronize(s->stream));
     s->free_problem();
@@ -1543,29 +1584,26 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
     // two FP32 var workgroups per CU (LDS for both; the FP64 kernel's
     // registers allow one) and more groups than CUs: stagger the pairs
     s->var_stagger = 0;
-    if (lds_var && s->esize == 4 && 2LL * s->var_lds <= VAR_LDS_CU) {
-        int dev = 0, ncu = 0;
-        HIP_CHECK(hipGetDevice(&dev));
-        HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        if (ncu > 0 && P.ngroups > ncu) s->var_stagger = ncu;
-    }
+    if (lds_var && s->esize == 4 && 2LL * s->var_lds <= VAR_LDS_CU && ncu > 0 && P.ngroups > ncu)
+        s->var_stagger = ncu;
     // clause arrays padded to whole tiles (padding literal 0, position -1)
-    P.lit.resize((size_t)round_up(P.m, TC_MAX) * P.k, 0);
+    const size_t mpad = (size_t)round_up(P.m, P.tc);
+    P.lit.resize(mpad * P.k, 0);
     upload(s->d_lit, P.lit);
     if (tiled) {
         upload(s->d_spos, P.spos);
         upload(s->d_srank, P.srank);
         if (P.split) {
-            P.pvar.resize((size_t)round_up(P.m, TC_MAX), 0);   // padding clauses: variable 0
+            P.pvar.resize(mpad, 0);   // padding clauses: variable 0
             upload(s->d_pvar, P.pvar);
             upload(s->d_pc0, P.pc0);
             // one term per clause padded to whole tiles, plus a 16-byte unit
-            const size_t np = (size_t)round_up(P.m, TC_MAX) + 16;
+            const size_t np = mpad + 16;
             HIP_CHECK(hipMalloc(&s->d_pcontrib, (size_t)s->esize * np));
             HIP_CHECK(hipMemset(s->d_pcontrib, 0, (size_t)s->esize * np));
         }
     } else {
-        P.pos.resize((size_t)round_up(P.m, TC_MAX) * P.k, -1);
+        P.pos.resize(mpad * P.k, -1);
         upload(s->d_pos, P.pos);
     }
     upload(s->d_csr_ptr, P.csr_ptr);
@@ -1664,6 +1702,7 @@ static StageArgs<T> stage_args(Sys* s, T* xa, T* xb, T* const* kk, T* tmp, doubl
     a.slot_spins = 0;
     a.n = s->n; a.m = s->m; a.k = s->k; a.aoff = s->aoff;
     a.ntiles = ceil_div(s->m, s->tc);
+    a.tile_clauses = s->tc;
     a.clause_per_xcd = ceil_div(s->tiled ? a.ntiles : ceil_div(s->m, BLK), 8);
     a.nblk_clause = clause_grid(s);
     a.vg = s->vg;
@@ -1692,7 +1731,7 @@ static void launch_clause(const StageArgs<T>& a, const Sys* s, hipStream_t st)
     require(!ss || ps, ASAT_ERR_INVALID_ARGUMENT, "slot spins without direct primary slots");
     const dim3 tb(TILE_BT);
     constexpr int TB = TC_BIG, TS = TC;
-    const bool big = s->tc == TC_BIG;
+    const bool big = s->tc != TC;
     require(!big || (sizeof(T) == 4 && ps && a.k == 3), ASAT_ERR_INVALID_ARGUMENT,
             "4096-clause tiles without their layout");
     if constexpr (sizeof(T) == 4) {

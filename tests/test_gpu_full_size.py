@@ -52,8 +52,9 @@ def test_c2_layout_rhs_and_steps_bitexact():
     say("C2: layout %s" % lay)
     assert lay["clause_pass"] == 0 and lay["var_pass"] == 0 and lay["slot_spins"] == 1
     assert lay["groups"] == 1024 and lay["group_vars"] == 977
-    assert lay["direct_primary"] == 1 and lay["tile_clauses"] == 4096
-    assert lay["tiles"] == -(-m // 4096)
+    # tiles sized to fill whole rounds of two workgroups per CU (256 CUs)
+    assert lay["direct_primary"] == 1 and lay["tile_clauses"] == 4160
+    assert lay["tiles"] == -(-m // 4160) == 1010
     # standalone RHS (spin gathers): bit-identical to the float oracle
     x, d = sat.MakeState(), sat.MakeState()
     x.CopyFrom(pub)
