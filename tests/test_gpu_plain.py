@@ -503,3 +503,42 @@ def test_new_problem_invalidates_states_and_integrator():
     a, b = sat.MakeState(), sat.MakeState()
     a.CopyFrom(O.random_state_f64(2, p2.n, p2.m))
     assert rk.Step(a, b, 3)
+
+
+@pytest.mark.parametrize("n, m, tile", [(140000, 600000, 1216), (520000, 2200000, 4352)])
+def test_big_tile_sizes_bitexact(n, m, tile):
+    """FP32 3-literal systems use tiles sized per problem to fill whole rounds
+    of two workgroups per CU (up to 4608 clauses, five for some threads; the
+    kernel takes a thread's clauses two at a time).  Tile sizes that are not
+    a multiple of the workgroup size: the standalone RHS and 12 integrator
+    steps bit-identical to the oracle's mixed mode."""
+    say("big tiles N=%d M=%d" % (n, m))
+    p = O.fastsat(n, m, 3, 31)
+    sat = _sys_with(p.literals, p.lengths, "f32")
+    lay = sat.Layout()
+    say("  layout %s" % lay)
+    assert lay["direct_primary"] == 1 and lay["tile_clauses"] % 64 == 0 and lay["tile_clauses"] <= 4608
+    assert lay["tiles"] * lay["tile_clauses"] >= m
+    if lay["tile_clauses"] % 1024 == 0:   # the size follows the CU count (`tile` on 256 CUs)
+        pytest.skip("this device sizes the tiles to a multiple of the workgroup")
+    pub = np.concatenate([np.random.default_rng(8).uniform(-1, 1, n), np.ones(m)]).astype(np.float32)
+    x, d = sat.MakeState(), sat.MakeState()
+    x.CopyFrom(pub)
+    sat.GetDerivatives(d, x, 0.0)
+    want = O.rhs(p, np.concatenate([[-1.0], pub]).astype(np.float32), mode="mix")[1:]
+    assert np.array_equal(d.CopyTo(), want)
+    rk = A.HipRungeKutta()
+    rk.SetEpsilon(1e-6)
+    rk.Configure(sat)
+    old, new = sat.MakeState(), sat.MakeState()
+    old.CopyFrom(pub)
+    with O.pow_mode("cr"):
+        orc = O.RK(p, eps=1e-6, mode="mix")
+        orc.set_state(pub)
+        for blk in range(3):
+            rk.Step(old, new, 4)
+            orc.step(4)
+            assert np.array_equal(old.CopyTo(), orc.get_state()), blk
+            st = rk.stats()
+            assert (st.hdid, st.attempts, st.rhscount) == (orc.hdid, orc.attempts, orc.rhscount)
+    assert sat.GetClauseViolationCount(old) == orc.violations()
