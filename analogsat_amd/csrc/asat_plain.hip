@@ -1210,16 +1210,6 @@ __global__ void k_ctrl_begin(Ctrl* c, long long count, int parity)
     }
 }
 
-__global__ void k_ctrl_reset(Ctrl* c, double eps, double floor, double* hlog, long long hlog_len)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        c->h = 1e-9; c->hnext = 1e-9; c->time = 0.0; c->hdid = 0.0; c->errmax = 0.0; c->eps = eps;
-        c->floor = floor; c->hlog = hlog; c->hlog_len = hlog_len;
-        c->steps = 0; c->attempts = 0; c->rhscount = 0; c->target = 0;
-        c->parity = 0; c->step_start = 1; c->active = 0; c->underflow = 0; c->violations = 0;
-    }
-}
-
 __global__ void k_ctrl_set_eps(Ctrl* c, double eps, double floor)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -1904,16 +1894,23 @@ struct RK {
     }
 };
 
+// CudaRungeKutta::Reset (h = 1e-9, clock and counters cleared): the control
+// block is set on the host and copied in stream order (pinned memory), so a
+// solve that follows waits for no round trip.  Every later host write of
+// h_ctrl is a device-to-host copy behind this one on the same stream.
 static void rk_reset(RK* r)
 {
     r->require_configured();
-    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, r->sys->stream, r->d_ctrl, r->eps, r->floor, r->hlog,
-                       r->hlog_len);
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(r->h_ctrl, r->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, r->sys->stream));
-    HIP_CHECK(hipStreamSynchronize(r->sys->stream));
+    Ctrl* c = r->h_ctrl;
+    memset(c, 0, sizeof(Ctrl));
+    c->h = 1e-9; c->hnext = 1e-9; c->eps = r->eps;
+    c->floor = r->floor; c->hlog = r->hlog; c->hlog_len = r->hlog_len;
+    c->step_start = 1;
+    HIP_CHECK(hipMemcpyAsync(r->d_ctrl, c, sizeof(Ctrl), hipMemcpyHostToDevice, r->sys->stream));
     r->pair[0] = r->pair[1] = nullptr;
 }
+
+static bool small_path(const RK* r);
 
 static void rk_configure(RK* r, Sys* s)
 {
@@ -1927,10 +1924,14 @@ static void rk_configure(RK* r, Sys* s)
         r->drop_graph();
         r->sys = s;
         r->gen = s->gen;
-        const size_t bytes = (size_t)s->esize * s->alloc;
-        for (auto& p : r->k) HIP_CHECK(hipMemsetAsync(p, 0, bytes, s->stream));
-        HIP_CHECK(hipMemsetAsync(r->tmp, 0, (size_t)s->esize * (s->n + 1), s->stream));
-        HIP_CHECK(hipMemsetAsync(r->partial, 0, sizeof(double) * r->npartial, s->stream));
+        // the one-workgroup path keeps everything in LDS and reads none of
+        // these buffers (C1: a caller solving start after start)
+        if (!small_path(r)) {
+            const size_t bytes = (size_t)s->esize * s->alloc;
+            for (auto& p : r->k) HIP_CHECK(hipMemsetAsync(p, 0, bytes, s->stream));
+            HIP_CHECK(hipMemsetAsync(r->tmp, 0, (size_t)s->esize * (s->n + 1), s->stream));
+            HIP_CHECK(hipMemsetAsync(r->partial, 0, sizeof(double) * r->npartial, s->stream));
+        }
         rk_reset(r);
         return;
     }
@@ -2752,6 +2753,8 @@ int asat_rk_reset(asat_rk_t* rk)
 {
     API_BEGIN
     CHECK_PTR(rk);
+    // no copy into the host control block may be pending when it is rewritten
+    if (rk->r.configured) HIP_CHECK(hipStreamSynchronize(rk->r.sys->stream));
     rk_reset(&rk->r);
     API_END
 }
