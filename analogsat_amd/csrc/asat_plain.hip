@@ -95,6 +95,13 @@ struct HostProblem {
     std::vector<uint16_t> csr_loc; // the same relative to the variable's region start
     // tiled clause pass: per tile of TC clauses, its slots sorted by position
     std::vector<int> spos;         // ntiles*TC*k positions (-1 past the tile's last slot)
+    // big tiles with few groups: per tile, a run table in place of spos.
+    // Word g of tile t: bits 0-15 the first rank of group g's run in the
+    // tile's sorted list, bits 16-31 the run's first position relative to
+    // the region start gstart[g]; word ngroups holds the tile's slot count
+    // (run_stride words per tile)
+    std::vector<uint32_t> runs;
+    int run_stride = 0;
     std::vector<uint16_t> srank;   // per slot: its rank in the tile's sorted list, the literal's sign
                                    // in bit 15, 0xFFFF for padding; per tile, literal q of every
                                    // clause first (index tile*TC*K + q*TC + clause in the tile)
@@ -288,7 +295,19 @@ static void build_tile_order(HostProblem& P)
     const int NR = P.split ? K - 1 : K;
     const long long ntiles = ceil_div((long long)P.m, TCt);
     const size_t per = (size_t)TCt * K, nper = (size_t)TCt * NR;
-    P.spos.assign((size_t)ntiles * nper, -1);
+    // run tables: a tile's positions in one region are consecutive (tiles are
+    // ranges of slots, region positions follow slot order), so the sorted
+    // list is one run per group.  Used for big tiles when the table (a word
+    // per group) is at most a quarter of the positions it replaces
+    const bool rt = P.split && TCt != TC && (long long)(P.ngroups + 1) * 4 <= (long long)nper;
+    P.runs.clear();
+    P.run_stride = 0;
+    if (rt) {
+        P.run_stride = (int)round_up((long long)P.ngroups + 1, 16);
+        P.runs.assign((size_t)ntiles * P.run_stride, 0);
+    } else {
+        P.spos.assign((size_t)ntiles * nper, -1);
+    }
     P.srank.assign((size_t)ntiles * per, 0xFFFF);
     std::vector<int> cnt((size_t)P.ngroups + 1, 0);
     std::vector<int> touched;
@@ -305,6 +324,18 @@ static void build_tile_order(HostProblem& P)
         }
         std::sort(touched.begin(), touched.end());
         int acc = 0;
+        if (rt) {
+            // every group's word: its first rank (untouched groups: that of
+            // the next run); region offsets are filled from the slots below
+            uint32_t* w = P.runs.data() + (size_t)t * P.run_stride;
+            size_t i = 0;
+            for (int g = 0; g <= P.ngroups; g++) {
+                while (i < touched.size() && touched[i] < g) acc += cnt[touched[i++]];
+                w[g] = (uint32_t)acc;
+            }
+            require(acc <= (int)nper, ASAT_ERR_UNSUPPORTED, "run table exceeds the tile");
+            acc = 0;
+        }
         for (int g : touched) { int c = cnt[g]; cnt[g] = acc; acc += c; }
         for (size_t s = s0; s < s1; s++) {
             int l = P.lit[s];
@@ -318,7 +349,16 @@ static void build_tile_order(HostProblem& P)
             }
             int g = (std::abs(l) - 1) / P.vg;
             const size_t j = (size_t)cnt[g]++;
-            P.spos[r0 + j] = P.pos[s];
+            if (rt) {
+                uint32_t& w = P.runs[(size_t)t * P.run_stride + g];
+                if (j == (w & 0xFFFFu)) {   // the run's first slot
+                    const long long off = P.pos[s] - P.gstart[g];
+                    require(off >= 0 && off < 65536, ASAT_ERR_UNSUPPORTED, "run offset exceeds 16 bits");
+                    w |= (uint32_t)off << 16;
+                }
+            } else {
+                P.spos[r0 + j] = P.pos[s];
+            }
             rk = (uint16_t)(j | sign);
         }
         for (int g : touched) cnt[g] = 0;
@@ -334,6 +374,8 @@ struct StageArgs {
     const int* pos;        // contribution position of each slot (-1: padding); generic-K pass
     const int* spos;       // tiled pass: per tile, positions in ascending order (-1: none)
     const uint16_t* srank; // tiled pass: rank of each slot in its tile's sorted list
+    const unsigned* runs;  // big tiles: per tile, run table in place of spos (null: spos)
+    int run_stride;        // words per tile of the run tables
     const int* csr_ptr;
     const int* csr_pos;    // per-variable contribution positions (gather var pass)
     const uint16_t* csr_loc;   // the same relative to the region start (LDS var pass)
@@ -622,22 +664,51 @@ void k_clause_tile(StageArgs<T> a)
         const long long c0 = (long long)tile * TS;
         int ps[NP];
         T psv[CPTG];
-        const int* sp_t = a.spos + c0 * NR;
+        int pv[CPTG];
+        if constexpr (SS && PS) {
+            // padding clauses have primary variable 0: x[0] / tmp[0], unused
 #pragma unroll
-        for (int i = 0; i < NP; i++) {
-            const int lr = i * BT + (int)threadIdx.x;
-            ps[i] = lr < TS * NR ? ld_g(sp_t + lr) : -1;
+            for (int u = 0; u < CPTG; u++) {
+                const int lc = u * BT + (int)threadIdx.x;
+                pv[u] = lc < TS ? ld_g(a.pvar + c0 + lc) : 0;
+            }
+        }
+        bool runs = false;
+        if constexpr (PL) runs = a.runs != nullptr;
+        if (runs) {
+            // the sorted positions from the tile's run table: group g's
+            // ranks [j0(g), j0(g+1)) sit at consecutive positions from
+            // gstart[g] + off(g); ranks past the tile's slots get -1
+            const unsigned* rt = a.runs + (long long)tile * a.run_stride;
+            for (int g = (int)threadIdx.x; g < a.ngroups; g += BT) {
+                const unsigned w0 = ld_g(rt + g), w1 = ld_g(rt + g + 1);
+                const int j0 = (int)(w0 & 0xFFFFu), j1 = (int)(w1 & 0xFFFFu);
+                const int base = (int)ld_g(a.gstart + g) + (int)(w0 >> 16) - j0;
+                for (int j = j0; j < j1; j++) s_pos[j] = base + j;
+            }
+            const int nsl = (int)(ld_g(rt + a.ngroups) & 0xFFFFu);
+#pragma unroll
+            for (int i = 0; i < NP; i++) {
+                const int lr = i * BT + (int)threadIdx.x;
+                if (lr >= nsl && lr < TS * NR) s_pos[lr] = -1;
+            }
+            if constexpr (SS) __syncthreads();
+#pragma unroll
+            for (int i = 0; i < NP; i++) {
+                const int lr = i * BT + (int)threadIdx.x;
+                ps[i] = (SS && lr < TS * NR) ? s_pos[lr] : -1;
+            }
+        } else {
+            const int* sp_t = a.spos + c0 * NR;
+#pragma unroll
+            for (int i = 0; i < NP; i++) {
+                const int lr = i * BT + (int)threadIdx.x;
+                ps[i] = lr < TS * NR ? ld_g(sp_t + lr) : -1;
+            }
         }
         if constexpr (SS) {
             if constexpr (PS) {
-                // padding clauses have primary variable 0: x[0] / tmp[0], unused
                 const T* svec = (J == 1) ? x : a.tmp;
-                int pv[CPTG];
-#pragma unroll
-                for (int u = 0; u < CPTG; u++) {
-                    const int lc = u * BT + (int)threadIdx.x;
-                    pv[u] = lc < TS ? ld_g(a.pvar + c0 + lc) : 0;
-                }
 #pragma unroll
                 for (int u = 0; u < CPTG; u++) psv[u] = ld_g(svec + pv[u]);
             }
@@ -653,10 +724,12 @@ void k_clause_tile(StageArgs<T> a)
             }
         }
         if constexpr (PL) {
+            if (!runs) {
 #pragma unroll
-            for (int i = 0; i < NP; i++) {
-                const int lr = i * BT + (int)threadIdx.x;
-                if (lr < TS * NR) s_pos[lr] = ps[i];
+                for (int i = 0; i < NP; i++) {
+                    const int lr = i * BT + (int)threadIdx.x;
+                    if (lr < TS * NR) s_pos[lr] = ps[i];
+                }
             }
         }
         if constexpr (SS) __syncthreads();
@@ -1387,6 +1460,8 @@ struct Sys {
     int* d_pos = nullptr;        // generic-K clause pass only
     int* d_spos = nullptr;       // tiled clause pass (K = 3, 4)
     uint16_t* d_srank = nullptr;
+    unsigned* d_runs = nullptr;  // big tiles with few groups: run tables in place of d_spos
+    int run_stride = 0;
     int* d_csr_ptr = nullptr;
     int* d_csr_pos = nullptr;    // gather var pass only
     uint16_t* d_csr_loc = nullptr;   // LDS var pass only
@@ -1428,6 +1503,8 @@ struct Sys {
     void free_problem()
     {
         dfree(d_lit); dfree(d_pos); dfree(d_spos); dfree(d_srank); dfree(d_csr_ptr); dfree(d_csr_pos);
+        dfree(d_runs);
+        run_stride = 0;
         dfree(d_csr_loc); dfree(d_gstart); dfree(d_perm); dfree(d_contrib); dfree(d_apart);
         dfree(d_pvar); dfree(d_pc0); dfree(d_pcontrib);
         split = false;
@@ -1581,7 +1658,9 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
     P.lit.resize(mpad * P.k, 0);
     upload(s->d_lit, P.lit);
     if (tiled) {
-        upload(s->d_spos, P.spos);
+        if (P.runs.empty()) upload(s->d_spos, P.spos);
+        else upload(s->d_runs, P.runs);
+        s->run_stride = P.run_stride;
         upload(s->d_srank, P.srank);
         if (P.split) {
             P.pvar.resize(mpad, 0);   // padding clauses: variable 0
@@ -1683,6 +1762,7 @@ static StageArgs<T> stage_args(Sys* s, T* xa, T* xb, T* const* kk, T* tmp, doubl
     StageArgs<T> a;
     memset(&a, 0, sizeof(a));   // padding too: the arguments are compared bytewise (captured attempts)
     a.lit = s->d_lit; a.pos = s->d_pos; a.spos = s->d_spos; a.srank = s->d_srank;
+    a.runs = s->d_runs; a.run_stride = s->run_stride;
     a.csr_ptr = s->d_csr_ptr; a.csr_pos = s->d_csr_pos; a.csr_loc = s->d_csr_loc; a.gstart = s->d_gstart;
     a.pvar = s->d_pvar; a.pc0 = s->d_pc0; a.pcontrib = (T*)s->d_pcontrib; a.split = s->split ? 1 : 0;
     a.contrib = (T*)s->d_contrib;
@@ -2595,6 +2675,7 @@ int asat_sys_layout(const asat_sys_t* sys, asat_layout_t* out)
     out->var_lds_bytes = s->var_lds;
     out->slot_spins = (s->tiled && s->lds_var) ? 1 : 0;
     out->direct_primary = s->split ? 1 : 0;
+    out->run_tables = s->d_runs ? 1 : 0;
     API_END
 }
 

@@ -505,13 +505,17 @@ def test_new_problem_invalidates_states_and_integrator():
     assert rk.Step(a, b, 3)
 
 
-@pytest.mark.parametrize("n, m, tile", [(140000, 600000, 1216), (520000, 2200000, 4352)])
-def test_big_tile_sizes_bitexact(n, m, tile):
+@pytest.mark.parametrize("n, m, runs", [(140000, 600000, 1), (520000, 2200000, 1), (600000, 500000, 0)])
+def test_big_tile_sizes_bitexact(n, m, runs):
     """FP32 3-literal systems use tiles sized per problem to fill whole rounds
     of two workgroups per CU (up to 4608 clauses, five for some threads; the
-    kernel takes a thread's clauses two at a time).  Tile sizes that are not
-    a multiple of the workgroup size: the standalone RHS and 12 integrator
-    steps bit-identical to the oracle's mixed mode."""
+    kernel takes a thread's clauses two at a time).  On 256 CUs the first two
+    get tiles of 1216 and 4352 clauses, not multiples of the workgroup size,
+    and rebuild their sorted positions from run tables (256 groups); the third
+    gets 1024-clause tiles over 512 groups, where a run table would exceed a
+    quarter of the positions, and reads the positions themselves.  The
+    standalone RHS and 12 integrator steps bit-identical to the oracle's
+    mixed mode."""
     say("big tiles N=%d M=%d" % (n, m))
     p = O.fastsat(n, m, 3, 31)
     sat = _sys_with(p.literals, p.lengths, "f32")
@@ -519,8 +523,10 @@ def test_big_tile_sizes_bitexact(n, m, tile):
     say("  layout %s" % lay)
     assert lay["direct_primary"] == 1 and lay["tile_clauses"] % 64 == 0 and lay["tile_clauses"] <= 4608
     assert lay["tiles"] * lay["tile_clauses"] >= m
-    if lay["tile_clauses"] % 1024 == 0:   # the size follows the CU count (`tile` on 256 CUs)
+    assert lay["run_tables"] == int(4 * (lay["groups"] + 1) <= 2 * lay["tile_clauses"])
+    if lay["tile_clauses"] % 1024 == 0 and runs:   # the size follows the CU count (256 CUs here)
         pytest.skip("this device sizes the tiles to a multiple of the workgroup")
+    assert lay["run_tables"] == runs
     pub = np.concatenate([np.random.default_rng(8).uniform(-1, 1, n), np.ones(m)]).astype(np.float32)
     x, d = sat.MakeState(), sat.MakeState()
     x.CopyFrom(pub)
