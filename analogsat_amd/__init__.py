@@ -87,7 +87,8 @@ class LayoutInfo(ctypes.Structure):
     """asat_layout_t: the passes and launch shapes SetProblem chose."""
     _fields_ = [(nm, ctypes.c_int) for nm in ("clause_pass", "var_pass", "groups", "group_vars", "tiles",
                                               "clause_grid", "var_grid", "var_lds_bytes", "slot_spins",
-                                              "direct_primary", "tile_clauses", "run_tables")]
+                                              "direct_primary", "tile_clauses", "run_tables",
+                                              "primary_words")]
 
     def as_dict(self):
         return {nm: getattr(self, nm) for nm, _ in self._fields_}

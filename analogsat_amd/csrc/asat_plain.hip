@@ -17,6 +17,12 @@
 //                     order (int32), and each slot's rank in that order (uint16,
 //                     the literal's sign in bit 15; literal q of the tile's
 //                     clauses at q*TC + clause, so each load is one run)
+//   runs              big FP32 tiles (up to 4608 clauses): a word per variable
+//                     group in place of spos (the tile's positions in one
+//                     region are consecutive: first rank, first offset)
+//   pw16, pv0         big FP32 tiles: per clause its primary variable relative
+//                     to the tile's first (pv0), the primary literal's position
+//                     and sign, in place of pvar and the primary slots' ranks
 //   csr_ptr[0..N+1]   var-major CSR row pointers
 //   csr_loc[e]        uint16 offset of each of a variable's contributions in
 //                     its group's region, in slot order
@@ -72,6 +78,8 @@ constexpr int TC = 2048;                  // clauses per tile of the tiled claus
 constexpr int TC_BIG = 4608;              // FP32 3-literal systems with direct primary slots: the most
                                           // clauses a tile may have (its size is chosen per problem)
 constexpr int TC_MAX = TC_BIG;
+constexpr unsigned PW_VMASK = 0x1FFFu;    // big tiles' primary words: relative variable bits,
+constexpr int PW_QSHIFT = 13;             // then the primary literal's position, then its sign
 constexpr int TILE_BT = 1024;             // threads per tile workgroup (a clause each)
 constexpr int VBLK = 1024;                // var-pass workgroup (one variable group)
 constexpr int VAR_LDS_BYTES = 147456;     // LDS holding one group's contribution region
@@ -102,6 +110,13 @@ struct HostProblem {
     // (run_stride words per tile)
     std::vector<uint32_t> runs;
     int run_stride = 0;
+    // big tiles: primary words in place of pvar and of the primary slots'
+    // ranks.  pw16[j] holds the clause's primary variable relative to its
+    // tile's first (pv0[tile]) in bits 0-12, the primary literal's position
+    // in bits 13-14 (3: padding clause) and its sign in bit 15; srank then
+    // holds K - 1 ranks per clause, the other literals in clause order
+    std::vector<uint16_t> pw16;
+    std::vector<int> pv0;
     std::vector<uint16_t> srank;   // per slot: its rank in the tile's sorted list, the literal's sign
                                    // in bit 15, 0xFFFF for padding; per tile, literal q of every
                                    // clause first (index tile*TC*K + q*TC + clause in the tile)
@@ -308,7 +323,29 @@ static void build_tile_order(HostProblem& P)
     } else {
         P.spos.assign((size_t)ntiles * nper, -1);
     }
-    P.srank.assign((size_t)ntiles * per, 0xFFFF);
+    // big tiles store K - 1 ranks per clause (the primary one is in pw16)
+    const bool cp = P.split && TCt != TC;
+    const int KR = cp ? NR : K;
+    const size_t rper = (size_t)TCt * KR;
+    P.srank.assign((size_t)ntiles * rper, 0xFFFF);
+    P.pw16.clear();
+    P.pv0.clear();
+    if (cp) {
+        P.pw16.assign((size_t)ntiles * TCt, (uint16_t)(3u << PW_QSHIFT));
+        P.pv0.assign((size_t)ntiles, 0);
+        for (long long t = 0; t < ntiles; t++) {
+            const long long j0 = t * TCt, j1 = std::min<long long>(P.m, j0 + TCt);
+            const int v0 = P.pvar[(size_t)j0];
+            P.pv0[(size_t)t] = v0;
+            for (long long j = j0; j < j1; j++) {
+                const int rel = P.pvar[(size_t)j] - v0;
+                require(rel >= 0 && rel <= (int)PW_VMASK, ASAT_ERR_UNSUPPORTED, "tile's primary variables too far apart");
+                const int q = P.pq[(size_t)j];
+                const unsigned sg = P.lit[(size_t)j * K + q] < 0 ? 0x8000u : 0u;
+                P.pw16[(size_t)j] = (uint16_t)(rel | (q << PW_QSHIFT) | sg);
+            }
+        }
+    }
     std::vector<int> cnt((size_t)P.ngroups + 1, 0);
     std::vector<int> touched;
     touched.reserve(per);
@@ -342,11 +379,13 @@ static void build_tile_order(HostProblem& P)
             if (!l) continue;
             const size_t ls = s - s0;   // tile-local slot: clause ls / K, literal ls % K
             const unsigned sign = l < 0 ? 0x8000u : 0u;
-            uint16_t& rk = P.srank[s0 + (ls % K) * TCt + ls / K];
             if (P.pos[s] < 0) {   // a direct primary slot
-                rk = (uint16_t)(0x4000u | sign);
+                if (!cp) P.srank[(size_t)t * rper + (ls % K) * TCt + ls / K] = (uint16_t)(0x4000u | sign);
                 continue;
             }
+            size_t qs = ls % K;   // the slot's rank row: with primary words, among the other literals
+            if (cp && qs > P.pq[(size_t)t * TCt + ls / K]) qs--;
+            uint16_t& rk = P.srank[(size_t)t * rper + qs * TCt + ls / K];
             int g = (std::abs(l) - 1) / P.vg;
             const size_t j = (size_t)cnt[g]++;
             if (rt) {
@@ -381,6 +420,8 @@ struct StageArgs {
     const uint16_t* csr_loc;   // the same relative to the region start (LDS var pass)
     const long long* gstart;   // region starts, ngroups+1
     const int* pvar;       // direct primary slots: primary variable of each clause (processing order)
+    const uint16_t* pw16;  // big tiles: primary word of each clause (in place of pvar)
+    const int* pv0;        // big tiles: each tile's first primary variable
     const int* pc0;        // direct primary slots: first clause of each group's primaries, ngroups+1
     T* pcontrib;           // direct primary slots: each clause's primary term (processing order)
     T* contrib;
@@ -514,7 +555,10 @@ __device__ __forceinline__ void slot_terms(const int (&l)[K], const T (&f)[K], T
 // psv the clauses' direct primary spins; the literal signs come from the
 // ranks' bit 15.  Otherwise sp is the N-element stage spin vector, gathered
 // by variable.
-template <typename T, int K, int J, int BT, bool SS, bool PS, int G, int CPT>
+// CP (primary words, big tiles, K = 3): a clause's two stored ranks are its
+// other literals' in clause order; its primary word names the primary
+// literal's position and sign, from which the three slot ranks are formed.
+template <typename T, int K, int J, int BT, bool SS, bool PS, int G, int CPT, bool CP>
 __device__ __forceinline__ void tile_group(const StageArgs<T>& a, long long c0, int TS, int u0, const T* x, T* xnew,
                                            const T* sp, const T (&psv)[CPT], T hs, double h, double hnext,
                                            double floor, bool tanh_family, T* s_out, double& err, double& asum)
@@ -523,6 +567,8 @@ __device__ __forceinline__ void tile_group(const StageArgs<T>& a, long long c0, 
     T sv[G][K];
     T ax[G], a1[G], a2[G], a3[G], a4[G], a5[G];
     uint16_t rk[G][K];
+    static_assert(!CP || K == 3, "primary words: 3-literal clauses");
+    unsigned pw[G], r2[G][2];
 #pragma unroll
     for (int g = 0; g < G; g++) {
         const int lc = (u0 + g) * BT + (int)threadIdx.x;   // the clause in the tile (< TS: a clause of it)
@@ -531,8 +577,14 @@ __device__ __forceinline__ void tile_group(const StageArgs<T>& a, long long c0, 
 #pragma unroll
             for (int q = 0; q < K; q++) l[g][q] = lc < TS ? ld_g(a.lit + m * K + q) : 0;
         }
+        if constexpr (CP) {
+            pw[g] = lc < TS ? (unsigned)ld_g(a.pw16 + m) : (3u << PW_QSHIFT);
 #pragma unroll
-        for (int q = 0; q < K; q++) rk[g][q] = lc < TS ? ld_g(a.srank + c0 * K + (long long)q * TS + lc) : 0xFFFF;
+            for (int q = 0; q < 2; q++) r2[g][q] = lc < TS ? (unsigned)ld_g(a.srank + c0 * 2 + (long long)q * TS + lc) : 0xFFFFu;
+        } else {
+#pragma unroll
+            for (int q = 0; q < K; q++) rk[g][q] = lc < TS ? ld_g(a.srank + c0 * K + (long long)q * TS + lc) : 0xFFFF;
+        }
     }
 #pragma unroll
     for (int g = 0; g < G; g++) {
@@ -547,6 +599,17 @@ __device__ __forceinline__ void tile_group(const StageArgs<T>& a, long long c0, 
             if constexpr (J > 3) a3[g] = ld_g(a.k3 + ai);
             if constexpr (J > 4) a4[g] = ld_g(a.k4 + ai);
             if constexpr (J > 5) a5[g] = ld_g(a.k5 + ai);
+        }
+    }
+    if constexpr (CP) {
+        // primary position 3: a padding clause (its stored ranks are 0xFFFF)
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const unsigned pq = (pw[g] >> PW_QSHIFT) & 3u;
+            const unsigned pr = RANK_PRIMARY | (pw[g] & 0x8000u);
+            rk[g][0] = (uint16_t)(pq == 0 ? pr : r2[g][0]);
+            rk[g][1] = (uint16_t)(pq == 1 ? pr : (pq == 0 ? r2[g][0] : r2[g][1]));
+            rk[g][2] = (uint16_t)(pq == 2 ? pr : (pq == 3 ? 0xFFFFu : r2[g][1]));
         }
     }
     if constexpr (SS) {
@@ -666,11 +729,21 @@ void k_clause_tile(StageArgs<T> a)
         T psv[CPTG];
         int pv[CPTG];
         if constexpr (SS && PS) {
-            // padding clauses have primary variable 0: x[0] / tmp[0], unused
+            // padding clauses have primary variable 0 (or the tile's first
+            // primary variable): x[0] / tmp[0], unused
+            if constexpr (PL) {
+                const int v0 = a.pv0[tile];
 #pragma unroll
-            for (int u = 0; u < CPTG; u++) {
-                const int lc = u * BT + (int)threadIdx.x;
-                pv[u] = lc < TS ? ld_g(a.pvar + c0 + lc) : 0;
+                for (int u = 0; u < CPTG; u++) {
+                    const int lc = u * BT + (int)threadIdx.x;
+                    pv[u] = lc < TS ? v0 + (int)(ld_g(a.pw16 + c0 + lc) & PW_VMASK) : 0;
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < CPTG; u++) {
+                    const int lc = u * BT + (int)threadIdx.x;
+                    pv[u] = lc < TS ? ld_g(a.pvar + c0 + lc) : 0;
+                }
             }
         }
         bool runs = false;
@@ -735,8 +808,8 @@ void k_clause_tile(StageArgs<T> a)
         if constexpr (SS) __syncthreads();
 #pragma unroll
         for (int u0 = 0; u0 < CPT; u0 += G)
-            tile_group<T, K, J, BT, SS, PS, G, CPTG>(a, c0, TS, u0, x, xnew, sp, psv, hs, h, hnext, floor,
-                                                      tanh_family, s_out, err, asum);
+            tile_group<T, K, J, BT, SS, PS, G, CPTG, PL>(a, c0, TS, u0, x, xnew, sp, psv, hs, h, hnext, floor,
+                                                          tanh_family, s_out, err, asum);
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < NP; i++) {
@@ -1467,6 +1540,8 @@ struct Sys {
     uint16_t* d_csr_loc = nullptr;   // LDS var pass only
     long long* d_gstart = nullptr;
     int* d_pvar = nullptr;       // direct primary slots: primary variable per clause
+    uint16_t* d_pw16 = nullptr;  // big tiles: primary word per clause (in place of d_pvar)
+    int* d_pv0 = nullptr;        // big tiles: first primary variable of each tile
     int* d_pc0 = nullptr;        // direct primary slots: first clause of each group's primaries
     void* d_pcontrib = nullptr;  // direct primary slots: primary term per clause
     bool split = false;          // direct primary slots (tiled pass + LDS var pass)
@@ -1506,7 +1581,7 @@ struct Sys {
         dfree(d_runs);
         run_stride = 0;
         dfree(d_csr_loc); dfree(d_gstart); dfree(d_perm); dfree(d_contrib); dfree(d_apart);
-        dfree(d_pvar); dfree(d_pc0); dfree(d_pcontrib);
+        dfree(d_pvar); dfree(d_pc0); dfree(d_pcontrib); dfree(d_pw16); dfree(d_pv0);
         split = false;
         dfree(d_vptr); dfree(d_spos_small);
         perm.clear();
@@ -1624,6 +1699,12 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
         const long long rounds = std::max<long long>(1, ceil_div((long long)P.m, slots * TC_BIG));
         const long long ts = round_up(ceil_div((long long)P.m, rounds * slots), 64);
         P.tc = (int)std::min<long long>(std::max<long long>(ts, 1024), TC_BIG);
+        // primary words hold a clause's primary variable relative to its
+        // tile's first: tiles spanning more variables keep 2048 clauses
+        for (long long j0 = 0; P.tc != TC && j0 < P.m; j0 += P.tc) {
+            const long long j1 = std::min<long long>(P.m, j0 + P.tc) - 1;
+            if (P.pvar[(size_t)j1] - P.pvar[(size_t)j0] > (int)PW_VMASK) P.tc = TC;
+        }
     }
     if (tiled) build_tile_order(P);
     HIP_CHECK(hipStreamSynchronize(s->stream));
@@ -1663,8 +1744,13 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
         s->run_stride = P.run_stride;
         upload(s->d_srank, P.srank);
         if (P.split) {
-            P.pvar.resize(mpad, 0);   // padding clauses: variable 0
-            upload(s->d_pvar, P.pvar);
+            if (P.pw16.empty()) {
+                P.pvar.resize(mpad, 0);   // padding clauses: variable 0
+                upload(s->d_pvar, P.pvar);
+            } else {
+                upload(s->d_pw16, P.pw16);
+                upload(s->d_pv0, P.pv0);
+            }
             upload(s->d_pc0, P.pc0);
             // one term per clause padded to whole tiles, plus a 16-byte unit
             const size_t np = mpad + 16;
@@ -1764,7 +1850,7 @@ static StageArgs<T> stage_args(Sys* s, T* xa, T* xb, T* const* kk, T* tmp, doubl
     a.lit = s->d_lit; a.pos = s->d_pos; a.spos = s->d_spos; a.srank = s->d_srank;
     a.runs = s->d_runs; a.run_stride = s->run_stride;
     a.csr_ptr = s->d_csr_ptr; a.csr_pos = s->d_csr_pos; a.csr_loc = s->d_csr_loc; a.gstart = s->d_gstart;
-    a.pvar = s->d_pvar; a.pc0 = s->d_pc0; a.pcontrib = (T*)s->d_pcontrib; a.split = s->split ? 1 : 0;
+    a.pvar = s->d_pvar; a.pw16 = s->d_pw16; a.pv0 = s->d_pv0; a.pc0 = s->d_pc0; a.pcontrib = (T*)s->d_pcontrib; a.split = s->split ? 1 : 0;
     a.contrib = (T*)s->d_contrib;
     a.xa = xa; a.xb = xb;
     a.k1 = kk[0]; a.k2 = kk[1]; a.k3 = kk[2]; a.k4 = kk[3]; a.k5 = kk[4];
@@ -1802,8 +1888,8 @@ static void launch_clause(const StageArgs<T>& a, const Sys* s, hipStream_t st)
     const dim3 tb(TILE_BT);
     constexpr int TB = TC_BIG, TS = TC;
     const bool big = s->tc != TC;
-    require(!big || (sizeof(T) == 4 && ps && a.k == 3), ASAT_ERR_INVALID_ARGUMENT,
-            "4096-clause tiles without their layout");
+    require(!big || (sizeof(T) == 4 && ps && a.k == 3 && a.pw16 && a.pv0), ASAT_ERR_INVALID_ARGUMENT,
+            "big clause tiles without their layout");
     if constexpr (sizeof(T) == 4) {
         if (tiled && big) {
             if (ss) hipLaunchKernelGGL((k_clause_tile<T, 3, J, TILE_BT, TB, true, true>), g, tb, 0, st, a);
@@ -2676,6 +2762,7 @@ int asat_sys_layout(const asat_sys_t* sys, asat_layout_t* out)
     out->slot_spins = (s->tiled && s->lds_var) ? 1 : 0;
     out->direct_primary = s->split ? 1 : 0;
     out->run_tables = s->d_runs ? 1 : 0;
+    out->primary_words = s->d_pw16 ? 1 : 0;
     API_END
 }
 

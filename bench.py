@@ -173,7 +173,7 @@ def measure_traffic(args, kind, log):
     return (rd, wr, detail), None
 
 
-def kernel_bytes(n, m, k, nslots, esize=4):
+def kernel_bytes(n, m, k, nslots, esize=4, layout=None):
     """Bytes each plain-solver kernel moves by design (every array the kernel
     touches, once; index arrays int32/uint16, state arrays of esize bytes),
     reported as bytes_impl beside the algorithmic figure.
@@ -181,7 +181,12 @@ def kernel_bytes(n, m, k, nslots, esize=4):
     clause pass J (tiled, K = 3, 4, slot spins, direct primary slots): per
     slot a rank with the literal's sign (2) read; per slot other than its
     clause's primary one a sorted position (4) and the slot's stage spin
-    (esize) read through it and one contribution (esize) stored; per clause
+    (esize) read through it and one contribution (esize) stored; with run
+    tables (layout["run_tables"]) the sorted positions are replaced by a
+    word per variable group and tile (groups + 1 rounded up to 16) and the
+    region starts (8 per group); with primary words (layout["primary_words"])
+    the primary slots have no rank and a clause's primary variable takes 2
+    bytes, plus 4 per tile; per clause
     its primary variable (4) read and its primary term (esize) stored, plus
     the N-element stage spin vector the primary spins come from (esize per
     variable); per clause A_J aux values (x_a and k_1..k_{J-1} read, k_J or
@@ -193,9 +198,16 @@ def kernel_bytes(n, m, k, nslots, esize=4):
     L = nslots
     A = (2, 3, 4, 5, 6, 7)
     B = (3, 4, 5, 6, 7, 6)
+    idx = 4 * (L - m)
+    if layout and layout.get("run_tables"):
+        g = layout["groups"]
+        idx = 4 * layout["tiles"] * (-(-(g + 1) // 16) * 16) + 8 * (g + 1)
+    rank, prim = 2 * L, 4 * m
+    if layout and layout.get("primary_words"):
+        rank, prim = 2 * (L - m), 2 * m + 4 * layout["tiles"]
     out = {}
     for j in range(6):
-        out["clause%d" % (j + 1)] = (2 * L + (4 + 2 * esize) * (L - m) + (4 + esize) * m + esize * n
+        out["clause%d" % (j + 1)] = (rank + idx + 2 * esize * (L - m) + prim + esize * m + esize * n
                                      + esize * m * A[j])
         out["var%d" % (j + 1)] = (2 + esize) * L + esize * (L - m) + 4 * (n + 1) + esize * n * B[j]
     out["control"] = 8 * (-(-m // 1024) + -(-n // 1024))
@@ -217,9 +229,9 @@ def adjoint_bytes(L, D):
     return 96 * L + 136 * D, 96 * L + 136 * D + 6 * (24 * L + 20 * D) + 160 * D
 
 
-def attempt_bytes(n, m, k, nslots, esize=4):
+def attempt_bytes(n, m, k, nslots, esize=4, layout=None):
     """Bytes one attempt moves by design (sum over the 13 kernels)."""
-    return sum(kernel_bytes(n, m, k, nslots, esize).values())
+    return sum(kernel_bytes(n, m, k, nslots, esize, layout).values())
 
 
 def host_cores():
@@ -724,7 +736,8 @@ def main():
 
     nslots = int(np.count_nonzero(prob.literals)) if len(prob.literals) else 0
     esz = 4 if args.precision == "f32" else 8
-    kb = kernel_bytes(n, m, k, nslots, esz)
+    lay = sat.Layout()
+    kb = kernel_bytes(n, m, k, nslots, esz, lay)
     kernels = {nm: {"us": round(float(us[i]), 2), "GBps": round(float(kb[nm] / (us[i] * 1e-6) / 1e9), 1)}
                for i, nm in enumerate(names)}
     log("kernels (one kind at a time): " + ", ".join("%s %.1fus %.0fGB/s" % (nm, v["us"], v["GBps"])
@@ -734,7 +747,7 @@ def main():
     ab = alg_kernel_bytes(n, m, nslots, esz)
     achieved = float(ab[dom] / (dom_us * 1e-6) / 1e9)
     attempt_us = float(us[:13].sum())
-    att_bytes = attempt_bytes(n, m, k, nslots, esz)
+    att_bytes = attempt_bytes(n, m, k, nslots, esz, lay)
     b_alg, b_floor = alg_attempt_bytes(n, m, nslots, esz)
     att_rate = attempts / wall
     traffic, traffic_detail = None, None

@@ -139,6 +139,7 @@ typedef struct {
     int direct_primary;  /* 1: each clause's primary slot bypasses the regions (own spin read and term store) */
     int tile_clauses;    /* clauses per tile of the tiled clause pass (0 otherwise) */
     int run_tables;      /* 1: the tiles' sorted positions are rebuilt from per-tile run tables */
+    int primary_words;   /* 1: a 16-bit word per clause names its primary variable, literal and sign */
 } asat_layout_t;
 int asat_sys_layout(const asat_sys_t* sys, asat_layout_t* out);
 

@@ -55,7 +55,7 @@ def test_c2_layout_rhs_and_steps_bitexact():
     # tiles sized to fill whole rounds of two workgroups per CU (256 CUs)
     assert lay["direct_primary"] == 1 and lay["tile_clauses"] == 4160
     assert lay["tiles"] == -(-m // 4160) == 1010
-    assert lay["run_tables"] == 1
+    assert lay["run_tables"] == 1 and lay["primary_words"] == 1
     # standalone RHS (spin gathers): bit-identical to the float oracle
     x, d = sat.MakeState(), sat.MakeState()
     x.CopyFrom(pub)

@@ -524,15 +524,44 @@ def test_big_tile_sizes_bitexact(n, m, runs):
     assert lay["direct_primary"] == 1 and lay["tile_clauses"] % 64 == 0 and lay["tile_clauses"] <= 4608
     assert lay["tiles"] * lay["tile_clauses"] >= m
     assert lay["run_tables"] == int(4 * (lay["groups"] + 1) <= 2 * lay["tile_clauses"])
+    assert lay["primary_words"] == 1
     if lay["tile_clauses"] % 1024 == 0 and runs:   # the size follows the CU count (256 CUs here)
         pytest.skip("this device sizes the tiles to a multiple of the workgroup")
     assert lay["run_tables"] == runs
+    _rhs_and_steps_bitexact(sat, p, n, m)
+
+
+def test_far_apart_primaries_keep_2048_clause_tiles():
+    """N = 1e6, M = 1e5 (FP32, 3 literals): a tile of 1024 clauses would span
+    about 10 000 primary variables, more than a 16-bit primary word's 13
+    relative-variable bits hold, so the system keeps 2048-clause tiles with
+    a rank for every slot and 4-byte primary variables.  The standalone RHS
+    and 12 integrator steps bit-identical to the oracle's mixed mode."""
+    n, m = 1000000, 100000
+    say("far-apart primaries N=%d M=%d" % (n, m))
+    p = O.fastsat(n, m, 3, 32)
+    sat = _sys_with(p.literals, p.lengths, "f32")
+    lay = sat.Layout()
+    say("  layout %s" % lay)
+    assert lay["direct_primary"] == 1 and lay["tile_clauses"] == 2048
+    assert lay["primary_words"] == 0 and lay["run_tables"] == 0
+    _rhs_and_steps_bitexact(sat, p, n, m)
+
+
+def _rhs_and_steps_bitexact(sat, p, n, m):
+    # N is the largest variable the clauses name (SatProblem), which random
+    # instances with M << N can leave below the generator's N
+    n, m = p.n, p.m
+    assert (sat.Get_N(), sat.Get_M()) == (n, m)
     pub = np.concatenate([np.random.default_rng(8).uniform(-1, 1, n), np.ones(m)]).astype(np.float32)
     x, d = sat.MakeState(), sat.MakeState()
     x.CopyFrom(pub)
     sat.GetDerivatives(d, x, 0.0)
     want = O.rhs(p, np.concatenate([[-1.0], pub]).astype(np.float32), mode="mix")[1:]
-    assert np.array_equal(d.CopyTo(), want)
+    got = d.CopyTo()
+    bad = np.flatnonzero(got != want)
+    assert bad.size == 0, ("%d spin / %d aux entries differ; first %s; got %s want %s"
+                           % ((bad < n).sum(), (bad >= n).sum(), bad[:6], got[bad[:3]], want[bad[:3]]))
     rk = A.HipRungeKutta()
     rk.SetEpsilon(1e-6)
     rk.Configure(sat)
