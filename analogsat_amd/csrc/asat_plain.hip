@@ -690,14 +690,15 @@ __device__ __forceinline__ void tile_group(const StageArgs<T>& a, long long c0, 
 // problem (tile_clauses) so that the tiles fill whole rounds of resident
 // workgroups: about 4100-clause tiles give each region runs of about 8
 // entries per tile in place of 4, and no nearly empty last round (one
-// extra tile running alone cost about 5 us per pass).  Taking their clauses
-// two at a time in phase 1 keeps the registers of 2048-clause tiles.
+// extra tile running alone cost about 5 us per pass).  Phase 1 takes their
+// clauses one at a time, which keeps them at 8 waves per SIMD without
+// spills; two at a time spilled at J = 6 and ran 1-4 us slower per pass.
 template <typename T, int K, int J, int BT, int TCt, bool SS, bool PS>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(TCt > TC ? 8 : 1)))
 void k_clause_tile(StageArgs<T> a)
 {
     constexpr int CPT = (TCt + BT - 1) / BT;
-    constexpr int G = CPT < 2 ? CPT : 2;
+    constexpr int G = TCt > TC ? 1 : (CPT < 2 ? CPT : 2);
     constexpr int CPTG = (CPT + G - 1) / G * G;   // clause slots of the groups (past CPT: none of the tile)
     constexpr int NR = PS ? K - 1 : K;
     constexpr int NP = CPT * NR;
@@ -1690,7 +1691,7 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
         HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     }
     // FP32 3-literal clauses with direct primary slots: big tiles (the
-    // kernel takes their clauses two at a time, and two workgroups still fit
+    // kernel takes their clauses one at a time, and two workgroups still fit
     // on a CU), sized so that they fill whole rounds of the 2 x CUs resident
     // workgroups: C2 (M = 4.2e6) gets 1010 tiles of 4160 clauses, two rounds
     P.tc = TC;
