@@ -103,7 +103,7 @@ struct HostProblem {
     std::vector<uint16_t> csr_loc; // the same relative to the variable's region start
     // tiled clause pass: per tile of TC clauses, its slots sorted by position
     std::vector<int> spos;         // ntiles*TC*k positions (-1 past the tile's last slot)
-    // big tiles with few groups: per tile, a run table in place of spos.
+    // big tiles: per tile, a run table in place of spos.
     // Word g of tile t: bits 0-15 the first rank of group g's run in the
     // tile's sorted list, bits 16-31 the run's first position relative to
     // the region start gstart[g]; word ngroups holds the tile's slot count
@@ -310,11 +310,12 @@ static void build_tile_order(HostProblem& P)
     const int NR = P.split ? K - 1 : K;
     const long long ntiles = ceil_div((long long)P.m, TCt);
     const size_t per = (size_t)TCt * K, nper = (size_t)TCt * NR;
-    // run tables: a tile's positions in one region are consecutive (tiles are
-    // ranges of slots, region positions follow slot order), so the sorted
-    // list is one run per group.  Used for big tiles when the table (a word
-    // per group) is at most a quarter of the positions it replaces
-    const bool rt = P.split && TCt != TC && (long long)(P.ngroups + 1) * 4 <= (long long)nper;
+    // run tables (big tiles): a tile's positions in one region are
+    // consecutive (tiles are ranges of slots, region positions follow slot
+    // order), so the sorted list is one run per group
+    const bool rt = P.split && TCt != TC;
+    require(!rt || (long long)(P.ngroups + 1) * 4 <= (long long)nper, ASAT_ERR_UNSUPPORTED,
+            "run tables larger than the positions they replace");
     P.runs.clear();
     P.run_stride = 0;
     if (rt) {
@@ -413,7 +414,7 @@ struct StageArgs {
     const int* pos;        // contribution position of each slot (-1: padding); generic-K pass
     const int* spos;       // tiled pass: per tile, positions in ascending order (-1: none)
     const uint16_t* srank; // tiled pass: rank of each slot in its tile's sorted list
-    const unsigned* runs;  // big tiles: per tile, run table in place of spos (null: spos)
+    const unsigned* runs;  // big tiles: per tile, run table in place of spos
     int run_stride;        // words per tile of the run tables
     const int* csr_ptr;
     const int* csr_pos;    // per-variable contribution positions (gather var pass)
@@ -747,9 +748,7 @@ void k_clause_tile(StageArgs<T> a)
                 }
             }
         }
-        bool runs = false;
-        if constexpr (PL) runs = a.runs != nullptr;
-        if (runs) {
+        if constexpr (PL) {
             // the sorted positions from the tile's run table: group g's
             // ranks [j0(g), j0(g+1)) sit at consecutive positions from
             // gstart[g] + off(g); ranks past the tile's slots get -1
@@ -795,15 +794,6 @@ void k_clause_tile(StageArgs<T> a)
             for (int i = 0; i < NP; i++) {
                 const int lr = i * BT + (int)threadIdx.x;
                 if (lr < TS * NR) s_out[lr] = so[i];
-            }
-        }
-        if constexpr (PL) {
-            if (!runs) {
-#pragma unroll
-                for (int i = 0; i < NP; i++) {
-                    const int lr = i * BT + (int)threadIdx.x;
-                    if (lr < TS * NR) s_pos[lr] = ps[i];
-                }
             }
         }
         if constexpr (SS) __syncthreads();
@@ -1534,7 +1524,7 @@ struct Sys {
     int* d_pos = nullptr;        // generic-K clause pass only
     int* d_spos = nullptr;       // tiled clause pass (K = 3, 4)
     uint16_t* d_srank = nullptr;
-    unsigned* d_runs = nullptr;  // big tiles with few groups: run tables in place of d_spos
+    unsigned* d_runs = nullptr;  // big tiles: run tables in place of d_spos
     int run_stride = 0;
     int* d_csr_ptr = nullptr;
     int* d_csr_pos = nullptr;    // gather var pass only
@@ -1693,13 +1683,21 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
     // FP32 3-literal clauses with direct primary slots: big tiles (the
     // kernel takes their clauses one at a time, and two workgroups still fit
     // on a CU), sized so that they fill whole rounds of the 2 x CUs resident
-    // workgroups: C2 (M = 4.2e6) gets 1010 tiles of 4160 clauses, two rounds
+    // workgroups: C2 (M = 4.2e6) gets 1010 tiles of 4160 clauses, two rounds.
+    // Problems too small to give every resident workgroup 2048 clauses keep
+    // 2048-clause tiles: with few waves per CU a thread's clauses taken one
+    // after another expose their load latency (N = 2000: 16-18 us per pass
+    // against 5-7 us; M = 6e5 in 1216-clause tiles: 11 us against 8 us)
     P.tc = TC;
     if (tiled && P.split && s->esize == 4 && P.k == 3) {
         const long long slots = 2LL * (ncu > 0 ? ncu : 1);
         const long long rounds = std::max<long long>(1, ceil_div((long long)P.m, slots * TC_BIG));
         const long long ts = round_up(ceil_div((long long)P.m, rounds * slots), 64);
-        P.tc = (int)std::min<long long>(std::max<long long>(ts, 1024), TC_BIG);
+        P.tc = ts < TC ? TC : (int)std::min<long long>(ts, TC_BIG);
+        // big tiles rebuild their sorted positions from run tables, a word
+        // per variable group: worth it while that is at most a quarter of
+        // the positions (ngroups + 1 <= tc (K - 1) / 4)
+        if (P.tc != TC && (long long)(P.ngroups + 1) * 4 > (long long)P.tc * (P.k - 1)) P.tc = TC;
         // primary words hold a clause's primary variable relative to its
         // tile's first: tiles spanning more variables keep 2048 clauses
         for (long long j0 = 0; P.tc != TC && j0 < P.m; j0 += P.tc) {
@@ -1889,7 +1887,7 @@ static void launch_clause(const StageArgs<T>& a, const Sys* s, hipStream_t st)
     const dim3 tb(TILE_BT);
     constexpr int TB = TC_BIG, TS = TC;
     const bool big = s->tc != TC;
-    require(!big || (sizeof(T) == 4 && ps && a.k == 3 && a.pw16 && a.pv0), ASAT_ERR_INVALID_ARGUMENT,
+    require(!big || (sizeof(T) == 4 && ps && a.k == 3 && a.pw16 && a.pv0 && a.runs), ASAT_ERR_INVALID_ARGUMENT,
             "big clause tiles without their layout");
     if constexpr (sizeof(T) == 4) {
         if (tiled && big) {

@@ -505,47 +505,63 @@ def test_new_problem_invalidates_states_and_integrator():
     assert rk.Step(a, b, 3)
 
 
-@pytest.mark.parametrize("n, m, runs", [(140000, 600000, 1), (520000, 2200000, 1), (600000, 500000, 0)])
-def test_big_tile_sizes_bitexact(n, m, runs):
-    """FP32 3-literal systems use tiles sized per problem to fill whole rounds
-    of two workgroups per CU (up to 4608 clauses, five for some threads; the
-    kernel takes a thread's clauses two at a time).  On 256 CUs the first two
-    get tiles of 1216 and 4352 clauses, not multiples of the workgroup size,
-    and rebuild their sorted positions from run tables (256 groups); the third
-    gets 1024-clause tiles over 512 groups, where a run table would exceed a
-    quarter of the positions, and reads the positions themselves.  The
-    standalone RHS and 12 integrator steps bit-identical to the oracle's
-    mixed mode."""
+@pytest.mark.parametrize("n, m", [(310000, 1300000), (520000, 2200000)])
+def test_big_tile_sizes_bitexact(n, m):
+    """FP32 3-literal systems large enough to give every resident workgroup
+    (two per CU) at least 2048 clauses use tiles sized per problem to fill
+    whole rounds (up to 4608 clauses, five for some threads; the kernel
+    takes a thread's clauses one at a time), rebuild their sorted positions
+    from per-tile run tables and carry 16-bit primary words.  On 256 CUs
+    these get tiles of 2560 and 4352 clauses, not multiples of the
+    workgroup size.  The standalone RHS and 12 integrator steps
+    bit-identical to the oracle's mixed mode."""
     say("big tiles N=%d M=%d" % (n, m))
     p = O.fastsat(n, m, 3, 31)
     sat = _sys_with(p.literals, p.lengths, "f32")
     lay = sat.Layout()
     say("  layout %s" % lay)
-    assert lay["direct_primary"] == 1 and lay["tile_clauses"] % 64 == 0 and lay["tile_clauses"] <= 4608
+    assert lay["direct_primary"] == 1 and lay["tile_clauses"] % 64 == 0 and 2048 < lay["tile_clauses"] <= 4608
     assert lay["tiles"] * lay["tile_clauses"] >= m
-    assert lay["run_tables"] == int(4 * (lay["groups"] + 1) <= 2 * lay["tile_clauses"])
-    assert lay["primary_words"] == 1
-    if lay["tile_clauses"] % 1024 == 0 and runs:   # the size follows the CU count (256 CUs here)
+    assert lay["run_tables"] == 1 and lay["primary_words"] == 1
+    assert 4 * (lay["groups"] + 1) <= 2 * lay["tile_clauses"]
+    if lay["tile_clauses"] % 1024 == 0:   # the size follows the CU count (256 CUs here)
         pytest.skip("this device sizes the tiles to a multiple of the workgroup")
-    assert lay["run_tables"] == runs
     _rhs_and_steps_bitexact(sat, p, n, m)
 
 
 def test_far_apart_primaries_keep_2048_clause_tiles():
-    """N = 1e6, M = 1e5 (FP32, 3 literals): a tile of 1024 clauses would span
-    about 10 000 primary variables, more than a 16-bit primary word's 13
-    relative-variable bits hold, so the system keeps 2048-clause tiles with
-    a rank for every slot and 4-byte primary variables.  The standalone RHS
-    and 12 integrator steps bit-identical to the oracle's mixed mode."""
-    n, m = 1000000, 100000
+    """N = 2.2e6, M = 1.1e6 (FP32, 3 literals): big enough for tiles of 2176
+    clauses, but some of those tiles would span up to about 18 000 primary
+    variables (half of them about 3 400), more than a 16-bit primary word's
+    13 relative-variable bits hold; its 1280 groups would also make a run
+    table larger than a quarter of the positions.  The system keeps
+    2048-clause tiles with sorted positions, a rank for every slot and
+    4-byte primary variables.  The standalone RHS and 12 integrator steps
+    bit-identical to the oracle's mixed mode."""
+    n, m = 2200000, 1100000
     say("far-apart primaries N=%d M=%d" % (n, m))
-    p = O.fastsat(n, m, 3, 32)
+    p = O.fastsat(n, m, 3, 31)
     sat = _sys_with(p.literals, p.lengths, "f32")
     lay = sat.Layout()
     say("  layout %s" % lay)
-    assert lay["direct_primary"] == 1 and lay["tile_clauses"] == 2048
+    assert lay["direct_primary"] == 1 and lay["tile_clauses"] == 2048 and lay["groups"] == 1280
     assert lay["primary_words"] == 0 and lay["run_tables"] == 0
     _rhs_and_steps_bitexact(sat, p, n, m)
+
+
+def test_small_systems_keep_2048_clause_tiles():
+    """N = 2000, M = 8500 and N = 140000, M = 6e5 (FP32, 3 literals): too
+    few clauses for 2048 per resident workgroup, so 2048-clause tiles (the
+    per-problem size would be 1024 and 1216); RHS and 12 steps
+    bit-identical to the oracle's mixed mode."""
+    for n, m in ((2000, 8500), (140000, 600000)):
+        say("small system N=%d M=%d" % (n, m))
+        p = O.fastsat(n, m, 3, 33)
+        sat = _sys_with(p.literals, p.lengths, "f32")
+        lay = sat.Layout()
+        assert lay["direct_primary"] == 1 and lay["tile_clauses"] == 2048, lay
+        assert lay["primary_words"] == 0 and lay["run_tables"] == 0
+        _rhs_and_steps_bitexact(sat, p, n, m)
 
 
 def _rhs_and_steps_bitexact(sat, p, n, m):
