@@ -20,8 +20,8 @@
 //   u_s[p] = w_a (-2 a c_p K K_p) + w_p (-2 a K_p^2) + sum_{q!=p} w_q (-4 a c_q c_p K K_qp)
 // Per-variable sums run over each variable's slots in slot order
 // (deterministic).  FP32 terms are written at their var-major positions (a
-// variable's terms are one contiguous run; 510 KB per C5 trajectory, L2-
-// resident); FP64 terms stay in clause order and are gathered through the
+// variable's terms are one contiguous run; 510 KB per trajectory at
+// N = 1e4, M = 42 500, L2-resident); FP64 terms stay in clause order and are gathered through the
 // var-major slot list (scattered FP64 writes cost more than the gathers).
 // Internal layout of every vector: [0 | s_1..s_N | a_1..a_M], stride D.
 #include "asat_common.h"

@@ -1683,7 +1683,7 @@ static void sys_set_problem(Sys* s, const int* lits, const int* lens, int nclaus
     // FP32 3-literal clauses with direct primary slots: big tiles (the
     // kernel takes their clauses one at a time, and two workgroups still fit
     // on a CU), sized so that they fill whole rounds of the 2 x CUs resident
-    // workgroups: C2 (M = 4.2e6) gets 1010 tiles of 4160 clauses, two rounds.
+    // workgroups: N = 1e6, M = 4.2e6 gets 1010 tiles of 4160 clauses, two rounds.
     // Problems too small to give every resident workgroup 2048 clauses keep
     // 2048-clause tiles: with few waves per CU a thread's clauses taken one
     // after another expose their load latency (N = 2000: 16-18 us per pass
@@ -2090,7 +2090,7 @@ static void rk_configure(RK* r, Sys* s)
         r->sys = s;
         r->gen = s->gen;
         // the one-workgroup path keeps everything in LDS and reads none of
-        // these buffers (C1: a caller solving start after start)
+        // these buffers (small instances: a caller solving start after start)
         if (!small_path(r)) {
             const size_t bytes = (size_t)s->esize * s->alloc;
             for (auto& p : r->k) HIP_CHECK(hipMemsetAsync(p, 0, bytes, s->stream));

@@ -1,6 +1,6 @@
 // One-workgroup persistent integrator for problems that fit in LDS.
 //
-// The plain solver's attempt is 13 kernels; for a small instance (C1-sized:
+// The plain solver's attempt is 13 kernels; for a small instance (such as
 // N = 50, M = 213) every one of them is a few microseconds of dispatch for
 // well under a microsecond of work.  Here one workgroup holds the whole
 // integrator in LDS -- state x, x_new, k1..k5, the stage spin input, the slot
