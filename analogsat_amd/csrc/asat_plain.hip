@@ -703,6 +703,7 @@ void k_clause_tile(StageArgs<T> a)
     constexpr int CPTG = (CPT + G - 1) / G * G;   // clause slots of the groups (past CPT: none of the tile)
     constexpr int NR = PS ? K - 1 : K;
     constexpr int NP = CPT * NR;
+    static_assert(TCt * NR <= (int)RANK_MASK + 1, "a tile's ranks must fit the rank field");
     // the sorted positions of big tiles wait in LDS for phase 2
     constexpr bool PL = TCt > TC;
     __shared__ double sh[BT / 64];
